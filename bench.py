@@ -1,0 +1,200 @@
+"""Intersection throughput benchmark (one JSON line on stdout).
+
+Workload: 2^24 rays against the SAH BVH of a seeded noise-displaced icosphere
+of 1,310,720 triangles (subdivision level 8). Two ray sets live in HBM before
+timing starts: coherent (pinhole camera, 4096 x 4096 pixel-centre rays,
+generated by the camera kernel) and incoherent (origins uniform in the scene
+box, directions uniform on the sphere). One step = four launches over the
+whole set: closest-hit and any-hit on each ray set, i.e. 4 * 2^24 BVH queries.
+
+`value` is queries per second over the timed steps (Mrays/s). The roofline
+object is computed for the slowest of the four kernels from its algorithmic
+bytes per launch (32 B ray + 16 B hit per ray, 32 B per box test, 36 B per
+triangle test, 64 B per sphere test, counted by the kernel's counting mode)
+over its HIP-event-timed average duration. The CPU baseline is the oracle
+(C++ restatement of the reference traversal) on a bounded random sample of
+the same four query sets, using every host core available to this process.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Mrays/s + time-to-relMSE≤0.02 at 1080p, neural-guided vs plain GPU path"
+HBM_PEAK_GBS = 8000.0
+LEVEL = 8
+N_RAYS = 1 << 24
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def alg_bytes(n_rays: int, st) -> float:
+    return 32.0 * n_rays + 16.0 * n_rays + 32.0 * st.nodes_visited + 36.0 * st.tris_tested + 64.0 * st.spheres_tested
+
+
+def cpu_baseline(scene, bvh, sets, target_s: float):
+    """Times the oracle's reference-order traversal on a random sample of each query set."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import cpu_oracle as O
+
+    cores = len(os.sched_getaffinity(0))
+    o = O.OracleScene(scene)
+    nodes, slots = bvh.bvh()
+    o.use_bvh(nodes, slots)
+    rng = np.random.default_rng(1234)
+    # calibrate on a small sample, then size the sample to about target_s of CPU time
+    probe = 1 << 13
+    t0 = time.perf_counter()
+    for name, rays, any_hit in sets:
+        o.intersect(rays[rng.integers(0, rays.shape[0], probe)], any_hit=any_hit, threads=cores)
+    dt = time.perf_counter() - t0
+    per_set = int(min(N_RAYS, max(probe, probe * target_s / max(dt, 1e-6))))
+    samples = [(rays[rng.integers(0, rays.shape[0], per_set)], any_hit) for _, rays, any_hit in sets]
+    t0 = time.perf_counter()
+    outs = [o.intersect(r, any_hit=a, threads=cores) for r, a in samples]
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(per_set * len(sets) / dt / 1e6, 3),
+        "unit": "Mrays/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{per_set} random rays from each of the 4 query sets ({per_set * len(sets)} queries, {dt:.1f} s)",
+    }, samples, outs
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--stack", type=int, default=0)
+    ap.add_argument("--blocks-per-cu", type=int, default=-1)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        return 2
+
+    import torch
+
+    from bonsai_amd import Context, GpuBvh
+    from bonsai_amd.scenes import incoherent_rays, intersection_camera, intersection_scene
+
+    ctx = Context(0)
+    if args.block or args.stack or args.blocks_per_cu >= 0:
+        b, s, c = ctx.launch()
+        ctx.set_launch(args.block or b, args.stack or s, c if args.blocks_per_cu < 0 else args.blocks_per_cu)
+    t0 = time.perf_counter()
+    scene = intersection_scene(LEVEL)
+    gen_s = time.perf_counter() - t0
+    bvh = GpuBvh(ctx, scene)
+    log(f"scene: {scene.n_tris} tris, {bvh.info.n_nodes} nodes, height {bvh.info.max_depth}, "
+        f"build+upload {bvh.info.build_ms:.0f} ms, generation {gen_s * 1e3:.0f} ms")
+
+    dev = torch.device("cuda:0")
+    cam = intersection_camera(4096, 4096)
+    coh = torch.empty((N_RAYS, 8), dtype=torch.float32, device=dev)
+    ctx.camera_rays_device(cam, coh.data_ptr())
+    lo, hi = scene.bounds()
+    inc_host = incoherent_rays(N_RAYS, lo, hi, 99)
+    inc = torch.from_numpy(inc_host).to(dev)
+    hits = torch.empty((N_RAYS, 4), dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    queries = [("coherent_closest", coh, False), ("coherent_any", coh, True),
+               ("incoherent_closest", inc, False), ("incoherent_any", inc, True)]
+
+    # algorithmic bytes per launch from the kernel's counting mode (untimed)
+    counts = {}
+    for name, rays, any_hit in queries:
+        st = bvh.intersect_device(rays.data_ptr(), N_RAYS, hits.data_ptr(), any_hit, count=True)
+        counts[name] = (alg_bytes(N_RAYS, st), st.nodes_visited / N_RAYS, st.tris_tested / N_RAYS)
+        log(f"{name}: {st.nodes_visited / N_RAYS:.1f} box tests/ray, {st.tris_tested / N_RAYS:.1f} tri tests/ray")
+
+    def step(record):
+        for name, rays, any_hit in queries:
+            st = bvh.intersect_device(rays.data_ptr(), N_RAYS, hits.data_ptr(), any_hit)
+            if record is not None:
+                record.setdefault(name, []).append(st.kernel_ms)
+
+    for _ in range(args.warmup):
+        step(None)
+    torch.cuda.synchronize()
+    kms: dict = {}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(kms)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    total = N_RAYS * len(queries) * args.steps
+    value = total / wall / 1e6
+    breakdown = {}
+    for name, _, _ in queries:
+        ms = float(np.mean(kms[name]))
+        b = counts[name][0]
+        breakdown[name] = {"Mrays_s": round(N_RAYS / ms / 1e3, 1), "kernel_ms": round(ms, 3),
+                           "alg_GB_s": round(b / ms / 1e6, 1), "box_tests_per_ray": round(counts[name][1], 2),
+                           "tri_tests_per_ray": round(counts[name][2], 2)}
+    dom = max(breakdown, key=lambda k: breakdown[k]["kernel_ms"])
+    achieved = breakdown[dom]["alg_GB_s"]
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": f"k_traverse ({dom})",
+                "bytes_per_launch": counts[dom][0]}
+
+    cpu = None
+    if not args.no_cpu:
+        sets = [(name, (coh if name.startswith("coherent") else None), a) for name, _, a in queries]
+        coh_host = coh.cpu().numpy()
+        sets = [(n, coh_host if n.startswith("coherent") else inc_host, a) for n, _, a in sets]
+        cpu, samples, outs = cpu_baseline(scene, bvh, sets, args.cpu_seconds)
+        # the sampled GPU results must equal the oracle's
+        for (r, a), ref in zip(samples, outs):
+            g = (bvh.intersect_test(r) if a else bvh.intersect(r)).view(np.uint32).reshape(-1, 4)
+            if not np.array_equal(g[:, 0], ref[:, 0]):
+                log("PARITY FAILURE on the CPU-baseline sample")
+                cpu["parity"] = False
+                break
+        else:
+            cpu["parity"] = True
+
+    b, s_, c = ctx.launch()
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "intersection microbenchmark: 2^24 coherent + 2^24 incoherent rays, closest-hit "
+                               "and any-hit, SAH BVH over a seeded displaced icosphere",
+                   "triangles": scene.n_tris, "rays_per_set": N_RAYS, "queries_per_step": N_RAYS * len(queries),
+                   "launch": {"block": b, "stack": s_, "blocks_per_cu": c}},
+        "breakdown": breakdown,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

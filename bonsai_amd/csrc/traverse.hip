@@ -1,0 +1,177 @@
+// Closest-hit and any-hit BVH traversal kernels for gfx950.
+//
+// One lane traces one ray through the flattened BVH in exactly the order of
+// accelerators/bvh.rs:281-423: each visited node's box is tested with the
+// ray's current t_max; interior nodes push the far child and descend to the
+// near one (chosen by the direction sign on the split axis); leaves test
+// their primitives in slot order and, for closest hit, shrink t_max on every
+// accepted hit (primitives/geometric.rs:32). Because each lane replays that
+// sequence exactly, the accepted primitive, t and barycentrics equal the
+// reference's, ties included.
+//
+// The per-lane traversal stack lives in LDS, laid out [entry][lane] so lanes
+// of a wavefront hit distinct banks. It is a ring of `STACK` entries; on
+// overflow its oldest entry moves to a per-thread global spill row (sized
+// from the tree height at scene build), so any depth is handled.
+#include <hip/hip_runtime.h>
+
+#include "bg_internal.h"
+#include "dev_geom.h"
+
+namespace bg {
+namespace {
+
+using namespace dev;
+
+template <int BLOCK, int STACK, bool ANY, bool COUNT>
+__global__ __launch_bounds__(BLOCK) void k_traverse(TraverseArgs a) {
+    __shared__ uint32_t lds[STACK * BLOCK];
+    const int tid = threadIdx.x;
+    const uint64_t gtid = (uint64_t)blockIdx.x * BLOCK + tid;
+    const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+    uint32_t* __restrict__ spill = a.spill + gtid * a.spill_depth;
+    uint32_t* __restrict__ ring = lds + tid;
+    constexpr uint32_t kMask = STACK - 1;
+    uint64_t c_nodes = 0, c_tris = 0, c_sph = 0;
+
+    for (uint64_t ray = gtid; ray < a.n_rays; ray += stride) {
+        const float4 r0 = a.rays[2 * ray];
+        const float4 r1 = a.rays[2 * ray + 1];
+        RayState rs;
+        rs.ox = r0.x; rs.oy = r0.y; rs.oz = r0.z; rs.t_max = r0.w;
+        rs.dx = r1.x; rs.dy = r1.y; rs.dz = r1.z;
+        ray_setup(rs);
+
+        int best = -1;
+        float bt = 0.0f, bb1 = 0.0f, bb2 = 0.0f;
+        uint32_t sp = 0, base = 0;  // logical stack size, first logical entry held in LDS
+        uint32_t cur = 0;
+        for (;;) {
+            const float4 n0 = a.nodes[2 * cur];
+            const float4 n1 = a.nodes[2 * cur + 1];
+            if (COUNT) ++c_nodes;
+            bool descend = false;
+            if (slab(rs, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y)) {
+                const uint32_t off = __float_as_uint(n1.z);
+                const uint32_t meta = __float_as_uint(n1.w);
+                const uint32_t count = meta >> 3;
+                if (count > 0) {
+                    const bool mixed = (meta & 4u) != 0;
+                    bool done = false;
+                    for (uint32_t s = off; s < off + count; ++s) {
+                        float t, b1, b2;
+                        bool hit;
+                        uint32_t pid = 0;
+                        if (mixed && (pid = a.slot_prim[s]) >= a.n_tris) {
+                            if (COUNT) ++c_sph;
+                            const SphereDev& sp_ = a.spheres[pid - a.n_tris];
+                            SphereHit h;
+                            hit = sphere_hit(rs, sp_.w2o, sp_.radius, sp_.z_min, sp_.z_max, sp_.phi_max, &h);
+                            if (hit) {
+                                t = h.t;
+                                b1 = h.phi / sp_.phi_max;
+                                const SphereAux ax = a.sphere_aux[pid - a.n_tris];
+                                float c = h.pz / sp_.radius;
+                                c = c < -1.0f ? -1.0f : (c > 1.0f ? 1.0f : c);
+                                const float theta = acosf(c);
+                                b2 = (theta - ax.theta_min) / (ax.theta_max - ax.theta_min);
+                            }
+                        } else {
+                            if (COUNT) ++c_tris;
+                            hit = tri_hit(rs, a.tri_verts + 9 * (uint64_t)s, &t, &b1, &b2);
+                        }
+                        if (hit) {
+                            best = (int)s;
+                            bt = t;
+                            if (ANY) {
+                                bb1 = 0.0f; bb2 = 0.0f;
+                                done = true;
+                                break;
+                            }
+                            bb1 = b1; bb2 = b2;
+                            rs.t_max = t;
+                        }
+                    }
+                    if (ANY && done) break;
+                } else {
+                    // interior: push far child, go to near child (bvh.rs:331-339)
+                    const uint32_t axis = meta & 3u;
+                    const int neg = axis == 0 ? rs.neg0 : (axis == 1 ? rs.neg1 : rs.neg2);
+                    const uint32_t far = neg ? cur + 1 : off;
+                    cur = neg ? off : cur + 1;
+                    if (sp - base == STACK) {
+                        spill[base] = ring[(base & kMask) * BLOCK];
+                        ++base;
+                    }
+                    ring[(sp & kMask) * BLOCK] = far;
+                    ++sp;
+                    descend = true;
+                }
+            }
+            if (!descend) {
+                if (sp == 0) break;
+                --sp;
+                if (sp >= base) {
+                    cur = ring[(sp & kMask) * BLOCK];
+                } else {
+                    cur = spill[sp];
+                    base = sp;
+                }
+            }
+        }
+        uint4 out;
+        out.x = best >= 0 ? a.slot_prim[best] : 0xFFFFFFFFu;
+        out.y = __float_as_uint(bt);
+        out.z = __float_as_uint(bb1);
+        out.w = __float_as_uint(bb2);
+        a.hits[ray] = out;
+    }
+    if (COUNT) {
+        atomicAdd(&a.counters[0], (unsigned long long)c_nodes);
+        atomicAdd(&a.counters[1], (unsigned long long)c_tris);
+        atomicAdd(&a.counters[2], (unsigned long long)c_sph);
+    }
+}
+
+template <int BLOCK, int STACK>
+hipError_t launch2(const TraverseArgs& a, bool any, bool count, int grid, hipStream_t s) {
+    if (count) {
+        if (any) hipLaunchKernelGGL((k_traverse<BLOCK, STACK, true, true>), dim3(grid), dim3(BLOCK), 0, s, a);
+        else hipLaunchKernelGGL((k_traverse<BLOCK, STACK, false, true>), dim3(grid), dim3(BLOCK), 0, s, a);
+    } else {
+        if (any) hipLaunchKernelGGL((k_traverse<BLOCK, STACK, true, false>), dim3(grid), dim3(BLOCK), 0, s, a);
+        else hipLaunchKernelGGL((k_traverse<BLOCK, STACK, false, false>), dim3(grid), dim3(BLOCK), 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+template <int BLOCK>
+hipError_t launch1(const TraverseArgs& a, bool any, bool count, int stack, int grid, hipStream_t s) {
+    switch (stack) {
+        case 8: return launch2<BLOCK, 8>(a, any, count, grid, s);
+        case 16: return launch2<BLOCK, 16>(a, any, count, grid, s);
+        case 32: return launch2<BLOCK, 32>(a, any, count, grid, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+bool traverse_shape_supported(const LaunchShape& s) {
+    const bool b = s.block == 64 || s.block == 128 || s.block == 256;
+    const bool st = s.stack == 8 || s.stack == 16 || s.stack == 32;
+    return b && st && s.blocks_per_cu >= 0;
+}
+
+hipError_t launch_traverse(const TraverseArgs& a, bool any_hit, bool count, const LaunchShape& shape, int grid,
+                           hipStream_t stream) {
+    if (a.n_rays == 0) return hipSuccess;
+    switch (shape.block) {
+        case 64: return launch1<64>(a, any_hit, count, shape.stack, grid, stream);
+        case 128: return launch1<128>(a, any_hit, count, shape.stack, grid, stream);
+        case 256: return launch1<256>(a, any_hit, count, shape.stack, grid, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace bg

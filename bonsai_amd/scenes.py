@@ -1,0 +1,242 @@
+"""Scene descriptions and the seeded generators for the benchmark configurations.
+
+A ``SceneData`` holds the arrays of a ``bg_scene_desc``: a non-indexed
+triangle soup (triangle.rs:96), spheres with their object_to_world
+transform and its inverse (sphere.rs:38-65), materials and per-primitive
+diffuse area lights (diffuse.rs:98-123).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+
+
+@dataclass
+class Camera:
+    """PerspectiveCameraOptions (cameras/perspective.rs:36-46) with a static look_at transform."""
+
+    pos: tuple
+    look: tuple
+    up: tuple
+    fov: float
+    width: int
+    height: int
+    near: float = 0.01
+    far: float = 1000.0
+    lens_radius: float = 0.0
+    focal_distance: float = 1e6
+    shutter_open: float = 0.0
+    shutter_close: float = 1.0
+
+    def matrices(self) -> tuple[np.ndarray, np.ndarray]:
+        m = np.zeros(16, np.float32)
+        mi = np.zeros(16, np.float32)
+        p = np.asarray(self.pos, np.float32)
+        lk = np.asarray(self.look, np.float32)
+        u = np.asarray(self.up, np.float32)
+        N.check(N.lib().bg_look_at(p.ctypes.data, lk.ctypes.data, u.ctypes.data, m.ctypes.data, mi.ctypes.data))
+        return m, mi
+
+    def desc(self) -> N.bg_camera_desc:
+        m, mi = self.matrices()
+        d = N.bg_camera_desc()
+        d.camera_to_world[:] = [float(x) for x in m]
+        d.camera_to_world_inv[:] = [float(x) for x in mi]
+        d.fov = self.fov
+        d.near_clip = self.near
+        d.far_clip = self.far
+        d.lens_radius = self.lens_radius
+        d.focal_distance = self.focal_distance
+        d.shutter_open = self.shutter_open
+        d.shutter_close = self.shutter_close
+        d.width = self.width
+        d.height = self.height
+        return d
+
+
+@dataclass
+class SceneData:
+    tri_positions: np.ndarray = field(default_factory=lambda: np.zeros((0, 9), np.float32))
+    tri_reverse: np.ndarray | None = None
+    tri_material: np.ndarray | None = None
+    sphere_o2w: np.ndarray = field(default_factory=lambda: np.zeros((0, 16), np.float32))
+    sphere_w2o: np.ndarray = field(default_factory=lambda: np.zeros((0, 16), np.float32))
+    sphere_params: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.float32))
+    sphere_reverse: np.ndarray | None = None
+    sphere_material: np.ndarray | None = None
+    materials: list = field(default_factory=list)   # bg_material
+    lights: list = field(default_factory=list)      # bg_area_light
+    camera: Camera | None = None
+
+    @property
+    def n_tris(self) -> int:
+        return int(self.tri_positions.shape[0])
+
+    @property
+    def n_spheres(self) -> int:
+        return int(self.sphere_params.shape[0])
+
+    def bounds(self) -> tuple[np.ndarray, np.ndarray]:
+        lo = np.full(3, np.inf, np.float32)
+        hi = np.full(3, -np.inf, np.float32)
+        if self.n_tris:
+            v = self.tri_positions.reshape(-1, 3)
+            lo = np.minimum(lo, v.min(0))
+            hi = np.maximum(hi, v.max(0))
+        for i in range(self.n_spheres):
+            c = self.sphere_o2w[i].reshape(4, 4)[:3, 3]
+            r = self.sphere_params[i, 0]
+            lo = np.minimum(lo, c - r)
+            hi = np.maximum(hi, c + r)
+        return lo.astype(np.float32), hi.astype(np.float32)
+
+    def desc(self) -> tuple[N.bg_scene_desc, list]:
+        """Returns the C descriptor and the arrays it points into (keep them alive)."""
+        keep = []
+
+        def arr(a, dt):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dt)
+            keep.append(a)
+            return a.ctypes.data
+
+        d = N.bg_scene_desc()
+        d.n_tris = self.n_tris
+        d.tri_positions = arr(self.tri_positions, np.float32)
+        d.tri_reverse_orientation = arr(self.tri_reverse, np.uint8)
+        d.tri_material = arr(self.tri_material, np.uint32)
+        d.n_spheres = self.n_spheres
+        d.sphere_object_to_world = arr(self.sphere_o2w, np.float32)
+        d.sphere_world_to_object = arr(self.sphere_w2o, np.float32)
+        d.sphere_params = arr(self.sphere_params, np.float32)
+        d.sphere_reverse_orientation = arr(self.sphere_reverse, np.uint8)
+        d.sphere_material = arr(self.sphere_material, np.uint32)
+        if self.materials:
+            mats = (N.bg_material * len(self.materials))(*self.materials)
+            keep.append(mats)
+            d.n_materials = len(self.materials)
+            d.materials = C.addressof(mats)
+        if self.lights:
+            ls = (N.bg_area_light * len(self.lights))(*self.lights)
+            keep.append(ls)
+            d.n_lights = len(self.lights)
+            d.lights = C.addressof(ls)
+        return d, keep
+
+
+def matte(r, g, b) -> N.bg_material:
+    m = N.bg_material()
+    m.type = N.BG_MAT_MATTE
+    m.reflectance[:] = [r, g, b]
+    return m
+
+
+# Gold-like conductor constants (RGB eta, k) recorded for the conductor materials.
+GOLD_ETA = (0.143119, 0.374957, 1.44248)
+GOLD_K = (3.98316, 2.38572, 1.60322)
+
+
+def conductor(alpha, eta=GOLD_ETA, k=GOLD_K, r=(1.0, 1.0, 1.0)) -> N.bg_material:
+    m = N.bg_material()
+    m.type = N.BG_MAT_CONDUCTOR
+    m.reflectance[:] = list(r)
+    m.alpha = alpha
+    m.eta[:] = list(eta)
+    m.k[:] = list(k)
+    return m
+
+
+def area_light(prim_id, radiance, double_sided=False) -> N.bg_area_light:
+    a = N.bg_area_light()
+    a.prim_id = prim_id
+    a.radiance[:] = list(radiance)
+    a.double_sided = 1 if double_sided else 0
+    return a
+
+
+def translate(c) -> tuple[np.ndarray, np.ndarray]:
+    """Transform::translate (transform.rs:87-97): m and m_inverse, row-major."""
+    m = np.eye(4, dtype=np.float32)
+    mi = np.eye(4, dtype=np.float32)
+    m[:3, 3] = np.asarray(c, np.float32)
+    mi[:3, 3] = -np.asarray(c, np.float32)
+    return m.reshape(16), mi.reshape(16)
+
+
+def icosphere(level: int, center, radius: float, amplitude: float, frequency: float, seed: int) -> np.ndarray:
+    n = int(N.lib().bgx_icosphere_tri_count(level))
+    out = np.empty((n, 9), np.float32)
+    c = np.asarray(center, np.float32)
+    N.check(N.lib().bgx_icosphere(level, c.ctypes.data, radius, amplitude, frequency, seed, out.ctypes.data))
+    return out
+
+
+def incoherent_rays(n: int, lo, hi, seed: int) -> np.ndarray:
+    """(n, 8) float32 rays: origins uniform in [lo, hi], directions uniform on the sphere."""
+    out = np.empty((n, 8), np.float32)
+    lo = np.asarray(lo, np.float32)
+    hi = np.asarray(hi, np.float32)
+    N.check(N.lib().bgx_rays_incoherent(n, lo.ctypes.data, hi.ctypes.data, seed, out.ctypes.data))
+    return out
+
+
+def add_spheres(s: SceneData, centers, radii, material_ids=None) -> None:
+    o2w, w2o, prm = [], [], []
+    for c, r in zip(centers, radii):
+        m, mi = translate(c)
+        o2w.append(m)
+        w2o.append(mi)
+        prm.append([r, -r, r, 360.0])
+    s.sphere_o2w = np.concatenate([s.sphere_o2w, np.asarray(o2w, np.float32).reshape(-1, 16)])
+    s.sphere_w2o = np.concatenate([s.sphere_w2o, np.asarray(w2o, np.float32).reshape(-1, 16)])
+    s.sphere_params = np.concatenate([s.sphere_params, np.asarray(prm, np.float32).reshape(-1, 4)])
+    if material_ids is not None:
+        prev = s.sphere_material if s.sphere_material is not None else np.zeros(0, np.uint32)
+        s.sphere_material = np.concatenate([prev, np.asarray(material_ids, np.uint32)])
+
+
+def quad(p0, p1, p2, p3) -> np.ndarray:
+    """Two triangles (p0, p1, p2), (p0, p2, p3)."""
+    return np.array([[*p0, *p1, *p2], [*p0, *p2, *p3]], np.float32)
+
+
+def cornell_box(double_sided_light: bool = False, width: int = 256, height: int = 256) -> SceneData:
+    """Small Cornell-style box in [0,1]^3: 5 walls as 10 triangles, a 0.25 x 0.25
+    quad light (2 triangles, radiance 17) under the ceiling facing down, two
+    r = 0.3 spheres (matte albedo 0.75, conductor alpha 0.2)."""
+    walls = [
+        quad((0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1)),        # floor y=0
+        quad((0, 1, 0), (0, 1, 1), (1, 1, 1), (1, 1, 0)),        # ceiling y=1
+        quad((0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)),        # back z=1
+        quad((0, 0, 0), (0, 0, 1), (0, 1, 1), (0, 1, 0)),        # left x=0
+        quad((1, 0, 0), (1, 1, 0), (1, 1, 1), (1, 0, 1)),        # right x=1
+    ]
+    light = quad((0.375, 0.999, 0.375), (0.625, 0.999, 0.375), (0.625, 0.999, 0.625), (0.375, 0.999, 0.625))
+    tris = np.concatenate(walls + [light]).astype(np.float32)
+    s = SceneData(tri_positions=tris)
+    # materials: 0 white, 1 red, 2 green, 3 black (light), 4 sphere matte, 5 conductor
+    s.materials = [matte(0.73, 0.73, 0.73), matte(0.65, 0.05, 0.05), matte(0.12, 0.45, 0.15),
+                   matte(0.0, 0.0, 0.0), matte(0.75, 0.75, 0.75), conductor(0.2)]
+    s.tri_material = np.array([0, 0, 0, 0, 0, 0, 1, 1, 2, 2, 3, 3], np.uint32)
+    add_spheres(s, [(0.3, 0.3, 0.6), (0.7, 0.3, 0.35)], [0.3, 0.3], [4, 5])
+    s.lights = [area_light(10, (17.0, 17.0, 17.0), double_sided_light),
+                area_light(11, (17.0, 17.0, 17.0), double_sided_light)]
+    s.camera = Camera(pos=(0.5, 0.5, -1.4), look=(0.5, 0.5, 0.5), up=(0.0, 1.0, 0.0), fov=40.0,
+                      width=width, height=height)
+    return s
+
+
+def intersection_scene(level: int = 8, seed: int = 7) -> SceneData:
+    """Seeded noise-displaced icosphere (20 * 4^level triangles; level 8 -> 1,310,720)
+    of radius 1 at the origin, displacement amplitude 0.15, frequency 4."""
+    return SceneData(tri_positions=icosphere(level, (0.0, 0.0, 0.0), 1.0, 0.15, 4.0, seed))
+
+
+def intersection_camera(width: int = 4096, height: int = 4096) -> Camera:
+    return Camera(pos=(0.0, 0.3, -3.2), look=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=45.0,
+                  width=width, height=height)

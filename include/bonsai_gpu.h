@@ -1,0 +1,211 @@
+/*
+ * bonsai_gpu.h — C ABI of the MI355X (gfx950) ray-traversal and
+ * path-integration library.
+ *
+ * The entry points replace, for batches, the reference crate's hot-path
+ * trait methods:
+ *   bg_scene_create      ~ accelerators::BVH::new            (src/accelerators/bvh.rs:53-89)
+ *                          over GeometricPrimitive{Triangle,Sphere}
+ *                          (src/primitives/geometric.rs:15-20,
+ *                           src/shapes/triangle.rs:93-105, src/shapes/sphere.rs:38-65)
+ *   bg_intersect(CLOSEST) ~ Primitive::intersect for BVH      (src/accelerators/bvh.rs:281-352)
+ *   bg_intersect(ANY)     ~ Primitive::intersect_test for BVH (src/accelerators/bvh.rs:354-423)
+ *   bg_camera_rays       ~ PerspectiveCamera::generate_ray    (src/cameras/perspective.rs:117-175)
+ *   bg_render            ~ SamplerIntegrator::render with PathIntegrator::radiance
+ *                          (src/base/integrator.rs:35-133, src/integrators/path.rs:51-146)
+ *
+ * Conventions: every function returns 0 (BG_OK) or a negative error code;
+ * the message is available from bg_last_error(ctx). Host buffers belong to
+ * the caller and are copied; device memory made by the library is freed by
+ * the matching destroy call. A context is used by one host thread at a time.
+ */
+#ifndef BONSAI_GPU_H
+#define BONSAI_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BG_OK 0
+#define BG_ERR_INVALID (-1)
+#define BG_ERR_HIP (-2)
+#define BG_ERR_NOMEM (-3)
+#define BG_ERR_UNSUPPORTED (-4)
+
+#define BG_ABI_VERSION 1
+
+/* Query flags for bg_intersect / bg_intersect_device. */
+#define BG_CLOSEST 0u
+#define BG_ANY 1u
+/* Also count box and primitive tests into bg_stats (slower; for measuring
+ * the algorithmic bytes of a ray set). */
+#define BG_COUNT 2u
+
+#define BG_MISS 0xFFFFFFFFu
+
+typedef struct bg_ctx bg_ctx;
+typedef struct bg_scene bg_scene;
+
+/* A ray: origin, t_max, direction, time (src/geometries/ray.rs:10-17 minus
+ * the differentials, which no hot-path result depends on). 32 bytes. */
+typedef struct bg_ray {
+    float o[3];
+    float t_max;
+    float d[3];
+    float time;
+} bg_ray;
+
+/* A query result. prim_id is the index of the primitive in scene order
+ * (triangles [0, n_tris), then spheres), BG_MISS on a miss. For a closest
+ * hit on a triangle (b1, b2) are the barycentrics of p1 and p2
+ * (triangle.rs:213-217) and t the hit distance; on a sphere (b1, b2) are the
+ * surface (u, v) (sphere.rs:166-168). For an any-hit query prim_id is the
+ * first primitive, in traversal order, that reported a hit, and t its hit
+ * distance. 16 bytes. */
+typedef struct bg_hit {
+    uint32_t prim_id;
+    float t;
+    float b1;
+    float b2;
+} bg_hit;
+
+/* Flattened BVH node as stored in HBM (32 bytes). For a leaf, `offset` is the
+ * first slot of its primitive range and meta = count << 3 | has_sphere << 2.
+ * For an interior node, `offset` is the second child (the first child is the
+ * next node) and meta = axis (count 0). Same DFS order as bvh.rs:249-269. */
+typedef struct bg_bvh_node {
+    float bmin[3];
+    float bmax[3];
+    uint32_t offset;
+    uint32_t meta;
+} bg_bvh_node;
+
+/* Materials (src/materials/matte.rs:20-41 and a conductor assembled from
+ * src/bxdfs/microfacet.rs:68-151, src/base/microfacet.rs:32-154,
+ * src/base/fresnel.rs:13-46). */
+#define BG_MAT_MATTE 0u
+#define BG_MAT_CONDUCTOR 1u
+typedef struct bg_material {
+    uint32_t type;
+    float reflectance[3]; /* matte Kd / conductor R */
+    float alpha;          /* Trowbridge-Reitz alpha (conductor) */
+    float eta[3];         /* conductor eta */
+    float k[3];           /* conductor absorption */
+} bg_material;
+
+/* A diffuse area light on one primitive (src/lights/diffuse.rs:98-123). */
+typedef struct bg_area_light {
+    uint32_t prim_id;
+    float radiance[3];
+    uint32_t double_sided;
+} bg_area_light;
+
+typedef struct bg_scene_desc {
+    uint64_t n_tris;
+    const float* tri_positions;           /* 9 floats per triangle, world space (non-indexed mesh, triangle.rs:96) */
+    const uint8_t* tri_reverse_orientation; /* optional, 1 byte per triangle */
+    const uint32_t* tri_material;         /* optional, material index per triangle */
+    uint64_t n_spheres;
+    const float* sphere_object_to_world;  /* 16 floats per sphere: Transform::m, row-major, affine */
+    const float* sphere_world_to_object;  /* 16 floats per sphere: the same Transform's m_inverse */
+    const float* sphere_params;           /* 4 floats per sphere: radius, z_min, z_max, phi_max (degrees) */
+    const uint8_t* sphere_reverse_orientation; /* optional */
+    const uint32_t* sphere_material;      /* optional */
+    uint32_t n_materials;
+    const bg_material* materials;         /* optional */
+    uint32_t n_lights;
+    const bg_area_light* lights;          /* optional */
+} bg_scene_desc;
+
+typedef struct bg_scene_info {
+    uint64_t n_prims;
+    uint64_t n_nodes;
+    uint32_t max_depth;   /* largest number of pending far children any traversal can hold */
+    uint32_t max_leaf;
+    double build_ms;
+    uint64_t device_bytes;
+} bg_scene_info;
+
+typedef struct bg_stats {
+    double kernel_ms;     /* summed duration of the hot kernel launches (HIP events) */
+    double total_ms;      /* wall time of the call on the device stream */
+    uint64_t rays;        /* BVH queries issued */
+    uint64_t launches;
+    /* Filled only with BG_COUNT: box tests (bvh.rs:308-313) and primitive
+     * tests (bvh.rs:316-321, 388-392) summed over all rays. */
+    uint64_t nodes_visited;
+    uint64_t tris_tested;
+    uint64_t spheres_tested;
+} bg_stats;
+
+/* Library */
+int bg_abi_version(void);
+int bg_device_count(int* count);
+
+/* Context: binds a device and owns a HIP stream. */
+int bg_create(int device, bg_ctx** out);
+void bg_destroy(bg_ctx* ctx);
+const char* bg_last_error(const bg_ctx* ctx);
+/* Stream the context launches on (hipStream_t), for callers sharing it. */
+void* bg_stream(bg_ctx* ctx);
+
+/* Scene: builds the SAH BVH on the host (bvh.rs:91-247 with the bucket index
+ * and bucket initialisation fixed), flattens it and uploads it. */
+int bg_scene_create(bg_ctx* ctx, const bg_scene_desc* desc, bg_scene** out);
+void bg_scene_destroy(bg_scene* scene);
+int bg_scene_info_get(const bg_scene* scene, bg_scene_info* info);
+/* Copies the flattened BVH out: nodes (n_nodes) and slot -> primitive id
+ * (n_prims). Either pointer may be NULL. */
+int bg_scene_bvh(const bg_scene* scene, bg_bvh_node* nodes, uint32_t* slot_prim);
+
+/* Host-only BVH build (no device needed): the same builder bg_scene_create
+ * runs. Call with nodes == NULL to learn the sizes (*n_nodes, *n_prims), then
+ * again with buffers of those sizes. */
+int bg_build_bvh(const bg_scene_desc* desc, bg_bvh_node* nodes, uint64_t* n_nodes, uint32_t* slot_prim,
+                 uint64_t* n_prims, uint32_t* max_depth);
+
+/* Batched queries, host buffers (copied in and out; includes PCIe). */
+int bg_intersect(bg_ctx* ctx, const bg_scene* scene, const bg_ray* rays, size_t n, bg_hit* hits,
+                 uint32_t flags, bg_stats* stats);
+/* Batched queries on device-resident buffers (rays/hits in HBM), launched on
+ * the context stream; returns after the kernel completes. */
+int bg_intersect_device(bg_ctx* ctx, const bg_scene* scene, const void* d_rays, size_t n, void* d_hits,
+                        uint32_t flags, bg_stats* stats);
+
+/* Traversal launch shape (chosen by the autotuner or the caller). */
+typedef struct bg_launch_config {
+    uint32_t block;       /* threads per block: 64, 128, 256 */
+    uint32_t stack;       /* LDS stack entries per lane: 8, 16, 32 */
+    uint32_t blocks_per_cu; /* persistent blocks per CU; 0 = one block per ray tile */
+} bg_launch_config;
+int bg_set_launch_config(bg_ctx* ctx, const bg_launch_config* cfg);
+int bg_get_launch_config(const bg_ctx* ctx, bg_launch_config* cfg);
+
+/* Camera (src/cameras/perspective.rs:49-114 with src/base/transform.rs). */
+typedef struct bg_camera_desc {
+    float camera_to_world[16];      /* row-major m of the (static) camera transform */
+    float camera_to_world_inv[16];  /* its m_inverse */
+    float fov;                      /* degrees */
+    float near_clip, far_clip;
+    float lens_radius, focal_distance;
+    float shutter_open, shutter_close;
+    uint32_t width, height;         /* film full resolution */
+} bg_camera_desc;
+
+/* Transform::look_at (src/base/transform.rs:152-189): camera_to_world m and
+ * m_inverse, row-major. */
+int bg_look_at(const float pos[3], const float look[3], const float up[3], float m[16], float m_inv[16]);
+
+/* One camera ray per pixel through film point (x + fx, y + fy), pixels in
+ * row-major order, written to device memory. */
+int bg_camera_rays_device(bg_ctx* ctx, const bg_camera_desc* cam, float fx, float fy, void* d_rays);
+int bg_camera_rays(bg_ctx* ctx, const bg_camera_desc* cam, float fx, float fy, bg_ray* rays);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BONSAI_GPU_H */

@@ -1,0 +1,247 @@
+// Oracle (test infrastructure only): C entry points used by tests/ (ctypes)
+// and by bench.py's CPU-baseline leg. Not linked into the shipped library.
+#include <atomic>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "bvh.h"
+#include "camera.h"
+
+using namespace orc;
+
+namespace {
+
+struct OrcScene {
+    Scene scene;
+    RefBVH bvh;
+};
+
+// Packed node layout shared with the GPU library's export (32 bytes):
+// bmin[3], bmax[3], offset, meta (count << 3 | has_sphere << 2 | axis).
+struct PackedNode {
+    float bmin[3], bmax[3];
+    uint32_t offset, meta;
+};
+static_assert(sizeof(PackedNode) == 32, "packed node is 32 bytes");
+
+template <class F>
+void parallel_for(size_t n, int threads, F fn) {
+    if (threads <= 1 || n < 1024) {
+        for (size_t i = 0; i < n; ++i) fn(i);
+        return;
+    }
+    std::atomic<size_t> next(0);
+    const size_t chunk = 256;
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t) {
+        pool.emplace_back([&] {
+            for (;;) {
+                size_t b = next.fetch_add(chunk);
+                if (b >= n) break;
+                size_t e = b + chunk < n ? b + chunk : n;
+                for (size_t i = b; i < e; ++i) fn(i);
+            }
+        });
+    }
+    for (auto& th : pool) th.join();
+}
+
+inline Ray load_ray(const float* r) {
+    return Ray{V3(r[0], r[1], r[2]), V3(r[4], r[5], r[6]), r[3], r[7]};
+}
+
+}  // namespace
+
+extern "C" {
+
+void* orc_scene_new(const float* tri_pos, uint64_t n_tris, const uint8_t* tri_flip, const float* sph_o2w,
+                    const float* sph_w2o, const float* sph_params, const uint8_t* sph_rev, uint64_t n_sph) {
+    OrcScene* s = new OrcScene();
+    s->scene.tris.resize(n_tris);
+    s->scene.tri_flip.assign(n_tris, 0);
+    for (uint64_t i = 0; i < n_tris; ++i) {
+        const float* v = tri_pos + 9 * i;
+        s->scene.tris[i] = TriVerts{V3(v[0], v[1], v[2]), V3(v[3], v[4], v[5]), V3(v[6], v[7], v[8])};
+        if (tri_flip) s->scene.tri_flip[i] = tri_flip[i];
+    }
+    for (uint64_t i = 0; i < n_sph; ++i) {
+        Transform o2w{mat4_from(sph_o2w + 16 * i), mat4_from(sph_w2o + 16 * i)};
+        const float* p = sph_params + 4 * i;
+        s->scene.spheres.push_back(make_sphere(o2w, sph_rev ? sph_rev[i] != 0 : false, p[0], p[1], p[2], p[3]));
+    }
+    return s;
+}
+
+void orc_scene_free(void* h) { delete static_cast<OrcScene*>(h); }
+
+// Builds the reference BVH; returns the node count.
+uint64_t orc_bvh_build(void* h) {
+    OrcScene* s = static_cast<OrcScene*>(h);
+    s->bvh = build_bvh(s->scene);
+    return s->bvh.nodes.size();
+}
+
+uint32_t orc_bvh_height(void* h) {
+    OrcScene* s = static_cast<OrcScene*>(h);
+    // height = max number of interior ancestors of any node
+    std::vector<uint32_t> depth(s->bvh.nodes.size(), 0);
+    uint32_t best = 0;
+    for (size_t i = 0; i < s->bvh.nodes.size(); ++i) {
+        const RefNode& n = s->bvh.nodes[i];
+        if (depth[i] > best) best = depth[i];
+        if (n.count == 0) {
+            depth[i + 1] = depth[i] + 1;
+            depth[n.second_child_offset] = depth[i] + 1;
+        }
+    }
+    return best;
+}
+
+void orc_bvh_export(void* h, void* nodes_out, uint32_t* ordered_out) {
+    OrcScene* s = static_cast<OrcScene*>(h);
+    PackedNode* out = static_cast<PackedNode*>(nodes_out);
+    for (size_t i = 0; i < s->bvh.nodes.size(); ++i) {
+        const RefNode& n = s->bvh.nodes[i];
+        PackedNode p;
+        p.bmin[0] = n.bounds.mn.x; p.bmin[1] = n.bounds.mn.y; p.bmin[2] = n.bounds.mn.z;
+        p.bmax[0] = n.bounds.mx.x; p.bmax[1] = n.bounds.mx.y; p.bmax[2] = n.bounds.mx.z;
+        if (n.count > 0) {
+            bool sph = false;
+            for (uint32_t k = 0; k < n.count; ++k) sph = sph || s->scene.is_sphere(s->bvh.ordered[n.primitive_offset + k]);
+            p.offset = n.primitive_offset;
+            p.meta = (n.count << 3) | (sph ? 4u : 0u);
+        } else {
+            p.offset = n.second_child_offset;
+            p.meta = n.axis;
+        }
+        out[i] = p;
+    }
+    if (ordered_out) std::memcpy(ordered_out, s->bvh.ordered.data(), s->bvh.ordered.size() * 4);
+}
+
+// Replaces the oracle's BVH with an externally supplied flattened one.
+void orc_bvh_import(void* h, const void* nodes_in, uint64_t n_nodes, const uint32_t* ordered, uint64_t n_ordered) {
+    OrcScene* s = static_cast<OrcScene*>(h);
+    const PackedNode* in = static_cast<const PackedNode*>(nodes_in);
+    s->bvh.nodes.resize(n_nodes);
+    for (uint64_t i = 0; i < n_nodes; ++i) {
+        RefNode& n = s->bvh.nodes[i];
+        n.bounds.mn = V3(in[i].bmin[0], in[i].bmin[1], in[i].bmin[2]);
+        n.bounds.mx = V3(in[i].bmax[0], in[i].bmax[1], in[i].bmax[2]);
+        n.count = in[i].meta >> 3;
+        if (n.count > 0) {
+            n.primitive_offset = in[i].offset;
+            n.second_child_offset = 0;
+            n.axis = 0;
+        } else {
+            n.second_child_offset = in[i].offset;
+            n.primitive_offset = 0;
+            n.axis = in[i].meta & 3u;
+        }
+    }
+    s->bvh.ordered.assign(ordered, ordered + n_ordered);
+}
+
+// rays: 8 floats each (o.xyz, t_max, d.xyz, time). hits: 4 x 32-bit each
+// (prim, t, b1, b2). counters (optional): 3 x u32 per ray (nodes, tris, spheres).
+void orc_intersect(void* h, const float* rays, uint64_t n, uint32_t any, uint32_t* hits, uint32_t* counters,
+                   int threads) {
+    OrcScene* s = static_cast<OrcScene*>(h);
+    parallel_for(n, threads, [&](size_t i) {
+        Ray r = load_ray(rays + 8 * i);
+        Hit hit;
+        Counters c;
+        if (any) bvh_intersect_test(s->scene, s->bvh, r, &hit, counters ? &c : nullptr);
+        else bvh_intersect(s->scene, s->bvh, &r, &hit, nullptr, counters ? &c : nullptr);
+        hits[4 * i + 0] = hit.prim;
+        std::memcpy(&hits[4 * i + 1], &hit.t, 4);
+        std::memcpy(&hits[4 * i + 2], &hit.b1, 4);
+        std::memcpy(&hits[4 * i + 3], &hit.b2, 4);
+        if (counters) {
+            counters[3 * i + 0] = c.nodes;
+            counters[3 * i + 1] = c.tris;
+            counters[3 * i + 2] = c.spheres;
+        }
+    });
+}
+
+// Brute force over every primitive in the given order with the closest-hit
+// acceptance rule (a ground truth for small scenes, independent of the BVH).
+void orc_intersect_brute(void* h, const float* rays, uint64_t n, const uint32_t* order, uint32_t* hits) {
+    OrcScene* s = static_cast<OrcScene*>(h);
+    const size_t np = s->scene.n_prims();
+    for (uint64_t i = 0; i < n; ++i) {
+        Ray r = load_ray(rays + 8 * i);
+        Hit hit;
+        for (size_t k = 0; k < np; ++k) {
+            uint32_t id = order[k];
+            Float t, b[3];
+            bool ok;
+            if (id < s->scene.tris.size()) {
+                ok = tri_intersect(s->scene.tris[id], r, false, &t, b, nullptr);
+                if (ok) { hit.b1 = b[1]; hit.b2 = b[2]; }
+            } else {
+                ok = sphere_intersect(s->scene.spheres[id - s->scene.tris.size()], r, &t, b, nullptr);
+                if (ok) { hit.b1 = b[0]; hit.b2 = b[1]; }
+            }
+            if (ok) { r.t_max = t; hit.prim = id; hit.t = t; }
+        }
+        hits[4 * i + 0] = hit.prim;
+        std::memcpy(&hits[4 * i + 1], &hit.t, 4);
+        std::memcpy(&hits[4 * i + 2], &hit.b1, 4);
+        std::memcpy(&hits[4 * i + 3], &hit.b2, 4);
+    }
+}
+
+// Single-primitive tests, for closed-form unit checks. kind 0 = closest
+// (intersect), 1 = test. Returns 1 on hit and fills out[0..3] = t, b0|u, b1|v, b2.
+int orc_prim_intersect(void* h, uint32_t id, const float* ray, uint32_t kind, float* out) {
+    OrcScene* s = static_cast<OrcScene*>(h);
+    Ray r = load_ray(ray);
+    Float t = 0, b[3] = {0, 0, 0};
+    bool ok;
+    if (id < s->scene.tris.size()) {
+        ok = kind ? tri_intersect_test(s->scene.tris[id], r, &t) : tri_intersect(s->scene.tris[id], r, false, &t, b, nullptr);
+    } else {
+        const Sphere& sp = s->scene.spheres[id - s->scene.tris.size()];
+        ok = kind ? sphere_intersect_test(sp, r, &t) : sphere_intersect(sp, r, &t, b, nullptr);
+    }
+    out[0] = t; out[1] = b[0]; out[2] = b[1]; out[3] = b[2];
+    return ok ? 1 : 0;
+}
+
+// Camera rays for a pinhole/thin-lens camera (perspective.rs:49-175); see camera.h.
+void orc_camera_rays(const float* c2w, const float* c2w_inv, float fov, float near_clip, float far_clip,
+                     uint32_t width, uint32_t height, float fx, float fy, float* rays_out) {
+    PerspCamera cam = make_camera(mat4_from(c2w), mat4_from(c2w_inv), fov, near_clip, far_clip, 0.0f, 1e6f, 0.0f,
+                                  0.0f, (Float)width, (Float)height);
+    for (uint32_t y = 0; y < height; ++y)
+        for (uint32_t x = 0; x < width; ++x) {
+            Ray r = camera_generate_ray(cam, (Float)x + fx, (Float)y + fy, 0.5f, 0.5f, 0.0f);
+            float* o = rays_out + 8 * ((uint64_t)y * width + x);
+            o[0] = r.o.x; o[1] = r.o.y; o[2] = r.o.z; o[3] = r.t_max;
+            o[4] = r.d.x; o[5] = r.d.y; o[6] = r.d.z; o[7] = r.time;
+        }
+}
+
+// Helpers for closed-form tests of the numerics.
+float orc_gamma(float n) { return gamma(n); }
+float orc_next_float_up(float v) { return next_float_up(v); }
+float orc_next_float_down(float v) { return next_float_down(v); }
+// look_at (transform.rs:152-189): writes m and m_inverse (16 floats each).
+void orc_look_at(const float* pos, const float* look, const float* up, float* m, float* minv) {
+    Transform t = look_at(V3(pos[0], pos[1], pos[2]), V3(look[0], look[1], look[2]), V3(up[0], up[1], up[2]));
+    for (int i = 0; i < 16; ++i) { m[i] = t.m.m[i / 4][i % 4]; minv[i] = t.minv.m[i / 4][i % 4]; }
+}
+// Transform::translate then ::scale products for sphere placement: T(c) * S(r,r,r)
+void orc_translate_scale(const float* c, float sx, float sy, float sz, float* m, float* minv) {
+    Transform t = tmul(translate(V3(c[0], c[1], c[2])), scale(sx, sy, sz));
+    for (int i = 0; i < 16; ++i) { m[i] = t.m.m[i / 4][i % 4]; minv[i] = t.minv.m[i / 4][i % 4]; }
+}
+void orc_mat4_inverse(const float* in, float* out) {
+    Mat4 r = inverse(mat4_from(in));
+    for (int i = 0; i < 16; ++i) out[i] = r.m[i / 4][i % 4];
+}
+
+}  // extern "C"

@@ -1,0 +1,218 @@
+"""GPU traversal parity: the HIP closest-hit / any-hit kernels through the C ABI
+against the oracle's reference-order traversal on the same flattened BVH.
+Primitive ids, t and triangle barycentrics must be bit-exact; sphere (u, v)
+come from atan2/acos and are compared at 1e-6 relative."""
+import numpy as np
+import pytest
+
+import cpu_oracle as O
+from bonsai_amd import GpuBvh
+from bonsai_amd.scenes import (SceneData, add_spheres, cornell_box, icosphere, incoherent_rays,
+                               intersection_camera, intersection_scene)
+
+pytestmark = pytest.mark.gpu
+MISS = 0xFFFFFFFF
+
+
+def oracle_for(bvh: GpuBvh, scene: SceneData) -> O.OracleScene:
+    o = O.OracleScene(scene)
+    nodes, slots = bvh.bvh()
+    o.use_bvh(nodes, slots)
+    return o
+
+
+def same_bits(a, b):
+    """Bitwise equal f32 words, except that any NaN equals any NaN (x86 and gfx950 produce different
+    default NaN payloads for the same invalid operation)."""
+    fa, fb = a.view(np.float32), b.view(np.float32)
+    return np.all((a == b) | (np.isnan(fa) & np.isnan(fb)))
+
+
+def assert_hits_equal(gpu_hits, ref, n_tris, any_hit=False):
+    g = O.hits_as_words(gpu_hits)
+    assert np.array_equal(g[:, 0], ref[:, 0]), f"prim ids differ on {np.sum(g[:, 0] != ref[:, 0])} rays"
+    hit = g[:, 0] != MISS
+    assert same_bits(g[hit, 1], ref[hit, 1]), "t differs"
+    if any_hit:
+        return
+    tri = hit & (g[:, 0] < n_tris)
+    assert same_bits(g[tri, 2:4], ref[tri, 2:4]), "triangle barycentrics differ"
+    sph = hit & (g[:, 0] >= n_tris)
+    if sph.any():
+        gf = g[sph, 2:4].view(np.float32)
+        rf = ref[sph, 2:4].view(np.float32)
+        np.testing.assert_allclose(gf, rf, rtol=1e-6, atol=1e-6)
+
+
+def run_both(ctx, scene, rays):
+    bvh = GpuBvh(ctx, scene)
+    o = oracle_for(bvh, scene)
+    for any_hit in (False, True):
+        g = bvh.intersect_test(rays) if any_hit else bvh.intersect(rays)
+        r = o.intersect(rays, any_hit=any_hit)
+        assert_hits_equal(g, r, scene.n_tris, any_hit)
+    return bvh, o
+
+
+def box_rays(n, lo, hi, seed, aim_lo=None, aim_hi=None):
+    rng = np.random.default_rng(seed)
+    r = np.zeros((n, 8), np.float32)
+    r[:, 0:3] = rng.uniform(lo, hi, (n, 3))
+    r[:, 3] = np.inf
+    if aim_lo is None:
+        r[:, 4:7] = rng.standard_normal((n, 3))
+    else:
+        r[:, 4:7] = rng.uniform(aim_lo, aim_hi, (n, 3)) - r[:, 0:3]
+    return r
+
+
+def test_cornell_geometry(ctx):
+    s = cornell_box()
+    rays = box_rays(200_000, 0.01, 0.99, 1)
+    bvh, o = run_both(ctx, s, rays)
+    g = O.hits_as_words(bvh.intersect(rays))
+    assert (g[:, 0] != MISS).mean() > 0.99  # closed box: almost every interior ray hits
+
+
+def test_icosphere_incoherent(ctx):
+    s = SceneData(tri_positions=icosphere(6, (0.0, 0.0, 0.0), 1.0, 0.15, 4.0, 7))
+    lo, hi = s.bounds()
+    rays = incoherent_rays(300_000, lo, hi, 5)
+    run_both(ctx, s, rays)
+
+
+def test_icosphere_coherent_camera(ctx):
+    s = SceneData(tri_positions=icosphere(6, (0.0, 0.0, 0.0), 1.0, 0.15, 4.0, 7))
+    cam = intersection_camera(384, 384)
+    rays = ctx.camera_rays(cam)
+    m, mi = cam.matrices()
+    ref = O.camera_rays(m, mi, cam.fov, cam.width, cam.height)
+    assert np.array_equal(rays.view(np.uint32), ref.view(np.uint32)), "camera rays differ"
+    run_both(ctx, s, rays)
+
+
+def test_spheres_and_triangles_mixed(ctx):
+    rng = np.random.default_rng(8)
+    n = 3000
+    c = rng.uniform(-2, 2, (n, 1, 3))
+    tris = (c + rng.uniform(-0.2, 0.2, (n, 3, 3))).astype(np.float32).reshape(n, 9)
+    s = SceneData(tri_positions=tris)
+    add_spheres(s, rng.uniform(-2, 2, (300, 3)), rng.uniform(0.01, 0.5, 300))
+    rays = box_rays(200_000, -3, 3, 9, -2, 2)
+    bvh, _ = run_both(ctx, s, rays)
+    g = O.hits_as_words(bvh.intersect(rays))
+    assert (g[:, 0] >= s.n_tris).sum() > 1000 and ((g[:, 0] < s.n_tris) & (g[:, 0] != MISS)).sum() > 1000
+
+
+def test_edge_case_rays(ctx):
+    s = cornell_box()
+    bvh = GpuBvh(ctx, s)
+    o = oracle_for(bvh, s)
+    rays = [
+        O.ray((0.5, 0.5, 0.5), (1, 0, 0)), O.ray((0.5, 0.5, 0.5), (0, -1, 0)),   # axis-aligned (0 components)
+        O.ray((0.5, 0.5, 0.5), (0, 0, 1)), O.ray((0.5, 0.5, 0.5), (-0.0, 0, -1)),
+        O.ray((0.5, 0.5, 0.5), (0, 0, 0)),                                        # zero direction
+        O.ray((0.5, 0.5, 0.5), (np.nan, 0, 1)),                                   # NaN direction
+        O.ray((0.5, 0.5, 0.5), (1, 1, 1), t_max=0.0),                             # empty interval
+        O.ray((0.5, 0.5, 0.5), (1, 1, 1), t_max=-1.0),
+        O.ray((0.5, 0.5, 0.5), (1, 1, 1), t_max=0.1),                             # stops short of walls
+        O.ray((0.0, 0.5, 0.5), (1, 0, 0)),                                        # origin on a wall plane
+        O.ray((0.5, 0.0, 0.5), (0, 1, 0)),
+        O.ray((0.5, 0.5, 0.5), (1, 0, 1)),                                        # towards an edge of two walls
+        O.ray((0.5, 0.5, 0.5), (1, 1, 1)),                                        # towards a corner
+        O.ray((0.5, 0.5, -5), (0, 0, 1)),                                         # from outside through the open front
+        O.ray((2, 2, 2), (1, 1, 1)),                                              # outside, away
+        O.ray((0.5, 0.5, 0.5), (np.inf, 0, 0)),
+    ]
+    rays = np.array(rays, np.float32)
+    for any_hit in (False, True):
+        g = bvh.intersect_test(rays) if any_hit else bvh.intersect(rays)
+        assert_hits_equal(g, o.intersect(rays, any_hit=any_hit), s.n_tris, any_hit)
+
+
+def test_empty_scene_and_empty_batch(ctx):
+    bvh = GpuBvh(ctx, SceneData())
+    rays = box_rays(100, -1, 1, 3)
+    assert np.all(O.hits_as_words(bvh.intersect(rays))[:, 0] == MISS)
+    assert np.all(O.hits_as_words(bvh.intersect_test(rays))[:, 0] == MISS)
+    s = cornell_box()
+    b2 = GpuBvh(ctx, s)
+    assert b2.intersect(np.zeros((0, 8), np.float32)).shape[0] == 0
+
+
+def deep_scene(n=400):
+    """Triangles at exponentially shrinking scale along x: the SAH peels one off per level, giving a tree
+    far deeper than any LDS stack, so the spill path runs."""
+    tris = []
+    for k in range(n):
+        x = 1.0 - 0.5 ** (k / 8.0)
+        tris.append([x, 0, 0, x, 1, 0, x, 0, 1])
+    return SceneData(tri_positions=np.asarray(tris, np.float32))
+
+
+@pytest.mark.parametrize("block,stack", [(64, 8), (128, 16), (256, 32), (128, 8), (64, 32)])
+def test_launch_shapes_do_not_change_results(ctx, block, stack):
+    saved = ctx.launch()
+    try:
+        ctx.set_launch(block, stack, 4)
+        s = deep_scene()
+        bvh = GpuBvh(ctx, s)
+        rays = box_rays(50_000, -0.5, 1.5, 17, 0, 1)
+        o = oracle_for(bvh, s)
+        for any_hit in (False, True):
+            g = bvh.intersect_test(rays) if any_hit else bvh.intersect(rays)
+            assert_hits_equal(g, o.intersect(rays, any_hit=any_hit), s.n_tris, any_hit)
+        s2 = SceneData(tri_positions=icosphere(4, (0.0, 0.0, 0.0), 1.0, 0.15, 4.0, 3))
+        lo, hi = s2.bounds()
+        run_both(ctx, s2, incoherent_rays(50_000, lo, hi, 2))
+    finally:
+        ctx.set_launch(*saved)
+
+
+@pytest.mark.parametrize("any_hit", [False, True])
+def test_visit_counts_equal_oracle(ctx, any_hit):
+    """The kernel's box/primitive test counts (the basis of the algorithmic-bytes figure) equal the oracle's
+    reference-order counts summed over the ray set, which holds only if every lane replays the reference visit
+    sequence."""
+    rng = np.random.default_rng(8)
+    n = 2000
+    c = rng.uniform(-2, 2, (n, 1, 3))
+    tris = (c + rng.uniform(-0.2, 0.2, (n, 3, 3))).astype(np.float32).reshape(n, 9)
+    s = SceneData(tri_positions=tris)
+    add_spheres(s, rng.uniform(-2, 2, (100, 3)), rng.uniform(0.01, 0.5, 100))
+    rays = box_rays(100_000, -3, 3, 19, -2, 2)
+    bvh = GpuBvh(ctx, s)
+    o = oracle_for(bvh, s)
+    st = bvh.count(rays, any_hit)
+    _, cnt = o.intersect(rays, any_hit=any_hit, counters=True)
+    tot = cnt.astype(np.uint64).sum(0)
+    assert (st.nodes_visited, st.tris_tested, st.spheres_tested) == tuple(int(x) for x in tot)
+
+
+def test_deep_tree_really_is_deep(ctx):
+    bvh = GpuBvh(ctx, deep_scene())
+    assert bvh.info.max_depth > 40
+
+
+def test_full_size_intersection_benchmark_subset(ctx):
+    """The 2^24-ray, 1.31M-triangle workload: every ray on the GPU, a 40k-ray random subset against the
+    oracle, and a size-independent property on all rays (any-hit reports a hit exactly when closest-hit does)."""
+    s = intersection_scene(8)
+    assert s.n_tris == 1_310_720
+    bvh = GpuBvh(ctx, s)
+    o = oracle_for(bvh, s)
+    cam = intersection_camera(4096, 4096)
+    rays = ctx.camera_rays(cam)
+    closest = O.hits_as_words(bvh.intersect(rays))
+    anyh = O.hits_as_words(bvh.intersect_test(rays))
+    assert np.array_equal(closest[:, 0] != MISS, anyh[:, 0] != MISS)
+    rng = np.random.default_rng(0)
+    idx = rng.choice(rays.shape[0], 40_000, replace=False)
+    ref = o.intersect(rays[idx])
+    assert np.array_equal(closest[idx, 0], ref[:, 0]) and same_bits(closest[idx, 1:], ref[:, 1:])
+    lo, hi = s.bounds()
+    inc = incoherent_rays(1 << 20, lo, hi, 11)
+    g = O.hits_as_words(bvh.intersect(inc))
+    idx = rng.choice(inc.shape[0], 20_000, replace=False)
+    ref = o.intersect(inc[idx])
+    assert np.array_equal(g[idx, 0], ref[:, 0]) and same_bits(g[idx, 1:], ref[:, 1:])
