@@ -70,7 +70,8 @@ class bg_scene_info(C.Structure):
 
 class bg_stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("rays", C.c_uint64), ("launches", C.c_uint64),
-                ("nodes_visited", C.c_uint64), ("tris_tested", C.c_uint64), ("spheres_tested", C.c_uint64)]
+                ("nodes_visited", C.c_uint64), ("tris_tested", C.c_uint64), ("spheres_tested", C.c_uint64),
+                ("samples", C.c_uint64), ("shadow_rays", C.c_uint64)]
 
 
 class bg_launch_config(C.Structure):
@@ -92,6 +93,27 @@ class bg_camera_desc(C.Structure):
         ("height", C.c_uint32),
     ]
 
+
+class bg_render_desc(C.Structure):
+    _fields_ = [
+        ("camera", bg_camera_desc),
+        ("spp", C.c_uint32),
+        ("first_sample", C.c_uint32),
+        ("max_depth", C.c_uint32),
+        ("rr_threshold", C.c_float),
+        ("mode", C.c_uint32),
+        ("seed", C.c_uint64),
+        ("x0", C.c_uint32),
+        ("y0", C.c_uint32),
+        ("x1", C.c_uint32),
+        ("y1", C.c_uint32),
+        ("paths_per_pass", C.c_uint32),
+    ]
+
+
+BG_MODE_REF = 0
+BG_MODE_CORRECTED = 1
+BG_FILM_F64 = 1
 
 P = C.c_void_p
 # name -> (restype, argtypes); the list every header declaration must appear in.
@@ -115,6 +137,12 @@ SIGNATURES = {
     "bg_camera_rays_device": (C.c_int, [P, C.POINTER(bg_camera_desc), C.c_float, C.c_float, P]),
     "bg_camera_rays": (C.c_int, [P, C.POINTER(bg_camera_desc), C.c_float, C.c_float, P]),
     "bg_look_at": (C.c_int, [P, P, P, P, P]),
+    "bg_film_create": (C.c_int, [P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(P)]),
+    "bg_film_destroy": (None, [P]),
+    "bg_film_clear": (C.c_int, [P]),
+    "bg_film_read": (C.c_int, [P, P, P]),
+    "bg_render_film": (C.c_int, [P, P, C.POINTER(bg_render_desc), P, C.POINTER(bg_stats)]),
+    "bg_render": (C.c_int, [P, P, C.POINTER(bg_render_desc), P, C.POINTER(bg_stats)]),
     "bgx_icosphere_tri_count": (C.c_uint64, [C.c_int]),
     "bgx_icosphere": (C.c_int, [C.c_int, P, C.c_float, C.c_float, C.c_float, C.c_uint64, P]),
     "bgx_rays_incoherent": (C.c_int, [C.c_uint64, P, P, C.c_uint64, P]),

@@ -79,7 +79,89 @@ struct CameraDev {
 void camera_setup(const bg_camera_desc& d, CameraDev* out);
 hipError_t launch_camera_rays(const CameraDev& cam, float fx, float fy, float4* rays, hipStream_t stream);
 
+// Shading records in HBM.
+struct MaterialDev {
+    uint32_t type;
+    float r[3];
+    float alpha;
+    float eta[3];
+    float k[3];
+    uint32_t pad;
+};
+struct LightDev {
+    uint32_t prim;
+    float L[3];
+    uint32_t double_sided;
+    float area;
+    uint32_t pad[2];
+};
+struct SphereShadeDev {
+    float o2w[12];
+    float w2o[12];
+    float radius, z_min, z_max, phi_max, theta_min, theta_max;
+    uint32_t flip, pad;
+};
+struct ShadeSceneDev {
+    const float* tri_orig;      // 9 floats per triangle, scene order
+    const uint8_t* tri_flip;
+    const uint32_t* prim_mat;
+    const int32_t* prim_light;
+    const MaterialDev* mats;
+    const LightDev* lights;
+    const SphereShadeDev* spheres;
+    uint32_t n_tris, n_lights;
+};
+
+// Wavefront path state (structure of arrays) for up to `cap` paths.
+struct PathBuffers {
+    size_t cap = 0;
+    void* mem = nullptr;
+    float4* ray = nullptr;     // 2 per path: (o, t_max), (d, time)
+    uint4* hit = nullptr;
+    float4* beta = nullptr;    // rgb, bounce (as float bits unused)
+    float4* L = nullptr;       // rgb
+    uint2* key = nullptr;      // sample-stream key (lo, hi)
+    uint2* state = nullptr;    // x: draw index, y: flags | bounce << 8
+    float2* film = nullptr;    // film point
+    float4* sray = nullptr;    // shadow ray, 2 per path
+    uint4* shit = nullptr;
+    float4* mray = nullptr;    // MIS-branch ray, 2 per path
+    uint4* mhit = nullptr;
+    float4* nee_a = nullptr;   // light-branch term rgb, flags
+    float4* nee_b = nullptr;   // BSDF-branch f rgb, MIS weight
+    float4* nee_c = nullptr;   // beta at the vertex rgb, light pdf
+    float4* nee_d = nullptr;   // BSDF-branch pdf, light index, -, -
+    unsigned long long* counters = nullptr;  // [queries, shadow queries]
+};
+
+struct RenderParamsDev {
+    CameraDev cam;
+    uint64_t seed;
+    uint32_t x0, y0, x1, y1;  // window
+    uint32_t first_sample, n_samples;  // samples of this pass
+    uint32_t max_depth;
+    float rr;
+    uint32_t mode;
+    uint64_t n_paths;
+};
+
+// path.hip
+hipError_t launch_init_paths(const RenderParamsDev& p, const PathBuffers& b, hipStream_t s);
+hipError_t launch_shade(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b, uint32_t bounce,
+                        hipStream_t s);
+hipError_t launch_nee_resolve(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b,
+                              hipStream_t s);
+hipError_t launch_film(const RenderParamsDev& p, const PathBuffers& b, uint32_t width, uint32_t height, float4* film,
+                       double4* film64, hipStream_t s);
+
 }  // namespace bg
+
+struct bg_film {
+    bg_ctx* ctx = nullptr;
+    uint32_t width = 0, height = 0, flags = 0;
+    float4* sums = nullptr;
+    double4* sums64 = nullptr;
+};
 
 struct bg_scene {
     bg_ctx* ctx = nullptr;
@@ -92,6 +174,14 @@ struct bg_scene {
     uint32_t* d_slot_prim = nullptr;
     bg::SphereDev* d_spheres = nullptr;
     bg::SphereAux* d_sphere_aux = nullptr;
+    // shading data
+    float* d_tri_orig = nullptr;
+    uint8_t* d_tri_flip = nullptr;
+    uint32_t* d_prim_mat = nullptr;
+    int32_t* d_prim_light = nullptr;
+    bg::MaterialDev* d_mats = nullptr;
+    bg::LightDev* d_lights = nullptr;
+    bg::SphereShadeDev* d_sph_shade = nullptr;
     // host copies of scene data the integrator needs
     std::vector<float> tri_positions;
     std::vector<uint8_t> tri_flip;
@@ -116,4 +206,5 @@ struct bg_ctx {
     size_t spill_bytes = 0;
     void* io = nullptr;
     size_t io_bytes = 0;
+    bg::PathBuffers paths;
 };

@@ -268,7 +268,7 @@ int bg_scene_create(bg_ctx* ctx, const bg_scene_desc* d, bg_scene** out) {
         if (!(d->sphere_params[4 * i] > 0.0f)) return fail(ctx, BG_ERR_INVALID, "sphere radius must be > 0");
     }
     for (uint32_t i = 0; i < d->n_lights; ++i)
-        if (d->lights[i].prim_id >= np) return fail(ctx, BG_ERR_INVALID, "light prim_id out of range");
+        if (d->lights[i].prim_id >= nt) return fail(ctx, BG_ERR_INVALID, "area lights must be on triangles");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     auto t0 = std::chrono::steady_clock::now();
 
@@ -326,6 +326,51 @@ int bg_scene_create(bg_ctx* ctx, const bg_scene_desc* d, bg_scene** out) {
     if (!rc) rc = upload(ctx, &s->d_slot_prim, s->bvh.slot_prim.data(), nslot, &bytes);
     if (!rc) rc = upload(ctx, &s->d_spheres, sph.data(), sph.size(), &bytes);
     if (!rc) rc = upload(ctx, &s->d_sphere_aux, aux.data(), aux.size(), &bytes);
+    // shading records
+    if (!rc) rc = upload(ctx, &s->d_tri_orig, d->tri_positions, 9 * nt, &bytes);
+    if (!rc) rc = upload(ctx, &s->d_tri_flip, s->tri_flip.data(), nt, &bytes);
+    if (!rc) rc = upload(ctx, &s->d_prim_mat, s->prim_material.data(), np, &bytes);
+    if (!rc) rc = upload(ctx, &s->d_prim_light, s->prim_light.data(), np, &bytes);
+    std::vector<bg::MaterialDev> mats(s->materials.size());
+    for (size_t i = 0; i < mats.size(); ++i) {
+        const bg_material& m = s->materials[i];
+        mats[i] = bg::MaterialDev{m.type, {m.reflectance[0], m.reflectance[1], m.reflectance[2]}, m.alpha,
+                                  {m.eta[0], m.eta[1], m.eta[2]}, {m.k[0], m.k[1], m.k[2]}, 0u};
+    }
+    if (!rc) rc = upload(ctx, &s->d_mats, mats.data(), mats.size(), &bytes);
+    std::vector<bg::LightDev> lts(s->lights.size());
+    for (size_t i = 0; i < lts.size(); ++i) {
+        const bg_area_light& l = s->lights[i];
+        const float* v = d->tri_positions + 9 * (uint64_t)l.prim_id;
+        // triangle.rs:561-566: 0.5 * |(p1 - p0) x (p2 - p0)| with the f64 cross of vec3.rs:108-122
+        const double ax = v[3] - v[0], ay = v[4] - v[1], az = v[5] - v[2];
+        const double bx = v[6] - v[0], by = v[7] - v[1], bz = v[8] - v[2];
+        const float ax_ = (float)ax, ay_ = (float)ay, az_ = (float)az, bx_ = (float)bx, by_ = (float)by, bz_ = (float)bz;
+        const float cx = (float)((double)ay_ * bz_ - (double)az_ * by_);
+        const float cy = (float)((double)az_ * bx_ - (double)ax_ * bz_);
+        const float cz = (float)((double)ax_ * by_ - (double)ay_ * bx_);
+        lts[i] = bg::LightDev{l.prim_id, {l.radiance[0], l.radiance[1], l.radiance[2]}, l.double_sided ? 1u : 0u,
+                              0.5f * std::sqrt(cx * cx + cy * cy + cz * cz), {0u, 0u}};
+    }
+    if (!rc) rc = upload(ctx, &s->d_lights, lts.data(), lts.size(), &bytes);
+    std::vector<bg::SphereShadeDev> ssh(ns);
+    for (uint64_t i = 0; i < ns; ++i) {
+        const float* m = d->sphere_object_to_world + 16 * i;
+        std::memcpy(ssh[i].o2w, m, 12 * sizeof(float));
+        std::memcpy(ssh[i].w2o, sph[i].w2o, 12 * sizeof(float));
+        ssh[i].radius = sph[i].radius;
+        ssh[i].z_min = sph[i].z_min;
+        ssh[i].z_max = sph[i].z_max;
+        ssh[i].phi_max = sph[i].phi_max;
+        ssh[i].theta_min = aux[i].theta_min;
+        ssh[i].theta_max = aux[i].theta_max;
+        // transform.rs:218-224
+        const float det = m[0] * (m[5] * m[10] - m[6] * m[9]) - m[1] * (m[4] * m[10] - m[6] * m[8]) +
+                          m[2] * (m[4] * m[9] - m[5] * m[8]);
+        ssh[i].flip = ((s->sphere_flags[i] != 0) != (det < 0.0f)) ? 1u : 0u;
+        ssh[i].pad = 0;
+    }
+    if (!rc) rc = upload(ctx, &s->d_sph_shade, ssh.data(), ssh.size(), &bytes);
     if (rc) {
         bg_scene_destroy(s);
         return rc;
@@ -377,6 +422,9 @@ void bg_scene_destroy(bg_scene* s) {
     if (s->d_slot_prim) hipFree(s->d_slot_prim);
     if (s->d_spheres) hipFree(s->d_spheres);
     if (s->d_sphere_aux) hipFree(s->d_sphere_aux);
+    void* extra[] = {s->d_tri_orig, s->d_tri_flip, s->d_prim_mat, s->d_prim_light, s->d_mats, s->d_lights, s->d_sph_shade};
+    for (void* p : extra)
+        if (p) hipFree(p);
     delete s;
 }
 

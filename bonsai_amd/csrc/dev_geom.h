@@ -143,9 +143,9 @@ __device__ __noinline__ void edges_f64(float p0x, float p0y, float p1x, float p1
 }
 
 // triangle.rs:135-240 (+ the degenerate exit): on success returns t and the
-// barycentrics b1, b2.
-__device__ __forceinline__ bool tri_hit(const RayState& r, const float* __restrict__ v, float* t_out, float* b1_out,
-                                        float* b2_out) {
+// barycentrics b0, b1, b2.
+__device__ __forceinline__ bool tri_hit_full(const RayState& r, const float* __restrict__ v, float* t_out,
+                                             float* b0_out, float* b1_out, float* b2_out) {
     const float p0x = v[0], p0y = v[1], p0z = v[2];
     const float p1x = v[3], p1y = v[4], p1z = v[5];
     const float p2x = v[6], p2y = v[7], p2z = v[8];
@@ -188,9 +188,16 @@ __device__ __forceinline__ bool tri_hit(const RayState& r, const float* __restri
     if (t <= delta_t) return false;
     if (tri_degenerate(p0x, p0y, p0z, p1x, p1y, p1z, p2x, p2y, p2z)) return false;
     *t_out = t;
+    *b0_out = e0 * inv_det;
     *b1_out = e1 * inv_det;
     *b2_out = e2 * inv_det;
     return true;
+}
+
+__device__ __forceinline__ bool tri_hit(const RayState& r, const float* __restrict__ v, float* t_out, float* b1_out,
+                                        float* b2_out) {
+    float b0;
+    return tri_hit_full(r, v, t_out, &b0, b1_out, b2_out);
 }
 
 // ---------------------------------------------------------------- spheres

@@ -1,0 +1,386 @@
+// Wavefront path integrator for gfx950 (integrators/path.rs:51-146 with
+// base/integrator.rs:35-133, 309-437).
+//
+// A pass holds n_paths camera samples (window pixels x samples of the pass),
+// path-major state in HBM. Each bounce runs: extend (closest-hit traversal of
+// every live path's ray), shade (surface interaction, emission, light and
+// BSDF sampling for next-event estimation, BSDF sampling of the continuation,
+// Russian roulette), the two next-event queries (any-hit shadow rays and
+// closest-hit MIS-branch rays, both plain traversal launches), and resolve
+// (adds the estimate to the path's radiance). The film kernel then gathers
+// every sample whose box-filter footprint covers a pixel, in a fixed order,
+// so the film is deterministic without atomics.
+#include <hip/hip_runtime.h>
+
+#include "bg_internal.h"
+#include "dev_camera.h"
+#include "dev_shade.h"
+
+namespace bg {
+namespace {
+
+using namespace dev;
+
+constexpr uint32_t kAlive = 1u, kSpecFlag = 2u;
+// nee_a.w flags
+constexpr uint32_t kHasNee = 1u, kHasLight = 2u, kHasBsdf = 4u;
+
+__device__ __forceinline__ uint64_t key_of(uint2 k) { return (uint64_t)k.x | ((uint64_t)k.y << 32); }
+
+__device__ __forceinline__ void write_ray(float4* rays, uint64_t i, V o, float t_max, V d, float time) {
+    rays[2 * i] = make_float4(o.x, o.y, o.z, t_max);
+    rays[2 * i + 1] = make_float4(d.x, d.y, d.z, time);
+}
+__device__ __forceinline__ void kill_ray(float4* rays, uint64_t i) {
+    rays[2 * i] = make_float4(0.0f, 0.0f, 0.0f, -__builtin_inff());
+    rays[2 * i + 1] = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+}
+
+__device__ __forceinline__ C3 emission(const LightDev& l, V n, V w) {
+    return (l.double_sided || dot(n, w) > 0.0f) ? C3{l.L[0], l.L[1], l.L[2]} : mkc(0.0f);
+}
+
+__device__ __forceinline__ void tri_verts(const float* __restrict__ v, V* p0, V* p1, V* p2) {
+    *p0 = mkv(v[0], v[1], v[2]);
+    *p1 = mkv(v[3], v[4], v[5]);
+    *p2 = mkv(v[6], v[7], v[8]);
+}
+
+// Interaction::spawn_ray (interaction.rs:115-121); t_max 1 - 1e-4 in the reference, +inf corrected.
+__device__ __forceinline__ void spawn(const Surf& s, V d, bool ref, V* o, float* t_max) {
+    *o = offset_ray_origin(s.p, s.perr, s.n, d);
+    *t_max = ref ? 1.0f - 0.0001f : __builtin_inff();
+}
+
+// shape.rs:40-57 on a light's triangle: solid-angle pdf of direction wi.
+__device__ __noinline__ float light_point_pdf(const ShadeSceneDev& sc, const LightDev& l, const Surf& s, V wi, bool ref) {
+    V o;
+    float tm;
+    spawn(s, wi, ref, &o, &tm);
+    RayState rs;
+    rs.ox = o.x; rs.oy = o.y; rs.oz = o.z; rs.t_max = tm;
+    rs.dx = wi.x; rs.dy = wi.y; rs.dz = wi.z;
+    ray_setup(rs);
+    const float* v = sc.tri_orig + 9 * (uint64_t)l.prim;
+    float t, b0, b1, b2;
+    if (!tri_hit_full(rs, v, &t, &b0, &b1, &b2)) return 0.0f;
+    V p0, p1, p2;
+    tri_verts(v, &p0, &p1, &p2);
+    const V hp = b0 * p0 + b1 * p1 + b2 * p2;
+    V hn = unit(cross(p0 - p2, p1 - p2));
+    if (sc.tri_flip[l.prim]) hn = -hn;
+    float pdf = len2(s.p - hp) / (__builtin_fabsf(dot(hn, -wi)) * l.area);
+    if (__builtin_isinf(pdf)) pdf = 0.0f;
+    return pdf;
+}
+
+__global__ __launch_bounds__(256) void k_init_paths(RenderParamsDev p, PathBuffers b) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= p.n_paths) return;
+    const uint32_t ww = p.x1 - p.x0, wh = p.y1 - p.y0;
+    const uint64_t npix = (uint64_t)ww * wh;
+    const uint32_t local = (uint32_t)(i % npix);
+    const uint32_t s = p.first_sample + (uint32_t)(i / npix);
+    const uint32_t px = p.x0 + local % ww, py = p.y0 + local / ww;
+    const uint64_t key = stream_key(p.seed, (uint64_t)py * p.cam.width + px, s);
+    uint32_t dim = 0;
+    // sampler.rs:14-20: film = pixel + get_2d, lens = get_2d, time = get_1d
+    const float fx = (float)px + draw(key, dim);
+    const float fy = (float)py + draw(key, dim);
+    draw(key, dim);
+    draw(key, dim);
+    const float tu = draw(key, dim);
+    const CamRay r = generate_ray(p.cam, fx, fy, tu);
+    b.ray[2 * i] = make_float4(r.ox, r.oy, r.oz, r.t_max);
+    b.ray[2 * i + 1] = make_float4(r.dx, r.dy, r.dz, r.time);
+    b.beta[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    b.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    b.key[i] = make_uint2((uint32_t)key, (uint32_t)(key >> 32));
+    b.state[i] = make_uint2(dim, kAlive);
+    b.film[i] = make_float2(fx, fy);
+}
+
+__global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev sc, PathBuffers b, uint32_t bounce) {
+    const uint64_t i = (uint64_t)blockIdx.x * 128 + threadIdx.x;
+    if (i >= p.n_paths) return;
+    uint2 st = b.state[i];
+    uint32_t flags = st.y;
+    b.nee_a[i].w = 0.0f;
+    kill_ray(b.sray, i);
+    kill_ray(b.mray, i);
+    if (!(flags & kAlive)) return;
+    const bool ref = p.mode == 0;
+    const uint4 h = b.hit[i];
+    const float4 r0 = b.ray[2 * i], r1 = b.ray[2 * i + 1];
+    if (h.x == 0xFFFFFFFFu) {
+        b.state[i] = make_uint2(st.x, 0u);
+        kill_ray(b.ray, i);
+        return;
+    }
+    const uint32_t prim = h.x;
+    RayState rs;
+    rs.ox = r0.x; rs.oy = r0.y; rs.oz = r0.z; rs.t_max = r0.w;
+    rs.dx = r1.x; rs.dy = r1.y; rs.dz = r1.z;
+    ray_setup(rs);
+    Surf s;
+    float t;
+    bool ok;
+    if (prim < sc.n_tris) ok = tri_interaction(rs, sc.tri_orig + 9 * (uint64_t)prim, sc.tri_flip[prim] != 0, &s, &t);
+    else ok = sphere_interaction(rs, sc.spheres[prim - sc.n_tris], &s, &t);
+    s.time = r1.w;
+    if (!ok) {  // unreachable for a primitive the traversal accepted on this ray
+        b.state[i] = make_uint2(st.x, 0u);
+        kill_ray(b.ray, i);
+        return;
+    }
+    const uint64_t key = key_of(b.key[i]);
+    uint32_t dim = st.x;
+    const float4 b4 = b.beta[i];
+    C3 beta = C3{b4.x, b4.y, b4.z};
+    const float4 l4 = b.L[i];
+    C3 L = C3{l4.x, l4.y, l4.z};
+    const V rd = mkv(r1.x, r1.y, r1.z);
+
+    // path.rs:78-86
+    if (bounce == 0 || (flags & kSpecFlag)) {
+        const int32_t li = sc.prim_light[prim];
+        if (li >= 0) L = L + beta * emission(sc.lights[li], s.n, -rd);
+    }
+    bool alive = bounce < p.max_depth;
+    if (alive) {
+        // matte.rs:20-41 / conductor; bsdf.rs:33-52
+        const MaterialDev& m = sc.mats[sc.prim_mat[prim]];
+        Bsdf bs;
+        bs.ns = s.ns;
+        bs.ng = s.n;
+        bs.ss = unit(s.dpdu);
+        bs.ts = cross(bs.ns, bs.ss);
+        bs.lobe.type = 0;
+        bs.lobe.ref = ref;
+        if (m.type == 0) {
+            const C3 kd = C3{clampf(m.r[0], 0.0f, __builtin_inff()), clampf(m.r[1], 0.0f, __builtin_inff()),
+                             clampf(m.r[2], 0.0f, __builtin_inff())};
+            if (!black(kd)) { bs.lobe.type = kRefl | kDiffuse; bs.lobe.r = kd; }
+        } else {
+            bs.lobe.type = kRefl | kGlossy;
+            bs.lobe.r = C3{m.r[0], m.r[1], m.r[2]};
+            bs.lobe.ax = rmax(m.alpha, 0.001f);
+            bs.lobe.ay = rmax(m.alpha, 0.001f);
+            bs.lobe.eta = C3{m.eta[0], m.eta[1], m.eta[2]};
+            bs.lobe.k = C3{m.k[0], m.k[1], m.k[2]};
+        }
+        const uint32_t nonspec = kAll & ~kSpecular;
+        if (bs.ncomp(nonspec) > 0 && sc.n_lights > 0) {
+            // integrator.rs:309-328
+            const float nl = (float)sc.n_lights;
+            const float u = draw(key, dim);
+            const uint32_t idx = (uint32_t)rmin(u * nl, nl - 1.0f);
+            const float light_pdf = 1.0f / nl;
+            const float ul0 = draw(key, dim), ul1 = draw(key, dim);
+            const float us0 = draw(key, dim), us1 = draw(key, dim);
+            const LightDev& light = sc.lights[idx];
+            uint32_t nee = kHasNee;
+            // diffuse.rs:134-167: sample a point on the light's triangle (triangle.rs:522-559)
+            V p0, p1, p2;
+            tri_verts(sc.tri_orig + 9 * (uint64_t)light.prim, &p0, &p1, &p2);
+            const float sq = __builtin_sqrtf(ul0);
+            const float c0 = 1.0f - sq, c1 = ul1 * sq, c2 = 1.0f - c0 - c1;
+            const V lp = c0 * p0 + c1 * p1 + c2 * p2;
+            V ln = unit(cross(p1 - p0, p2 - p0));
+            if (sc.tri_flip[light.prim]) ln = ln * -1.0f;
+            const V labs = vabs(c0 * p0) + vabs(c1 * p1) + vabs(c2 * p2);
+            const V lerr = gamma_n(6.0f) * labs;
+            const float pdf_area = 1.0f / light.area;
+            V wi = lp - s.p;
+            float lpdf;
+            if (len2(wi) == 0.0f) {
+                lpdf = 0.0f;
+            } else {
+                wi = unit(wi);
+                const float conv = len2(s.p - lp) / __builtin_fabsf(dot(ln, -wi));
+                lpdf = ref ? conv : pdf_area * conv;  // shape.rs:26-27 drops 1/area
+                if (__builtin_isinf(lpdf)) lpdf = 0.0f;
+            }
+            C3 lrad = mkc(0.0f);
+            V lwi = mkv(0.0f, 0.0f, 0.0f);
+            if (!(lpdf == 0.0f || len2(lp - s.p) == 0.0f)) {
+                lwi = unit(lp - s.p);
+                lrad = emission(light, ln, ref ? lwi : -lwi);  // diffuse.rs:67 evaluates toward +wi
+            } else {
+                lpdf = 0.0f;
+            }
+            float scattering_pdf = 0.0f;
+            C3 term = mkc(0.0f);
+            if (lpdf > 0.0f && !black(lrad)) {
+                const C3 f = bs.f(s.wo, lwi, nonspec) * __builtin_fabsf(dot(lwi, bs.ns));
+                scattering_pdf = bs.pdf(s.wo, lwi, nonspec);
+                if (!black(f)) {
+                    // VisibilityTester: spawn_ray_to_it (interaction.rs:132-143)
+                    const V o = offset_ray_origin(s.p, s.perr, s.n, lp - s.p);
+                    const V tgt = offset_ray_origin(lp, lerr, ln, o - lp);
+                    write_ray(b.sray, i, o, 1.0f - 0.0001f, tgt - o, s.time);
+                    const float w = power_heuristic(lpdf, scattering_pdf);
+                    term = f * lrad * w / lpdf;
+                    nee |= kHasLight;
+                }
+            }
+            // BSDF-sampling branch (integrator.rs:379-434)
+            const BxS bsm = bs.sample(s.wo, us0, us1, nonspec);
+            const C3 fb = bsm.f * __builtin_fabsf(dot(bsm.wi, bs.ns));
+            const float spdf = ref ? scattering_pdf : bsm.pdf;
+            const V mdir = ref ? lwi : bsm.wi;  // the reference traces the light-sample direction
+            if (!black(fb) && spdf > 0.0f) {
+                const float lpp = light_point_pdf(sc, light, s, mdir, ref);
+                if (lpp != 0.0f) {
+                    const float w = power_heuristic(spdf, lpp);
+                    V o;
+                    float tm;
+                    spawn(s, mdir, ref, &o, &tm);
+                    write_ray(b.mray, i, o, tm, mdir, s.time);
+                    b.nee_b[i] = make_float4(fb.r, fb.g, fb.b, w);
+                    b.nee_d[i] = make_float4(spdf, __uint_as_float(idx), 0.0f, 0.0f);
+                    nee |= kHasBsdf;
+                }
+            }
+            b.nee_a[i] = make_float4(term.r, term.g, term.b, __uint_as_float(nee));
+            b.nee_c[i] = make_float4(beta.r, beta.g, beta.b, light_pdf);
+        }
+        // path.rs:111-142
+        const float u0 = draw(key, dim), u1 = draw(key, dim);
+        const BxS bsc = bs.sample(-rd, u0, u1, kAll);
+        if (black(bsc.f) || bsc.pdf == 0.0f) {
+            alive = false;
+        } else {
+            beta = beta * (bsc.f * __builtin_fabsf(dot(bsc.wi, bs.ns)) / bsc.pdf);
+            flags = (bsc.type & kSpecular) ? (flags | kSpecFlag) : (flags & ~kSpecFlag);
+            if (cmax(beta) < p.rr && bounce > 3) {
+                const float q = rmax(1.0f - cmax(beta), 0.5f);
+                if (draw(key, dim) < q) alive = false;
+                else beta = beta / (1.0f - q);
+            }
+            // At bounce + 1 == max_depth the next intersection can add nothing (no specular lobes), so stop.
+            if (alive && bounce + 1 >= p.max_depth) alive = false;
+            if (alive) {
+                if (ref) {
+                    b.ray[2 * i].w = t;  // Q1: the camera ray is re-intersected with t_max = t_hit
+                } else {
+                    V o;
+                    float tm;
+                    spawn(s, bsc.wi, false, &o, &tm);
+                    write_ray(b.ray, i, o, tm, bsc.wi, s.time);
+                }
+            }
+        }
+    }
+    if (!alive) kill_ray(b.ray, i);
+    b.beta[i] = make_float4(beta.r, beta.g, beta.b, 0.0f);
+    b.L[i] = make_float4(L.r, L.g, L.b, 0.0f);
+    b.state[i] = make_uint2(dim, alive ? (flags | kAlive) : 0u);
+}
+
+__global__ __launch_bounds__(256) void k_nee_resolve(RenderParamsDev p, ShadeSceneDev sc, PathBuffers b) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= p.n_paths) return;
+    const float4 a = b.nee_a[i];
+    const uint32_t nee = __float_as_uint(a.w);
+    if (!(nee & kHasNee)) return;
+    C3 out = mkc(0.0f);
+    if ((nee & kHasLight) && b.shit[i].x == 0xFFFFFFFFu) out = out + C3{a.x, a.y, a.z};
+    if (nee & kHasBsdf) {
+        const uint32_t hp = b.mhit[i].x;
+        if (hp != 0xFFFFFFFFu) {
+            const int32_t li = sc.prim_light[hp];
+            const float4 dd = b.nee_d[i];
+            const bool same = p.mode == 0 ? (li >= 0) : (li == (int32_t)__float_as_uint(dd.y));
+            if (same) {
+                V p0, p1, p2;
+                tri_verts(sc.tri_orig + 9 * (uint64_t)hp, &p0, &p1, &p2);
+                V n = unit(cross(p0 - p2, p1 - p2));
+                if (sc.tri_flip[hp]) n = -n;
+                const float4 d = b.mray[2 * i + 1];
+                const C3 Le = emission(sc.lights[li], n, mkv(-d.x, -d.y, -d.z));
+                if (!black(Le)) {
+                    const float4 fw = b.nee_b[i];
+                    out = out + C3{fw.x, fw.y, fw.z} * Le * fw.w / dd.x;
+                }
+            }
+        }
+    }
+    const float4 c = b.nee_c[i];
+    const C3 ld = out / c.w;
+    const float4 l4 = b.L[i];
+    const C3 L = C3{l4.x, l4.y, l4.z} + C3{c.x, c.y, c.z} * ld;
+    b.L[i] = make_float4(L.r, L.g, L.b, 0.0f);
+}
+
+// film.rs:232-284 with a radius-0.5 box filter, gathered per pixel: a sample at
+// film point f covers pixels [ceil(f - 1), floor(f) + 1) clipped to its tile's
+// pixel bounds [tile_min - 1, tile_max) (film.rs:121-136) and the film.
+__global__ __launch_bounds__(256) void k_film(RenderParamsDev p, PathBuffers b, uint32_t W, uint32_t H, float4* film,
+                                              double4* film64) {
+    const uint32_t ww = p.x1 - p.x0, wh = p.y1 - p.y0;
+    const uint64_t npix = (uint64_t)ww * wh;
+    const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= npix) return;
+    const int px = (int)(p.x0 + t % ww), py = (int)(p.y0 + t / ww);
+    float4 acc = film[(uint64_t)py * W + px];
+    double4 acc64 = film64 ? film64[(uint64_t)py * W + px] : double4{0, 0, 0, 0};
+    for (uint32_t s = 0; s < p.n_samples; ++s) {
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int qy = py + dy;
+            if (qy < (int)p.y0 || qy >= (int)p.y1) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int qx = px + dx;
+                if (qx < (int)p.x0 || qx >= (int)p.x1) continue;
+                const uint64_t path = (uint64_t)s * npix + (uint64_t)(qy - (int)p.y0) * ww + (qx - (int)p.x0);
+                const float2 f = b.film[path];
+                const float fpx = f.x - 0.5f, fpy = f.y - 0.5f;
+                const int tx0 = (qx / 16) * 16, ty0 = (qy / 16) * 16;
+                const int lx = max((int)ceilf(fpx - 0.5f), max(tx0 - 1, 0));
+                const int ly = max((int)ceilf(fpy - 0.5f), max(ty0 - 1, 0));
+                const int hx = min((int)floorf(fpx + 0.5f) + 1, min(tx0 + 16, (int)W));
+                const int hy = min((int)floorf(fpy + 0.5f) + 1, min(ty0 + 16, (int)H));
+                if (px < lx || px >= hx || py < ly || py >= hy) continue;
+                const float4 l4 = b.L[path];
+                C3 L = C3{l4.x, l4.y, l4.z};
+                // integrator.rs:92-117
+                const float y = luminance(L);
+                if (L.r != L.r || L.g != L.g || L.b != L.b) L = mkc(0.0f);
+                else if (y < -1e-5f) L = mkc(0.0f);
+                else if (__builtin_isinf(y)) L = mkc(0.0f);
+                acc.x += L.r; acc.y += L.g; acc.z += L.b; acc.w += 1.0f;
+                if (film64) { acc64.x += L.r; acc64.y += L.g; acc64.z += L.b; acc64.w += 1.0; }
+            }
+        }
+    }
+    film[(uint64_t)py * W + px] = acc;
+    if (film64) film64[(uint64_t)py * W + px] = acc64;
+}
+
+}  // namespace
+
+hipError_t launch_init_paths(const RenderParamsDev& p, const PathBuffers& b, hipStream_t s) {
+    if (p.n_paths == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_init_paths, dim3((uint32_t)((p.n_paths + 255) / 256)), dim3(256), 0, s, p, b);
+    return hipGetLastError();
+}
+hipError_t launch_shade(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b, uint32_t bounce,
+                        hipStream_t s) {
+    if (p.n_paths == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_shade, dim3((uint32_t)((p.n_paths + 127) / 128)), dim3(128), 0, s, p, sc, b, bounce);
+    return hipGetLastError();
+}
+hipError_t launch_nee_resolve(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b, hipStream_t s) {
+    if (p.n_paths == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_nee_resolve, dim3((uint32_t)((p.n_paths + 255) / 256)), dim3(256), 0, s, p, sc, b);
+    return hipGetLastError();
+}
+hipError_t launch_film(const RenderParamsDev& p, const PathBuffers& b, uint32_t W, uint32_t H, float4* film,
+                       double4* film64, hipStream_t s) {
+    const uint64_t npix = (uint64_t)(p.x1 - p.x0) * (p.y1 - p.y0);
+    if (npix == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_film, dim3((uint32_t)((npix + 255) / 256)), dim3(256), 0, s, p, b, W, H, film, film64);
+    return hipGetLastError();
+}
+
+}  // namespace bg

@@ -1,0 +1,252 @@
+// Host driver of the wavefront path integrator and the film
+// (SamplerIntegrator::render, base/integrator.rs:35-133; Film, base/film.rs).
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "bg_internal.h"
+
+namespace {
+
+int fail(bg_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                      \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail((ctx), BG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Path-state arrays carved out of one allocation, 256-byte aligned each.
+int ensure_paths(bg_ctx* ctx, size_t cap) {
+    bg::PathBuffers& b = ctx->paths;
+    if (b.cap >= cap) return BG_OK;
+    if (b.mem) hipFree(b.mem);
+    b = bg::PathBuffers{};
+    const size_t per_path[] = {32, 16, 16, 16, 8, 8, 8, 32, 16, 32, 16, 16, 16, 16, 16};
+    size_t total = 256;
+    for (size_t s : per_path) total += ((s * cap + 255) / 256) * 256;
+    void* mem = nullptr;
+    HIP_TRY(ctx, hipMalloc(&mem, total));
+    char* p = (char*)mem;
+    auto take = [&](size_t bytes) {
+        void* r = p;
+        p += ((bytes + 255) / 256) * 256;
+        return r;
+    };
+    b.mem = mem;
+    b.ray = (float4*)take(32 * cap);
+    b.hit = (uint4*)take(16 * cap);
+    b.beta = (float4*)take(16 * cap);
+    b.L = (float4*)take(16 * cap);
+    b.key = (uint2*)take(8 * cap);
+    b.state = (uint2*)take(8 * cap);
+    b.film = (float2*)take(8 * cap);
+    b.sray = (float4*)take(32 * cap);
+    b.shit = (uint4*)take(16 * cap);
+    b.mray = (float4*)take(32 * cap);
+    b.mhit = (uint4*)take(16 * cap);
+    b.nee_a = (float4*)take(16 * cap);
+    b.nee_b = (float4*)take(16 * cap);
+    b.nee_c = (float4*)take(16 * cap);
+    b.nee_d = (float4*)take(16 * cap);
+    b.cap = cap;
+    return BG_OK;
+}
+
+int ensure_spill(bg_ctx* ctx, size_t need) {
+    if (ctx->spill_bytes >= need) return BG_OK;
+    if (ctx->spill) hipFree(ctx->spill);
+    ctx->spill = nullptr;
+    ctx->spill_bytes = 0;
+    HIP_TRY(ctx, hipMalloc((void**)&ctx->spill, need));
+    ctx->spill_bytes = need;
+    return BG_OK;
+}
+
+bg::TraverseArgs traverse_args(const bg_scene* s, bg_ctx* ctx, const float4* rays, uint4* hits, uint64_t n) {
+    bg::TraverseArgs a;
+    a.nodes = s->d_nodes;
+    a.tri_verts = s->d_tri_verts;
+    a.slot_prim = s->d_slot_prim;
+    a.spheres = s->d_spheres;
+    a.sphere_aux = s->d_sphere_aux;
+    a.n_tris = (uint32_t)s->n_tris;
+    a.rays = rays;
+    a.hits = hits;
+    a.n_rays = n;
+    a.spill = ctx->spill;
+    a.spill_depth = s->bvh.max_depth + 1;
+    a.counters = nullptr;
+    return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bg_film_create(bg_ctx* ctx, uint32_t width, uint32_t height, uint32_t flags, bg_film** out) {
+    if (!ctx || !out || width == 0 || height == 0 || (flags & ~BG_FILM_F64)) return fail(ctx, BG_ERR_INVALID, "bad film");
+    *out = nullptr;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    bg_film* f = new (std::nothrow) bg_film();
+    if (!f) return BG_ERR_NOMEM;
+    f->ctx = ctx;
+    f->width = width;
+    f->height = height;
+    f->flags = flags;
+    const size_t n = (size_t)width * height;
+    if (hipMalloc((void**)&f->sums, n * sizeof(float4)) != hipSuccess ||
+        ((flags & BG_FILM_F64) && hipMalloc((void**)&f->sums64, n * sizeof(double4)) != hipSuccess)) {
+        bg_film_destroy(f);
+        return fail(ctx, BG_ERR_NOMEM, "film allocation failed");
+    }
+    *out = f;
+    return bg_film_clear(f);
+}
+
+void bg_film_destroy(bg_film* f) {
+    if (!f) return;
+    if (f->sums) hipFree(f->sums);
+    if (f->sums64) hipFree(f->sums64);
+    delete f;
+}
+
+int bg_film_clear(bg_film* f) {
+    if (!f) return BG_ERR_INVALID;
+    bg_ctx* ctx = f->ctx;
+    const size_t n = (size_t)f->width * f->height;
+    HIP_TRY(ctx, hipMemsetAsync(f->sums, 0, n * sizeof(float4), ctx->stream));
+    if (f->sums64) HIP_TRY(ctx, hipMemsetAsync(f->sums64, 0, n * sizeof(double4), ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return BG_OK;
+}
+
+int bg_film_read(bg_film* f, float* rgbw, double* rgbw64) {
+    if (!f) return BG_ERR_INVALID;
+    bg_ctx* ctx = f->ctx;
+    const size_t n = (size_t)f->width * f->height;
+    if (rgbw64 && !f->sums64) return fail(ctx, BG_ERR_INVALID, "film has no FP64 sums");
+    if (rgbw) HIP_TRY(ctx, hipMemcpy(rgbw, f->sums, n * sizeof(float4), hipMemcpyDeviceToHost));
+    if (rgbw64) HIP_TRY(ctx, hipMemcpy(rgbw64, f->sums64, n * sizeof(double4), hipMemcpyDeviceToHost));
+    return BG_OK;
+}
+
+int bg_render_film(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film* film, bg_stats* stats) {
+    if (!ctx || !s || !d || !film || s->ctx != ctx || film->ctx != ctx) return fail(ctx, BG_ERR_INVALID, "null or foreign argument");
+    const bg_camera_desc& cam = d->camera;
+    if (cam.width != film->width || cam.height != film->height) return fail(ctx, BG_ERR_INVALID, "film size differs from camera");
+    if (cam.lens_radius > 0.0f) return fail(ctx, BG_ERR_UNSUPPORTED, "thin-lens cameras are not supported on the GPU path");
+    if (d->mode > BG_MODE_CORRECTED) return fail(ctx, BG_ERR_INVALID, "unknown mode");
+    if (s->materials.empty()) return fail(ctx, BG_ERR_INVALID, "scene has no materials");
+    uint32_t x0 = d->x0, y0 = d->y0, x1 = d->x1, y1 = d->y1;
+    if (x1 == 0 && y1 == 0) { x0 = 0; y0 = 0; x1 = cam.width; y1 = cam.height; }
+    if (x0 >= x1 || y0 >= y1 || x1 > cam.width || y1 > cam.height) return fail(ctx, BG_ERR_INVALID, "bad window");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (stats) *stats = bg_stats{};
+    if (d->spp == 0 || s->bvh.nodes.empty()) return BG_OK;
+
+    const uint64_t npix = (uint64_t)(x1 - x0) * (y1 - y0);
+    uint64_t target = d->paths_per_pass ? d->paths_per_pass : (uint64_t)4 << 20;
+    uint32_t spp_pass = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(d->spp, target / npix));
+    const uint64_t cap = npix * spp_pass;
+    int rc = ensure_paths(ctx, cap);
+    if (rc) return rc;
+    const int block = ctx->shape.block;
+    const uint64_t grid_cap = ctx->shape.blocks_per_cu > 0 ? (uint64_t)ctx->num_cus * ctx->shape.blocks_per_cu : (1u << 20);
+    const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((cap + block - 1) / block, std::min<uint64_t>(grid_cap, 1u << 20)));
+    rc = ensure_spill(ctx, (size_t)grid * block * (s->bvh.max_depth + 1) * sizeof(uint32_t));
+    if (rc) return rc;
+
+    bg::RenderParamsDev p;
+    bg::camera_setup(cam, &p.cam);
+    p.seed = d->seed;
+    p.x0 = x0; p.y0 = y0; p.x1 = x1; p.y1 = y1;
+    p.max_depth = d->max_depth;
+    p.rr = d->rr_threshold;
+    p.mode = d->mode;
+    bg::ShadeSceneDev sc;
+    sc.tri_orig = s->d_tri_orig;
+    sc.tri_flip = s->d_tri_flip;
+    sc.prim_mat = s->d_prim_mat;
+    sc.prim_light = s->d_prim_light;
+    sc.mats = s->d_mats;
+    sc.lights = s->d_lights;
+    sc.spheres = s->d_sph_shade;
+    sc.n_tris = (uint32_t)s->n_tris;
+    sc.n_lights = (uint32_t)s->lights.size();
+    const bg::PathBuffers& b = ctx->paths;
+    hipStream_t st = ctx->stream;
+
+    auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+    uint64_t launches = 0;
+    for (uint32_t s0 = 0; s0 < d->spp; s0 += spp_pass) {
+        const uint32_t ns = std::min(spp_pass, d->spp - s0);
+        p.first_sample = d->first_sample + s0;
+        p.n_samples = ns;
+        p.n_paths = npix * ns;
+        const int g = (int)std::min<uint64_t>(grid, (p.n_paths + block - 1) / block);
+        HIP_TRY(ctx, bg::launch_init_paths(p, b, st));
+        for (uint32_t bounce = 0; bounce <= d->max_depth; ++bounce) {
+            HIP_TRY(ctx, bg::launch_traverse(traverse_args(s, ctx, b.ray, b.hit, p.n_paths), false, false, ctx->shape, g, st));
+            HIP_TRY(ctx, bg::launch_shade(p, sc, b, bounce, st));
+            if (sc.n_lights > 0 && bounce < d->max_depth) {
+                HIP_TRY(ctx, bg::launch_traverse(traverse_args(s, ctx, b.sray, b.shit, p.n_paths), true, false, ctx->shape, g, st));
+                HIP_TRY(ctx, bg::launch_traverse(traverse_args(s, ctx, b.mray, b.mhit, p.n_paths), false, false, ctx->shape, g, st));
+                HIP_TRY(ctx, bg::launch_nee_resolve(p, sc, b, st));
+                launches += 3;
+            }
+            launches += 2;
+        }
+        HIP_TRY(ctx, bg::launch_film(p, b, film->width, film->height, film->sums, film->sums64, st));
+        launches += 2;
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    auto t1 = std::chrono::steady_clock::now();
+    float ms = 0.0f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (stats) {
+        stats->kernel_ms = ms;
+        stats->total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        stats->launches = launches;
+        stats->samples = npix * d->spp;
+    }
+    return BG_OK;
+}
+
+int bg_render(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, float* rgb_out, bg_stats* stats) {
+    if (!ctx || !d || !rgb_out) return fail(ctx, BG_ERR_INVALID, "null argument");
+    bg_film* film = nullptr;
+    int rc = bg_film_create(ctx, d->camera.width, d->camera.height, 0, &film);
+    if (rc) return rc;
+    rc = bg_render_film(ctx, s, d, film, stats);
+    const size_t n = (size_t)d->camera.width * d->camera.height;
+    std::vector<float> sums(4 * n);
+    if (!rc) rc = bg_film_read(film, sums.data(), nullptr);
+    bg_film_destroy(film);
+    if (rc) return rc;
+    // film.rs:138-202: RGB sums -> XYZ (merge) -> RGB, / weight, clamp >= 0, scale 1
+    for (size_t i = 0; i < n; ++i) {
+        const float r = sums[4 * i], g = sums[4 * i + 1], b = sums[4 * i + 2], w = sums[4 * i + 3];
+        const float X = 0.412453f * r + 0.357580f * g + 0.180423f * b;
+        const float Y = 0.212671f * r + 0.715160f * g + 0.072169f * b;
+        const float Z = 0.019334f * r + 0.119193f * g + 0.950227f * b;
+        float o[3] = {3.240479f * X - 1.537150f * Y - 0.498535f * Z, -0.969256f * X + 1.875991f * Y + 0.041556f * Z,
+                      0.055648f * X - 0.204043f * Y + 1.057311f * Z};
+        if (w != 0.0f) {
+            const float iw = 1.0f / w;
+            for (int k = 0; k < 3; ++k) o[k] = std::max(o[k] * iw, 0.0f);
+        }
+        for (int k = 0; k < 3; ++k) rgb_out[3 * i + k] = o[k];
+    }
+    return BG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,120 @@
+"""GpuPathIntegrator: stands where the reference's PathIntegrator stands
+(src/integrators/path.rs:19-40; render = SamplerIntegrator::render,
+src/base/integrator.rs:35-133), rendering on the GPU into a Film whose
+sums can be read back or resolved like Film::write_image (film.rs:156-202).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .gpu import Context, GpuBvh
+from .scenes import Camera
+
+MODE_REF = N.BG_MODE_REF
+MODE_CORRECTED = N.BG_MODE_CORRECTED
+
+_XYZ = np.array([[0.412453, 0.357580, 0.180423], [0.212671, 0.715160, 0.072169], [0.019334, 0.119193, 0.950227]])
+_RGB = np.array([[3.240479, -1.537150, -0.498535], [-0.969256, 1.875991, 0.041556], [0.055648, -0.204043, 1.057311]])
+
+
+def resolve(rgb_sum: np.ndarray, weight: np.ndarray) -> np.ndarray:
+    """Film::write_image: RGB sums -> XYZ -> RGB, divided by the weight sum, clamped at 0."""
+    xyz = rgb_sum @ _XYZ.T
+    rgb = xyz @ _RGB.T
+    w = weight[..., None]
+    out = np.where(w != 0, np.maximum(rgb / np.where(w != 0, w, 1), 0), rgb)
+    return out
+
+
+class Film:
+    def __init__(self, ctx: Context, width: int, height: int, f64: bool = False):
+        self.ctx = ctx
+        self.width, self.height = width, height
+        self.f64 = f64
+        h = C.c_void_p()
+        N.check(N.lib().bg_film_create(ctx.h, width, height, N.BG_FILM_F64 if f64 else 0, C.byref(h)), ctx.h)
+        self.h = h
+
+    def close(self):
+        if self.h:
+            N.lib().bg_film_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clear(self):
+        N.check(N.lib().bg_film_clear(self.h), self.ctx.h)
+
+    def sums(self, f64: bool | None = None) -> tuple[np.ndarray, np.ndarray]:
+        """(rgb_sum (h, w, 3), weight_sum (h, w)); FP64 sums when the film keeps them."""
+        use64 = self.f64 if f64 is None else f64
+        if use64:
+            a = np.empty((self.height, self.width, 4), np.float64)
+            N.check(N.lib().bg_film_read(self.h, None, a.ctypes.data), self.ctx.h)
+        else:
+            a = np.empty((self.height, self.width, 4), np.float32)
+            N.check(N.lib().bg_film_read(self.h, a.ctypes.data, None), self.ctx.h)
+        return a[..., :3], a[..., 3]
+
+    def image(self) -> np.ndarray:
+        rgb, w = self.sums()
+        return resolve(rgb.astype(np.float64), w.astype(np.float64))
+
+
+class GpuPathIntegrator:
+    """PathIntegrator::new(camera, sampler, max_depth, rr_threshold) with an independent
+    (sampled_dimensions = 0) sampler of `spp` samples per pixel."""
+
+    def __init__(self, ctx: Context, camera: Camera, spp: int, max_depth: int, rr_threshold: float = 1.0,
+                 mode: int = MODE_REF, seed: int = 1):
+        self.ctx = ctx
+        self.camera = camera
+        self.spp = spp
+        self.max_depth = max_depth
+        self.rr_threshold = rr_threshold
+        self.mode = mode
+        self.seed = seed
+
+    def desc(self, spp=None, first_sample=0, window=None, paths_per_pass=0) -> N.bg_render_desc:
+        d = N.bg_render_desc()
+        d.camera = self.camera.desc()
+        d.spp = self.spp if spp is None else spp
+        d.first_sample = first_sample
+        d.max_depth = self.max_depth
+        d.rr_threshold = self.rr_threshold
+        d.mode = self.mode
+        d.seed = self.seed
+        if window is not None:
+            d.x0, d.y0, d.x1, d.y1 = window
+        d.paths_per_pass = paths_per_pass
+        return d
+
+    def render_into(self, bvh: GpuBvh, film: Film, spp=None, first_sample=0, window=None,
+                    paths_per_pass=0) -> N.bg_stats:
+        d = self.desc(spp, first_sample, window, paths_per_pass)
+        st = N.bg_stats()
+        N.check(N.lib().bg_render_film(self.ctx.h, bvh.h, C.byref(d), film.h, C.byref(st)), self.ctx.h)
+        return st
+
+    def render(self, bvh: GpuBvh) -> np.ndarray:
+        """Integrator::render: the resolved (h, w, 3) image."""
+        d = self.desc()
+        out = np.empty((self.camera.height, self.camera.width, 3), np.float32)
+        st = N.bg_stats()
+        N.check(N.lib().bg_render(self.ctx.h, bvh.h, C.byref(d), out.ctypes.data, C.byref(st)), self.ctx.h)
+        self.last_stats = st
+        return out
+
+
+def rel_mse(img: np.ndarray, ref: np.ndarray) -> float:
+    """mean over pixels and channels of (I - R)^2 / (R^2 + 0.01), in FP64."""
+    i = img.astype(np.float64)
+    r = ref.astype(np.float64)
+    return float(np.mean((i - r) ** 2 / (r * r + 0.01)))

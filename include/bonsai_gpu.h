@@ -139,6 +139,9 @@ typedef struct bg_stats {
     uint64_t nodes_visited;
     uint64_t tris_tested;
     uint64_t spheres_tested;
+    /* Filled by the render calls. */
+    uint64_t samples;     /* camera samples completed */
+    uint64_t shadow_rays; /* any-hit queries among `rays` */
 } bg_stats;
 
 /* Library */
@@ -203,6 +206,48 @@ int bg_look_at(const float pos[3], const float look[3], const float up[3], float
  * row-major order, written to device memory. */
 int bg_camera_rays_device(bg_ctx* ctx, const bg_camera_desc* cam, float fx, float fy, void* d_rays);
 int bg_camera_rays(bg_ctx* ctx, const bg_camera_desc* cam, float fx, float fy, bg_ray* rays);
+
+/* ------------------------------------------------------------------ render */
+
+/* Integrator semantics. REF reproduces src/integrators/path.rs and
+ * src/base/integrator.rs as written (the camera ray is re-intersected with a
+ * shrinking t_max instead of spawning continuation rays, the light pdf lacks
+ * 1/area, light samples take emission toward +wi, the MIS branch traces the
+ * light-sample direction, G1 = Lambda, sample11 keeps slope_x_1). CORRECTED
+ * is the pbrt-v3 estimator those lines intend. */
+#define BG_MODE_REF 0u
+#define BG_MODE_CORRECTED 1u
+
+typedef struct bg_render_desc {
+    bg_camera_desc camera;
+    uint32_t spp;           /* samples per pixel of this call */
+    uint32_t first_sample;  /* sample index of the first one (progressive rendering) */
+    uint32_t max_depth;
+    float rr_threshold;     /* PathIntegrator rr_threshold (path.rs:133) */
+    uint32_t mode;          /* BG_MODE_REF / BG_MODE_CORRECTED */
+    uint64_t seed;
+    uint32_t x0, y0, x1, y1; /* pixel window; x1 == 0 and y1 == 0 means the full film */
+    uint32_t paths_per_pass; /* 0 = choose automatically */
+} bg_render_desc;
+
+/* Film: per-pixel RGB sums and filter-weight sums (src/base/film.rs). */
+typedef struct bg_film bg_film;
+#define BG_FILM_F64 1u  /* also keep FP64 sums (ground-truth renders) */
+int bg_film_create(bg_ctx* ctx, uint32_t width, uint32_t height, uint32_t flags, bg_film** out);
+void bg_film_destroy(bg_film* film);
+int bg_film_clear(bg_film* film);
+/* rgbw: width*height*4 floats (r, g, b, weight sums); rgbw64 likewise in
+ * double (requires BG_FILM_F64). Either may be NULL. */
+int bg_film_read(bg_film* film, float* rgbw, double* rgbw64);
+
+/* Renders desc->spp samples per pixel of the window into the film
+ * (SamplerIntegrator::render, src/base/integrator.rs:35-133). */
+int bg_render_film(bg_ctx* ctx, const bg_scene* scene, const bg_render_desc* desc, bg_film* film, bg_stats* stats);
+
+/* Convenience: a fresh film, bg_render_film, then Film::write_image
+ * (film.rs:156-202: XYZ round trip, divide by weight, clamp at 0) into
+ * rgb_out (width*height*3 floats of the full film). */
+int bg_render(bg_ctx* ctx, const bg_scene* scene, const bg_render_desc* desc, float* rgb_out, bg_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -7,6 +7,7 @@
 
 #include "bvh.h"
 #include "camera.h"
+#include "integrator.h"
 
 using namespace orc;
 
@@ -15,6 +16,7 @@ namespace {
 struct OrcScene {
     Scene scene;
     RefBVH bvh;
+    ShadeScene shade;
 };
 
 // Packed node layout shared with the GPU library's export (32 bytes):
@@ -223,6 +225,48 @@ void orc_camera_rays(const float* c2w, const float* c2w_inv, float fov, float ne
             o[0] = r.o.x; o[1] = r.o.y; o[2] = r.o.z; o[3] = r.t_max;
             o[4] = r.d.x; o[5] = r.d.y; o[6] = r.d.z; o[7] = r.time;
         }
+}
+
+// Materials: mat_type[n], mat_params[n * 10] = (r[3], alpha, eta[3], k[3]); prim_material[n_prims];
+// lights: light_prim[nl], light_L[nl * 3], light_double_sided[nl].
+void orc_scene_set_shading(void* h, uint32_t n_mat, const uint32_t* mat_type, const float* mat_params,
+                           const uint32_t* prim_material, uint32_t n_lights, const uint32_t* light_prim,
+                           const float* light_L, const uint8_t* light_ds) {
+    OrcScene* s = static_cast<OrcScene*>(h);
+    s->shade.materials.resize(n_mat);
+    for (uint32_t i = 0; i < n_mat; ++i) {
+        Material& m = s->shade.materials[i];
+        const float* q = mat_params + 10 * i;
+        m.type = mat_type[i];
+        m.r = RGB(q[0], q[1], q[2]);
+        m.alpha = q[3];
+        m.eta = RGB(q[4], q[5], q[6]);
+        m.k = RGB(q[7], q[8], q[9]);
+    }
+    const size_t np = s->scene.n_prims();
+    s->shade.prim_material.assign(prim_material, prim_material + np);
+    s->shade.prim_light.assign(np, -1);
+    s->shade.lights.clear();
+    for (uint32_t i = 0; i < n_lights; ++i) {
+        s->shade.lights.push_back(AreaLight{light_prim[i], RGB(light_L[3 * i], light_L[3 * i + 1], light_L[3 * i + 2]),
+                                            light_ds[i] != 0});
+        s->shade.prim_light[light_prim[i]] = (int32_t)i;
+    }
+}
+
+// Renders window [x0,x1) x [y0,y1) of a width x height film; writes per-pixel
+// double RGB sums (3 per pixel) and filter-weight sums.
+void orc_render(void* h, const float* c2w, const float* c2w_inv, float fov, float near_clip, float far_clip,
+                float lens_radius, float focal_distance, float shutter_open, float shutter_close, uint32_t width,
+                uint32_t height, uint32_t spp, uint32_t max_depth, float rr, uint64_t seed, uint32_t mode,
+                uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1, int threads, double* rgb_sum, double* weight_sum) {
+    OrcScene* s = static_cast<OrcScene*>(h);
+    s->shade.geom = &s->scene;
+    s->shade.bvh = &s->bvh;
+    PerspCamera cam = make_camera(mat4_from(c2w), mat4_from(c2w_inv), fov, near_clip, far_clip, shutter_open,
+                                  shutter_close, lens_radius, focal_distance, (Float)width, (Float)height);
+    RenderParams p{width, height, spp, max_depth, rr, seed, mode == 0 ? MODE_REF : MODE_CORRECTED, x0, y0, x1, y1};
+    render(s->shade, cam, p, threads, rgb_sum, weight_sum);
 }
 
 // Helpers for closed-form tests of the numerics.

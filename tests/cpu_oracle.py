@@ -40,6 +40,9 @@ def lib() -> C.CDLL:
             "orc_look_at": (None, [P, P, P, P, P]),
             "orc_translate_scale": (None, [P, C.c_float, C.c_float, C.c_float, P, P]),
             "orc_mat4_inverse": (None, [P, P]),
+            "orc_scene_set_shading": (None, [P, C.c_uint32, P, P, P, C.c_uint32, P, P, P]),
+            "orc_render": (None, [P, P, P] + [C.c_float] * 7 + [C.c_uint32] * 4 + [C.c_float, C.c_uint64] +
+                           [C.c_uint32] * 5 + [C.c_int, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -113,6 +116,37 @@ class OracleScene:
         hits = np.empty((rays.shape[0], 4), np.uint32)
         lib().orc_intersect_brute(self.h, rays.ctypes.data, rays.shape[0], order.ctypes.data, hits.ctypes.data)
         return hits
+
+    def set_shading(self, scene) -> None:
+        mats = scene.materials
+        mt = np.array([m.type for m in mats], np.uint32)
+        mp = np.array([[*m.reflectance, m.alpha, *m.eta, *m.k] for m in mats], np.float32).reshape(-1, 10)
+        n_prims = scene.n_tris + scene.n_spheres
+        pm = np.zeros(n_prims, np.uint32)
+        if scene.tri_material is not None:
+            pm[:scene.n_tris] = scene.tri_material
+        if scene.sphere_material is not None:
+            pm[scene.n_tris:] = scene.sphere_material
+        lp = np.array([l.prim_id for l in scene.lights], np.uint32)
+        lL = np.array([list(l.radiance) for l in scene.lights], np.float32).reshape(-1, 3)
+        lds = np.array([l.double_sided for l in scene.lights], np.uint8)
+        self._keep += [mt, mp, pm, lp, lL, lds]
+        lib().orc_scene_set_shading(self.h, len(mats), mt.ctypes.data, mp.ctypes.data, pm.ctypes.data, len(lp),
+                                    lp.ctypes.data, lL.ctypes.data, lds.ctypes.data)
+
+    def render(self, scene, spp: int, max_depth: int, mode: int = 0, seed: int = 1, rr: float = 1.0,
+               window=None, threads: int | None = None):
+        """Returns (rgb_sum (h, w, 3) float64, weight_sum (h, w)) for the window (x0, y0, x1, y1)."""
+        cam = scene.camera
+        m, mi = cam.matrices()
+        x0, y0, x1, y1 = window or (0, 0, cam.width, cam.height)
+        rgb = np.zeros((y1 - y0, x1 - x0, 3), np.float64)
+        w = np.zeros((y1 - y0, x1 - x0), np.float64)
+        th = threads if threads is not None else len(os.sched_getaffinity(0))
+        lib().orc_render(self.h, m.ctypes.data, mi.ctypes.data, cam.fov, cam.near, cam.far, cam.lens_radius,
+                         cam.focal_distance, cam.shutter_open, cam.shutter_close, cam.width, cam.height, spp,
+                         max_depth, rr, seed, mode, x0, y0, x1, y1, th, rgb.ctypes.data, w.ctypes.data)
+        return rgb, w
 
     def prim(self, prim_id: int, ray, kind: int = 0):
         r = np.ascontiguousarray(ray, np.float32).reshape(8)
