@@ -83,6 +83,7 @@ def main() -> int:
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=-1)
+    ap.add_argument("--variant", type=int, default=-1, help="0: per-ray rounds, 1: per-lane refill")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -95,9 +96,10 @@ def main() -> int:
     from bonsai_amd.scenes import incoherent_rays, intersection_camera, intersection_scene
 
     ctx = Context(0)
-    if args.block or args.stack or args.blocks_per_cu >= 0:
-        b, s, c = ctx.launch()
-        ctx.set_launch(args.block or b, args.stack or s, c if args.blocks_per_cu < 0 else args.blocks_per_cu)
+    if args.block or args.stack or args.blocks_per_cu >= 0 or args.variant >= 0:
+        b, s, c, v = ctx.launch()
+        ctx.set_launch(args.block or b, args.stack or s, c if args.blocks_per_cu < 0 else args.blocks_per_cu,
+                       v if args.variant < 0 else args.variant)
     t0 = time.perf_counter()
     scene = intersection_scene(LEVEL)
     gen_s = time.perf_counter() - t0
@@ -170,7 +172,7 @@ def main() -> int:
         else:
             cpu["parity"] = True
 
-    b, s_, c = ctx.launch()
+    b, s_, c, v_ = ctx.launch()
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -187,7 +189,7 @@ def main() -> int:
         "config": {"workload": "intersection microbenchmark: 2^24 coherent + 2^24 incoherent rays, closest-hit "
                                "and any-hit, SAH BVH over a seeded displaced icosphere",
                    "triangles": scene.n_tris, "rays_per_set": N_RAYS, "queries_per_step": N_RAYS * len(queries),
-                   "launch": {"block": b, "stack": s_, "blocks_per_cu": c}},
+                   "launch": {"block": b, "stack": s_, "blocks_per_cu": c, "variant": v_}},
         "breakdown": breakdown,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -62,6 +62,7 @@ struct LaunchShape {
     int block = 128;
     int stack = 16;
     int blocks_per_cu = 8;
+    int variant = 0;  // 0: one ray per loop round, 1: per-lane refill
 };
 
 // traverse.hip

@@ -133,24 +133,154 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(TraverseArgs a) {
     }
 }
 
+// Same traversal with per-lane refill: the loop advances one node per
+// iteration, and a lane whose ray is finished stores its hit and starts the
+// next ray of its grid-stride sequence at once instead of idling until every
+// lane of its wavefront has finished. The per-ray visit sequence is unchanged.
+template <int BLOCK, int STACK, bool ANY, bool COUNT>
+__global__ __launch_bounds__(BLOCK) void k_traverse_refill(TraverseArgs a) {
+    __shared__ uint32_t lds[STACK * BLOCK];
+    const int tid = threadIdx.x;
+    const uint64_t gtid = (uint64_t)blockIdx.x * BLOCK + tid;
+    const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+    uint32_t* __restrict__ spill = a.spill + gtid * a.spill_depth;
+    uint32_t* __restrict__ ring = lds + tid;
+    constexpr uint32_t kMask = STACK - 1;
+    uint64_t c_nodes = 0, c_tris = 0, c_sph = 0;
+
+    uint64_t ray = gtid;
+    if (ray < a.n_rays) {
+        RayState rs;
+        int best = -1;
+        float bt = 0.0f, bb1 = 0.0f, bb2 = 0.0f;
+        uint32_t sp = 0, base = 0, cur = 0;
+        auto start = [&](uint64_t r) {
+            const float4 r0 = a.rays[2 * r];
+            const float4 r1 = a.rays[2 * r + 1];
+            rs.ox = r0.x; rs.oy = r0.y; rs.oz = r0.z; rs.t_max = r0.w;
+            rs.dx = r1.x; rs.dy = r1.y; rs.dz = r1.z;
+            ray_setup(rs);
+            best = -1;
+            bt = 0.0f; bb1 = 0.0f; bb2 = 0.0f;
+            sp = 0; base = 0; cur = 0;
+        };
+        start(ray);
+        for (;;) {
+            const float4 n0 = a.nodes[2 * cur];
+            const float4 n1 = a.nodes[2 * cur + 1];
+            if (COUNT) ++c_nodes;
+            bool descend = false, finished = false;
+            if (slab(rs, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y)) {
+                const uint32_t off = __float_as_uint(n1.z);
+                const uint32_t meta = __float_as_uint(n1.w);
+                const uint32_t count = meta >> 3;
+                if (count > 0) {
+                    const bool mixed = (meta & 4u) != 0;
+                    for (uint32_t s = off; s < off + count; ++s) {
+                        float t, b1, b2;
+                        bool hit;
+                        uint32_t pid = 0;
+                        if (mixed && (pid = a.slot_prim[s]) >= a.n_tris) {
+                            if (COUNT) ++c_sph;
+                            const SphereDev& sp_ = a.spheres[pid - a.n_tris];
+                            SphereHit h;
+                            hit = sphere_hit(rs, sp_.w2o, sp_.radius, sp_.z_min, sp_.z_max, sp_.phi_max, &h);
+                            if (hit) {
+                                t = h.t;
+                                b1 = h.phi / sp_.phi_max;
+                                const SphereAux ax = a.sphere_aux[pid - a.n_tris];
+                                float c = h.pz / sp_.radius;
+                                c = c < -1.0f ? -1.0f : (c > 1.0f ? 1.0f : c);
+                                const float theta = acosf(c);
+                                b2 = (theta - ax.theta_min) / (ax.theta_max - ax.theta_min);
+                            }
+                        } else {
+                            if (COUNT) ++c_tris;
+                            hit = tri_hit(rs, a.tri_verts + 9 * (uint64_t)s, &t, &b1, &b2);
+                        }
+                        if (hit) {
+                            best = (int)s;
+                            bt = t;
+                            if (ANY) {
+                                bb1 = 0.0f; bb2 = 0.0f;
+                                finished = true;
+                                break;
+                            }
+                            bb1 = b1; bb2 = b2;
+                            rs.t_max = t;
+                        }
+                    }
+                } else {
+                    const uint32_t axis = meta & 3u;
+                    const int neg = axis == 0 ? rs.neg0 : (axis == 1 ? rs.neg1 : rs.neg2);
+                    const uint32_t far = neg ? cur + 1 : off;
+                    cur = neg ? off : cur + 1;
+                    if (sp - base == STACK) {
+                        spill[base] = ring[(base & kMask) * BLOCK];
+                        ++base;
+                    }
+                    ring[(sp & kMask) * BLOCK] = far;
+                    ++sp;
+                    descend = true;
+                }
+            }
+            if (!descend && !finished) {
+                if (sp == 0) {
+                    finished = true;
+                } else {
+                    --sp;
+                    if (sp >= base) {
+                        cur = ring[(sp & kMask) * BLOCK];
+                    } else {
+                        cur = spill[sp];
+                        base = sp;
+                    }
+                }
+            }
+            if (finished) {
+                uint4 out;
+                out.x = best >= 0 ? a.slot_prim[best] : 0xFFFFFFFFu;
+                out.y = __float_as_uint(bt);
+                out.z = __float_as_uint(bb1);
+                out.w = __float_as_uint(bb2);
+                a.hits[ray] = out;
+                ray += stride;
+                if (ray >= a.n_rays) break;
+                start(ray);
+            }
+        }
+    }
+    if (COUNT) {
+        atomicAdd(&a.counters[0], (unsigned long long)c_nodes);
+        atomicAdd(&a.counters[1], (unsigned long long)c_tris);
+        atomicAdd(&a.counters[2], (unsigned long long)c_sph);
+    }
+}
+
+template <int BLOCK, int STACK, bool ANY, bool COUNT>
+void launch4(const TraverseArgs& a, int variant, int grid, hipStream_t s) {
+    if (variant == 1) hipLaunchKernelGGL((k_traverse_refill<BLOCK, STACK, ANY, COUNT>), dim3(grid), dim3(BLOCK), 0, s, a);
+    else hipLaunchKernelGGL((k_traverse<BLOCK, STACK, ANY, COUNT>), dim3(grid), dim3(BLOCK), 0, s, a);
+}
+
 template <int BLOCK, int STACK>
-hipError_t launch2(const TraverseArgs& a, bool any, bool count, int grid, hipStream_t s) {
+hipError_t launch2(const TraverseArgs& a, bool any, bool count, int variant, int grid, hipStream_t s) {
     if (count) {
-        if (any) hipLaunchKernelGGL((k_traverse<BLOCK, STACK, true, true>), dim3(grid), dim3(BLOCK), 0, s, a);
-        else hipLaunchKernelGGL((k_traverse<BLOCK, STACK, false, true>), dim3(grid), dim3(BLOCK), 0, s, a);
+        if (any) launch4<BLOCK, STACK, true, true>(a, variant, grid, s);
+        else launch4<BLOCK, STACK, false, true>(a, variant, grid, s);
     } else {
-        if (any) hipLaunchKernelGGL((k_traverse<BLOCK, STACK, true, false>), dim3(grid), dim3(BLOCK), 0, s, a);
-        else hipLaunchKernelGGL((k_traverse<BLOCK, STACK, false, false>), dim3(grid), dim3(BLOCK), 0, s, a);
+        if (any) launch4<BLOCK, STACK, true, false>(a, variant, grid, s);
+        else launch4<BLOCK, STACK, false, false>(a, variant, grid, s);
     }
     return hipGetLastError();
 }
 
 template <int BLOCK>
-hipError_t launch1(const TraverseArgs& a, bool any, bool count, int stack, int grid, hipStream_t s) {
+hipError_t launch1(const TraverseArgs& a, bool any, bool count, int stack, int variant, int grid, hipStream_t s) {
     switch (stack) {
-        case 8: return launch2<BLOCK, 8>(a, any, count, grid, s);
-        case 16: return launch2<BLOCK, 16>(a, any, count, grid, s);
-        case 32: return launch2<BLOCK, 32>(a, any, count, grid, s);
+        case 8: return launch2<BLOCK, 8>(a, any, count, variant, grid, s);
+        case 16: return launch2<BLOCK, 16>(a, any, count, variant, grid, s);
+        case 32: return launch2<BLOCK, 32>(a, any, count, variant, grid, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -160,16 +290,16 @@ hipError_t launch1(const TraverseArgs& a, bool any, bool count, int stack, int g
 bool traverse_shape_supported(const LaunchShape& s) {
     const bool b = s.block == 64 || s.block == 128 || s.block == 256;
     const bool st = s.stack == 8 || s.stack == 16 || s.stack == 32;
-    return b && st && s.blocks_per_cu >= 0;
+    return b && st && s.blocks_per_cu >= 0 && (s.variant == 0 || s.variant == 1);
 }
 
 hipError_t launch_traverse(const TraverseArgs& a, bool any_hit, bool count, const LaunchShape& shape, int grid,
                            hipStream_t stream) {
     if (a.n_rays == 0) return hipSuccess;
     switch (shape.block) {
-        case 64: return launch1<64>(a, any_hit, count, shape.stack, grid, stream);
-        case 128: return launch1<128>(a, any_hit, count, shape.stack, grid, stream);
-        case 256: return launch1<256>(a, any_hit, count, shape.stack, grid, stream);
+        case 64: return launch1<64>(a, any_hit, count, shape.stack, shape.variant, grid, stream);
+        case 128: return launch1<128>(a, any_hit, count, shape.stack, shape.variant, grid, stream);
+        case 256: return launch1<256>(a, any_hit, count, shape.stack, shape.variant, grid, stream);
         default: return hipErrorInvalidValue;
     }
 }

@@ -46,14 +46,14 @@ class Context:
     def stream(self) -> int:
         return int(N.lib().bg_stream(self.h) or 0)
 
-    def set_launch(self, block: int, stack: int, blocks_per_cu: int) -> None:
-        cfg = N.bg_launch_config(block, stack, blocks_per_cu)
+    def set_launch(self, block: int, stack: int, blocks_per_cu: int, variant: int = 0) -> None:
+        cfg = N.bg_launch_config(block, stack, blocks_per_cu, variant)
         N.check(N.lib().bg_set_launch_config(self.h, C.byref(cfg)), self.h)
 
-    def launch(self) -> tuple[int, int, int]:
+    def launch(self) -> tuple[int, int, int, int]:
         cfg = N.bg_launch_config()
         N.check(N.lib().bg_get_launch_config(self.h, C.byref(cfg)), self.h)
-        return cfg.block, cfg.stack, cfg.blocks_per_cu
+        return cfg.block, cfg.stack, cfg.blocks_per_cu, cfg.variant
 
     def camera_rays(self, cam: Camera, fx: float = 0.5, fy: float = 0.5) -> np.ndarray:
         d = cam.desc()

@@ -237,6 +237,69 @@ def intersection_scene(level: int = 8, seed: int = 7) -> SceneData:
     return SceneData(tri_positions=icosphere(level, (0.0, 0.0, 0.0), 1.0, 0.15, 4.0, seed))
 
 
+def _uniform(seed: int, index: int) -> float:
+    return float(N.lib().bgx_uniform(seed, index))
+
+
+def path_tracer_scene(width: int = 1920, height: int = 1080, seed: int = 2024, small: bool = False) -> SceneData:
+    """The plain path-tracer configuration: 16 seeded displaced icospheres (4 at level 7, 8 at level 6,
+    4 at level 5: 2,048,000 triangles) and 256 spheres of radius log-uniform in [0.01, 0.5] scattered
+    over a ground quad, four double-sided quad area lights overhead, and per object either a matte
+    material (albedo U[0.1, 0.9]) or a Trowbridge-Reitz conductor (alpha U[0.05, 0.8]), half each.
+    `small` lowers every icosphere by three levels (for tests)."""
+    levels = [7] * 4 + [6] * 8 + [5] * 4
+    if small:
+        levels = [max(1, l - 3) for l in levels]
+    k = 0
+
+    def u():
+        nonlocal k
+        k += 1
+        return _uniform(seed, k)
+
+    s = SceneData()
+    s.materials = []
+    tris, tri_mat = [], []
+    # ground quad (matte 0.5) spanning the scene
+    g = quad((-14, 0, -14), (14, 0, -14), (14, 0, 14), (-14, 0, 14))
+    s.materials.append(matte(0.5, 0.5, 0.5))
+    tris.append(g)
+    tri_mat += [0, 0]
+    for i, lv in enumerate(levels):
+        r = 0.6 + 1.4 * u()
+        c = (-10 + 20 * u(), r * 0.9, -10 + 20 * u())
+        m = icosphere(lv, c, r, 0.12, 3.0, seed * 100 + i)
+        if u() < 0.5:
+            s.materials.append(matte(0.1 + 0.8 * u(), 0.1 + 0.8 * u(), 0.1 + 0.8 * u()))
+        else:
+            s.materials.append(conductor(0.05 + 0.75 * u()))
+        tris.append(m)
+        tri_mat += [len(s.materials) - 1] * m.shape[0]
+    # four quad lights (double-sided), 2 x 2 m at y = 9
+    light_base = sum(t.shape[0] for t in tris)
+    for (lx, lz) in [(-6, -6), (6, -6), (-6, 6), (6, 6)]:
+        tris.append(quad((lx - 1, 9, lz - 1), (lx + 1, 9, lz - 1), (lx + 1, 9, lz + 1), (lx - 1, 9, lz + 1)))
+        s.materials.append(matte(0.0, 0.0, 0.0))
+        tri_mat += [len(s.materials) - 1] * 2
+    s.tri_positions = np.concatenate(tris).astype(np.float32)
+    s.tri_material = np.asarray(tri_mat, np.uint32)
+    s.lights = [area_light(light_base + j, (12.0, 12.0, 11.0), True) for j in range(8)]
+    centers, radii, smat = [], [], []
+    for i in range(256):
+        r = float(np.exp(np.log(0.01) + (np.log(0.5) - np.log(0.01)) * u()))
+        centers.append((-12 + 24 * u(), r, -12 + 24 * u()))
+        radii.append(r)
+        if u() < 0.5:
+            s.materials.append(matte(0.1 + 0.8 * u(), 0.1 + 0.8 * u(), 0.1 + 0.8 * u()))
+        else:
+            s.materials.append(conductor(0.05 + 0.75 * u()))
+        smat.append(len(s.materials) - 1)
+    add_spheres(s, centers, radii, smat)
+    s.camera = Camera(pos=(0.0, 7.0, -22.0), look=(0.0, 1.0, 0.0), up=(0.0, 1.0, 0.0), fov=50.0,
+                      width=width, height=height)
+    return s
+
+
 def intersection_camera(width: int = 4096, height: int = 4096) -> Camera:
     return Camera(pos=(0.0, 0.3, -3.2), look=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=45.0,
                   width=width, height=height)

@@ -183,6 +183,8 @@ typedef struct bg_launch_config {
     uint32_t block;       /* threads per block: 64, 128, 256 */
     uint32_t stack;       /* LDS stack entries per lane: 8, 16, 32 */
     uint32_t blocks_per_cu; /* persistent blocks per CU; 0 = one block per ray tile */
+    uint32_t variant;     /* 0: a lane finishes its ray before the wavefront takes new rays;
+                             1: each lane starts its next ray as soon as its current one ends */
 } bg_launch_config;
 int bg_set_launch_config(bg_ctx* ctx, const bg_launch_config* cfg);
 int bg_get_launch_config(const bg_ctx* ctx, bg_launch_config* cfg);

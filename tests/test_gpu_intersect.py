@@ -71,7 +71,7 @@ def test_cornell_geometry(ctx):
     rays = box_rays(200_000, 0.01, 0.99, 1)
     bvh, o = run_both(ctx, s, rays)
     g = O.hits_as_words(bvh.intersect(rays))
-    assert (g[:, 0] != MISS).mean() > 0.99  # closed box: almost every interior ray hits
+    assert (g[:, 0] != MISS).mean() > 0.8  # five walls: only rays leaving through the open front miss
 
 
 def test_icosphere_incoherent(ctx):
@@ -140,24 +140,34 @@ def test_empty_scene_and_empty_batch(ctx):
     assert b2.intersect(np.zeros((0, 8), np.float32)).shape[0] == 0
 
 
-def deep_scene(n=400):
-    """Triangles at exponentially shrinking scale along x: the SAH peels one off per level, giving a tree
-    far deeper than any LDS stack, so the spill path runs."""
-    tris = []
-    for k in range(n):
-        x = 1.0 - 0.5 ** (k / 8.0)
-        tris.append([x, 0, 0, x, 1, 0, x, 0, 1])
+def deep_scene(n=120):
+    """Planar triangles at x = 2^k: the centroid buckets peel a few off per level, giving a 30-level tree,
+    deeper than the 8- and 16-entry LDS stacks, so the spill path runs."""
+    tris = [[2.0 ** k, 0, 0, 2.0 ** k, 1, 0, 2.0 ** k, 0, 1] for k in range(n)]
     return SceneData(tri_positions=np.asarray(tris, np.float32))
 
 
-@pytest.mark.parametrize("block,stack", [(64, 8), (128, 16), (256, 32), (128, 8), (64, 32)])
-def test_launch_shapes_do_not_change_results(ctx, block, stack):
+def deep_rays(n, seed):
+    """Rays along +x and -x through the triangles' (y, z) span, reaching the deepest leaves."""
+    rng = np.random.default_rng(seed)
+    r = np.zeros((n, 8), np.float32)
+    r[:, 0] = np.where(rng.random(n) < 0.5, -1.0, 2.0 ** 121)
+    r[:, 1:3] = rng.uniform(0.02, 0.45, (n, 2))
+    r[:, 3] = np.inf
+    r[:, 4] = np.where(r[:, 0] < 0, 1.0, -1.0)
+    r[:, 5:7] = rng.uniform(-1e-3, 1e-3, (n, 2))
+    return r
+
+
+@pytest.mark.parametrize("block,stack,variant", [(64, 8, 0), (128, 16, 0), (256, 32, 0), (128, 8, 0), (64, 32, 0),
+                                                  (64, 8, 1), (128, 16, 1), (256, 32, 1)])
+def test_launch_shapes_do_not_change_results(ctx, block, stack, variant):
     saved = ctx.launch()
     try:
-        ctx.set_launch(block, stack, 4)
+        ctx.set_launch(block, stack, 4, variant)
         s = deep_scene()
         bvh = GpuBvh(ctx, s)
-        rays = box_rays(50_000, -0.5, 1.5, 17, 0, 1)
+        rays = np.concatenate([deep_rays(20_000, 17), box_rays(20_000, -0.5, 1.5, 17, 0, 1)])
         o = oracle_for(bvh, s)
         for any_hit in (False, True):
             g = bvh.intersect_test(rays) if any_hit else bvh.intersect(rays)
@@ -191,7 +201,10 @@ def test_visit_counts_equal_oracle(ctx, any_hit):
 
 def test_deep_tree_really_is_deep(ctx):
     bvh = GpuBvh(ctx, deep_scene())
-    assert bvh.info.max_depth > 40
+    assert bvh.info.max_depth > 20
+    o = oracle_for(bvh, deep_scene())
+    _, cnt = o.intersect(deep_rays(2000, 3), counters=True)
+    assert cnt[:, 0].max() > 20  # rays really walk down the deep path
 
 
 def test_full_size_intersection_benchmark_subset(ctx):
