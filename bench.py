@@ -43,6 +43,39 @@ def alg_bytes(n_rays: int, st) -> float:
     return 32.0 * n_rays + 16.0 * n_rays + 32.0 * st.nodes_visited + 36.0 * st.tris_tested + 64.0 * st.spheres_tested
 
 
+def traffic_from(prof_dir: str, any_hit: bool, set_index: int):
+    """HBM bytes per launch of the timed traversal kernel for one query, from rocprofv3 --pmc runs of
+    `bench.py --no-cpu --steps 1 --warmup 0` (FETCH_SIZE and WRITE_SIZE, KiB per dispatch) scaled by the
+    factors a plain copy of known size gives (`bench.py --copy-bytes N`)."""
+    import csv
+    import glob
+
+    def rows(pattern):
+        files = glob.glob(os.path.join(prof_dir, pattern))
+        return list(csv.DictReader(open(files[0]))) if files else []
+
+    def kernel_values(pattern, counter):
+        tag = f"k_traverse<256, 16, {'true' if any_hit else 'false'}, false"
+        vals = [float(r["Counter_Value"]) for r in rows(pattern)
+                if r["Counter_Name"] == counter and tag in r["Kernel_Name"]]
+        return vals[set_index] if len(vals) > set_index else None
+
+    def copy_factor(pattern, counter):
+        vals = [float(r["Counter_Value"]) for r in rows(pattern)
+                if r["Counter_Name"] == counter and "copyBuffer" in r["Kernel_Name"]]
+        if not vals:
+            return None
+        return (1 << 30) / 1024.0 / (sum(vals) / len(vals))
+
+    f = kernel_values("prof_fetch/*counter_collection.csv", "FETCH_SIZE")
+    w = kernel_values("prof_write/*counter_collection.csv", "WRITE_SIZE")
+    ff = copy_factor("prof_copyf/*counter_collection.csv", "FETCH_SIZE")
+    wf = copy_factor("prof_copyw/*counter_collection.csv", "WRITE_SIZE")
+    if None in (f, w, ff, wf):
+        return None
+    return round((f * ff + w * wf) * 1024.0, 0)
+
+
 def cpu_baseline(scene, bvh, sets, target_s: float):
     """Times the oracle's reference-order traversal on a random sample of each query set."""
     sys.path.insert(0, str(ROOT / "tests"))
@@ -84,6 +117,12 @@ def main() -> int:
     ap.add_argument("--stack", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=-1)
     ap.add_argument("--variant", type=int, default=-1, help="0: per-ray rounds, 1: per-lane refill")
+    ap.add_argument("--traffic-from", default="",
+                    help="directory holding rocprofv3 --pmc outputs prof_fetch/, prof_write/, prof_copyf/, prof_copyw/ "
+                         "of this workload; fills roofline.traffic")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only copy a buffer of this many bytes device-to-device three times (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against a known traffic)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -91,6 +130,15 @@ def main() -> int:
         return 2
 
     import torch
+
+    if args.copy_bytes:
+        src = torch.ones(args.copy_bytes // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": args.copy_bytes, "copies": 3}), flush=True)
+        return 0
 
     from bonsai_amd import Context, GpuBvh
     from bonsai_amd.scenes import incoherent_rays, intersection_camera, intersection_scene
@@ -153,7 +201,10 @@ def main() -> int:
     dom = max(breakdown, key=lambda k: breakdown[k]["kernel_ms"])
     achieved = breakdown[dom]["alg_GB_s"]
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": f"k_traverse ({dom})",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": (traffic_from(args.traffic_from, dom.endswith("_any"), 1 if dom.startswith("incoherent") else 0)
+                            if args.traffic_from else None),
+                "kernel": f"k_traverse ({dom})",
                 "bytes_per_launch": counts[dom][0]}
 
     cpu = None

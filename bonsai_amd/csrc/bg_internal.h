@@ -58,10 +58,12 @@ struct TraverseArgs {
     unsigned long long* counters;  // [nodes, tris, spheres] when counting
 };
 
+// Default = the best of a sweep over block {64,128,256} x stack {8,16,32} x
+// blocks/CU {4,8,16} x variant on the 2^24-ray intersection workload.
 struct LaunchShape {
-    int block = 128;
+    int block = 256;
     int stack = 16;
-    int blocks_per_cu = 8;
+    int blocks_per_cu = 16;
     int variant = 0;  // 0: one ray per loop round, 1: per-lane refill
 };
 

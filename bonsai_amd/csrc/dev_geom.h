@@ -115,8 +115,8 @@ __device__ __forceinline__ float cross64_len2(float ax, float ay, float az, floa
 // triangle.rs:241-268 / 493-517 with the default uvs (0,0), (1,0), (1,1):
 // dpdu = (-1*dp02 - (-1)*dp12) * 1, dpdv = (-0*dp02 + (-1)*dp12) * 1.
 // Returns true when the triangle is degenerate (the intersection is rejected).
-__device__ __noinline__ bool tri_degenerate(float p0x, float p0y, float p0z, float p1x, float p1y, float p1z,
-                                            float p2x, float p2y, float p2z) {
+__device__ __forceinline__ bool tri_degenerate(float p0x, float p0y, float p0z, float p1x, float p1y, float p1z,
+                                               float p2x, float p2y, float p2z) {
     const float d02x = p0x - p2x, d02y = p0y - p2y, d02z = p0z - p2z;
     const float d12x = p1x - p2x, d12y = p1y - p2y, d12z = p1z - p2z;
     const float m1 = -1.0f, nz = -0.0f;
@@ -129,8 +129,8 @@ __device__ __noinline__ bool tri_degenerate(float p0x, float p0y, float p0z, flo
 }
 
 // triangle.rs:179-190 (operand order as written there).
-__device__ __noinline__ void edges_f64(float p0x, float p0y, float p1x, float p1y, float p2x, float p2y, float* e0,
-                                       float* e1, float* e2) {
+__device__ __forceinline__ void edges_f64(float p0x, float p0y, float p1x, float p1y, float p2x, float p2y, float* e0,
+                                          float* e1, float* e2) {
     const double p2txp1ty = (double)p2x * (double)p1y;
     const double p2typ1tx = (double)p2y * (double)p1x;
     *e0 = (float)(p2typ1tx - p2txp1ty);

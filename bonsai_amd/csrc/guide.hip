@@ -1,0 +1,407 @@
+// Path-guiding MLP on gfx950 matrix cores (no counterpart in the reference).
+//
+// Network: 64-dim encoding of (position, outgoing direction, normal) -> 4
+// hidden layers of 64 ReLU units -> 40 outputs = 8 von Mises-Fisher lobes x
+// (mixture logit, direction 3-vector, log-concentration). Weights are kept in
+// FP32 (master copy, Adam moments) and in BF16 for the matrix cores; every
+// product is v_mfma_f32_32x32x8_bf16 with FP32 accumulation.
+//
+// One wavefront evaluates 32 samples at a time as Y^T = W X^T: the weights
+// are the A operand (32 output rows x 8 inputs per instruction) and the
+// activations the B operand (8 inputs x 32 samples). The accumulator layout
+// (lane l, register r -> row 8*(r/4) + 4*(l/32) + r%4, column l%32) is
+// exactly the B-operand layout of the next layer's k-steps, so activations
+// stay in registers from layer to layer; only the bias, ReLU and the BF16
+// rounding touch them in between. The backward pass reuses the same scheme
+// with the transposed weights. Weight gradients (dW = dY X^T summed over the
+// batch) are split over sample chunks and reduced in the fused Adam kernel.
+#include <hip/hip_runtime.h>
+
+#include "bg_internal.h"
+
+namespace bg {
+namespace {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+constexpr int kIn = 64, kHid = 64, kOut = 40, kLayers = 5;
+constexpr float kPiF = 3.14159265358979323846f;
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x7FFFFFu)) return 0x7FC0;
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+__device__ __forceinline__ s4 pack4(float a, float b, float c, float d) {
+    s4 r;
+    r.x = (short)f2bf(a); r.y = (short)f2bf(b); r.z = (short)f2bf(c); r.w = (short)f2bf(d);
+    return r;
+}
+
+// Encoding feature f of a sample (matches oracle/guide_mlp.py encode()).
+__device__ __forceinline__ float feature(int f, const float q[3], const float wo[3], const float n[3]) {
+    if (f < 48) {
+        const int axis = f / 16, k = (f % 16) / 2;
+        const float a = (float)(1 << k) * kPiF * q[axis];
+        return (f & 1) ? cosf(a) : sinf(a);
+    }
+    if (f < 51) return wo[f - 48];
+    if (f < 54) return n[f - 51];
+    if (f < 60) {
+        const int i = (f - 54) / 2;
+        const float a = kPiF * wo[i];
+        return (f & 1) ? cosf(a) : sinf(a);
+    }
+    return 0.0f;
+}
+
+// Weights in LDS: BF16 [layer][64 rows][64 cols]; A operand of k-step s for
+// output block mb: row 32*mb + lane%32, columns 8s + 4*(lane/32) .. +3.
+__device__ __forceinline__ s4 load_a(const uint16_t* __restrict__ w, int mb, int s, int lane) {
+    const uint16_t* p = w + (32 * mb + (lane & 31)) * 64 + 8 * s + 4 * (lane >> 5);
+    return *reinterpret_cast<const s4*>(p);
+}
+
+// One layer: acc[mb] = W[32mb.., :] * X^T, with X^T given as B operands xb[s].
+__device__ __forceinline__ void layer_mm(const uint16_t* __restrict__ w, const s4 xb[8], int lane, f16v acc[2]) {
+    for (int mb = 0; mb < 2; ++mb) {
+        f16v c = {0};
+#pragma unroll
+        for (int s = 0; s < 8; ++s) c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(load_a(w, mb, s, lane), xb[s], c, 0, 0, 0);
+        acc[mb] = c;
+    }
+}
+
+// Row held by accumulator register r of lane `lane` in block mb.
+__device__ __forceinline__ int acc_row(int mb, int r, int lane) { return 32 * mb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+
+// Bias + optional ReLU, then repack the accumulators as the next layer's B operands.
+__device__ __forceinline__ void epilogue(f16v acc[2], const float* __restrict__ bias, bool relu, int lane, s4 xb[8]) {
+    for (int mb = 0; mb < 2; ++mb) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            float v = acc[mb][r] + bias[acc_row(mb, r, lane)];
+            if (relu) v = v > 0.0f ? v : 0.0f;
+            acc[mb][r] = v;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            xb[4 * mb + q] = pack4(acc[mb][4 * q], acc[mb][4 * q + 1], acc[mb][4 * q + 2], acc[mb][4 * q + 3]);
+    }
+}
+
+// B operands of the encoding for the sample of this lane.
+__device__ __forceinline__ void encode_b(const float q[3], const float wo[3], const float n[3], int lane, s4 xb[8]) {
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int f = 8 * s + 4 * (lane >> 5);
+        xb[s] = pack4(feature(f, q, wo, n), feature(f + 1, q, wo, n), feature(f + 2, q, wo, n), feature(f + 3, q, wo, n));
+    }
+}
+
+struct Sample {
+    float p[3], wo[3], n[3];
+};
+__device__ __forceinline__ Sample load_sample(const float4* __restrict__ in, uint64_t i) {
+    const float4 a = in[3 * i], b = in[3 * i + 1], c = in[3 * i + 2];
+    return Sample{{a.x, a.y, a.z}, {b.x, b.y, b.z}, {c.x, c.y, c.z}};
+}
+
+__device__ __forceinline__ void normalise_pos(const Sample& s, const GuideBox& box, float q[3]) {
+    for (int k = 0; k < 3; ++k) q[k] = (s.p[k] - box.lo[k]) / (box.hi[k] - box.lo[k]);
+}
+
+__device__ void load_weights_lds(const uint16_t* __restrict__ wg, uint16_t* __restrict__ lds, int n_u16) {
+    const uint4* src = reinterpret_cast<const uint4*>(wg);
+    uint4* dst = reinterpret_cast<uint4*>(lds);
+    for (int i = threadIdx.x; i < n_u16 / 8; i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
+}
+
+// ------------------------------------------------------------- inference
+// in: 3 float4 per sample (p, wo, n); out: 40 floats per sample.
+__global__ __launch_bounds__(256) void k_guide_infer(GuideDevice g, GuideBox box, const float4* __restrict__ in,
+                                                     uint64_t n, float* __restrict__ out) {
+    __shared__ uint16_t wl[kLayers * 64 * 64];
+    __shared__ float tile[4][32][65];
+    load_weights_lds(g.wbf, wl, kLayers * 64 * 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t groups = (n + 31) / 32;
+    for (uint64_t grp = (uint64_t)blockIdx.x * 4 + wave; grp < groups; grp += (uint64_t)gridDim.x * 4) {
+        const uint64_t i = grp * 32 + (lane & 31);
+        const bool valid = i < n;
+        Sample smp = valid ? load_sample(in, i) : Sample{{0, 0, 0}, {0, 0, 1}, {0, 0, 1}};
+        float q[3];
+        normalise_pos(smp, box, q);
+        s4 xb[8];
+        encode_b(q, smp.wo, smp.n, lane, xb);
+        f16v acc[2];
+        for (int l = 0; l < kLayers; ++l) {
+            layer_mm(wl + l * 4096, xb, lane, acc);
+            epilogue(acc, g.bias + l * 64, l < kLayers - 1, lane, xb);
+        }
+        for (int mb = 0; mb < 2; ++mb)
+            for (int r = 0; r < 16; ++r) tile[wave][lane & 31][acc_row(mb, r, lane)] = acc[mb][r];
+        __syncthreads();
+        if (lane < 32 && valid)
+            for (int k = 0; k < kOut; ++k) out[i * kOut + k] = tile[wave][lane][k];
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------- mixture math
+struct Lobe8 {
+    float pi[8], mu[8][3], kappa[8], nm[8], s[8];
+};
+__device__ __forceinline__ void mixture_of(const float* o, Lobe8& L) {
+    float amax = -3.0e38f;
+    for (int j = 0; j < 8; ++j) amax = fmaxf(amax, o[5 * j]);
+    float sum = 0.0f;
+    for (int j = 0; j < 8; ++j) { L.pi[j] = expf(o[5 * j] - amax); sum += L.pi[j]; }
+    for (int j = 0; j < 8; ++j) {
+        L.pi[j] /= sum;
+        const float mx = o[5 * j + 1], my = o[5 * j + 2], mz = o[5 * j + 3];
+        const float nm = sqrtf(mx * mx + my * my + mz * mz + 1e-12f);
+        L.nm[j] = nm;
+        L.mu[j][0] = mx / nm; L.mu[j][1] = my / nm; L.mu[j][2] = mz / nm;
+        L.s[j] = o[5 * j + 4];
+        L.kappa[j] = expf(fminf(fmaxf(L.s[j], -4.0f), 7.0f));
+    }
+}
+__device__ __forceinline__ float vmf(const float* mu, float kappa, const float* w) {
+    const float c = kappa / (2.0f * kPiF * (-expm1f(-2.0f * kappa)));
+    return c * expf(kappa * (mu[0] * w[0] + mu[1] * w[1] + mu[2] * w[2] - 1.0f));
+}
+
+// d(-weight * log p(w)) / d out, scaled by inv_batch (oracle loss_grad_out).
+__device__ void out_grad(const float* o, const float* w, float weight, float inv_batch, float* g, float* loss) {
+    Lobe8 L;
+    mixture_of(o, L);
+    float f[8], p = 0.0f;
+    for (int j = 0; j < 8; ++j) { f[j] = vmf(L.mu[j], L.kappa[j], w); p += L.pi[j] * f[j]; }
+    *loss = -weight * logf(p);
+    const float sc = -weight * inv_batch;
+    for (int j = 0; j < 8; ++j) {
+        const float r = L.pi[j] * f[j] / p;
+        g[5 * j] = sc * (r - L.pi[j]);
+        const float k = L.kappa[j];
+        const float muw = L.mu[j][0] * w[0] + L.mu[j][1] * w[1] + L.mu[j][2] * w[2];
+        const float dlogk = 1.0f / k - 2.0f * expf(-2.0f * k) / (-expm1f(-2.0f * k)) + (muw - 1.0f);
+        const bool inside = L.s[j] > -4.0f && L.s[j] < 7.0f;
+        g[5 * j + 4] = inside ? sc * r * dlogk * k : 0.0f;
+        for (int c = 0; c < 3; ++c) {
+            const float proj = k * w[c] - k * muw * L.mu[j][c];
+            g[5 * j + 1 + c] = sc * r * proj / L.nm[j];
+        }
+    }
+}
+
+// ------------------------------------------------------------- training
+// Forward + loss gradient + backward for a batch. Stores every layer's input
+// activations X_l and output gradients dY_l as BF16, feature-major [64][S],
+// for the weight-gradient kernel. Per-sample losses go to loss_out.
+__global__ __launch_bounds__(256) void k_guide_train(GuideDevice g, GuideBox box, const float4* __restrict__ in,
+                                                     const float4* __restrict__ target, uint64_t n, float inv_batch,
+                                                     uint16_t* __restrict__ xs, uint16_t* __restrict__ dys,
+                                                     float* __restrict__ loss_out) {
+    __shared__ uint16_t wl[kLayers * 64 * 64];
+    __shared__ uint16_t wt[kLayers * 64 * 64];
+    __shared__ float tile[4][32][65];
+    load_weights_lds(g.wbf, wl, kLayers * 64 * 64);
+    load_weights_lds(g.wbft, wt, kLayers * 64 * 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t groups = (n + 31) / 32;
+    for (uint64_t grp = (uint64_t)blockIdx.x * 4 + wave; grp < groups; grp += (uint64_t)gridDim.x * 4) {
+        const uint64_t i = grp * 32 + (lane & 31);
+        const bool valid = i < n;
+        Sample smp = valid ? load_sample(in, i) : Sample{{0, 0, 0}, {0, 0, 1}, {0, 0, 1}};
+        float q[3];
+        normalise_pos(smp, box, q);
+        s4 xb[8];
+        encode_b(q, smp.wo, smp.n, lane, xb);
+        f16v acc[2];
+        for (int l = 0; l < kLayers; ++l) {
+            // store X_l (B operand, feature-major) for dW
+            if (valid)
+                for (int s = 0; s < 8; ++s) {
+                    const int f = 8 * s + 4 * (lane >> 5);
+                    for (int j = 0; j < 4; ++j) xs[((uint64_t)l * 64 + f + j) * n + i] = (uint16_t)xb[s][j];
+                }
+            layer_mm(wl + l * 4096, xb, lane, acc);
+            epilogue(acc, g.bias + l * 64, l < kLayers - 1, lane, xb);
+        }
+        // gather the 40 outputs of each sample, form d loss / d out
+        for (int mb = 0; mb < 2; ++mb)
+            for (int r = 0; r < 16; ++r) tile[wave][lane & 31][acc_row(mb, r, lane)] = acc[mb][r];
+        __syncthreads();
+        if (lane < 32) {
+            float o[kOut], gr[kOut], ls = 0.0f;
+            for (int k = 0; k < kOut; ++k) o[k] = tile[wave][lane][k];
+            const float4 tg = valid ? target[i] : make_float4(0, 0, 1, 0);
+            const float w[3] = {tg.x, tg.y, tg.z};
+            out_grad(o, w, valid ? tg.w : 0.0f, inv_batch, gr, &ls);
+            for (int k = 0; k < 64; ++k) tile[wave][lane][k] = k < kOut ? gr[k] : 0.0f;
+            if (valid) loss_out[i] = ls;
+        }
+        __syncthreads();
+        // backward: d of layer-l input = W_l^T dY_l, masked by the ReLU of that input
+        s4 db[8];
+        for (int s = 0; s < 8; ++s) {
+            const int f = 8 * s + 4 * (lane >> 5);
+            db[s] = pack4(tile[wave][lane & 31][f], tile[wave][lane & 31][f + 1], tile[wave][lane & 31][f + 2],
+                          tile[wave][lane & 31][f + 3]);
+        }
+        __syncthreads();
+        for (int l = kLayers - 1; l >= 0; --l) {
+            if (valid)
+                for (int s = 0; s < 8; ++s) {
+                    const int f = 8 * s + 4 * (lane >> 5);
+                    for (int j = 0; j < 4; ++j) dys[((uint64_t)l * 64 + f + j) * n + i] = (uint16_t)db[s][j];
+                }
+            if (l == 0) break;
+            layer_mm(wt + l * 4096, db, lane, acc);
+            // mask by ReLU'(pre-activation of layer l-1) = (X_l > 0)
+            for (int mb = 0; mb < 2; ++mb)
+                for (int r = 0; r < 16; ++r) {
+                    const int row = acc_row(mb, r, lane);
+                    const uint16_t xv = valid ? xs[((uint64_t)l * 64 + row) * n + i] : 0;
+                    if (!(bf2f(xv) > 0.0f)) acc[mb][r] = 0.0f;
+                }
+            for (int mb = 0; mb < 2; ++mb)
+                for (int qq = 0; qq < 4; ++qq)
+                    db[4 * mb + qq] = pack4(acc[mb][4 * qq], acc[mb][4 * qq + 1], acc[mb][4 * qq + 2], acc[mb][4 * qq + 3]);
+        }
+    }
+}
+
+// Weight gradients: for layer l, output block mb, input block nb and sample
+// chunk c, partial[c][l][32mb + row][32nb + col] = sum over the chunk of
+// dY_l[row][s] * X_l[col][s]; bias partials = sum of dY_l over the chunk.
+__global__ __launch_bounds__(64) void k_guide_dw(const uint16_t* __restrict__ xs, const uint16_t* __restrict__ dys,
+                                                 uint64_t n, uint64_t chunk, float* __restrict__ partial,
+                                                 float* __restrict__ bpartial) {
+    const int lane = threadIdx.x;
+    const int l = blockIdx.y / 4, mb = (blockIdx.y / 2) & 1, nb = blockIdx.y & 1;
+    const uint64_t c = blockIdx.x;
+    const uint64_t s0 = c * chunk, s1 = s0 + chunk < n ? s0 + chunk : n;
+    f16v acc = {0};
+    float bsum = 0.0f;
+    const uint64_t rowA = (uint64_t)l * 64 + 32 * mb + (lane & 31);
+    const uint64_t rowB = (uint64_t)l * 64 + 32 * nb + (lane & 31);
+    for (uint64_t s = s0; s < s1; s += 8) {
+        s4 a, b;
+        for (int j = 0; j < 4; ++j) {
+            const uint64_t k = s + 4 * (lane >> 5) + j;
+            a[j] = (short)(k < s1 ? dys[rowA * n + k] : 0);
+            b[j] = (short)(k < s1 ? xs[rowB * n + k] : 0);
+            if (nb == 0) bsum += bf2f((uint16_t)a[j]);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    float* dst = partial + ((c * kLayers + l) * 64) * 64;
+    for (int r = 0; r < 16; ++r) dst[(uint64_t)(32 * mb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)) * 64 + 32 * nb + (lane & 31)] = acc[r];
+    if (nb == 0) {
+        // lanes l and l+32 hold halves of the same row's sum
+        const float other = __shfl_down(bsum, 32);
+        if (lane < 32) bpartial[(c * kLayers + l) * 64 + 32 * mb + lane] = bsum + other;
+    }
+}
+
+// Sums the partials into the gradient of every parameter (oracle flat order),
+// takes one Adam step on the FP32 master weights and refreshes the BF16 copies.
+__global__ __launch_bounds__(256) void k_guide_adam(GuideDevice g, const float* __restrict__ partial,
+                                                    const float* __restrict__ bpartial, uint64_t chunks, float lr,
+                                                    float b1, float b2, float eps, int t, float* __restrict__ grad_out) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= kParamsTotal) return;
+    // locate (layer, row, col) of flat index idx: per layer W (rows x cols) then b (rows)
+    int l = 0, base = 0;
+    int rows = 64, cols = 64;
+    for (;;) {
+        rows = l == kLayers - 1 ? kOut : 64;
+        cols = 64;
+        const int sz = rows * cols + rows;
+        if (idx < base + sz) break;
+        base += sz;
+        ++l;
+    }
+    const int off = idx - base;
+    float gsum = 0.0f;
+    int row, col = -1;
+    if (off < rows * cols) {
+        row = off / cols;
+        col = off % cols;
+        for (uint64_t c = 0; c < chunks; ++c) gsum += partial[((c * kLayers + l) * 64 + row) * 64 + col];
+    } else {
+        row = off - rows * cols;
+        for (uint64_t c = 0; c < chunks; ++c) gsum += bpartial[(c * kLayers + l) * 64 + row];
+    }
+    if (grad_out) grad_out[idx] = gsum;
+    // Adam (Kingma & Ba 2015) with bias correction
+    const float m = b1 * g.m[idx] + (1.0f - b1) * gsum;
+    const float v = b2 * g.v[idx] + (1.0f - b2) * gsum * gsum;
+    g.m[idx] = m;
+    g.v[idx] = v;
+    const float mh = m / (1.0f - powf(b1, (float)t));
+    const float vh = v / (1.0f - powf(b2, (float)t));
+    const float w = g.master[idx] - lr * mh / (sqrtf(vh) + eps);
+    g.master[idx] = w;
+    if (col >= 0) {
+        g.wbf[l * 4096 + row * 64 + col] = f2bf(w);
+        g.wbft[l * 4096 + col * 64 + row] = f2bf(w);
+    } else {
+        g.bias[l * 64 + row] = w;
+    }
+}
+
+// BF16 copies and biases from the FP32 master parameters.
+__global__ __launch_bounds__(256) void k_guide_refresh(GuideDevice g) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= kLayers * 64 * 64) return;
+    const int l = idx / 4096, row = (idx / 64) % 64, col = idx % 64;
+    const int rows = l == kLayers - 1 ? kOut : 64;
+    int base = 0;
+    for (int k = 0; k < l; ++k) base += 64 * 64 + 64;
+    const float w = row < rows ? g.master[base + row * 64 + col] : 0.0f;
+    g.wbf[l * 4096 + row * 64 + col] = f2bf(w);
+    g.wbft[l * 4096 + col * 64 + row] = f2bf(w);
+    if (col == 0) g.bias[l * 64 + row] = row < rows ? g.master[base + rows * 64 + row] : 0.0f;
+}
+
+}  // namespace
+
+hipError_t guide_refresh(const GuideDevice& g, hipStream_t s) {
+    hipLaunchKernelGGL(k_guide_refresh, dim3((kLayers * 4096 + 255) / 256), dim3(256), 0, s, g);
+    return hipGetLastError();
+}
+
+hipError_t guide_infer(const GuideDevice& g, const GuideBox& box, const float4* in, uint64_t n, float* out,
+                       int grid, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const uint64_t need = (n + 127) / 128;
+    const int gr = (int)(need < (uint64_t)grid ? need : (uint64_t)grid);
+    hipLaunchKernelGGL(k_guide_infer, dim3(gr), dim3(256), 0, s, g, box, in, n, out);
+    return hipGetLastError();
+}
+
+hipError_t guide_train_step(const GuideDevice& g, const GuideBox& box, const float4* in, const float4* target,
+                            uint64_t n, const GuideScratch& sc, float lr, int t, int grid, float* grad_out,
+                            hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const uint64_t need = (n + 127) / 128;
+    const int gr = (int)(need < (uint64_t)grid ? need : (uint64_t)grid);
+    hipLaunchKernelGGL(k_guide_train, dim3(gr), dim3(256), 0, s, g, box, in, target, n, 1.0f / (float)n, sc.xs,
+                       sc.dys, sc.loss);
+    const uint64_t chunks = (n + sc.chunk - 1) / sc.chunk;
+    hipLaunchKernelGGL(k_guide_dw, dim3((uint32_t)chunks, kLayers * 4), dim3(64), 0, s, sc.xs, sc.dys, n, sc.chunk,
+                       sc.partial, sc.bpartial);
+    hipLaunchKernelGGL(k_guide_adam, dim3((kParamsTotal + 255) / 256), dim3(256), 0, s, g, sc.partial, sc.bpartial,
+                       chunks, lr, 0.9f, 0.999f, 1e-8f, t, grad_out);
+    return hipGetLastError();
+}
+
+}  // namespace bg

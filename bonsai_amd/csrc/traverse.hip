@@ -23,7 +23,7 @@ namespace {
 
 using namespace dev;
 
-template <int BLOCK, int STACK, bool ANY, bool COUNT>
+template <int BLOCK, int STACK, bool ANY, bool COUNT, bool SPH>
 __global__ __launch_bounds__(BLOCK) void k_traverse(TraverseArgs a) {
     __shared__ uint32_t lds[STACK * BLOCK];
     const int tid = threadIdx.x;
@@ -62,7 +62,7 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(TraverseArgs a) {
                         float t, b1, b2;
                         bool hit;
                         uint32_t pid = 0;
-                        if (mixed && (pid = a.slot_prim[s]) >= a.n_tris) {
+                        if (SPH && mixed && (pid = a.slot_prim[s]) >= a.n_tris) {
                             if (COUNT) ++c_sph;
                             const SphereDev& sp_ = a.spheres[pid - a.n_tris];
                             SphereHit h;
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(TraverseArgs a) {
 // iteration, and a lane whose ray is finished stores its hit and starts the
 // next ray of its grid-stride sequence at once instead of idling until every
 // lane of its wavefront has finished. The per-ray visit sequence is unchanged.
-template <int BLOCK, int STACK, bool ANY, bool COUNT>
+template <int BLOCK, int STACK, bool ANY, bool COUNT, bool SPH>
 __global__ __launch_bounds__(BLOCK) void k_traverse_refill(TraverseArgs a) {
     __shared__ uint32_t lds[STACK * BLOCK];
     const int tid = threadIdx.x;
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_refill(TraverseArgs a) {
                         float t, b1, b2;
                         bool hit;
                         uint32_t pid = 0;
-                        if (mixed && (pid = a.slot_prim[s]) >= a.n_tris) {
+                        if (SPH && mixed && (pid = a.slot_prim[s]) >= a.n_tris) {
                             if (COUNT) ++c_sph;
                             const SphereDev& sp_ = a.spheres[pid - a.n_tris];
                             SphereHit h;
@@ -259,8 +259,14 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_refill(TraverseArgs a) {
 
 template <int BLOCK, int STACK, bool ANY, bool COUNT>
 void launch4(const TraverseArgs& a, int variant, int grid, hipStream_t s) {
-    if (variant == 1) hipLaunchKernelGGL((k_traverse_refill<BLOCK, STACK, ANY, COUNT>), dim3(grid), dim3(BLOCK), 0, s, a);
-    else hipLaunchKernelGGL((k_traverse<BLOCK, STACK, ANY, COUNT>), dim3(grid), dim3(BLOCK), 0, s, a);
+    // scenes without spheres use instantiations with the sphere code compiled out
+    if (a.spheres != nullptr) {
+        if (variant == 1) hipLaunchKernelGGL((k_traverse_refill<BLOCK, STACK, ANY, COUNT, true>), dim3(grid), dim3(BLOCK), 0, s, a);
+        else hipLaunchKernelGGL((k_traverse<BLOCK, STACK, ANY, COUNT, true>), dim3(grid), dim3(BLOCK), 0, s, a);
+    } else {
+        if (variant == 1) hipLaunchKernelGGL((k_traverse_refill<BLOCK, STACK, ANY, COUNT, false>), dim3(grid), dim3(BLOCK), 0, s, a);
+        else hipLaunchKernelGGL((k_traverse<BLOCK, STACK, ANY, COUNT, false>), dim3(grid), dim3(BLOCK), 0, s, a);
+    }
 }
 
 template <int BLOCK, int STACK>
