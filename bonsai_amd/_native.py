@@ -71,7 +71,7 @@ class bg_scene_info(C.Structure):
 class bg_stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("rays", C.c_uint64), ("launches", C.c_uint64),
                 ("nodes_visited", C.c_uint64), ("tris_tested", C.c_uint64), ("spheres_tested", C.c_uint64),
-                ("samples", C.c_uint64), ("shadow_rays", C.c_uint64)]
+                ("samples", C.c_uint64), ("shadow_rays", C.c_uint64), ("train_steps", C.c_uint64)]
 
 
 class bg_launch_config(C.Structure):
@@ -111,9 +111,16 @@ class bg_render_desc(C.Structure):
     ]
 
 
+class bg_guiding_desc(C.Structure):
+    _fields_ = [("select_prob", C.c_float), ("train", C.c_uint32), ("train_paths", C.c_uint32),
+                ("batch", C.c_uint32), ("steps", C.c_uint32), ("lr", C.c_float), ("replay", C.c_uint32)]
+
+
 BG_MODE_REF = 0
 BG_MODE_CORRECTED = 1
 BG_FILM_F64 = 1
+BG_GUIDE_PARAMS = 19240
+BG_GUIDE_OUTPUTS = 40
 
 P = C.c_void_p
 # name -> (restype, argtypes); the list every header declaration must appear in.
@@ -143,6 +150,16 @@ SIGNATURES = {
     "bg_film_read": (C.c_int, [P, P, P]),
     "bg_render_film": (C.c_int, [P, P, C.POINTER(bg_render_desc), P, C.POINTER(bg_stats)]),
     "bg_render": (C.c_int, [P, P, C.POINTER(bg_render_desc), P, C.POINTER(bg_stats)]),
+    "bg_guide_create": (C.c_int, [P, P, C.c_uint64, C.POINTER(P)]),
+    "bg_guide_destroy": (None, [P]),
+    "bg_guide_set_params": (C.c_int, [P, P]),
+    "bg_guide_get_params": (C.c_int, [P, P, P, P]),
+    "bg_guide_set_bounds": (C.c_int, [P, P, P]),
+    "bg_guide_eval": (C.c_int, [P, P, C.c_uint64, P]),
+    "bg_guide_train": (C.c_int, [P, P, P, C.c_uint64, C.c_float, C.POINTER(C.c_double), P]),
+    "bg_render_film_guided": (C.c_int, [P, P, C.POINTER(bg_render_desc), P, C.POINTER(bg_guiding_desc), P,
+                                        C.POINTER(bg_stats)]),
+    "bg_guide_records": (C.c_int, [P, P, P, C.c_uint64, C.POINTER(C.c_uint64)]),
     "bgx_icosphere_tri_count": (C.c_uint64, [C.c_int]),
     "bgx_icosphere": (C.c_int, [C.c_int, P, C.c_float, C.c_float, C.c_float, C.c_uint64, P]),
     "bgx_rays_incoherent": (C.c_int, [C.c_uint64, P, P, C.c_uint64, P]),

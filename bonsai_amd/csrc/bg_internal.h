@@ -135,6 +135,25 @@ struct PathBuffers {
     float4* nee_c = nullptr;   // beta at the vertex rgb, light pdf
     float4* nee_d = nullptr;   // BSDF-branch pdf, light index, -, -
     unsigned long long* counters = nullptr;  // [queries, shadow queries]
+    // path guiding (allocated only for guided renders)
+    size_t guide_cap = 0, rec_cap = 0;
+    void* guide_mem = nullptr;
+    float4* gin = nullptr;     // 3 per path: network input (p, wo, n) at the current hit
+    float* gout = nullptr;     // 40 per path: network output
+    float4* rec_in = nullptr;  // 3 per training record: the input the direction was sampled with
+    float4* rec_tgt = nullptr; // per record: sampled direction, then (after the pass) its weight in .w
+    float4* rec_aux = nullptr; // 2 per record: throughput before scattering, radiance after the vertex
+    uint32_t* rec_blk = nullptr; // per 256 records: count of nonzero weights, then its exclusive prefix
+    uint64_t last_records = 0; // records of the last trained pass (their weights are final)
+};
+
+// Replay ring of training samples with nonzero weight, kept by the guide
+// across passes; state = [head, count, base of the last append, last append size].
+struct Replay {
+    float4* in = nullptr;   // 3 per sample
+    float4* tgt = nullptr;  // direction, weight
+    uint32_t* state = nullptr;
+    uint32_t cap = 0;
 };
 
 struct RenderParamsDev {
@@ -146,6 +165,11 @@ struct RenderParamsDev {
     float rr;
     uint32_t mode;
     uint64_t n_paths;
+    // path guiding: one-sample MIS between the network's mixture (probability guide_alpha) and
+    // the BSDF; every rec_stride-th path (up to rec_paths of them) records its vertices
+    uint32_t guided = 0;
+    float guide_alpha = 0.0f;
+    uint32_t rec_stride = 1, rec_paths = 0;
 };
 
 // path.hip
@@ -153,11 +177,63 @@ hipError_t launch_init_paths(const RenderParamsDev& p, const PathBuffers& b, hip
 hipError_t launch_shade(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b, uint32_t bounce,
                         hipStream_t s);
 hipError_t launch_nee_resolve(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b,
-                              hipStream_t s);
+                              uint32_t bounce, hipStream_t s);
+hipError_t launch_count_rays(const RenderParamsDev& p, const PathBuffers& b, unsigned long long* counters,
+                             hipStream_t s);
+hipError_t launch_guide_inputs(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b, hipStream_t s);
+hipError_t launch_guide_targets(const RenderParamsDev& p, const PathBuffers& b, hipStream_t s);
+// Appends the pass's nonzero-weight records to the ring (order-preserving,
+// deterministic: block counts, one-block exclusive scan, scatter).
+hipError_t launch_replay_append(const PathBuffers& b, uint64_t n_records, const Replay& r, hipStream_t s);
+// Minibatch k = 0..n-1 takes ring sample ((first + k) * 2654435761) mod count:
+// a stride-permutation of the ring (the prime exceeds any count).
+hipError_t launch_replay_gather(const Replay& r, uint64_t first, uint64_t n, float4* in, float4* tgt, hipStream_t s);
 hipError_t launch_film(const RenderParamsDev& p, const PathBuffers& b, uint32_t width, uint32_t height, float4* film,
                        double4* film64, hipStream_t s);
 
+// ---------------------------------------------------------------- guiding MLP
+constexpr int kParamsTotal = 4 * (64 * 64 + 64) + (40 * 64 + 40);  // 19,240
+
+struct GuideDevice {
+    uint16_t* wbf;   // BF16 weights [5][64 out][64 in], rows >= 40 of layer 5 zero
+    uint16_t* wbft;  // the same transposed [5][64 in][64 out]
+    float* bias;     // [5][64]
+    float* master;   // FP32 parameters in flat order (per layer W row-major, then b)
+    float* m;        // Adam moments
+    float* v;
+};
+struct GuideBox {
+    float lo[3], hi[3];
+};
+struct GuideScratch {
+    uint16_t* xs = nullptr;    // [5][64][cap] layer inputs, BF16
+    uint16_t* dys = nullptr;   // [5][64][cap] layer output gradients, BF16
+    float* loss = nullptr;     // [cap]
+    float* partial = nullptr;  // [chunks][5][64][64]
+    float* bpartial = nullptr; // [chunks][5][64]
+    uint64_t cap = 0, chunk = 2048;
+};
+hipError_t guide_refresh(const GuideDevice& g, hipStream_t s);
+hipError_t guide_infer(const GuideDevice& g, const GuideBox& box, const float4* in, uint64_t n, float* out, int grid,
+                       hipStream_t s);
+hipError_t guide_train_step(const GuideDevice& g, const GuideBox& box, const float4* in, const float4* target,
+                            uint64_t n, const GuideScratch& sc, float lr, int t, int grid, float* grad_out,
+                            hipStream_t s);
+
 }  // namespace bg
+
+struct bg_guide {
+    bg_ctx* ctx = nullptr;
+    bg::GuideDevice dev{};
+    bg::GuideScratch scratch;
+    bg::GuideBox box{};
+    void* mem = nullptr;  // one allocation holding parameters, Adam state and batch buffers
+    float4* batch_in = nullptr;   // minibatch buffers in `mem` (max_batch samples)
+    float4* batch_tgt = nullptr;
+    int step = 0;         // Adam steps taken (1-based inside the step)
+    bg::Replay replay;    // training samples gathered by guided renders
+    void* replay_mem = nullptr;
+};
 
 struct bg_film {
     bg_ctx* ctx = nullptr;

@@ -2,7 +2,8 @@
 //
 // Network: 64-dim encoding of (position, outgoing direction, normal) -> 4
 // hidden layers of 64 ReLU units -> 40 outputs = 8 von Mises-Fisher lobes x
-// (mixture logit, direction 3-vector, log-concentration). Weights are kept in
+// (mixture logit, direction 3-vector, concentration parameter s with log kappa
+// = 1.5 + 5.5 tanh((s - 1.5) / 5.5)). Weights are kept in
 // FP32 (master copy, Adam moments) and in BF16 for the matrix cores; every
 // product is v_mfma_f32_32x32x8_bf16 with FP32 accumulation.
 //
@@ -25,7 +26,9 @@ namespace {
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 
-constexpr int kIn = 64, kHid = 64, kOut = 40, kLayers = 5;
+constexpr int kOut = 40, kLayers = 5;
+// log kappa = C + R tanh((s - C) / R): kappa in (e^-4, e^7) with a gradient everywhere
+constexpr float kLogKappaC = 1.5f, kLogKappaR = 5.5f;
 constexpr float kPiF = 3.14159265358979323846f;
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
@@ -59,10 +62,14 @@ __device__ __forceinline__ float feature(int f, const float q[3], const float wo
     return 0.0f;
 }
 
-// Weights in LDS: BF16 [layer][64 rows][64 cols]; A operand of k-step s for
+// Weights in LDS: BF16 [layer][64 rows][kLd], rows padded from 64 to 68
+// elements (136 B = 34 banks) so the 32 rows one A-operand read touches spread
+// over the banks instead of all starting on bank 0. A operand of k-step s for
 // output block mb: row 32*mb + lane%32, columns 8s + 4*(lane/32) .. +3.
+constexpr int kLd = 68;
+constexpr int kLayerLds = 64 * kLd;
 __device__ __forceinline__ s4 load_a(const uint16_t* __restrict__ w, int mb, int s, int lane) {
-    const uint16_t* p = w + (32 * mb + (lane & 31)) * 64 + 8 * s + 4 * (lane >> 5);
+    const uint16_t* p = w + (32 * mb + (lane & 31)) * kLd + 8 * s + 4 * (lane >> 5);
     return *reinterpret_cast<const s4*>(p);
 }
 
@@ -115,10 +122,14 @@ __device__ __forceinline__ void normalise_pos(const Sample& s, const GuideBox& b
     for (int k = 0; k < 3; ++k) q[k] = (s.p[k] - box.lo[k]) / (box.hi[k] - box.lo[k]);
 }
 
-__device__ void load_weights_lds(const uint16_t* __restrict__ wg, uint16_t* __restrict__ lds, int n_u16) {
-    const uint4* src = reinterpret_cast<const uint4*>(wg);
-    uint4* dst = reinterpret_cast<uint4*>(lds);
-    for (int i = threadIdx.x; i < n_u16 / 8; i += blockDim.x) dst[i] = src[i];
+// Copies the 5 BF16 weight matrices (dense rows of 64) into padded LDS rows,
+// 8 bytes per thread and step.
+__device__ void load_weights_lds(const uint16_t* __restrict__ wg, uint16_t* __restrict__ lds) {
+    const uint2* src = reinterpret_cast<const uint2*>(wg);
+    for (int i = threadIdx.x; i < kLayers * 64 * 16; i += blockDim.x) {
+        const int row = i / 16, q = i % 16;  // row over all layers, 4-element quad in the row
+        *reinterpret_cast<uint2*>(lds + row * kLd + 4 * q) = src[i];
+    }
     __syncthreads();
 }
 
@@ -126,13 +137,14 @@ __device__ void load_weights_lds(const uint16_t* __restrict__ wg, uint16_t* __re
 // in: 3 float4 per sample (p, wo, n); out: 40 floats per sample.
 __global__ __launch_bounds__(256) void k_guide_infer(GuideDevice g, GuideBox box, const float4* __restrict__ in,
                                                      uint64_t n, float* __restrict__ out) {
-    __shared__ uint16_t wl[kLayers * 64 * 64];
+    __shared__ uint16_t wl[kLayers * kLayerLds];
     __shared__ float tile[4][32][65];
-    load_weights_lds(g.wbf, wl, kLayers * 64 * 64);
+    load_weights_lds(g.wbf, wl);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t groups = (n + 31) / 32;
-    for (uint64_t grp = (uint64_t)blockIdx.x * 4 + wave; grp < groups; grp += (uint64_t)gridDim.x * 4) {
-        const uint64_t i = grp * 32 + (lane & 31);
+    // every wave of the block runs the same number of rounds (the rounds hold block barriers)
+    for (uint64_t base = (uint64_t)blockIdx.x * 4; base < groups; base += (uint64_t)gridDim.x * 4) {
+        const uint64_t i = (base + wave) * 32 + (lane & 31);
         const bool valid = i < n;
         Sample smp = valid ? load_sample(in, i) : Sample{{0, 0, 0}, {0, 0, 1}, {0, 0, 1}};
         float q[3];
@@ -141,7 +153,7 @@ __global__ __launch_bounds__(256) void k_guide_infer(GuideDevice g, GuideBox box
         encode_b(q, smp.wo, smp.n, lane, xb);
         f16v acc[2];
         for (int l = 0; l < kLayers; ++l) {
-            layer_mm(wl + l * 4096, xb, lane, acc);
+            layer_mm(wl + l * kLayerLds, xb, lane, acc);
             epilogue(acc, g.bias + l * 64, l < kLayers - 1, lane, xb);
         }
         for (int mb = 0; mb < 2; ++mb)
@@ -155,7 +167,7 @@ __global__ __launch_bounds__(256) void k_guide_infer(GuideDevice g, GuideBox box
 
 // ------------------------------------------------------------- mixture math
 struct Lobe8 {
-    float pi[8], mu[8][3], kappa[8], nm[8], s[8];
+    float pi[8], mu[8][3], kappa[8], nm[8], s[8];  // s: tanh((s_raw - C) / R)
 };
 __device__ __forceinline__ void mixture_of(const float* o, Lobe8& L) {
     float amax = -3.0e38f;
@@ -168,8 +180,8 @@ __device__ __forceinline__ void mixture_of(const float* o, Lobe8& L) {
         const float nm = sqrtf(mx * mx + my * my + mz * mz + 1e-12f);
         L.nm[j] = nm;
         L.mu[j][0] = mx / nm; L.mu[j][1] = my / nm; L.mu[j][2] = mz / nm;
-        L.s[j] = o[5 * j + 4];
-        L.kappa[j] = expf(fminf(fmaxf(L.s[j], -4.0f), 7.0f));
+        L.s[j] = tanhf((o[5 * j + 4] - kLogKappaC) / kLogKappaR);  // kept for the chain rule
+        L.kappa[j] = expf(kLogKappaC + kLogKappaR * L.s[j]);
     }
 }
 __device__ __forceinline__ float vmf(const float* mu, float kappa, const float* w) {
@@ -179,10 +191,16 @@ __device__ __forceinline__ float vmf(const float* mu, float kappa, const float* 
 
 // d(-weight * log p(w)) / d out, scaled by inv_batch (oracle loss_grad_out).
 __device__ void out_grad(const float* o, const float* w, float weight, float inv_batch, float* g, float* loss) {
+    if (weight == 0.0f) {  // empty record: no loss, no gradient (even where the mixture underflows)
+        for (int k = 0; k < kOut; ++k) g[k] = 0.0f;
+        *loss = 0.0f;
+        return;
+    }
     Lobe8 L;
     mixture_of(o, L);
     float f[8], p = 0.0f;
     for (int j = 0; j < 8; ++j) { f[j] = vmf(L.mu[j], L.kappa[j], w); p += L.pi[j] * f[j]; }
+    p = fmaxf(p, 1e-30f);  // a direction where every lobe underflows in FP32
     *loss = -weight * logf(p);
     const float sc = -weight * inv_batch;
     for (int j = 0; j < 8; ++j) {
@@ -191,8 +209,7 @@ __device__ void out_grad(const float* o, const float* w, float weight, float inv
         const float k = L.kappa[j];
         const float muw = L.mu[j][0] * w[0] + L.mu[j][1] * w[1] + L.mu[j][2] * w[2];
         const float dlogk = 1.0f / k - 2.0f * expf(-2.0f * k) / (-expm1f(-2.0f * k)) + (muw - 1.0f);
-        const bool inside = L.s[j] > -4.0f && L.s[j] < 7.0f;
-        g[5 * j + 4] = inside ? sc * r * dlogk * k : 0.0f;
+        g[5 * j + 4] = sc * r * dlogk * k * (1.0f - L.s[j] * L.s[j]);
         for (int c = 0; c < 3; ++c) {
             const float proj = k * w[c] - k * muw * L.mu[j][c];
             g[5 * j + 1 + c] = sc * r * proj / L.nm[j];
@@ -208,15 +225,16 @@ __global__ __launch_bounds__(256) void k_guide_train(GuideDevice g, GuideBox box
                                                      const float4* __restrict__ target, uint64_t n, float inv_batch,
                                                      uint16_t* __restrict__ xs, uint16_t* __restrict__ dys,
                                                      float* __restrict__ loss_out) {
-    __shared__ uint16_t wl[kLayers * 64 * 64];
-    __shared__ uint16_t wt[kLayers * 64 * 64];
+    __shared__ uint16_t wl[kLayers * kLayerLds];
+    __shared__ uint16_t wt[kLayers * kLayerLds];
     __shared__ float tile[4][32][65];
-    load_weights_lds(g.wbf, wl, kLayers * 64 * 64);
-    load_weights_lds(g.wbft, wt, kLayers * 64 * 64);
+    load_weights_lds(g.wbf, wl);
+    load_weights_lds(g.wbft, wt);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t groups = (n + 31) / 32;
-    for (uint64_t grp = (uint64_t)blockIdx.x * 4 + wave; grp < groups; grp += (uint64_t)gridDim.x * 4) {
-        const uint64_t i = grp * 32 + (lane & 31);
+    // every wave of the block runs the same number of rounds (the rounds hold block barriers)
+    for (uint64_t base = (uint64_t)blockIdx.x * 4; base < groups; base += (uint64_t)gridDim.x * 4) {
+        const uint64_t i = (base + wave) * 32 + (lane & 31);
         const bool valid = i < n;
         Sample smp = valid ? load_sample(in, i) : Sample{{0, 0, 0}, {0, 0, 1}, {0, 0, 1}};
         float q[3];
@@ -231,7 +249,7 @@ __global__ __launch_bounds__(256) void k_guide_train(GuideDevice g, GuideBox box
                     const int f = 8 * s + 4 * (lane >> 5);
                     for (int j = 0; j < 4; ++j) xs[((uint64_t)l * 64 + f + j) * n + i] = (uint16_t)xb[s][j];
                 }
-            layer_mm(wl + l * 4096, xb, lane, acc);
+            layer_mm(wl + l * kLayerLds, xb, lane, acc);
             epilogue(acc, g.bias + l * 64, l < kLayers - 1, lane, xb);
         }
         // gather the 40 outputs of each sample, form d loss / d out
@@ -263,7 +281,7 @@ __global__ __launch_bounds__(256) void k_guide_train(GuideDevice g, GuideBox box
                     for (int j = 0; j < 4; ++j) dys[((uint64_t)l * 64 + f + j) * n + i] = (uint16_t)db[s][j];
                 }
             if (l == 0) break;
-            layer_mm(wt + l * 4096, db, lane, acc);
+            layer_mm(wt + l * kLayerLds, db, lane, acc);
             // mask by ReLU'(pre-activation of layer l-1) = (X_l > 0)
             for (int mb = 0; mb < 2; ++mb)
                 for (int r = 0; r < 16; ++r) {

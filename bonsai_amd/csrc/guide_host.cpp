@@ -1,0 +1,184 @@
+// Host side of the path-guiding MLP: parameter upload, batched evaluation and
+// one Adam step per call (kernels in guide.hip). The reference crate has no
+// path guiding; these entry points have no counterpart there.
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "bg_internal.h"
+
+namespace {
+
+int fail(bg_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                      \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail((ctx), BG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct GuideHostBufs {
+    float4* in;
+    float4* target;
+    float* out;
+    float* grad;
+};
+
+// Device views of the guide's single allocation.
+GuideHostBufs carve(bg_guide* g, size_t* total_out) {
+    const uint64_t cap = g->scratch.cap;
+    const uint64_t chunks = (cap + g->scratch.chunk - 1) / g->scratch.chunk;
+    char* p = (char*)g->mem;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        void* r = p ? p + off : nullptr;
+        off += ((bytes + 255) / 256) * 256;
+        return r;
+    };
+    g->dev.wbf = (uint16_t*)take(5 * 4096 * 2);
+    g->dev.wbft = (uint16_t*)take(5 * 4096 * 2);
+    g->dev.bias = (float*)take(5 * 64 * 4);
+    g->dev.master = (float*)take(bg::kParamsTotal * 4);
+    g->dev.m = (float*)take(bg::kParamsTotal * 4);
+    g->dev.v = (float*)take(bg::kParamsTotal * 4);
+    GuideHostBufs b;
+    b.grad = (float*)take(bg::kParamsTotal * 4);
+    b.in = (float4*)take(48 * cap);
+    b.target = (float4*)take(16 * cap);
+    b.out = (float*)take(4 * BG_GUIDE_OUTPUTS * cap);
+    g->scratch.xs = (uint16_t*)take(5 * 64 * 2 * cap);
+    g->scratch.dys = (uint16_t*)take(5 * 64 * 2 * cap);
+    g->scratch.loss = (float*)take(4 * cap);
+    g->scratch.partial = (float*)take(chunks * 5 * 64 * 64 * 4);
+    g->scratch.bpartial = (float*)take(chunks * 5 * 64 * 4);
+    if (total_out) *total_out = off;
+    return b;
+}
+
+int grid_of(const bg_guide* g) { return 2 * (g->ctx->num_cus > 0 ? g->ctx->num_cus : 1); }
+
+}  // namespace
+
+extern "C" {
+
+int bg_guide_create(bg_ctx* ctx, const float* params, uint64_t max_batch, bg_guide** out) {
+    if (!ctx || !params || !out) return fail(ctx, BG_ERR_INVALID, "bg_guide_create: null argument");
+    *out = nullptr;
+    if (max_batch == 0 || max_batch > (1ull << 26))
+        return fail(ctx, BG_ERR_INVALID, "bg_guide_create: max_batch must be in [1, 2^26]");
+    bg_guide* g = new (std::nothrow) bg_guide;
+    if (!g) return fail(ctx, BG_ERR_NOMEM, "bg_guide_create: out of host memory");
+    g->ctx = ctx;
+    g->scratch.cap = max_batch;
+    size_t total = 0;
+    carve(g, &total);
+    hipError_t e = hipMalloc(&g->mem, total);
+    if (e != hipSuccess) {
+        delete g;
+        return fail(ctx, BG_ERR_NOMEM, std::string("bg_guide_create: hipMalloc: ") + hipGetErrorString(e));
+    }
+    GuideHostBufs hb = carve(g, nullptr);
+    g->batch_in = hb.in;
+    g->batch_tgt = hb.target;
+    int rc = bg_guide_set_params(g, params);
+    if (rc != BG_OK) {
+        bg_guide_destroy(g);
+        return rc;
+    }
+    const float lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+    bg_guide_set_bounds(g, lo, hi);
+    *out = g;
+    return BG_OK;
+}
+
+void bg_guide_destroy(bg_guide* g) {
+    if (!g) return;
+    if (g->mem) hipFree(g->mem);
+    if (g->replay_mem) hipFree(g->replay_mem);
+    delete g;
+}
+
+int bg_guide_set_params(bg_guide* g, const float* params) {
+    if (!g || !params) return BG_ERR_INVALID;
+    bg_ctx* ctx = g->ctx;
+    hipStream_t s = ctx->stream;
+    HIP_TRY(ctx, hipMemcpyAsync(g->dev.master, params, bg::kParamsTotal * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipMemsetAsync(g->dev.m, 0, bg::kParamsTotal * 4, s));
+    HIP_TRY(ctx, hipMemsetAsync(g->dev.v, 0, bg::kParamsTotal * 4, s));
+    HIP_TRY(ctx, bg::guide_refresh(g->dev, s));
+    if (g->replay.state) HIP_TRY(ctx, hipMemsetAsync(g->replay.state, 0, 16, s));  // training starts afresh
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    g->step = 0;
+    return BG_OK;
+}
+
+int bg_guide_get_params(bg_guide* g, float* params, float* adam_m, float* adam_v) {
+    if (!g) return BG_ERR_INVALID;
+    bg_ctx* ctx = g->ctx;
+    hipStream_t s = ctx->stream;
+    if (params) HIP_TRY(ctx, hipMemcpyAsync(params, g->dev.master, bg::kParamsTotal * 4, hipMemcpyDeviceToHost, s));
+    if (adam_m) HIP_TRY(ctx, hipMemcpyAsync(adam_m, g->dev.m, bg::kParamsTotal * 4, hipMemcpyDeviceToHost, s));
+    if (adam_v) HIP_TRY(ctx, hipMemcpyAsync(adam_v, g->dev.v, bg::kParamsTotal * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    return BG_OK;
+}
+
+int bg_guide_set_bounds(bg_guide* g, const float lo[3], const float hi[3]) {
+    if (!g || !lo || !hi) return BG_ERR_INVALID;
+    for (int k = 0; k < 3; ++k)
+        if (!(hi[k] > lo[k])) return fail(g->ctx, BG_ERR_INVALID, "bg_guide_set_bounds: empty box");
+    for (int k = 0; k < 3; ++k) {
+        g->box.lo[k] = lo[k];
+        g->box.hi[k] = hi[k];
+    }
+    return BG_OK;
+}
+
+int bg_guide_eval(bg_guide* g, const float* samples, uint64_t n, float* out) {
+    if (!g || (n && (!samples || !out))) return BG_ERR_INVALID;
+    bg_ctx* ctx = g->ctx;
+    if (n > g->scratch.cap) return fail(ctx, BG_ERR_INVALID, "bg_guide_eval: n exceeds max_batch");
+    if (n == 0) return BG_OK;
+    GuideHostBufs b = carve(g, nullptr);
+    hipStream_t s = ctx->stream;
+    HIP_TRY(ctx, hipMemcpyAsync(b.in, samples, 48 * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, bg::guide_infer(g->dev, g->box, b.in, n, b.out, grid_of(g), s));
+    HIP_TRY(ctx, hipMemcpyAsync(out, b.out, 4 * BG_GUIDE_OUTPUTS * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    return BG_OK;
+}
+
+int bg_guide_train(bg_guide* g, const float* samples, const float* targets, uint64_t n, float lr,
+                   double* mean_loss, float* grad) {
+    if (!g || (n && (!samples || !targets))) return BG_ERR_INVALID;
+    bg_ctx* ctx = g->ctx;
+    if (n > g->scratch.cap) return fail(ctx, BG_ERR_INVALID, "bg_guide_train: n exceeds max_batch");
+    if (n == 0) return BG_OK;
+    GuideHostBufs b = carve(g, nullptr);
+    hipStream_t s = ctx->stream;
+    HIP_TRY(ctx, hipMemcpyAsync(b.in, samples, 48 * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipMemcpyAsync(b.target, targets, 16 * n, hipMemcpyHostToDevice, s));
+    ++g->step;
+    HIP_TRY(ctx, bg::guide_train_step(g->dev, g->box, b.in, b.target, n, g->scratch, lr, g->step, grid_of(g),
+                                      grad ? b.grad : nullptr, s));
+    std::vector<float> loss;
+    if (mean_loss) {
+        loss.resize(n);
+        HIP_TRY(ctx, hipMemcpyAsync(loss.data(), g->scratch.loss, 4 * n, hipMemcpyDeviceToHost, s));
+    }
+    if (grad) HIP_TRY(ctx, hipMemcpyAsync(grad, b.grad, bg::kParamsTotal * 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    if (mean_loss) {
+        double sum = 0.0;
+        for (float v : loss) sum += v;
+        *mean_loss = sum / (double)n;
+    }
+    return BG_OK;
+}
+
+}  // extern "C"

@@ -14,6 +14,7 @@
 
 #include "bg_internal.h"
 #include "dev_camera.h"
+#include "dev_guide.h"
 #include "dev_shade.h"
 
 namespace bg {
@@ -26,6 +27,16 @@ constexpr uint32_t kAlive = 1u, kSpecFlag = 2u;
 constexpr uint32_t kHasNee = 1u, kHasLight = 2u, kHasBsdf = 4u;
 
 __device__ __forceinline__ uint64_t key_of(uint2 k) { return (uint64_t)k.x | ((uint64_t)k.y << 32); }
+
+// Training-record slot of path i at this bounce: every rec_stride-th path up to
+// rec_paths of them records one slot per bounce < max_depth, path-major.
+__device__ __forceinline__ bool rec_slot(const RenderParamsDev& p, uint64_t i, uint32_t bounce, uint64_t* r) {
+    if (!p.guided || p.rec_paths == 0 || bounce >= p.max_depth || i % p.rec_stride != 0) return false;
+    const uint64_t k = i / p.rec_stride;
+    if (k >= p.rec_paths) return false;
+    *r = k * p.max_depth + bounce;
+    return true;
+}
 
 __device__ __forceinline__ void write_ray(float4* rays, uint64_t i, V o, float t_max, V d, float time) {
     rays[2 * i] = make_float4(o.x, o.y, o.z, t_max);
@@ -93,7 +104,7 @@ __global__ __launch_bounds__(256) void k_init_paths(RenderParamsDev p, PathBuffe
     const CamRay r = generate_ray(p.cam, fx, fy, tu);
     b.ray[2 * i] = make_float4(r.ox, r.oy, r.oz, r.t_max);
     b.ray[2 * i + 1] = make_float4(r.dx, r.dy, r.dz, r.time);
-    b.beta[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    b.beta[i] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(1u));  // .w: rays issued (camera ray)
     b.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     b.key[i] = make_uint2((uint32_t)key, (uint32_t)(key >> 32));
     b.state[i] = make_uint2(dim, kAlive);
@@ -108,6 +119,9 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
     b.nee_a[i].w = 0.0f;
     kill_ray(b.sray, i);
     kill_ray(b.mray, i);
+    uint64_t rec = 0;
+    const bool recording = rec_slot(p, i, bounce, &rec);
+    if (recording) b.rec_tgt[rec] = make_float4(0.0f, 0.0f, 1.0f, 0.0f);  // .w = 0: no sample at this slot
     if (!(flags & kAlive)) return;
     const bool ref = p.mode == 0;
     const uint4 h = b.hit[i];
@@ -136,6 +150,8 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
     const uint64_t key = key_of(b.key[i]);
     uint32_t dim = st.x;
     const float4 b4 = b.beta[i];
+    // rays this path has issued: low 16 bits all, high 16 bits any-hit (shadow) queries
+    uint32_t nrays = __float_as_uint(b4.w);
     C3 beta = C3{b4.x, b4.y, b4.z};
     const float4 l4 = b.L[i];
     C3 L = C3{l4.x, l4.y, l4.z};
@@ -222,6 +238,7 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
                     const float w = power_heuristic(lpdf, scattering_pdf);
                     term = f * lrad * w / lpdf;
                     nee |= kHasLight;
+                    nrays += 0x10001u;
                 }
             }
             // BSDF-sampling branch (integrator.rs:379-434)
@@ -240,19 +257,60 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
                     b.nee_b[i] = make_float4(fb.r, fb.g, fb.b, w);
                     b.nee_d[i] = make_float4(spdf, __uint_as_float(idx), 0.0f, 0.0f);
                     nee |= kHasBsdf;
+                    nrays += 1u;
                 }
             }
             b.nee_a[i] = make_float4(term.r, term.g, term.b, __uint_as_float(nee));
             b.nee_c[i] = make_float4(beta.r, beta.g, beta.b, light_pdf);
         }
-        // path.rs:111-142
-        const float u0 = draw(key, dim), u1 = draw(key, dim);
-        const BxS bsc = bs.sample(-rd, u0, u1, kAll);
-        if (black(bsc.f) || bsc.pdf == 0.0f) {
+        // path.rs:111-142; guided: one-sample MIS of the network's mixture and the BSDF
+        V wi;
+        C3 fs;
+        float pdf;
+        uint32_t stype = 0;
+        if (p.guided) {
+            // the BSDF draws come first, so with guide_alpha == 0 the streams match the plain integrator's
+            const float u0 = draw(key, dim), u1 = draw(key, dim);
+            const float uc = p.guide_alpha > 0.0f ? draw(key, dim) : 1.0f;
+            const float ul = p.guide_alpha > 0.0f ? draw(key, dim) : 0.0f;
+            GuideMix gm;
+            const bool gok = guide_mix(b.gout + 40 * i, &gm);
+            const float a = (gok && !(bs.lobe.type & kSpecular)) ? p.guide_alpha : 0.0f;
+            float pb;
+            if (uc < a) {
+                wi = guide_sample(gm, ul, u0, u1);
+                fs = bs.f(-rd, wi, kAll);
+                pb = bs.pdf(-rd, wi, kAll);
+            } else {
+                const BxS bsc = bs.sample(-rd, u0, u1, kAll);
+                wi = bsc.wi;
+                fs = bsc.f;
+                pb = bsc.pdf;
+                stype = bsc.type;
+            }
+            pdf = (1.0f - a) * pb + (a > 0.0f ? a * guide_pdf(gm, wi) : 0.0f);
+            if (recording && !black(fs) && pdf > 0.0f) {
+                const float4* gi = b.gin + 3 * i;
+                b.rec_in[3 * rec] = gi[0];
+                b.rec_in[3 * rec + 1] = gi[1];
+                b.rec_in[3 * rec + 2] = gi[2];
+                b.rec_tgt[rec] = make_float4(wi.x, wi.y, wi.z, 1.0f);
+                b.rec_aux[2 * rec] = make_float4(beta.r, beta.g, beta.b, 0.0f);
+                b.rec_aux[2 * rec + 1] = make_float4(L.r, L.g, L.b, 0.0f);
+            }
+        } else {
+            const float u0 = draw(key, dim), u1 = draw(key, dim);
+            const BxS bsc = bs.sample(-rd, u0, u1, kAll);
+            wi = bsc.wi;
+            fs = bsc.f;
+            pdf = bsc.pdf;
+            stype = bsc.type;
+        }
+        if (black(fs) || !(pdf > 0.0f)) {
             alive = false;
         } else {
-            beta = beta * (bsc.f * __builtin_fabsf(dot(bsc.wi, bs.ns)) / bsc.pdf);
-            flags = (bsc.type & kSpecular) ? (flags | kSpecFlag) : (flags & ~kSpecFlag);
+            beta = beta * (fs * __builtin_fabsf(dot(wi, bs.ns)) / pdf);
+            flags = (stype & kSpecular) ? (flags | kSpecFlag) : (flags & ~kSpecFlag);
             if (cmax(beta) < p.rr && bounce > 3) {
                 const float q = rmax(1.0f - cmax(beta), 0.5f);
                 if (draw(key, dim) < q) alive = false;
@@ -266,19 +324,21 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
                 } else {
                     V o;
                     float tm;
-                    spawn(s, bsc.wi, false, &o, &tm);
-                    write_ray(b.ray, i, o, tm, bsc.wi, s.time);
+                    spawn(s, wi, false, &o, &tm);
+                    write_ray(b.ray, i, o, tm, wi, s.time);
                 }
             }
         }
     }
     if (!alive) kill_ray(b.ray, i);
-    b.beta[i] = make_float4(beta.r, beta.g, beta.b, 0.0f);
+    else nrays += 1u;
+    b.beta[i] = make_float4(beta.r, beta.g, beta.b, __uint_as_float(nrays));
     b.L[i] = make_float4(L.r, L.g, L.b, 0.0f);
     b.state[i] = make_uint2(dim, alive ? (flags | kAlive) : 0u);
 }
 
-__global__ __launch_bounds__(256) void k_nee_resolve(RenderParamsDev p, ShadeSceneDev sc, PathBuffers b) {
+__global__ __launch_bounds__(256) void k_nee_resolve(RenderParamsDev p, ShadeSceneDev sc, PathBuffers b,
+                                                     uint32_t bounce) {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= p.n_paths) return;
     const float4 a = b.nee_a[i];
@@ -311,6 +371,87 @@ __global__ __launch_bounds__(256) void k_nee_resolve(RenderParamsDev p, ShadeSce
     const float4 l4 = b.L[i];
     const C3 L = C3{l4.x, l4.y, l4.z} + C3{c.x, c.y, c.z} * ld;
     b.L[i] = make_float4(L.r, L.g, L.b, 0.0f);
+    uint64_t rec;
+    if (rec_slot(p, i, bounce, &rec) && b.rec_tgt[rec].w != 0.0f) b.rec_aux[2 * rec + 1] = make_float4(L.r, L.g, L.b, 0.0f);
+}
+
+// Network input at each live path's hit: position, outgoing direction and the
+// geometric normal turned toward it.
+__global__ __launch_bounds__(256) void k_guide_inputs(RenderParamsDev p, ShadeSceneDev sc, PathBuffers b) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= p.n_paths) return;
+    float4 gp = make_float4(0.0f, 0.0f, 0.0f, 0.0f), gw = make_float4(0.0f, 0.0f, 1.0f, 0.0f), gn = gw;
+    const uint4 h = b.hit[i];
+    if ((b.state[i].y & kAlive) && h.x != 0xFFFFFFFFu) {
+        const float4 r0 = b.ray[2 * i], r1 = b.ray[2 * i + 1];
+        const float t = __uint_as_float(h.y);
+        const V d = mkv(r1.x, r1.y, r1.z);
+        const V hp = mkv(r0.x, r0.y, r0.z) + d * t;
+        V n;
+        if (h.x < sc.n_tris) {
+            V p0, p1, p2;
+            tri_verts(sc.tri_orig + 9 * (uint64_t)h.x, &p0, &p1, &p2);
+            n = cross(p1 - p0, p2 - p0);
+        } else {
+            const float* m = sc.spheres[h.x - sc.n_tris].o2w;
+            n = hp - mkv(m[3], m[7], m[11]);
+        }
+        const float l2 = len2(n);
+        n = l2 > 0.0f ? n * (1.0f / __builtin_sqrtf(l2)) : mkv(0.0f, 0.0f, 1.0f);
+        if (dot(n, d) > 0.0f) n = -n;
+        gp = make_float4(hp.x, hp.y, hp.z, 0.0f);
+        gw = make_float4(-d.x, -d.y, -d.z, 0.0f);
+        gn = make_float4(n.x, n.y, n.z, 0.0f);
+    }
+    b.gin[3 * i] = gp;
+    b.gin[3 * i + 1] = gw;
+    b.gin[3 * i + 2] = gn;
+}
+
+// After a pass: the training weight of each recorded direction,
+// luminance((L_final - L_after_vertex) / beta_before_scattering)
+// = luminance(f cos L_i / pdf) of the sampled direction (0 for empty slots).
+__global__ __launch_bounds__(256) void k_guide_targets(RenderParamsDev p, PathBuffers b) {
+    const uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= (uint64_t)p.rec_paths * p.max_depth) return;
+    float4 t = b.rec_tgt[r];
+    float w = 0.0f;
+    if (t.w != 0.0f) {
+        const uint64_t i = (r / p.max_depth) * p.rec_stride;
+        const float4 lf = b.L[i], be = b.rec_aux[2 * r], ls = b.rec_aux[2 * r + 1];
+        const float cr = be.x > 0.0f ? fmaxf(lf.x - ls.x, 0.0f) / be.x : 0.0f;
+        const float cg = be.y > 0.0f ? fmaxf(lf.y - ls.y, 0.0f) / be.y : 0.0f;
+        const float cb = be.z > 0.0f ? fmaxf(lf.z - ls.z, 0.0f) / be.z : 0.0f;
+        w = luminance(C3{cr, cg, cb});
+        if (!__builtin_isfinite(w)) w = 0.0f;
+    } else {
+        b.rec_in[3 * r] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        b.rec_in[3 * r + 1] = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+        b.rec_in[3 * r + 2] = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+    }
+    t.w = w;
+    b.rec_tgt[r] = t;
+}
+
+// Sums the per-path ray counts of a pass into counters[0] (all queries) and
+// counters[1] (any-hit queries): one atomic per block.
+__global__ __launch_bounds__(256) void k_count_rays(uint64_t n, const float4* __restrict__ beta,
+                                                    unsigned long long* __restrict__ counters) {
+    __shared__ unsigned long long part[2][4];
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint32_t c = i < n ? __float_as_uint(beta[i].w) : 0u;
+    unsigned long long all = c & 0xFFFFu, shadow = c >> 16;
+    for (int off = 32; off > 0; off >>= 1) {
+        all += __shfl_down(all, off);
+        shadow += __shfl_down(shadow, off);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) { part[0][wave] = all; part[1][wave] = shadow; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(&counters[0], part[0][0] + part[0][1] + part[0][2] + part[0][3]);
+        atomicAdd(&counters[1], part[1][0] + part[1][1] + part[1][2] + part[1][3]);
+    }
 }
 
 // film.rs:232-284 with a radius-0.5 box filter, gathered per pixel: a sample at
@@ -370,9 +511,28 @@ hipError_t launch_shade(const RenderParamsDev& p, const ShadeSceneDev& sc, const
     hipLaunchKernelGGL(k_shade, dim3((uint32_t)((p.n_paths + 127) / 128)), dim3(128), 0, s, p, sc, b, bounce);
     return hipGetLastError();
 }
-hipError_t launch_nee_resolve(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b, hipStream_t s) {
+hipError_t launch_nee_resolve(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b,
+                              uint32_t bounce, hipStream_t s) {
     if (p.n_paths == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_nee_resolve, dim3((uint32_t)((p.n_paths + 255) / 256)), dim3(256), 0, s, p, sc, b);
+    hipLaunchKernelGGL(k_nee_resolve, dim3((uint32_t)((p.n_paths + 255) / 256)), dim3(256), 0, s, p, sc, b, bounce);
+    return hipGetLastError();
+}
+hipError_t launch_count_rays(const RenderParamsDev& p, const PathBuffers& b, unsigned long long* counters,
+                             hipStream_t s) {
+    if (p.n_paths == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_count_rays, dim3((uint32_t)((p.n_paths + 255) / 256)), dim3(256), 0, s, p.n_paths, b.beta,
+                       counters);
+    return hipGetLastError();
+}
+hipError_t launch_guide_inputs(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b, hipStream_t s) {
+    if (p.n_paths == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_guide_inputs, dim3((uint32_t)((p.n_paths + 255) / 256)), dim3(256), 0, s, p, sc, b);
+    return hipGetLastError();
+}
+hipError_t launch_guide_targets(const RenderParamsDev& p, const PathBuffers& b, hipStream_t s) {
+    const uint64_t n = (uint64_t)p.rec_paths * p.max_depth;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_guide_targets, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, p, b);
     return hipGetLastError();
 }
 hipError_t launch_film(const RenderParamsDev& p, const PathBuffers& b, uint32_t W, uint32_t H, float4* film,

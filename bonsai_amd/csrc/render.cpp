@@ -55,6 +55,7 @@ int ensure_paths(bg_ctx* ctx, size_t cap) {
     b.nee_b = (float4*)take(16 * cap);
     b.nee_c = (float4*)take(16 * cap);
     b.nee_d = (float4*)take(16 * cap);
+    b.counters = (unsigned long long*)take(256);  // the 256 spare bytes of `total`
     b.cap = cap;
     return BG_OK;
 }
@@ -66,6 +67,54 @@ int ensure_spill(bg_ctx* ctx, size_t need) {
     ctx->spill_bytes = 0;
     HIP_TRY(ctx, hipMalloc((void**)&ctx->spill, need));
     ctx->spill_bytes = need;
+    return BG_OK;
+}
+
+// Guiding buffers: network input/output per path and training records.
+int ensure_guide_bufs(bg_ctx* ctx, size_t cap, size_t rec_cap) {
+    bg::PathBuffers& b = ctx->paths;
+    if (b.guide_cap >= cap && b.rec_cap >= rec_cap) return BG_OK;
+    if (b.guide_mem) hipFree(b.guide_mem);
+    b.guide_mem = nullptr;
+    b.guide_cap = b.rec_cap = 0;
+    cap = std::max(cap, (size_t)1);
+    rec_cap = std::max(rec_cap, (size_t)1);
+    const size_t sizes[] = {48 * cap, 160 * cap, 48 * rec_cap, 16 * rec_cap, 32 * rec_cap, 4 * (rec_cap / 256 + 1)};
+    size_t total = 0;
+    for (size_t z : sizes) total += ((z + 255) / 256) * 256;
+    HIP_TRY(ctx, hipMalloc(&b.guide_mem, total));
+    char* q = (char*)b.guide_mem;
+    void* parts[6];
+    for (int k = 0; k < 6; ++k) {
+        parts[k] = q;
+        q += ((sizes[k] + 255) / 256) * 256;
+    }
+    b.gin = (float4*)parts[0];
+    b.gout = (float*)parts[1];
+    b.rec_in = (float4*)parts[2];
+    b.rec_tgt = (float4*)parts[3];
+    b.rec_aux = (float4*)parts[4];
+    b.rec_blk = (uint32_t*)parts[5];
+    b.guide_cap = cap;
+    b.rec_cap = rec_cap;
+    return BG_OK;
+}
+
+// The guide's replay ring (allocated at its first trained render).
+int ensure_replay(bg_guide* g, uint32_t cap) {
+    bg_ctx* ctx = g->ctx;
+    if (g->replay.cap == cap) return BG_OK;
+    if (g->replay_mem) hipFree(g->replay_mem);
+    g->replay_mem = nullptr;
+    g->replay = bg::Replay{};
+    const size_t bytes = 256 + ((48ull * cap + 255) / 256) * 256 + 16ull * cap;
+    HIP_TRY(ctx, hipMalloc(&g->replay_mem, bytes));
+    char* q = (char*)g->replay_mem;
+    g->replay.state = (uint32_t*)q;
+    g->replay.in = (float4*)(q + 256);
+    g->replay.tgt = (float4*)(q + 256 + ((48ull * cap + 255) / 256) * 256);
+    g->replay.cap = cap;
+    HIP_TRY(ctx, hipMemsetAsync(g->replay.state, 0, 16, ctx->stream));
     return BG_OK;
 }
 
@@ -137,7 +186,12 @@ int bg_film_read(bg_film* f, float* rgbw, double* rgbw64) {
     return BG_OK;
 }
 
-int bg_render_film(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film* film, bg_stats* stats) {
+}  // extern "C"
+
+namespace {
+
+int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film* film, bg_stats* stats,
+                bg_guide* guide, const bg_guiding_desc* gd) {
     if (!ctx || !s || !d || !film || s->ctx != ctx || film->ctx != ctx) return fail(ctx, BG_ERR_INVALID, "null or foreign argument");
     const bg_camera_desc& cam = d->camera;
     if (cam.width != film->width || cam.height != film->height) return fail(ctx, BG_ERR_INVALID, "film size differs from camera");
@@ -170,6 +224,36 @@ int bg_render_film(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_f
     p.max_depth = d->max_depth;
     p.rr = d->rr_threshold;
     p.mode = d->mode;
+    const bool guided = guide != nullptr;
+    uint64_t train_paths = 0, batch = 0;
+    uint32_t steps = 0;
+    float lr = 1e-2f;
+    if (guided) {
+        if (guide->ctx != ctx) return fail(ctx, BG_ERR_INVALID, "guide belongs to another context");
+        if (d->mode != BG_MODE_CORRECTED) return fail(ctx, BG_ERR_INVALID, "guiding needs BG_MODE_CORRECTED");
+        if (!(gd->select_prob >= 0.0f && gd->select_prob <= 1.0f)) return fail(ctx, BG_ERR_INVALID, "select_prob outside [0, 1]");
+        p.guided = 1;
+        p.guide_alpha = gd->select_prob;
+        if (gd->train) {
+            train_paths = gd->train_paths ? gd->train_paths : (1u << 20);
+            batch = std::min<uint64_t>(gd->batch ? gd->batch : 16384, guide->scratch.cap);
+            steps = gd->steps ? gd->steps : 8;
+            lr = gd->lr > 0.0f ? gd->lr : 1e-2f;
+            rc = ensure_replay(guide, gd->replay ? gd->replay : (1u << 20));
+            if (rc) return rc;
+        }
+        rc = ensure_guide_bufs(ctx, cap, train_paths * d->max_depth);
+        if (rc) return rc;
+        // positions are normalised to the scene's bounds
+        const bg_bvh_node& root = s->bvh.nodes[0];
+        for (int k = 0; k < 3; ++k) {
+            const float ext = std::max(root.bmax[k] - root.bmin[k], 1e-6f);
+            guide->box.lo[k] = root.bmin[k] - 1e-3f * ext;
+            guide->box.hi[k] = root.bmax[k] + 1e-3f * ext;
+        }
+    }
+    const int guide_grid = 2 * std::max(ctx->num_cus, 1);
+    uint64_t train_steps = 0;
     bg::ShadeSceneDev sc;
     sc.tri_orig = s->d_tri_orig;
     sc.tri_flip = s->d_tri_flip;
@@ -183,6 +267,7 @@ int bg_render_film(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_f
     const bg::PathBuffers& b = ctx->paths;
     hipStream_t st = ctx->stream;
 
+    HIP_TRY(ctx, hipMemsetAsync(b.counters, 0, 2 * sizeof(unsigned long long), st));
     auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
     uint64_t launches = 0;
@@ -192,32 +277,96 @@ int bg_render_film(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_f
         p.n_samples = ns;
         p.n_paths = npix * ns;
         const int g = (int)std::min<uint64_t>(grid, (p.n_paths + block - 1) / block);
+        if (train_paths) {
+            // spread the recording paths over the whole pass (paths are pixel-major within a sample)
+            p.rec_stride = (uint32_t)std::max<uint64_t>(1, (p.n_paths + train_paths - 1) / train_paths);
+            p.rec_paths = (uint32_t)std::min<uint64_t>(train_paths, (p.n_paths + p.rec_stride - 1) / p.rec_stride);
+        }
         HIP_TRY(ctx, bg::launch_init_paths(p, b, st));
         for (uint32_t bounce = 0; bounce <= d->max_depth; ++bounce) {
             HIP_TRY(ctx, bg::launch_traverse(traverse_args(s, ctx, b.ray, b.hit, p.n_paths), false, false, ctx->shape, g, st));
+            if (guided && bounce < d->max_depth) {
+                HIP_TRY(ctx, bg::launch_guide_inputs(p, sc, b, st));
+                HIP_TRY(ctx, bg::guide_infer(guide->dev, guide->box, b.gin, p.n_paths, b.gout, guide_grid, st));
+                launches += 2;
+            }
             HIP_TRY(ctx, bg::launch_shade(p, sc, b, bounce, st));
             if (sc.n_lights > 0 && bounce < d->max_depth) {
                 HIP_TRY(ctx, bg::launch_traverse(traverse_args(s, ctx, b.sray, b.shit, p.n_paths), true, false, ctx->shape, g, st));
                 HIP_TRY(ctx, bg::launch_traverse(traverse_args(s, ctx, b.mray, b.mhit, p.n_paths), false, false, ctx->shape, g, st));
-                HIP_TRY(ctx, bg::launch_nee_resolve(p, sc, b, st));
+                HIP_TRY(ctx, bg::launch_nee_resolve(p, sc, b, bounce, st));
                 launches += 3;
             }
             launches += 2;
         }
         HIP_TRY(ctx, bg::launch_film(p, b, film->width, film->height, film->sums, film->sums64, st));
-        launches += 2;
+        HIP_TRY(ctx, bg::launch_count_rays(p, b, b.counters, st));
+        launches += 3;
+        if (train_paths && p.rec_paths) {
+            // one training round on this pass's records
+            HIP_TRY(ctx, bg::launch_guide_targets(p, b, st));
+            const uint64_t nrec = (uint64_t)p.rec_paths * d->max_depth;
+            ctx->paths.last_records = nrec;
+            HIP_TRY(ctx, bg::launch_replay_append(b, nrec, guide->replay, st));
+            uint32_t rs[4];
+            HIP_TRY(ctx, hipMemcpyAsync(rs, guide->replay.state, sizeof(rs), hipMemcpyDeviceToHost, st));
+            HIP_TRY(ctx, hipStreamSynchronize(st));
+            const uint64_t avail = rs[1];
+            const uint64_t nb = std::min<uint64_t>(batch, avail);
+            for (uint32_t k = 0; nb > 0 && k < steps; ++k) {
+                HIP_TRY(ctx, bg::launch_replay_gather(guide->replay, (uint64_t)guide->step * nb, nb, guide->batch_in,
+                                                      guide->batch_tgt, st));
+                ++guide->step;
+                HIP_TRY(ctx, bg::guide_train_step(guide->dev, guide->box, guide->batch_in, guide->batch_tgt, nb,
+                                                  guide->scratch, lr, guide->step, guide_grid, nullptr, st));
+                ++train_steps;
+                launches += 4;
+            }
+            launches += 4;
+        }
     }
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
     HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
     auto t1 = std::chrono::steady_clock::now();
     float ms = 0.0f;
     HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    unsigned long long nr[2] = {0, 0};
+    HIP_TRY(ctx, hipMemcpy(nr, b.counters, sizeof(nr), hipMemcpyDeviceToHost));
     if (stats) {
+        stats->rays = nr[0];
+        stats->shadow_rays = nr[1];
         stats->kernel_ms = ms;
         stats->total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         stats->launches = launches;
         stats->samples = npix * d->spp;
+        stats->train_steps = train_steps;
     }
+    return BG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bg_render_film(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film* film, bg_stats* stats) {
+    return render_impl(ctx, s, d, film, stats, nullptr, nullptr);
+}
+
+int bg_render_film_guided(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_guide* guide,
+                          const bg_guiding_desc* gd, bg_film* film, bg_stats* stats) {
+    if (!guide || !gd) return fail(ctx, BG_ERR_INVALID, "null guide or guiding descriptor");
+    return render_impl(ctx, s, d, film, stats, guide, gd);
+}
+
+int bg_guide_records(bg_ctx* ctx, float* inputs, float* targets, uint64_t max_records, uint64_t* n) {
+    if (!ctx || !n) return fail(ctx, BG_ERR_INVALID, "null argument");
+    const bg::PathBuffers& b = ctx->paths;
+    const uint64_t m = std::min<uint64_t>(max_records, b.last_records);
+    *n = b.last_records;
+    if (m == 0) return BG_OK;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (inputs) HIP_TRY(ctx, hipMemcpy(inputs, b.rec_in, 48 * m, hipMemcpyDeviceToHost));
+    if (targets) HIP_TRY(ctx, hipMemcpy(targets, b.rec_tgt, 16 * m, hipMemcpyDeviceToHost));
     return BG_OK;
 }
 

@@ -113,6 +113,46 @@ class GpuPathIntegrator:
         return out
 
 
+class GuidedPathIntegrator(GpuPathIntegrator):
+    """The corrected path integrator with a learned directional proposal: at every vertex the
+    continuation is drawn from the guiding network's vMF mixture with probability ``select_prob``,
+    else from the BSDF, and weighted by the one-sample MIS density, so the image converges to the
+    plain integrator's for any network state. With ``train`` the network is trained online after
+    every pass on that pass's own path vertices (no counterpart in the reference)."""
+
+    def __init__(self, ctx: Context, camera: Camera, spp: int, max_depth: int, guide, rr_threshold: float = 1.0,
+                 seed: int = 1, select_prob: float = 0.5, train: bool = True, train_paths: int = 1 << 20,
+                 batch: int = 16384, steps: int = 8, lr: float = 1e-2, replay: int = 1 << 20):
+        super().__init__(ctx, camera, spp, max_depth, rr_threshold, MODE_CORRECTED, seed)
+        self.guide = guide
+        self.guiding = N.bg_guiding_desc(select_prob, 1 if train else 0, train_paths, batch, steps, lr, replay)
+
+    def render_into(self, bvh: GpuBvh, film: Film, spp=None, first_sample=0, window=None,
+                    paths_per_pass=0) -> N.bg_stats:
+        d = self.desc(spp, first_sample, window, paths_per_pass)
+        st = N.bg_stats()
+        N.check(N.lib().bg_render_film_guided(self.ctx.h, bvh.h, C.byref(d), self.guide.h, C.byref(self.guiding),
+                                              film.h, C.byref(st)), self.ctx.h)
+        return st
+
+    def records(self, max_records: int = 1 << 22) -> tuple[np.ndarray, np.ndarray]:
+        """(inputs (n, 12), targets (n, 4)) of the last trained pass."""
+        n = C.c_uint64(0)
+        N.check(N.lib().bg_guide_records(self.ctx.h, None, None, 0, C.byref(n)), self.ctx.h)
+        m = min(n.value, max_records)
+        inp = np.empty((m, 12), np.float32)
+        tgt = np.empty((m, 4), np.float32)
+        N.check(N.lib().bg_guide_records(self.ctx.h, inp.ctypes.data, tgt.ctypes.data, m, C.byref(n)), self.ctx.h)
+        return inp, tgt
+
+    def render(self, bvh: GpuBvh) -> np.ndarray:
+        film = Film(self.ctx, self.camera.width, self.camera.height)
+        self.last_stats = self.render_into(bvh, film)
+        img = film.image().astype(np.float32)
+        film.close()
+        return img
+
+
 def rel_mse(img: np.ndarray, ref: np.ndarray) -> float:
     """mean over pixels and channels of (I - R)^2 / (R^2 + 0.01), in FP64."""
     i = img.astype(np.float64)

@@ -300,6 +300,101 @@ def path_tracer_scene(width: int = 1920, height: int = 1080, seed: int = 2024, s
     return s
 
 
+def grid(origin, eu, ev, h: float) -> np.ndarray:
+    """The rectangle origin + [0,1] eu + [0,1] ev as a grid of cells no larger than h, two triangles
+    per cell (winding as quad())."""
+    o, eu, ev = (np.asarray(x, np.float64) for x in (origin, eu, ev))
+    nu = max(1, int(np.ceil(np.linalg.norm(eu) / h)))
+    nv = max(1, int(np.ceil(np.linalg.norm(ev) / h)))
+    iu, iv = np.meshgrid(np.arange(nu), np.arange(nv), indexing="ij")
+    iu, iv = iu.ravel()[:, None], iv.ravel()[:, None]
+    p = lambda a, b: o + (a / nu) * eu + (b / nv) * ev  # noqa: E731
+    p0, p1, p2, p3 = p(iu, iv), p(iu + 1, iv), p(iu + 1, iv + 1), p(iu, iv + 1)
+    t = np.stack([np.concatenate([p0, p1, p2], 1), np.concatenate([p0, p2, p3], 1)], 1)
+    return t.reshape(-1, 9).astype(np.float32)
+
+
+def box_faces(lo, hi, h: float, skip=()) -> list[np.ndarray]:
+    """The six faces of an axis-aligned box as grids (names in `skip` left out: -x, +x, -y, +y, -z, +z)."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    faces = {
+        "-x": ((x0, y0, z0), (0, y1 - y0, 0), (0, 0, z1 - z0)),
+        "+x": ((x1, y0, z0), (0, 0, z1 - z0), (0, y1 - y0, 0)),
+        "-y": ((x0, y0, z0), (0, 0, z1 - z0), (x1 - x0, 0, 0)),
+        "+y": ((x0, y1, z0), (x1 - x0, 0, 0), (0, 0, z1 - z0)),
+        "-z": ((x0, y0, z0), (x1 - x0, 0, 0), (0, y1 - y0, 0)),
+        "+z": ((x0, y0, z1), (0, y1 - y0, 0), (x1 - x0, 0, 0)),
+    }
+    return [grid(*faces[k], h) for k in faces if k not in skip]
+
+
+AL_ETA = (1.657, 0.880, 0.521)
+AL_K = (9.224, 6.270, 4.837)
+
+
+def door_gap_room(width: int = 1920, height: int = 1080, cell: float = 0.0179, seed: int = 5,
+                  L: float = 3000.0) -> SceneData:
+    """Hard indirect interior: a 5 x 3 x 6 m room lit only through a 5 cm gap beside a nearly closed
+    door from a neighbouring 2.9 x 3 x 6 m room whose ceiling holds the single quad area light
+    (1 x 1 m, radiance L, facing down). The camera room's walls are glossy Trowbridge-Reitz
+    conductors (alpha 0.1, 0.2, 0.3), its floor and ceiling matte; two displaced icospheres stand in
+    it. Every surface is tessellated into cells of at most `cell` metres (0.0179 -> 1,513,386
+    triangles; larger cells give small test scenes with the same lighting)."""
+    s = SceneData()
+    s.materials = [matte(0.6, 0.6, 0.58),                               # 0 floor / ceiling (camera room)
+                   conductor(0.1, AL_ETA, AL_K, (0.75, 0.72, 0.68)),    # 1 glossy wall
+                   conductor(0.2, AL_ETA, AL_K, (0.7, 0.75, 0.72)),     # 2 glossy wall
+                   conductor(0.3, AL_ETA, AL_K, (0.72, 0.7, 0.75)),     # 3 glossy wall / partition
+                   matte(0.8, 0.78, 0.75),                              # 4 light room
+                   matte(0.45, 0.3, 0.2),                               # 5 door
+                   matte(0.0, 0.0, 0.0),                                # 6 light
+                   conductor(0.15),                                      # 7 gold sculpture
+                   matte(0.3, 0.5, 0.7)]                                # 8 matte sculpture
+    parts, mats = [], []
+
+    def add(tris, m):
+        parts.append(tris)
+        mats.append(np.full(len(tris), m, np.uint32))
+
+    # camera room x in [0, 5], y in [0, 3], z in [0, 6]
+    add(grid((0, 0, 0), (0, 0, 6), (5, 0, 0), cell), 0)                # floor
+    add(grid((0, 3, 0), (5, 0, 0), (0, 0, 6), cell), 0)                # ceiling
+    add(grid((0, 0, 0), (0, 3, 0), (0, 0, 6), cell), 1)                # x = 0 wall
+    add(grid((0, 0, 0), (5, 0, 0), (0, 3, 0), cell), 2)                # z = 0 wall
+    add(grid((0, 0, 6), (0, 3, 0), (5, 0, 0), cell), 2)                # z = 6 wall
+    # partition x in [5, 5.1] with a door opening z in [2.5, 3.5], y in [0, 2.1]
+    for x in (5.0, 5.1):
+        m = 3 if x == 5.0 else 4
+        add(grid((x, 0, 0), (0, 0, 2.5), (0, 3, 0), cell), m)
+        add(grid((x, 0, 3.5), (0, 0, 2.5), (0, 3, 0), cell), m)
+        add(grid((x, 2.1, 2.5), (0, 0, 1.0), (0, 0.9, 0), cell), m)
+    add(grid((5.0, 0, 2.5), (0.1, 0, 0), (0, 2.1, 0), cell), 3)        # jambs and lintel
+    add(grid((5.0, 0, 3.5), (0.1, 0, 0), (0, 2.1, 0), cell), 3)
+    add(grid((5.0, 2.1, 2.5), (0.1, 0, 0), (0, 0, 1.0), cell), 3)
+    # door panel leaves the gap z in [2.5, 2.55]
+    for f in box_faces((5.02, 0.0, 2.55), (5.08, 2.09, 3.5), cell, skip=("-y",)):
+        add(f, 5)
+    # light room x in [5.1, 8]
+    add(grid((5.1, 0, 0), (0, 0, 6), (2.9, 0, 0), cell), 4)
+    add(grid((5.1, 3, 0), (2.9, 0, 0), (0, 0, 6), cell), 4)
+    add(grid((8, 0, 0), (0, 0, 6), (0, 3, 0), cell), 4)
+    add(grid((5.1, 0, 0), (2.9, 0, 0), (0, 3, 0), cell), 4)
+    add(grid((5.1, 0, 6), (0, 3, 0), (2.9, 0, 0), cell), 4)
+    # sculptures
+    lv = 6 if cell < 0.05 else 2
+    add(icosphere(lv, (2.2, 0.6, 3.8), 0.6, 0.1, 3.0, seed), 7)
+    add(icosphere(lv, (3.4, 0.45, 1.6), 0.45, 0.1, 3.0, seed + 1), 8)
+    base = sum(len(t) for t in parts)
+    add(quad((6.0, 2.95, 2.5), (7.0, 2.95, 2.5), (7.0, 2.95, 3.5), (6.0, 2.95, 3.5)), 6)
+    s.tri_positions = np.concatenate(parts).astype(np.float32)
+    s.tri_material = np.concatenate(mats)
+    s.lights = [area_light(base, (L, L, L)), area_light(base + 1, (L, L, L))]
+    s.camera = Camera(pos=(0.6, 1.6, 0.8), look=(5.0, 0.9, 3.3), up=(0.0, 1.0, 0.0), fov=62.0,
+                      width=width, height=height)
+    return s
+
+
 def intersection_camera(width: int = 4096, height: int = 4096) -> Camera:
     return Camera(pos=(0.0, 0.3, -3.2), look=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=45.0,
                   width=width, height=height)

@@ -142,6 +142,7 @@ typedef struct bg_stats {
     /* Filled by the render calls. */
     uint64_t samples;     /* camera samples completed */
     uint64_t shadow_rays; /* any-hit queries among `rays` */
+    uint64_t train_steps; /* guiding-network Adam steps taken by the call */
 } bg_stats;
 
 /* Library */
@@ -250,6 +251,65 @@ int bg_render_film(bg_ctx* ctx, const bg_scene* scene, const bg_render_desc* des
  * (film.rs:156-202: XYZ round trip, divide by weight, clamp at 0) into
  * rgb_out (width*height*3 floats of the full film). */
 int bg_render(bg_ctx* ctx, const bg_scene* scene, const bg_render_desc* desc, float* rgb_out, bg_stats* stats);
+
+/* ------------------------------------------------------------ path guiding */
+
+/* Learned directional proposal. The reference has no path guiding; these
+ * entry points have no counterpart there. Network: 64-dim frequency encoding
+ * of (position normalised to a box, outgoing direction, normal) -> 4 x 64 ReLU
+ * -> 40 outputs = 8 von Mises-Fisher lobes x (mixture logit, direction
+ * 3-vector, concentration parameter s: kappa = exp(1.5 + 5.5 tanh((s - 1.5) /
+ * 5.5))). BF16 matrix-core products, FP32 accumulation,
+ * FP32 master parameters and Adam state. Parameter order: per layer the weight
+ * matrix (out x in, row-major) then the bias. */
+#define BG_GUIDE_PARAMS 19240u
+#define BG_GUIDE_OUTPUTS 40u
+typedef struct bg_guide bg_guide;
+
+/* params: BG_GUIDE_PARAMS floats, copied. max_batch bounds n of eval/train. */
+int bg_guide_create(bg_ctx* ctx, const float* params, uint64_t max_batch, bg_guide** out);
+void bg_guide_destroy(bg_guide* guide);
+/* Replaces the parameters, restarts Adam (moments 0, step 0) and empties the
+ * replay ring of training samples. */
+int bg_guide_set_params(bg_guide* guide, const float* params);
+/* Any output pointer may be NULL. */
+int bg_guide_get_params(bg_guide* guide, float* params, float* adam_m, float* adam_v);
+/* Box the positions are normalised to (each axis maps to [0, 1]). */
+int bg_guide_set_bounds(bg_guide* guide, const float lo[3], const float hi[3]);
+/* samples: n x 12 floats (p.xyz, -, wo.xyz, -, n.xyz, -); out: n x 40. */
+int bg_guide_eval(bg_guide* guide, const float* samples, uint64_t n, float* out);
+/* One Adam step on the batch mean of -weight * log p(direction).
+ * targets: n x 4 floats (direction.xyz, weight). mean_loss (before the step)
+ * and grad (BG_GUIDE_PARAMS floats) may be NULL. */
+int bg_guide_train(bg_guide* guide, const float* samples, const float* targets, uint64_t n, float lr,
+                   double* mean_loss, float* grad);
+
+/* Guided rendering (BG_MODE_CORRECTED only). At every path vertex the
+ * continuation direction comes from the network's mixture with probability
+ * select_prob, else from the BSDF, and is weighted by f |cos| / (select_prob *
+ * p_guide + (1 - select_prob) * p_bsdf) (one-sample MIS: the estimator stays
+ * unbiased for any network). Light sampling is unchanged. With train != 0,
+ * every pass records the vertices of up to train_paths paths, weighs each
+ * sampled direction by w = luminance(f |cos| L_i / pdf), appends those with
+ * w > 0 to the guide's replay ring, and takes `steps` Adam steps on
+ * minibatches of the ring (loss: batch mean of -w log p_guide(direction)).
+ * The guide's position box is set to the scene bounds. */
+typedef struct bg_guiding_desc {
+    float select_prob;     /* in [0, 1] */
+    uint32_t train;        /* nonzero: train online after every pass */
+    uint32_t train_paths;  /* paths per pass that record, spread over the pass (0: 2^20) */
+    uint32_t batch;        /* Adam minibatch (0: 16384; at most the guide's max_batch) */
+    uint32_t steps;        /* Adam steps after each pass (0: 8) */
+    float lr;              /* Adam learning rate (0: 1e-2) */
+    uint32_t replay;       /* replay ring capacity in samples (0: 2^20) */
+} bg_guiding_desc;
+int bg_render_film_guided(bg_ctx* ctx, const bg_scene* scene, const bg_render_desc* desc, bg_guide* guide,
+                          const bg_guiding_desc* guiding, bg_film* film, bg_stats* stats);
+/* The training records of the last trained pass, in the layouts bg_guide_train
+ * takes: inputs n x 12 floats, targets n x 4 (direction, weight; weight 0 for
+ * vertices that did not occur). *n receives the record count; at most
+ * max_records are copied. */
+int bg_guide_records(bg_ctx* ctx, float* inputs, float* targets, uint64_t max_records, uint64_t* n);
 
 #ifdef __cplusplus
 }

@@ -3,11 +3,14 @@
 The reference crate has no path guiding; this module is the specification
 the HIP kernels are checked against. It is pinned by tests/test_guide_oracle.py
 (finite differences for every gradient, quadrature for the vMF normalisation,
-the closed-form Adam step, sampling histograms).
+the closed-form Adam step, sampling histograms, bfloat16 rounding on exactly
+representable and tie values, and the bfloat16 variants reducing to the FP64
+ones when the inputs are already bfloat16 and the network is linear enough).
 
 Network: a 64-dim encoding of (position, outgoing direction, normal) ->
 4 hidden layers of 64 ReLU units -> 40 outputs = 8 von Mises-Fisher lobes x
-(mixture logit, 3-vector direction, log-concentration).
+(mixture logit, 3-vector direction, concentration parameter s with
+log kappa = 1.5 + 5.5 tanh((s - 1.5) / 5.5), i.e. kappa in (e^-4, e^7)).
 """
 from __future__ import annotations
 
@@ -20,6 +23,10 @@ N_OUT = 5 * N_LOBES
 LAYERS = [(N_HID, N_IN), (N_HID, N_HID), (N_HID, N_HID), (N_HID, N_HID), (N_OUT, N_HID)]
 N_PARAMS = sum(o * i + o for o, i in LAYERS)  # 19,240
 LOG_KAPPA_MIN, LOG_KAPPA_MAX = -4.0, 7.0
+# log kappa = C + R tanh((s - C) / R): the identity near C, saturating smoothly at (MIN, MAX), so no
+# lobe's concentration is ever stuck where its gradient vanishes
+LOG_KAPPA_C = 0.5 * (LOG_KAPPA_MIN + LOG_KAPPA_MAX)
+LOG_KAPPA_R = 0.5 * (LOG_KAPPA_MAX - LOG_KAPPA_MIN)
 N_FREQ = 8
 
 
@@ -79,7 +86,7 @@ def mixture(out):
     m = o[..., 1:4]
     nm = np.sqrt((m * m).sum(-1) + 1e-12)
     mu = m / nm[..., None]
-    s = np.clip(o[..., 4], LOG_KAPPA_MIN, LOG_KAPPA_MAX)
+    s = LOG_KAPPA_C + LOG_KAPPA_R * np.tanh((o[..., 4] - LOG_KAPPA_C) / LOG_KAPPA_R)
     kappa = np.exp(s)
     return pi, mu, kappa, nm
 
@@ -113,9 +120,8 @@ def loss_grad_out(out, w, weight):
     dlogp[..., 0] = r - pi                             # softmax logits
     muw = (mu * w[:, None, :]).sum(-1)
     dlogk = 1.0 / kappa - 2.0 * np.exp(-2.0 * kappa) / (-np.expm1(-2.0 * kappa)) + (muw - 1.0)
-    s = o[..., 4]
-    inside = (s > LOG_KAPPA_MIN) & (s < LOG_KAPPA_MAX)
-    dlogp[..., 4] = r * dlogk * kappa * inside         # kappa = exp(s)
+    th = np.tanh((o[..., 4] - LOG_KAPPA_C) / LOG_KAPPA_R)
+    dlogp[..., 4] = r * dlogk * kappa * (1.0 - th * th)  # kappa = exp(C + R tanh((s - C) / R))
     dmu = kappa[..., None] * w[:, None, :]             # d log f / d mu
     proj = dmu - (dmu * mu).sum(-1, keepdims=True) * mu
     dlogp[..., 1:4] = r[..., None] * proj / nm[..., None]
@@ -137,6 +143,47 @@ def backward(flat, cache, dout):
         gb = d.sum(0)
         grads.append((gW, gb))
         d = d @ W
+    grads.reverse()
+    return np.concatenate([np.concatenate([gW.ravel(), gb]) for gW, gb in grads])
+
+
+def bf16(x):
+    """Rounds to the nearest bfloat16 (ties to even), returned as float64. NaN stays NaN."""
+    f = np.asarray(x, np.float64).astype(np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    out = r.astype(np.uint32).view(np.float32).astype(np.float64)
+    return np.where(np.isnan(f), np.nan, out)
+
+
+def forward_bf16(flat, X):
+    """forward() with the rounding points of the matrix-core kernels: the encoding, every weight
+    and every hidden activation (after bias and ReLU) are rounded to bfloat16; products and
+    biases accumulate exactly (the kernels accumulate in FP32); outputs are not rounded."""
+    layers = unpack(np.asarray(flat, np.float64))
+    h = bf16(X)
+    acts, pre = [h], []
+    for li, (W, b) in enumerate(layers):
+        z = h @ bf16(W).T + b
+        pre.append(z)
+        h = bf16(np.maximum(z, 0.0)) if li < len(layers) - 1 else z
+        acts.append(h)
+    return h, (acts, pre)
+
+
+def backward_bf16(flat, cache, dout):
+    """backward() with the kernels' rounding points: d out and every back-propagated pre-activation
+    gradient are rounded to bfloat16, products use the bfloat16 weights, the ReLU mask is taken
+    from the rounded activations (X_l > 0), and dW, db accumulate exactly."""
+    acts, _ = cache
+    layers = unpack(np.asarray(flat, np.float64))
+    grads = []
+    d = bf16(dout)
+    for li in range(len(layers) - 1, -1, -1):
+        W, _ = layers[li]
+        grads.append((d.T @ acts[li], d.sum(0)))
+        if li > 0:
+            d = bf16((d @ bf16(W)) * (acts[li] > 0))
     grads.reverse()
     return np.concatenate([np.concatenate([gW.ravel(), gb]) for gW, gb in grads])
 

@@ -1,0 +1,198 @@
+"""GPU path-guiding MLP (BF16 matrix cores, FP32 accumulation) against the
+FP64 oracle in oracle/guide_mlp.py: forward outputs, loss, every parameter
+gradient, the Adam step, ragged batch sizes and training progress."""
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from bonsai_amd import _native as N
+from bonsai_amd.guide import N_PARAMS, GuideMLP, pack_samples
+
+sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "oracle"))
+import guide_mlp as G  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+LO = np.array([-1.0, -0.5, -2.0])
+HI = np.array([2.0, 1.5, 1.0])
+
+
+def unit(rng, n):
+    v = rng.standard_normal((n, 3))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def batch(seed, n):
+    rng = np.random.default_rng(seed)
+    p = LO + (HI - LO) * rng.random((n, 3))
+    wo, nrm, w = unit(rng, n), unit(rng, n), unit(rng, n)
+    weight = rng.uniform(0.1, 2.0, n)
+    return p.astype(np.float32), wo.astype(np.float32), nrm.astype(np.float32), w, weight
+
+
+def params(seed):
+    """He-uniform weights plus small non-zero biases (so the bias path is exercised)."""
+    flat = G.init_params(seed)
+    rng = np.random.default_rng(seed + 100)
+    o = 0
+    for no, ni in G.LAYERS:
+        o += no * ni
+        flat[o:o + no] = rng.uniform(-0.1, 0.1, no)
+        o += no
+    return flat.astype(np.float32)
+
+
+def oracle_inputs(p, wo, nrm):
+    return G.encode(p.astype(np.float64), wo.astype(np.float64), nrm.astype(np.float64), LO, HI)
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def make(ctx, flat, cap):
+    g = GuideMLP(ctx, flat, cap)
+    g.set_bounds(LO, HI)
+    return g
+
+
+def test_forward_matches_fp64(ctx):
+    flat = params(3)
+    p, wo, nrm, w, _ = batch(1, 4099)  # ragged: not a multiple of 32 or 128
+    g = make(ctx, flat, 8192)
+    out = g.eval(pack_samples(p, wo, nrm))
+    ref, _ = G.forward(flat.astype(np.float64), oracle_inputs(p, wo, nrm))
+    err = rel_l2(out, ref)
+    emu, _ = G.forward_bf16(flat.astype(np.float64), oracle_inputs(p, wo, nrm))
+    emu_err = rel_l2(out, emu)
+    pdf_gpu = G.mixture_pdf(out.astype(np.float64), w)
+    pdf_ref = G.mixture_pdf(ref, w)
+    pdf_err = np.abs(pdf_gpu / pdf_ref - 1)
+    print(f"BF16-vs-FP64 guide outputs: rel L2 {err:.2e}, max abs {np.abs(out - ref).max():.2e}; "
+          f"mixture pdf rel err median {np.median(pdf_err):.2e}, p99 {np.quantile(pdf_err, 0.99):.2e}; "
+          f"vs BF16-emulating oracle rel L2 {emu_err:.2e}")
+    assert np.isfinite(out).all()
+    assert emu_err < 2e-3
+    assert err < 2e-2
+    assert np.median(pdf_err) < 2e-2
+
+
+def test_ragged_batches_are_slices_of_one_batch(ctx):
+    flat = params(4)
+    p, wo, nrm, _, _ = batch(2, 700)
+    s = pack_samples(p, wo, nrm)
+    g = make(ctx, flat, 1024)
+    full = g.eval(s)
+    for n in (1, 31, 33, 129, 257):
+        np.testing.assert_array_equal(g.eval(s[:n]), full[:n])
+    np.testing.assert_array_equal(g.eval(s[300:333]), full[300:333])
+
+
+def test_gradient_matches_fp64(ctx):
+    flat = params(5)
+    n = 5000  # three sample chunks of the weight-gradient reduction, the last one ragged
+    p, wo, nrm, w, weight = batch(6, n)
+    g = make(ctx, flat, n)
+    targets = np.concatenate([w, weight[:, None]], 1).astype(np.float32)
+    loss, grad = g.train(pack_samples(p, wo, nrm), targets, lr=1e-3, want_grad=True)
+    X = oracle_inputs(p, wo, nrm)
+    # oracle on the GPU's own target directions (rounded to f32, as the GPU sees them)
+    w32 = targets[:, :3].astype(np.float64)
+    out, cache = G.forward(flat.astype(np.float64), X)
+    ref_loss = G.loss(out, w32, targets[:, 3].astype(np.float64))
+    wt = targets[:, 3].astype(np.float64)
+    ref = G.backward(flat.astype(np.float64), cache, G.loss_grad_out(out, w32, wt))
+    eout, ecache = G.forward_bf16(flat.astype(np.float64), X)
+    emu = G.backward_bf16(flat.astype(np.float64), ecache, G.loss_grad_out(eout, w32, wt))
+    assert abs(loss - ref_loss) < 1e-2 * abs(ref_loss)
+
+    def per_part(r):
+        o, errs = 0, []
+        for li, (no, ni) in enumerate(G.LAYERS):
+            for name, size in (("W", no * ni), ("b", no)):
+                a, b = grad[o:o + size], r[o:o + size]
+                cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
+                errs.append((li, name, rel_l2(a, b), cos))
+                o += size
+        return errs
+
+    for label, r, bound in (("BF16-emulating oracle", emu, (1.5e-2, 0.9999)), ("FP64 oracle", ref, (0.15, 0.99))):
+        errs = per_part(r)
+        print(f"gradients vs {label} (layer+part rel-L2/cosine):",
+              ", ".join(f"{li}{nm} {e:.1e}/{c:.5f}" for li, nm, e, c in errs))
+        for li, nm, e, c in errs:
+            assert e < bound[0] and c > bound[1], (label, li, nm, e, c)
+
+
+def test_adam_step_matches_closed_form(ctx):
+    flat = params(7)
+    p, wo, nrm, w, weight = batch(8, 300)
+    g = make(ctx, flat, 512)
+    targets = np.concatenate([w, weight[:, None]], 1).astype(np.float32)
+    for t in (1, 2, 3):  # the kernel's FP32 constants 0.9f, 0.999f, 1e-8f, as the oracle's b1, b2, eps
+        before, m0, v0 = g.params()
+        _, grad = g.train(pack_samples(p, wo, nrm), targets, lr=1e-2, want_grad=True)
+        after, m1, v1 = g.params()
+        exp_p, exp_m, exp_v = G.adam(before.astype(np.float64), grad.astype(np.float64), m0.astype(np.float64),
+                                     v0.astype(np.float64), t, lr=1e-2, b1=float(np.float32(0.9)),
+                                     b2=float(np.float32(0.999)), eps=float(np.float32(1e-8)))
+        # m = 0.9 m0 + 0.1 g can cancel: FP32 rounding of the two terms bounds the absolute error
+        np.testing.assert_allclose(m1, exp_m, rtol=1e-5, atol=1e-6 * np.abs(grad).max())
+        np.testing.assert_allclose(v1, exp_v, rtol=1e-5, atol=1e-18)
+        np.testing.assert_allclose(after, exp_p, rtol=1e-5, atol=2e-6)
+    # set_params restarts Adam
+    g.set_params(flat)
+    p1, m, v = g.params()
+    np.testing.assert_array_equal(p1, flat)
+    assert not m.any() and not v.any()
+
+
+def test_training_is_deterministic(ctx):
+    flat = params(9)
+    p, wo, nrm, w, weight = batch(10, 3000)
+    targets = np.concatenate([w, weight[:, None]], 1).astype(np.float32)
+    res = []
+    for _ in range(2):
+        g = make(ctx, flat, 4096)
+        for _ in range(3):
+            g.train(pack_samples(p, wo, nrm), targets)
+        res.append(g.params()[0])
+        g.close()
+    np.testing.assert_array_equal(res[0], res[1])
+
+
+def test_training_reduces_loss(ctx):
+    """Two position halves with one bright direction each: 200 Adam steps at lr 1e-2 on noisy
+    targets must concentrate the mixture on them (scored by the FP64 oracle)."""
+    rng = np.random.default_rng(7)
+    S = 256
+    p = rng.random((S, 3)).astype(np.float32)
+    wo = np.tile([0.0, 0.0, 1.0], (S, 1)).astype(np.float32)
+    nrm = np.tile([0.0, 1.0, 0.0], (S, 1)).astype(np.float32)
+    target = np.where(p[:, :1] < 0.5, [[0.0, 1.0, 0.0]], [[1.0, 0.0, 0.0]])
+    flat = G.init_params(8).astype(np.float32)
+    g = GuideMLP(ctx, flat, S)
+    g.set_bounds(np.zeros(3), np.ones(3))
+    X = G.encode(p.astype(np.float64), wo.astype(np.float64), nrm.astype(np.float64), np.zeros(3), np.ones(3))
+    first = G.loss(G.forward(flat.astype(np.float64), X)[0], target, np.ones(S))
+    s = pack_samples(p, wo, nrm)
+    losses = []
+    for _ in range(200):
+        wdir = target + 0.05 * rng.standard_normal((S, 3))
+        wdir /= np.linalg.norm(wdir, axis=1, keepdims=True)
+        tg = np.concatenate([wdir, np.ones((S, 1))], 1).astype(np.float32)
+        losses.append(g.train(s, tg)[0])
+    final = G.loss(G.forward(g.params()[0].astype(np.float64), X)[0], target, np.ones(S))
+    print(f"guide training: oracle loss {first:.3f} -> {final:.3f}; GPU batch loss {losses[0]:.3f} -> {losses[-1]:.3f}")
+    assert final < first - 2.0
+    assert losses[-1] < losses[0] - 2.0
+
+
+def test_batch_bound_is_enforced(ctx):
+    g = make(ctx, params(1), 64)
+    p, wo, nrm, _, _ = batch(3, 65)
+    with pytest.raises(N.BonsaiError):
+        g.eval(pack_samples(p, wo, nrm))
+    assert N_PARAMS == G.N_PARAMS

@@ -130,3 +130,31 @@ def test_training_reduces_loss_on_a_fixed_target():
         flat, m, v = G.adam(flat, g, m, v, t, lr=1e-2)
     out, _ = G.forward(flat, X)
     assert G.loss(out, target, np.ones(S)) < first - 2.0
+
+
+def test_bf16_rounding():
+    e7 = 2.0 ** -7
+    vals = np.array([1.0, 1 + e7 / 2, 1 + 3 * e7 / 2, 1 + e7 / 2 + 2.0 ** -20, -(1 + 3 * e7 / 2), 3.0e38, 1e-40, 0.0])
+    exp = np.array([1.0, 1.0, 1 + 2 * e7, 1 + e7, -(1 + 2 * e7), 2.9999e38, None, 0.0], dtype=object)
+    got = G.bf16(vals)
+    for g, e in zip(got[:5], exp[:5]):
+        assert g == e
+    assert abs(got[5] / 3.0e38 - 1) < 2.0 ** -8
+    assert got[7] == 0.0 and np.isnan(G.bf16(np.nan))
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal(10000) * 10.0 ** rng.uniform(-20, 20, 10000)
+    r = G.bf16(x)
+    assert np.all(np.abs(r / x - 1) <= 2.0 ** -8)
+    np.testing.assert_array_equal(G.bf16(r), r)  # idempotent
+
+
+def test_bf16_variants_reduce_to_fp64_without_rounding(monkeypatch):
+    flat = G.init_params(4)
+    X, w, weight = batch(5, 24)
+    out, cache = G.forward(flat, X)
+    ref = G.backward(flat, cache, G.loss_grad_out(out, w, weight))
+    monkeypatch.setattr(G, "bf16", lambda x: np.asarray(x, np.float64))
+    out2, cache2 = G.forward_bf16(flat, X)
+    np.testing.assert_allclose(out2, out, rtol=1e-12, atol=1e-12)
+    got = G.backward_bf16(flat, cache2, G.loss_grad_out(out2, w, weight))
+    np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-14)
