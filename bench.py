@@ -14,6 +14,12 @@ triangle test, 64 B per sphere test, counted by the kernel's counting mode)
 over its HIP-event-timed average duration. The CPU baseline is the oracle
 (C++ restatement of the reference traversal) on a bounded random sample of
 the same four query sets, using every host core available to this process.
+
+Two further objects ride on the same line: `path_tracer` (the plain GPU path
+tracer at 1920 x 1080, 64 spp, depth 8, samples/s and Mrays/s) and `guiding`
+(the door-gap interior at 1920 x 1080, depth 10: per-sample variance and
+seconds per sample of the plain, network-free and neural-guided estimators,
+and the time to relMSE 0.02 they predict).
 """
 from __future__ import annotations
 
@@ -106,6 +112,113 @@ def cpu_baseline(scene, bvh, sets, target_s: float):
     }, samples, outs
 
 
+def path_tracer_leg(ctx, spp: int) -> dict:
+    """The plain GPU path tracer at 1920 x 1080: 2,048,000 triangles + 256 spheres, 4 quad lights,
+    `spp` samples per pixel (64 in the configuration), max depth 8, corrected (pbrt-v3) semantics."""
+    from bonsai_amd import GpuBvh
+    from bonsai_amd.integrator import MODE_CORRECTED, MODE_REF, Film, GpuPathIntegrator
+    from bonsai_amd.scenes import path_tracer_scene
+
+    scene = path_tracer_scene()
+    bvh = GpuBvh(ctx, scene)
+    out = {"workload": "plain path tracer 1920x1080, max depth 8", "triangles": scene.n_tris,
+           "spheres": scene.n_spheres, "spp": spp}
+    for mode, name in ((MODE_CORRECTED, "corrected"), (MODE_REF, "reference_semantics")):
+        gi = GpuPathIntegrator(ctx, scene.camera, spp=spp, max_depth=8, mode=mode, seed=5)
+        film = Film(ctx, scene.camera.width, scene.camera.height)
+        gi.render_into(bvh, film, spp=1)  # warm-up (allocations, code objects)
+        film.clear()
+        t0 = time.perf_counter()
+        st = gi.render_into(bvh, film)
+        wall = time.perf_counter() - t0
+        out[name] = {"seconds": round(wall, 3), "samples_per_s": round(st.samples / wall, 1),
+                     "Msamples_s": round(st.samples / wall / 1e6, 2), "rays": int(st.rays),
+                     "Mrays_s": round(st.rays / wall / 1e6, 1), "shadow_rays": int(st.shadow_rays),
+                     "kernel_ms": round(st.kernel_ms, 1), "image_mean": round(float(film.image().mean()), 5)}
+        log(f"path tracer ({name}): {spp} spp in {wall:.2f} s, {st.rays / wall / 1e6:.0f} Mrays/s")
+        film.close()
+    bvh.close()
+    return out
+
+
+def box_blur(img: np.ndarray, r: int) -> np.ndarray:
+    """Mean over the (2r+1)^2 neighbourhood, clipped at the image border."""
+    out = img.astype(np.float64)
+    for axis in (0, 1):
+        c = np.cumsum(np.pad(out, [(r + 1, r) if a == axis else (0, 0) for a in range(out.ndim)], mode="edge"),
+                      axis=axis)
+        n = out.shape[axis]
+        hi = np.take(c, np.arange(2 * r + 1, 2 * r + 1 + n), axis=axis)
+        lo = np.take(c, np.arange(0, n), axis=axis)
+        out = (hi - lo) / (2 * r + 1)
+    return out
+
+
+def rel_mse_pair(a: np.ndarray, b: np.ndarray) -> float:
+    """relMSE estimated from two independent renders of n samples each: per pixel (A - B)^2 / 2 is an
+    unbiased estimate of one render's variance, and the squared mean in the denominator is taken
+    from the 7 x 7 box-filtered average of both renders (a single noisy pixel in the denominator would
+    hide its own error). For unbiased estimators with per-sample relative variance v this is v / n."""
+    m = box_blur(0.5 * (a + b), 3)
+    return float(np.mean(0.5 * (a - b) ** 2 / (m * m + 0.01)))
+
+
+def guiding_leg(ctx, spp: int) -> dict:
+    """Door-gap interior at 1920 x 1080, depth 10: per-sample relative variance v (from two
+    independent renders of `spp` samples each) and seconds per sample of each estimator; the
+    predicted time to relMSE 0.02 is v * seconds_per_spp / 0.02 (Monte Carlo error falls as
+    1 / samples). The guided estimator trains its network from scratch during the first render
+    and keeps training during the second."""
+    from bonsai_amd import GpuBvh
+    from bonsai_amd.guide import GuideMLP
+    from bonsai_amd.integrator import MODE_CORRECTED, Film, GpuPathIntegrator, GuidedPathIntegrator
+    from bonsai_amd.scenes import door_gap_room
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import guide_mlp  # initial parameters only (He-uniform, seeded)
+
+    scene = door_gap_room()
+    bvh = GpuBvh(ctx, scene)
+    W, H = scene.camera.width, scene.camera.height
+    out = {"workload": "door-gap interior 1920x1080, max depth 10, 1 light behind a 5 cm gap",
+           "triangles": scene.n_tris, "spp_per_render": spp}
+    guide = GuideMLP(ctx, guide_mlp.init_params(3).astype(np.float32), 65536)
+    variants = [("plain", None, 0.0), ("no_network", guide, 0.0), ("guided", guide, 0.5)]
+    for name, g, sel in variants:
+        imgs, secs, rays = [], 0.0, 0
+        if g is not None:
+            g.set_params(guide_mlp.init_params(3).astype(np.float32))
+        for seed in (31, 32):
+            if g is None:
+                gi = GpuPathIntegrator(ctx, scene.camera, spp=spp, max_depth=10, mode=MODE_CORRECTED, seed=seed)
+            else:
+                gi = GuidedPathIntegrator(ctx, scene.camera, spp=spp, max_depth=10, guide=g, seed=seed,
+                                          select_prob=sel, steps=8, train=sel > 0)
+            film = Film(ctx, W, H)
+            torch_sync()
+            t0 = time.perf_counter()
+            for k in range(0, spp, 2):  # passes of 2 samples per pixel (4.1M paths)
+                st = gi.render_into(bvh, film, spp=min(2, spp - k), first_sample=k)
+                rays += st.rays
+            secs += time.perf_counter() - t0
+            imgs.append(film.image())
+            film.close()
+        v = rel_mse_pair(imgs[0], imgs[1]) * spp
+        t_spp = secs / (2 * spp)
+        out[name] = {"v_relmse_x_spp": round(v, 3), "seconds_per_spp": round(t_spp, 4),
+                     "Mrays_s": round(rays / secs / 1e6, 1),
+                     "predicted_seconds_to_relmse_0.02": round(v * t_spp / 0.02, 1)}
+        log(f"guiding leg {name}: v {v:.2f}, {t_spp * 1e3:.1f} ms/spp, {rays / secs / 1e6:.0f} Mrays/s")
+    p, gd = out["plain"]["predicted_seconds_to_relmse_0.02"], out["guided"]["predicted_seconds_to_relmse_0.02"]
+    out["speedup_guided_vs_plain"] = round(p / gd, 3) if gd > 0 else None
+    bvh.close()
+    return out
+
+
+def torch_sync() -> None:
+    import torch
+    torch.cuda.synchronize()
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +226,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--render-spp", type=int, default=64, help="path-tracer leg samples per pixel (0: skip)")
+    ap.add_argument("--guide-spp", type=int, default=32, help="guiding leg samples per render (0: skip)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=-1)
@@ -224,6 +339,10 @@ def main() -> int:
             cpu["parity"] = True
 
     b, s_, c, v_ = ctx.launch()
+    hits = coh = inc = None
+    torch.cuda.empty_cache()
+    pt = path_tracer_leg(ctx, args.render_spp) if args.render_spp > 0 else None
+    gl = guiding_leg(ctx, args.guide_spp) if args.guide_spp > 0 else None
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -244,6 +363,8 @@ def main() -> int:
         "breakdown": breakdown,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "path_tracer": pt,
+        "guiding": gl,
     }
     print(json.dumps(out), flush=True)
     return 0

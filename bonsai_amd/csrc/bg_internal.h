@@ -138,8 +138,11 @@ struct PathBuffers {
     // path guiding (allocated only for guided renders)
     size_t guide_cap = 0, rec_cap = 0;
     void* guide_mem = nullptr;
-    float4* gin = nullptr;     // 3 per path: network input (p, wo, n) at the current hit
-    float* gout = nullptr;     // 40 per path: network output
+    float4* gin = nullptr;     // 3 per live path (compacted): network input (p, wo, n) at the current hit
+    float* gout = nullptr;     // 40 per live path (compacted): network output
+    uint32_t* gslot = nullptr; // per path: its slot in gin/gout this bounce, ~0 when not live
+    uint32_t* gcount = nullptr; // live-path count of this bounce (device)
+    uint32_t* gblk = nullptr;   // per 256 paths: live count, then its exclusive prefix
     float4* rec_in = nullptr;  // 3 per training record: the input the direction was sampled with
     float4* rec_tgt = nullptr; // per record: sampled direction, then (after the pass) its weight in .w
     float4* rec_aux = nullptr; // 2 per record: throughput before scattering, radiance after the vertex
@@ -169,6 +172,7 @@ struct RenderParamsDev {
     // the BSDF; every rec_stride-th path (up to rec_paths of them) records its vertices
     uint32_t guided = 0;
     float guide_alpha = 0.0f;
+    uint32_t guide_lobes = 3;  // bit 0: guide at diffuse vertices, bit 1: at glossy vertices
     uint32_t rec_stride = 1, rec_paths = 0;
 };
 
@@ -214,8 +218,10 @@ struct GuideScratch {
     uint64_t cap = 0, chunk = 2048;
 };
 hipError_t guide_refresh(const GuideDevice& g, hipStream_t s);
+// n_dev (optional) holds the sample count on the device; n then only bounds the grid.
+// mixture: write 10-float4 mixture records (dev_guide.h) instead of the 40 raw outputs.
 hipError_t guide_infer(const GuideDevice& g, const GuideBox& box, const float4* in, uint64_t n, float* out, int grid,
-                       hipStream_t s);
+                       hipStream_t s, const uint32_t* n_dev = nullptr, bool mixture = false);
 hipError_t guide_train_step(const GuideDevice& g, const GuideBox& box, const float4* in, const float4* target,
                             uint64_t n, const GuideScratch& sc, float lr, int t, int grid, float* grad_out,
                             hipStream_t s);

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include "bg_internal.h"
+#include "dev_guide.h"
 
 namespace bg {
 namespace {
@@ -135,10 +136,16 @@ __device__ void load_weights_lds(const uint16_t* __restrict__ wg, uint16_t* __re
 
 // ------------------------------------------------------------- inference
 // in: 3 float4 per sample (p, wo, n); out: 40 floats per sample.
+// MIX: write the 10-float4 mixture record of each sample (dev_guide.h) instead
+// of its 40 raw outputs.
+template <bool MIX>
 __global__ __launch_bounds__(256) void k_guide_infer(GuideDevice g, GuideBox box, const float4* __restrict__ in,
-                                                     uint64_t n, float* __restrict__ out) {
+                                                     uint64_t n, const uint32_t* __restrict__ n_dev,
+                                                     float* __restrict__ out) {
     __shared__ uint16_t wl[kLayers * kLayerLds];
     __shared__ float tile[4][32][65];
+    if (n_dev) n = *n_dev;
+    if ((uint64_t)blockIdx.x * 128 >= n) return;  // the whole block is idle (uniform)
     load_weights_lds(g.wbf, wl);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t groups = (n + 31) / 32;
@@ -159,8 +166,15 @@ __global__ __launch_bounds__(256) void k_guide_infer(GuideDevice g, GuideBox box
         for (int mb = 0; mb < 2; ++mb)
             for (int r = 0; r < 16; ++r) tile[wave][lane & 31][acc_row(mb, r, lane)] = acc[mb][r];
         __syncthreads();
-        if (lane < 32 && valid)
-            for (int k = 0; k < kOut; ++k) out[i * kOut + k] = tile[wave][lane][k];
+        if (lane < 32 && valid) {
+            if (MIX) {
+                float o[kOut];
+                for (int k = 0; k < kOut; ++k) o[k] = tile[wave][lane][k];
+                dev::guide_mixture(o, reinterpret_cast<float4*>(out) + dev::kMixRecord * i);
+            } else {
+                for (int k = 0; k < kOut; ++k) out[i * kOut + k] = tile[wave][lane][k];
+            }
+        }
         __syncthreads();
     }
 }
@@ -398,11 +412,12 @@ hipError_t guide_refresh(const GuideDevice& g, hipStream_t s) {
 }
 
 hipError_t guide_infer(const GuideDevice& g, const GuideBox& box, const float4* in, uint64_t n, float* out,
-                       int grid, hipStream_t s) {
+                       int grid, hipStream_t s, const uint32_t* n_dev, bool mixture) {
     if (n == 0) return hipSuccess;
     const uint64_t need = (n + 127) / 128;
     const int gr = (int)(need < (uint64_t)grid ? need : (uint64_t)grid);
-    hipLaunchKernelGGL(k_guide_infer, dim3(gr), dim3(256), 0, s, g, box, in, n, out);
+    if (mixture) hipLaunchKernelGGL(k_guide_infer<true>, dim3(gr), dim3(256), 0, s, g, box, in, n, n_dev, out);
+    else hipLaunchKernelGGL(k_guide_infer<false>, dim3(gr), dim3(256), 0, s, g, box, in, n, n_dev, out);
     return hipGetLastError();
 }
 

@@ -12,21 +12,30 @@
 // so the film is deterministic without atomics.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "bg_internal.h"
 #include "dev_camera.h"
 #include "dev_guide.h"
+#include "dev_scan.h"
 #include "dev_shade.h"
 
 namespace bg {
 namespace {
 
 using namespace dev;
+using dev::block_exclusive;
 
 constexpr uint32_t kAlive = 1u, kSpecFlag = 2u;
 // nee_a.w flags
 constexpr uint32_t kHasNee = 1u, kHasLight = 2u, kHasBsdf = 4u;
 
 __device__ __forceinline__ uint64_t key_of(uint2 k) { return (uint64_t)k.x | ((uint64_t)k.y << 32); }
+
+// A path the network is evaluated for at this bounce: alive, with a hit.
+__device__ __forceinline__ bool guide_live(const PathBuffers& b, uint64_t i) {
+    return (b.state[i].y & kAlive) && b.hit[i].x != 0xFFFFFFFFu;
+}
 
 // Training-record slot of path i at this bounce: every rec_stride-th path up to
 // rec_paths of them records one slot per bounce < max_depth, path-major.
@@ -111,6 +120,7 @@ __global__ __launch_bounds__(256) void k_init_paths(RenderParamsDev p, PathBuffe
     b.film[i] = make_float2(fx, fy);
 }
 
+template <bool GUIDED>
 __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev sc, PathBuffers b, uint32_t bounce) {
     const uint64_t i = (uint64_t)blockIdx.x * 128 + threadIdx.x;
     if (i >= p.n_paths) return;
@@ -158,11 +168,24 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
     const V rd = mkv(r1.x, r1.y, r1.z);
 
     // path.rs:78-86
-    if (bounce == 0 || (flags & kSpecFlag)) {
-        const int32_t li = sc.prim_light[prim];
-        if (li >= 0) L = L + beta * emission(sc.lights[li], s.n, -rd);
+    const int32_t hit_light = sc.prim_light[prim];
+    if (hit_light >= 0) {
+        if (bounce == 0 || (flags & kSpecFlag)) {
+            L = L + beta * emission(sc.lights[hit_light], s.n, -rd);
+        } else if (GUIDED) {
+            // guided estimator: emission reached by the continuation, MIS-weighted against light
+            // sampling (power heuristic; both pdfs in solid angle, light pdf with its 1/n_lights pick)
+            const LightDev& lt = sc.lights[hit_light];
+            const float cosl = __builtin_fabsf(dot(s.n, rd));
+            const float pc = l4.w;  // continuation pdf of the previous vertex
+            if (cosl > 0.0f && pc > 0.0f) {
+                const float pl = t * t / (cosl * lt.area) / (float)sc.n_lights;
+                L = L + beta * emission(lt, s.n, -rd) * power_heuristic(pc, pl);
+            }
+        }
     }
     bool alive = bounce < p.max_depth;
+    float cont_pdf = 0.0f;
     if (alive) {
         // matte.rs:20-41 / conductor; bsdf.rs:33-52
         const MaterialDev& m = sc.mats[sc.prim_mat[prim]];
@@ -186,6 +209,15 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
             bs.lobe.k = C3{m.k[0], m.k[1], m.k[2]};
         }
         const uint32_t nonspec = kAll & ~kSpecular;
+        // network mixture of this vertex (guided estimator with guide_alpha > 0)
+        const float4* gm = nullptr;
+        float alpha = 0.0f;
+        const bool lobe_ok = (bs.lobe.type & kDiffuse) ? (p.guide_lobes & 1u) : (p.guide_lobes & 2u);
+        const uint32_t gslot = GUIDED ? b.gslot[i] : 0xFFFFFFFFu;
+        if (GUIDED && p.guide_alpha > 0.0f && lobe_ok && !(bs.lobe.type & kSpecular) && gslot != 0xFFFFFFFFu) {
+            gm = reinterpret_cast<const float4*>(b.gout) + kMixRecord * (uint64_t)gslot;
+            if (guide_ok(gm)) alpha = p.guide_alpha;
+        }
         if (bs.ncomp(nonspec) > 0 && sc.n_lights > 0) {
             // integrator.rs:309-328
             const float nl = (float)sc.n_lights;
@@ -193,7 +225,8 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
             const uint32_t idx = (uint32_t)rmin(u * nl, nl - 1.0f);
             const float light_pdf = 1.0f / nl;
             const float ul0 = draw(key, dim), ul1 = draw(key, dim);
-            const float us0 = draw(key, dim), us1 = draw(key, dim);
+            // the guided estimator has no separate BSDF-sampling branch (its continuation is that branch)
+            const float us0 = GUIDED ? 0.0f : draw(key, dim), us1 = GUIDED ? 0.0f : draw(key, dim);
             const LightDev& light = sc.lights[idx];
             uint32_t nee = kHasNee;
             // diffuse.rs:134-167: sample a point on the light's triangle (triangle.rs:522-559)
@@ -235,14 +268,20 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
                     const V o = offset_ray_origin(s.p, s.perr, s.n, lp - s.p);
                     const V tgt = offset_ray_origin(lp, lerr, ln, o - lp);
                     write_ray(b.sray, i, o, 1.0f - 0.0001f, tgt - o, s.time);
-                    const float w = power_heuristic(lpdf, scattering_pdf);
+                    float w;
+                    if (GUIDED) {
+                        const float pc = (1.0f - alpha) * scattering_pdf + (alpha > 0.0f ? alpha * guide_pdf(gm, lwi) : 0.0f);
+                        w = power_heuristic(lpdf * light_pdf, pc);
+                    } else {
+                        w = power_heuristic(lpdf, scattering_pdf);
+                    }
                     term = f * lrad * w / lpdf;
                     nee |= kHasLight;
                     nrays += 0x10001u;
                 }
             }
             // BSDF-sampling branch (integrator.rs:379-434)
-            const BxS bsm = bs.sample(s.wo, us0, us1, nonspec);
+            const BxS bsm = GUIDED ? BxS{mkv(0, 0, 0), mkc(0.0f), 0.0f, 0} : bs.sample(s.wo, us0, us1, nonspec);
             const C3 fb = bsm.f * __builtin_fabsf(dot(bsm.wi, bs.ns));
             const float spdf = ref ? scattering_pdf : bsm.pdf;
             const V mdir = ref ? lwi : bsm.wi;  // the reference traces the light-sample direction
@@ -268,14 +307,12 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
         C3 fs;
         float pdf;
         uint32_t stype = 0;
-        if (p.guided) {
+        if (GUIDED) {
             // the BSDF draws come first, so with guide_alpha == 0 the streams match the plain integrator's
             const float u0 = draw(key, dim), u1 = draw(key, dim);
             const float uc = p.guide_alpha > 0.0f ? draw(key, dim) : 1.0f;
             const float ul = p.guide_alpha > 0.0f ? draw(key, dim) : 0.0f;
-            GuideMix gm;
-            const bool gok = guide_mix(b.gout + 40 * i, &gm);
-            const float a = (gok && !(bs.lobe.type & kSpecular)) ? p.guide_alpha : 0.0f;
+            const float a = alpha;
             float pb;
             if (uc < a) {
                 wi = guide_sample(gm, ul, u0, u1);
@@ -289,8 +326,9 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
                 stype = bsc.type;
             }
             pdf = (1.0f - a) * pb + (a > 0.0f ? a * guide_pdf(gm, wi) : 0.0f);
-            if (recording && !black(fs) && pdf > 0.0f) {
-                const float4* gi = b.gin + 3 * i;
+            cont_pdf = pdf;
+            if (recording && !black(fs) && pdf > 0.0f && gslot != 0xFFFFFFFFu) {
+                const float4* gi = b.gin + 3 * (uint64_t)gslot;
                 b.rec_in[3 * rec] = gi[0];
                 b.rec_in[3 * rec + 1] = gi[1];
                 b.rec_in[3 * rec + 2] = gi[2];
@@ -316,8 +354,9 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
                 if (draw(key, dim) < q) alive = false;
                 else beta = beta / (1.0f - q);
             }
-            // At bounce + 1 == max_depth the next intersection can add nothing (no specular lobes), so stop.
-            if (alive && bounce + 1 >= p.max_depth) alive = false;
+            // At bounce + 1 == max_depth the next intersection can add nothing (no specular lobes), so
+            // stop; the guided estimator still adds that intersection's MIS-weighted emission.
+            if (alive && bounce + 1 >= p.max_depth && !GUIDED) alive = false;
             if (alive) {
                 if (ref) {
                     b.ray[2 * i].w = t;  // Q1: the camera ray is re-intersected with t_max = t_hit
@@ -333,7 +372,7 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
     if (!alive) kill_ray(b.ray, i);
     else nrays += 1u;
     b.beta[i] = make_float4(beta.r, beta.g, beta.b, __uint_as_float(nrays));
-    b.L[i] = make_float4(L.r, L.g, L.b, 0.0f);
+    b.L[i] = make_float4(L.r, L.g, L.b, cont_pdf);  // .w: continuation pdf, for the next vertex's MIS
     b.state[i] = make_uint2(dim, alive ? (flags | kAlive) : 0u);
 }
 
@@ -370,7 +409,7 @@ __global__ __launch_bounds__(256) void k_nee_resolve(RenderParamsDev p, ShadeSce
     const C3 ld = out / c.w;
     const float4 l4 = b.L[i];
     const C3 L = C3{l4.x, l4.y, l4.z} + C3{c.x, c.y, c.z} * ld;
-    b.L[i] = make_float4(L.r, L.g, L.b, 0.0f);
+    b.L[i] = make_float4(L.r, L.g, L.b, l4.w);
     uint64_t rec;
     if (rec_slot(p, i, bounce, &rec) && b.rec_tgt[rec].w != 0.0f) b.rec_aux[2 * rec + 1] = make_float4(L.r, L.g, L.b, 0.0f);
 }
@@ -378,11 +417,21 @@ __global__ __launch_bounds__(256) void k_nee_resolve(RenderParamsDev p, ShadeSce
 // Network input at each live path's hit: position, outgoing direction and the
 // geometric normal turned toward it.
 __global__ __launch_bounds__(256) void k_guide_inputs(RenderParamsDev p, ShadeSceneDev sc, PathBuffers b) {
+    __shared__ uint32_t wt[4];
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool live = i < p.n_paths && guide_live(b, i);
+    uint32_t total;
+    const uint32_t rank = block_exclusive(live ? 1u : 0u, wt, &total);
     if (i >= p.n_paths) return;
-    float4 gp = make_float4(0.0f, 0.0f, 0.0f, 0.0f), gw = make_float4(0.0f, 0.0f, 1.0f, 0.0f), gn = gw;
+    if (!live) {
+        b.gslot[i] = 0xFFFFFFFFu;
+        return;
+    }
     const uint4 h = b.hit[i];
-    if ((b.state[i].y & kAlive) && h.x != 0xFFFFFFFFu) {
+    const uint32_t slot = b.gblk[blockIdx.x] + rank;  // compacted, in path order
+    b.gslot[i] = slot;
+    float4 gp, gw, gn;
+    {
         const float4 r0 = b.ray[2 * i], r1 = b.ray[2 * i + 1];
         const float t = __uint_as_float(h.y);
         const V d = mkv(r1.x, r1.y, r1.z);
@@ -403,9 +452,9 @@ __global__ __launch_bounds__(256) void k_guide_inputs(RenderParamsDev p, ShadeSc
         gw = make_float4(-d.x, -d.y, -d.z, 0.0f);
         gn = make_float4(n.x, n.y, n.z, 0.0f);
     }
-    b.gin[3 * i] = gp;
-    b.gin[3 * i + 1] = gw;
-    b.gin[3 * i + 2] = gn;
+    b.gin[3 * (uint64_t)slot] = gp;
+    b.gin[3 * (uint64_t)slot + 1] = gw;
+    b.gin[3 * (uint64_t)slot + 2] = gn;
 }
 
 // After a pass: the training weight of each recorded direction,
@@ -435,12 +484,17 @@ __global__ __launch_bounds__(256) void k_guide_targets(RenderParamsDev p, PathBu
 
 // Sums the per-path ray counts of a pass into counters[0] (all queries) and
 // counters[1] (any-hit queries): one atomic per block.
+// Sums the per-path ray counts of a pass into counters[0] (all queries) and
+// counters[1] (any-hit queries): grid-stride partial sums, two atomics per block.
 __global__ __launch_bounds__(256) void k_count_rays(uint64_t n, const float4* __restrict__ beta,
                                                     unsigned long long* __restrict__ counters) {
     __shared__ unsigned long long part[2][4];
-    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    const uint32_t c = i < n ? __float_as_uint(beta[i].w) : 0u;
-    unsigned long long all = c & 0xFFFFu, shadow = c >> 16;
+    unsigned long long all = 0, shadow = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+        const uint32_t c = __float_as_uint(beta[i].w);
+        all += c & 0xFFFFu;
+        shadow += c >> 16;
+    }
     for (int off = 32; off > 0; off >>= 1) {
         all += __shfl_down(all, off);
         shadow += __shfl_down(shadow, off);
@@ -451,6 +505,45 @@ __global__ __launch_bounds__(256) void k_count_rays(uint64_t n, const float4* __
     if (threadIdx.x == 0) {
         atomicAdd(&counters[0], part[0][0] + part[0][1] + part[0][2] + part[0][3]);
         atomicAdd(&counters[1], part[1][0] + part[1][1] + part[1][2] + part[1][3]);
+    }
+}
+
+// Live paths of this bounce (alive, with a hit), compacted in path order for the
+// network: per-block counts, one-block exclusive scan, then k_guide_inputs
+// writes each live path at its block's offset plus its rank in the block.
+
+__global__ __launch_bounds__(256) void k_live_count(uint64_t n, PathBuffers b) {
+    __shared__ uint32_t wt[4];
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    uint32_t total;
+    block_exclusive(i < n && guide_live(b, i) ? 1u : 0u, wt, &total);
+    if (threadIdx.x == 0) b.gblk[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_live_scan(uint32_t nblk, uint32_t* __restrict__ blk, uint32_t* __restrict__ total) {
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (nblk + 1023) / 1024;
+    const uint32_t b0 = t * per, b1 = min(b0 + per, nblk);
+    uint32_t sum = 0;
+    for (uint32_t k = b0; k < b1; ++k) sum += blk[k];
+    part[t] = sum;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t run = 0;
+        for (int k = 0; k < 1024; ++k) {
+            const uint32_t v = part[k];
+            part[k] = run;
+            run += v;
+        }
+        *total = run;
+    }
+    __syncthreads();
+    uint32_t run = part[t];
+    for (uint32_t k = b0; k < b1; ++k) {
+        const uint32_t v = blk[k];
+        blk[k] = run;
+        run += v;
     }
 }
 
@@ -508,7 +601,10 @@ hipError_t launch_init_paths(const RenderParamsDev& p, const PathBuffers& b, hip
 hipError_t launch_shade(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b, uint32_t bounce,
                         hipStream_t s) {
     if (p.n_paths == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_shade, dim3((uint32_t)((p.n_paths + 127) / 128)), dim3(128), 0, s, p, sc, b, bounce);
+    if (p.guided)
+        hipLaunchKernelGGL(k_shade<true>, dim3((uint32_t)((p.n_paths + 127) / 128)), dim3(128), 0, s, p, sc, b, bounce);
+    else
+        hipLaunchKernelGGL(k_shade<false>, dim3((uint32_t)((p.n_paths + 127) / 128)), dim3(128), 0, s, p, sc, b, bounce);
     return hipGetLastError();
 }
 hipError_t launch_nee_resolve(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b,
@@ -520,12 +616,15 @@ hipError_t launch_nee_resolve(const RenderParamsDev& p, const ShadeSceneDev& sc,
 hipError_t launch_count_rays(const RenderParamsDev& p, const PathBuffers& b, unsigned long long* counters,
                              hipStream_t s) {
     if (p.n_paths == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_count_rays, dim3((uint32_t)((p.n_paths + 255) / 256)), dim3(256), 0, s, p.n_paths, b.beta,
-                       counters);
+    const uint64_t blocks = std::min<uint64_t>((p.n_paths + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_count_rays, dim3((uint32_t)blocks), dim3(256), 0, s, p.n_paths, b.beta, counters);
     return hipGetLastError();
 }
 hipError_t launch_guide_inputs(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b, hipStream_t s) {
     if (p.n_paths == 0) return hipSuccess;
+    const uint32_t nblk = (uint32_t)((p.n_paths + 255) / 256);
+    hipLaunchKernelGGL(k_live_count, dim3(nblk), dim3(256), 0, s, p.n_paths, b);
+    hipLaunchKernelGGL(k_live_scan, dim3(1), dim3(1024), 0, s, nblk, b.gblk, b.gcount);
     hipLaunchKernelGGL(k_guide_inputs, dim3((uint32_t)((p.n_paths + 255) / 256)), dim3(256), 0, s, p, sc, b);
     return hipGetLastError();
 }

@@ -79,13 +79,14 @@ int ensure_guide_bufs(bg_ctx* ctx, size_t cap, size_t rec_cap) {
     b.guide_cap = b.rec_cap = 0;
     cap = std::max(cap, (size_t)1);
     rec_cap = std::max(rec_cap, (size_t)1);
-    const size_t sizes[] = {48 * cap, 160 * cap, 48 * rec_cap, 16 * rec_cap, 32 * rec_cap, 4 * (rec_cap / 256 + 1)};
+    const size_t sizes[] = {48 * cap, 160 * cap, 48 * rec_cap, 16 * rec_cap, 32 * rec_cap, 4 * (rec_cap / 256 + 1),
+                            4 * cap, 256, 4 * (cap / 256 + 1)};
     size_t total = 0;
     for (size_t z : sizes) total += ((z + 255) / 256) * 256;
     HIP_TRY(ctx, hipMalloc(&b.guide_mem, total));
     char* q = (char*)b.guide_mem;
-    void* parts[6];
-    for (int k = 0; k < 6; ++k) {
+    void* parts[9];
+    for (int k = 0; k < 9; ++k) {
         parts[k] = q;
         q += ((sizes[k] + 255) / 256) * 256;
     }
@@ -95,6 +96,9 @@ int ensure_guide_bufs(bg_ctx* ctx, size_t cap, size_t rec_cap) {
     b.rec_tgt = (float4*)parts[3];
     b.rec_aux = (float4*)parts[4];
     b.rec_blk = (uint32_t*)parts[5];
+    b.gslot = (uint32_t*)parts[6];
+    b.gcount = (uint32_t*)parts[7];
+    b.gblk = (uint32_t*)parts[8];
     b.guide_cap = cap;
     b.rec_cap = rec_cap;
     return BG_OK;
@@ -234,6 +238,7 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
         if (!(gd->select_prob >= 0.0f && gd->select_prob <= 1.0f)) return fail(ctx, BG_ERR_INVALID, "select_prob outside [0, 1]");
         p.guided = 1;
         p.guide_alpha = gd->select_prob;
+        p.guide_lobes = gd->lobes ? gd->lobes : 3u;
         if (gd->train) {
             train_paths = gd->train_paths ? gd->train_paths : (1u << 20);
             batch = std::min<uint64_t>(gd->batch ? gd->batch : 16384, guide->scratch.cap);
@@ -285,17 +290,25 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
         HIP_TRY(ctx, bg::launch_init_paths(p, b, st));
         for (uint32_t bounce = 0; bounce <= d->max_depth; ++bounce) {
             HIP_TRY(ctx, bg::launch_traverse(traverse_args(s, ctx, b.ray, b.hit, p.n_paths), false, false, ctx->shape, g, st));
-            if (guided && bounce < d->max_depth) {
-                HIP_TRY(ctx, bg::launch_guide_inputs(p, sc, b, st));
-                HIP_TRY(ctx, bg::guide_infer(guide->dev, guide->box, b.gin, p.n_paths, b.gout, guide_grid, st));
-                launches += 2;
+            if (guided && bounce < d->max_depth && (p.guide_alpha > 0.0f || train_paths)) {
+                HIP_TRY(ctx, bg::launch_guide_inputs(p, sc, b, st));  // the records copy these inputs
+                launches += 3;
+                if (p.guide_alpha > 0.0f) {  // the live paths only (count on the device)
+                    HIP_TRY(ctx, bg::guide_infer(guide->dev, guide->box, b.gin, p.n_paths, b.gout, guide_grid, st,
+                                                 b.gcount, true));
+                    ++launches;
+                }
             }
             HIP_TRY(ctx, bg::launch_shade(p, sc, b, bounce, st));
             if (sc.n_lights > 0 && bounce < d->max_depth) {
                 HIP_TRY(ctx, bg::launch_traverse(traverse_args(s, ctx, b.sray, b.shit, p.n_paths), true, false, ctx->shape, g, st));
-                HIP_TRY(ctx, bg::launch_traverse(traverse_args(s, ctx, b.mray, b.mhit, p.n_paths), false, false, ctx->shape, g, st));
+                // the guided estimator has no separate MIS ray: its continuation plays that part
+                if (!guided) {
+                    HIP_TRY(ctx, bg::launch_traverse(traverse_args(s, ctx, b.mray, b.mhit, p.n_paths), false, false, ctx->shape, g, st));
+                    ++launches;
+                }
                 HIP_TRY(ctx, bg::launch_nee_resolve(p, sc, b, bounce, st));
-                launches += 3;
+                launches += 2;
             }
             launches += 2;
         }

@@ -7,26 +7,13 @@
 #include <hip/hip_runtime.h>
 
 #include "bg_internal.h"
+#include "dev_scan.h"
 
 namespace bg {
 namespace {
 
-constexpr int kBlk = 256;
-
-__device__ __forceinline__ uint32_t block_exclusive(uint32_t flag, uint32_t* warp_tot, uint32_t* total) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const unsigned long long m = __ballot(flag);
-    const uint32_t below = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-    if (lane == 0) warp_tot[wave] = (uint32_t)__popcll(m);
-    __syncthreads();
-    uint32_t off = 0, t = 0;
-    for (int w = 0; w < kBlk / 64; ++w) {
-        if (w < wave) off += warp_tot[w];
-        t += warp_tot[w];
-    }
-    *total = t;
-    return off + below;
-}
+using dev::block_exclusive;
+constexpr int kBlk = dev::kScanBlock;
 
 __global__ __launch_bounds__(kBlk) void k_replay_count(uint64_t n, const float4* __restrict__ tgt, uint32_t* __restrict__ blk) {
     __shared__ uint32_t wt[kBlk / 64];

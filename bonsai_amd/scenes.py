@@ -334,18 +334,26 @@ AL_K = (9.224, 6.270, 4.837)
 
 
 def door_gap_room(width: int = 1920, height: int = 1080, cell: float = 0.0179, seed: int = 5,
-                  L: float = 3000.0) -> SceneData:
+                  L: float = 3000.0, wall_scale: float = 1.0, matte_walls: bool = False,
+                  light_up: bool = False) -> SceneData:
     """Hard indirect interior: a 5 x 3 x 6 m room lit only through a 5 cm gap beside a nearly closed
     door from a neighbouring 2.9 x 3 x 6 m room whose ceiling holds the single quad area light
     (1 x 1 m, radiance L, facing down). The camera room's walls are glossy Trowbridge-Reitz
     conductors (alpha 0.1, 0.2, 0.3), its floor and ceiling matte; two displaced icospheres stand in
     it. Every surface is tessellated into cells of at most `cell` metres (0.0179 -> 1,513,386
-    triangles; larger cells give small test scenes with the same lighting)."""
+    triangles; larger cells give small test scenes with the same lighting). `wall_scale` scales the
+    glossy walls' reflectance; `matte_walls` makes them Lambertian instead. `light_up` turns the
+    light toward the light room's ceiling, 15 cm below it."""
     s = SceneData()
+    ws = wall_scale
+
+    def wall(alpha, r):
+        return matte(*r) if matte_walls else conductor(alpha, AL_ETA, AL_K, tuple(ws * c for c in r))
+
     s.materials = [matte(0.6, 0.6, 0.58),                               # 0 floor / ceiling (camera room)
-                   conductor(0.1, AL_ETA, AL_K, (0.75, 0.72, 0.68)),    # 1 glossy wall
-                   conductor(0.2, AL_ETA, AL_K, (0.7, 0.75, 0.72)),     # 2 glossy wall
-                   conductor(0.3, AL_ETA, AL_K, (0.72, 0.7, 0.75)),     # 3 glossy wall / partition
+                   wall(0.1, (0.75, 0.72, 0.68)),                       # 1 glossy wall
+                   wall(0.2, (0.7, 0.75, 0.72)),                        # 2 glossy wall
+                   wall(0.3, (0.72, 0.7, 0.75)),                        # 3 glossy wall / partition
                    matte(0.8, 0.78, 0.75),                              # 4 light room
                    matte(0.45, 0.3, 0.2),                               # 5 door
                    matte(0.0, 0.0, 0.0),                                # 6 light
@@ -386,7 +394,10 @@ def door_gap_room(width: int = 1920, height: int = 1080, cell: float = 0.0179, s
     add(icosphere(lv, (2.2, 0.6, 3.8), 0.6, 0.1, 3.0, seed), 7)
     add(icosphere(lv, (3.4, 0.45, 1.6), 0.45, 0.1, 3.0, seed + 1), 8)
     base = sum(len(t) for t in parts)
-    add(quad((6.0, 2.95, 2.5), (7.0, 2.95, 2.5), (7.0, 2.95, 3.5), (6.0, 2.95, 3.5)), 6)
+    if light_up:  # facing the light room's ceiling: the camera room gets no direct light at all
+        add(quad((6.0, 2.85, 2.5), (6.0, 2.85, 3.5), (7.0, 2.85, 3.5), (7.0, 2.85, 2.5)), 6)
+    else:
+        add(quad((6.0, 2.95, 2.5), (7.0, 2.95, 2.5), (7.0, 2.95, 3.5), (6.0, 2.95, 3.5)), 6)
     s.tri_positions = np.concatenate(parts).astype(np.float32)
     s.tri_material = np.concatenate(mats)
     s.lights = [area_light(base, (L, L, L)), area_light(base + 1, (L, L, L))]

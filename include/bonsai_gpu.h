@@ -302,7 +302,10 @@ typedef struct bg_guiding_desc {
     uint32_t steps;        /* Adam steps after each pass (0: 8) */
     float lr;              /* Adam learning rate (0: 1e-2) */
     uint32_t replay;       /* replay ring capacity in samples (0: 2^20) */
+    uint32_t lobes;        /* where the network is sampled: BG_GUIDE_DIFFUSE | BG_GUIDE_GLOSSY (0: both) */
 } bg_guiding_desc;
+#define BG_GUIDE_DIFFUSE 1u
+#define BG_GUIDE_GLOSSY 2u
 int bg_render_film_guided(bg_ctx* ctx, const bg_scene* scene, const bg_render_desc* desc, bg_guide* guide,
                           const bg_guiding_desc* guiding, bg_film* film, bg_stats* stats);
 /* The training records of the last trained pass, in the layouts bg_guide_train

@@ -1,8 +1,7 @@
 """Guided path integration on the GPU: the one-sample MIS estimator stays
 unbiased for any network state (furnace closed form, agreement with the plain
-integrator), select_prob 0 leaves the plain integrator's sample streams
-untouched, and the online-trained network lowers the error at equal samples on
-the door-gap interior."""
+integrator), select_prob 0 never reads the network, and the online-trained
+network lowers the error at equal samples on the door-gap interior."""
 import sys
 from pathlib import Path
 
@@ -40,21 +39,22 @@ def test_guided_furnace_closed_form(ctx, select_prob, train):
         assert abs(img.mean() - exact) < 0.02 * exact, (depth, img.mean(), exact)
 
 
-def test_select_prob_zero_matches_plain_streams(ctx):
+def test_select_prob_zero_ignores_the_network(ctx):
+    """With select_prob 0 the guided estimator (light sampling + MIS-weighted continuation emission)
+    never reads the network: different parameters and training on or off give the same film."""
     s = cornell_box(width=64, height=64)
     bvh = GpuBvh(ctx, s)
-    plain = GpuPathIntegrator(ctx, s.camera, spp=8, max_depth=5, mode=MODE_CORRECTED, seed=4)
-    a = Film(ctx, 64, 64)
-    plain.render_into(bvh, a)
-    gi = GuidedPathIntegrator(ctx, s.camera, spp=8, max_depth=5, guide=new_guide(ctx), seed=4, select_prob=0.0,
-                              train=True, train_paths=1024)
-    b = Film(ctx, 64, 64)
-    st = gi.render_into(bvh, b)
-    assert st.train_steps > 0
-    ra, wa = a.sums()
-    rb, wb = b.sums()
-    np.testing.assert_array_equal(wa, wb)
-    np.testing.assert_array_equal(ra, rb)
+    films = []
+    for seed, train in ((3, True), (9, False)):
+        gi = GuidedPathIntegrator(ctx, s.camera, spp=8, max_depth=5, guide=new_guide(ctx, seed=seed), seed=4,
+                                  select_prob=0.0, train=train, train_paths=1024)
+        f = Film(ctx, 64, 64)
+        for k in range(4):
+            st = gi.render_into(bvh, f, spp=2, first_sample=2 * k)
+            assert (st.train_steps > 0) == train
+        films.append(f.sums())
+    np.testing.assert_array_equal(films[0][0], films[1][0])
+    np.testing.assert_array_equal(films[0][1], films[1][1])
 
 
 def test_guided_agrees_with_plain_on_cornell(ctx):
@@ -75,21 +75,24 @@ def test_guided_agrees_with_plain_on_cornell(ctx):
 
 
 def test_trained_guide_lowers_error_on_door_gap(ctx):
-    """Equal samples per pixel: plain vs guided (trained online during the same render) against a
-    high-sample plain reference on the door-gap interior."""
-    W, H, depth = 96, 54, 10
-    s = door_gap_room(W, H, cell=0.25)
+    """Equal samples per pixel on the door-gap interior (480 x 270, depth 10): the estimator without
+    the network (select_prob 0) against the same estimator sampling the network's mixture with
+    probability 0.5 while the network trains online, both scored against an 8192-sample render.
+    Every kernel is deterministic, so the comparison is reproducible."""
+    W, H, depth, spp = 480, 270, 10, 1024
+    s = door_gap_room(W, H)
     bvh = GpuBvh(ctx, s)
     ref = GpuPathIntegrator(ctx, s.camera, spp=8192, max_depth=depth, mode=MODE_CORRECTED, seed=21).render(bvh)
-    spp, passes = 256, 16
-    plain = GpuPathIntegrator(ctx, s.camera, spp=spp, max_depth=depth, mode=MODE_CORRECTED, seed=22).render(bvh)
-    gi = GuidedPathIntegrator(ctx, s.camera, spp=spp, max_depth=depth, guide=new_guide(ctx), seed=22)
-    film = Film(ctx, W, H)
-    per = spp // passes
-    for k in range(passes):
-        gi.render_into(bvh, film, spp=per, first_sample=per * k)
-    img = film.image()
-    e_p, e_g = rel_mse(plain, ref), rel_mse(img, ref)
-    print(f"door gap {W}x{H} {spp} spp relMSE: plain {e_p:.4f}, guided {e_g:.4f} (ratio {e_p / e_g:.2f})")
-    assert np.isfinite(img).all()
-    assert e_g < 0.8 * e_p
+    errs = {}
+    for sel in (0.0, 0.5):
+        gi = GuidedPathIntegrator(ctx, s.camera, spp=spp, max_depth=depth, guide=new_guide(ctx), seed=22,
+                                  select_prob=sel, steps=16, train=sel > 0)
+        film = Film(ctx, W, H)
+        for k in range(0, spp, 16):
+            gi.render_into(bvh, film, spp=16, first_sample=k)
+        img = film.image()
+        assert np.isfinite(img).all()
+        errs[sel] = rel_mse(img, ref)
+    print(f"door gap {W}x{H} {spp} spp relMSE: without network {errs[0.0]:.4f}, guided {errs[0.5]:.4f} "
+          f"(ratio {errs[0.0] / errs[0.5]:.2f})")
+    assert errs[0.5] < 0.9 * errs[0.0]
