@@ -19,7 +19,8 @@ Two further objects ride on the same line: `path_tracer` (the plain GPU path
 tracer at 1920 x 1080, 64 spp, depth 8, samples/s and Mrays/s) and `guiding`
 (the door-gap interior at 1920 x 1080, depth 10: per-sample variance and
 seconds per sample of the plain, network-free and neural-guided estimators,
-and the time to relMSE 0.02 they predict).
+and the time to relMSE 0.02 they predict), and `batch` (64 seeded scenes
+rendered by one hipGraph-replayed wavefront loop against one call per scene).
 """
 from __future__ import annotations
 
@@ -154,6 +155,53 @@ def box_blur(img: np.ndarray, r: int) -> np.ndarray:
     return out
 
 
+def batch_leg(ctx, spp: int) -> dict:
+    """64 seeded scenes of uneven size (triangle budgets log-uniform in 1k-500k, square films of side
+    uniform in 128-1024), 16 spp, depth 6, mixed materials: one batched wavefront loop (hipGraph
+    replay of a captured pass, autotuned traversal launch shape) against one render call per scene."""
+    from bonsai_amd import GpuBvh
+    from bonsai_amd.integrator import MODE_CORRECTED, BatchRenderer, Film, GpuPathIntegrator
+    from bonsai_amd.scenes import batch_scene
+
+    scenes = [batch_scene(i) for i in range(64)]
+    bvhs = [GpuBvh(ctx, s) for s in scenes]
+    items = [(b, s.camera, 1000 + k) for k, (b, s) in enumerate(zip(bvhs, scenes))]
+    pixels = sum(s.camera.width * s.camera.height for s in scenes)
+    out = {"workload": "64 scenes, 16 spp, depth 6", "scenes": 64, "triangles": sum(s.n_tris for s in scenes),
+           "pixels": pixels, "spp": spp}
+    br = BatchRenderer(ctx, items)
+    t0 = time.perf_counter()
+    br.render(1, 6, mode=MODE_CORRECTED, autotune=True)  # tunes the launch shape; warms allocations
+    tune_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    st = br.render(spp, 6, mode=MODE_CORRECTED, autotune=False)
+    wall = time.perf_counter() - t0
+    b, s_, c, v = br.launch()
+    out["batched"] = {"seconds": round(wall, 3), "Msamples_s": round(st.samples / wall / 1e6, 2),
+                      "Mrays_s": round(st.rays / wall / 1e6, 1), "launch": {"block": b, "stack": s_,
+                      "blocks_per_cu": c, "variant": v}, "first_call_seconds_incl_autotune": round(tune_s, 2)}
+    br.close()
+    films = [Film(ctx, s.camera.width, s.camera.height) for s in scenes]
+    for (bvh, cam, seed), f in zip(items, films):  # warm-up
+        GpuPathIntegrator(ctx, cam, spp=1, max_depth=6, mode=MODE_CORRECTED, seed=seed).render_into(bvh, f)
+    torch_sync()
+    t0 = time.perf_counter()
+    rays = 0
+    for (bvh, cam, seed), f in zip(items, films):
+        f.clear()
+        rays += GpuPathIntegrator(ctx, cam, spp=spp, max_depth=6, mode=MODE_CORRECTED, seed=seed).render_into(bvh, f).rays
+    wall1 = time.perf_counter() - t0
+    out["per_scene"] = {"seconds": round(wall1, 3), "Msamples_s": round(pixels * spp / wall1 / 1e6, 2),
+                        "Mrays_s": round(rays / wall1 / 1e6, 1)}
+    out["speedup_batched_vs_per_scene"] = round(wall1 / wall, 3)
+    log(f"batch leg: batched {wall:.2f} s, per-scene {wall1:.2f} s")
+    for f in films:
+        f.close()
+    for b_ in bvhs:
+        b_.close()
+    return out
+
+
 def rel_mse_pair(a: np.ndarray, b: np.ndarray) -> float:
     """relMSE estimated from two independent renders of n samples each: per pixel (A - B)^2 / 2 is an
     unbiased estimate of one render's variance, and the squared mean in the denominator is taken
@@ -228,6 +276,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--render-spp", type=int, default=64, help="path-tracer leg samples per pixel (0: skip)")
     ap.add_argument("--guide-spp", type=int, default=32, help="guiding leg samples per render (0: skip)")
+    ap.add_argument("--batch-spp", type=int, default=16, help="scene-batch leg samples per pixel (0: skip)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=-1)
@@ -343,6 +392,7 @@ def main() -> int:
     torch.cuda.empty_cache()
     pt = path_tracer_leg(ctx, args.render_spp) if args.render_spp > 0 else None
     gl = guiding_leg(ctx, args.guide_spp) if args.guide_spp > 0 else None
+    bl = batch_leg(ctx, args.batch_spp) if args.batch_spp > 0 else None
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -365,6 +415,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "path_tracer": pt,
         "guiding": gl,
+        "batch": bl,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -117,6 +117,15 @@ class bg_guiding_desc(C.Structure):
                 ("lobes", C.c_uint32)]
 
 
+class bg_batch_item(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("camera", bg_camera_desc), ("seed", C.c_uint64)]
+
+
+class bg_batch_desc(C.Structure):
+    _fields_ = [("spp", C.c_uint32), ("max_depth", C.c_uint32), ("rr_threshold", C.c_float), ("mode", C.c_uint32),
+                ("paths_per_pass", C.c_uint32), ("use_graph", C.c_uint32), ("autotune", C.c_uint32)]
+
+
 BG_MODE_REF = 0
 BG_MODE_CORRECTED = 1
 BG_FILM_F64 = 1
@@ -161,6 +170,11 @@ SIGNATURES = {
     "bg_render_film_guided": (C.c_int, [P, P, C.POINTER(bg_render_desc), P, C.POINTER(bg_guiding_desc), P,
                                         C.POINTER(bg_stats)]),
     "bg_guide_records": (C.c_int, [P, P, P, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "bg_batch_create": (C.c_int, [P, P, C.c_uint32, C.POINTER(P)]),
+    "bg_batch_destroy": (None, [P]),
+    "bg_render_batch": (C.c_int, [P, P, C.POINTER(bg_batch_desc), C.POINTER(bg_stats)]),
+    "bg_batch_film_read": (C.c_int, [P, C.c_uint32, P]),
+    "bg_batch_launch_config": (C.c_int, [P, C.POINTER(bg_launch_config)]),
     "bgx_icosphere_tri_count": (C.c_uint64, [C.c_int]),
     "bgx_icosphere": (C.c_int, [C.c_int, P, C.c_float, C.c_float, C.c_float, C.c_uint64, P]),
     "bgx_rays_incoherent": (C.c_int, [C.c_uint64, P, P, C.c_uint64, P]),

@@ -1,0 +1,482 @@
+// Scene batches: many scenes, cameras and films rendered by one wavefront loop
+// (no counterpart in the reference, whose SamplerIntegrator::render,
+// base/integrator.rs:35-133, renders one scene per call).
+//
+// The scenes' device data are concatenated once: BVH nodes with their child
+// and leaf offsets rebased, leaf slots mapped to batch-wide primitive ids
+// (every triangle of every scene, then every sphere), shading records with
+// material and light indices rebased. A path then needs only its item (in its
+// flags) and its BVH root (per path) to run the single-scene kernels. One pass
+// (every item's film times samples-per-pass paths) is captured in a hipGraph
+// and replayed; the pass reads its first sample index from device memory and
+// its last node advances it. Before the first pass an autotuner times the
+// traversal launch shapes on the batch's own camera, secondary and shadow
+// rays and keeps the fastest.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "bg_internal.h"
+
+namespace {
+
+int fail(bg_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                      \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail((ctx), BG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+int dalloc(bg_ctx* ctx, T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) return BG_OK;
+    HIP_TRY(ctx, hipMalloc((void**)p, count * sizeof(T)));
+    return BG_OK;
+}
+
+}  // namespace
+
+struct bg_batch {
+    bg_ctx* ctx = nullptr;
+    std::vector<bg_batch_item> items;
+    std::vector<uint32_t> item_scene;  // item -> unique scene index
+    std::vector<uint64_t> item_pix_base;
+    uint64_t total_pix = 0;
+    // concatenated scene data
+    uint64_t n_tris = 0, n_spheres = 0, n_nodes = 0, n_slots = 0;
+    uint32_t n_lights = 0, max_depth = 0;
+    std::vector<uint32_t> scene_root, scene_light_base, scene_n_lights;
+    float4* nodes = nullptr;
+    float* tri_verts = nullptr;
+    uint32_t* slot_prim = nullptr;
+    bg::SphereDev* spheres = nullptr;
+    bg::SphereAux* sphere_aux = nullptr;
+    float* tri_orig = nullptr;
+    uint8_t* tri_flip = nullptr;
+    uint32_t* prim_mat = nullptr;
+    int32_t* prim_light = nullptr;
+    bg::MaterialDev* mats = nullptr;
+    bg::LightDev* lights = nullptr;
+    bg::SphereShadeDev* sph_shade = nullptr;
+    // per render
+    bg::BatchItemDev* d_items = nullptr;
+    float4* film = nullptr;
+    uint32_t* d_pass = nullptr;
+    unsigned long long* d_queue = nullptr;
+    uint32_t* d_root = nullptr;
+    size_t root_cap = 0;
+    bg::LaunchShape shape;
+    bool tuned = false;
+    double tune_ms = 0.0;
+};
+
+namespace {
+
+void free_batch(bg_batch* b) {
+    void* ptrs[] = {b->nodes, b->tri_verts, b->slot_prim, b->spheres, b->sphere_aux, b->tri_orig, b->tri_flip,
+                    b->prim_mat, b->prim_light, b->mats, b->lights, b->sph_shade, b->d_items, b->film,
+                    b->d_pass, b->d_queue, b->d_root};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+}
+
+bg::TraverseArgs batch_args(const bg_batch* bt, const bg_ctx* ctx, const float4* rays, uint4* hits, uint64_t n,
+                            const uint32_t* roots, bool queue) {
+    bg::TraverseArgs a;
+    a.nodes = bt->nodes;
+    a.tri_verts = bt->tri_verts;
+    a.slot_prim = bt->slot_prim;
+    a.spheres = bt->spheres;
+    a.sphere_aux = bt->sphere_aux;
+    a.n_tris = (uint32_t)bt->n_tris;
+    a.rays = rays;
+    a.hits = hits;
+    a.n_rays = n;
+    a.spill = ctx->spill;
+    a.spill_depth = bt->max_depth + 1;
+    a.counters = nullptr;
+    a.roots = roots;
+    a.queue = queue ? bt->d_queue : nullptr;
+    return a;
+}
+
+int grid_for(const bg_ctx* ctx, const bg::LaunchShape& sh, uint64_t n) {
+    const uint64_t tiles = (n + sh.block - 1) / sh.block;
+    const uint64_t cap = sh.blocks_per_cu > 0 ? (uint64_t)ctx->num_cus * sh.blocks_per_cu : tiles;
+    return (int)std::max<uint64_t>(1, std::min<uint64_t>(std::min(tiles, cap), 1u << 20));
+}
+
+hipError_t traverse(const bg_batch* bt, bg_ctx* ctx, const bg::LaunchShape& sh, const float4* rays, uint4* hits,
+                    uint64_t n, const uint32_t* roots, bool any, hipStream_t st) {
+    const bool queue = sh.variant == 2;
+    if (queue) {
+        hipError_t e = bg::launch_zero_u64(bt->d_queue, st);
+        if (e != hipSuccess) return e;
+    }
+    return bg::launch_traverse(batch_args(bt, ctx, rays, hits, n, roots, queue), any, false, sh, grid_for(ctx, sh, n), st);
+}
+
+}  // namespace
+
+extern "C" {
+
+int bg_batch_create(bg_ctx* ctx, const bg_batch_item* items, uint32_t n, bg_batch** out) {
+    if (!ctx || !out || (n && !items)) return fail(ctx, BG_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (n == 0 || n > 65535) return fail(ctx, BG_ERR_INVALID, "a batch holds 1 to 65535 items");
+    std::vector<const bg_scene*> uniq;
+    std::map<const bg_scene*, uint32_t> index;
+    for (uint32_t k = 0; k < n; ++k) {
+        const bg_scene* s = items[k].scene;
+        if (!s || s->ctx != ctx) return fail(ctx, BG_ERR_INVALID, "batch item with a null or foreign scene");
+        if (s->materials.empty()) return fail(ctx, BG_ERR_INVALID, "batch scene without materials");
+        if (s->bvh.nodes.empty()) return fail(ctx, BG_ERR_INVALID, "batch scene without primitives");
+        const bg_camera_desc& c = items[k].camera;
+        if (c.width == 0 || c.height == 0 || c.lens_radius > 0.0f)
+            return fail(ctx, BG_ERR_INVALID, "batch camera: empty film or thin lens");
+        if (!index.count(s)) {
+            index[s] = (uint32_t)uniq.size();
+            uniq.push_back(s);
+        }
+    }
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    bg_batch* bt = new (std::nothrow) bg_batch();
+    if (!bt) return fail(ctx, BG_ERR_NOMEM, "out of host memory");
+    bt->ctx = ctx;
+    bt->items.assign(items, items + n);
+    for (uint32_t k = 0; k < n; ++k) {
+        bt->item_scene.push_back(index[items[k].scene]);
+        bt->item_pix_base.push_back(bt->total_pix);
+        bt->total_pix += (uint64_t)items[k].camera.width * items[k].camera.height;
+    }
+    // bases of every unique scene in the concatenated arrays
+    const size_t U = uniq.size();
+    std::vector<uint64_t> node_base(U), slot_base(U), tri_base(U), sph_base(U), mat_base(U), light_base(U);
+    uint64_t n_mats = 0;
+    for (size_t u = 0; u < U; ++u) {
+        const bg_scene* s = uniq[u];
+        node_base[u] = bt->n_nodes;
+        slot_base[u] = bt->n_slots;
+        tri_base[u] = bt->n_tris;
+        sph_base[u] = bt->n_spheres;
+        mat_base[u] = n_mats;
+        light_base[u] = bt->n_lights;
+        bt->n_nodes += s->bvh.nodes.size();
+        bt->n_slots += s->bvh.slot_prim.size();
+        bt->n_tris += s->n_tris;
+        bt->n_spheres += s->n_spheres;
+        n_mats += s->materials.size();
+        bt->n_lights += (uint32_t)s->lights.size();
+        bt->max_depth = std::max(bt->max_depth, s->bvh.max_depth);
+        bt->scene_root.push_back((uint32_t)node_base[u]);
+        bt->scene_light_base.push_back((uint32_t)light_base[u]);
+        bt->scene_n_lights.push_back((uint32_t)s->lights.size());
+    }
+    if (bt->n_nodes >= 0xFFFFFFFFull || bt->n_tris + bt->n_spheres >= 0x7FFFFFFFull) {
+        delete bt;
+        return fail(ctx, BG_ERR_INVALID, "batch too large for 32-bit node and primitive ids");
+    }
+    const uint64_t np = bt->n_tris + bt->n_spheres;
+    std::vector<bg_bvh_node> nodes(bt->n_nodes);
+    std::vector<uint32_t> slots(bt->n_slots), pmat(np);
+    std::vector<int32_t> plight(np, -1);
+    std::vector<bg::LightDev> lts(bt->n_lights);
+    for (size_t u = 0; u < U; ++u) {
+        const bg_scene* s = uniq[u];
+        for (size_t k = 0; k < s->bvh.nodes.size(); ++k) {
+            bg_bvh_node nd = s->bvh.nodes[k];
+            nd.offset += (nd.meta >> 3) > 0 ? (uint32_t)slot_base[u] : (uint32_t)node_base[u];
+            nodes[node_base[u] + k] = nd;
+        }
+        const uint64_t nt = s->n_tris;
+        auto global = [&](uint64_t p) -> uint64_t {
+            return p < nt ? tri_base[u] + p : bt->n_tris + sph_base[u] + (p - nt);
+        };
+        for (size_t k = 0; k < s->bvh.slot_prim.size(); ++k) slots[slot_base[u] + k] = (uint32_t)global(s->bvh.slot_prim[k]);
+        for (uint64_t p = 0; p < nt + s->n_spheres; ++p) {
+            pmat[global(p)] = s->prim_material[p] + (uint32_t)mat_base[u];
+            if (s->prim_light[p] >= 0) plight[global(p)] = s->prim_light[p] + (int32_t)light_base[u];
+        }
+    }
+    uint64_t bytes = 0;
+    int rc = BG_OK;
+    auto up = [&](auto** dst, const auto* src, size_t count) {
+        if (rc || count == 0) return;
+        rc = dalloc(ctx, dst, count);
+        if (!rc && hipMemcpy(*dst, src, count * sizeof(**dst), hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(ctx, BG_ERR_HIP, "batch upload failed");
+        bytes += count * sizeof(**dst);
+    };
+    up((bg_bvh_node**)&bt->nodes, nodes.data(), nodes.size());
+    up(&bt->slot_prim, slots.data(), slots.size());
+    up(&bt->prim_mat, pmat.data(), pmat.size());
+    up(&bt->prim_light, plight.data(), plight.size());
+    // device-to-device copies of the per-scene arrays that need no rebasing
+    if (!rc) rc = dalloc(ctx, &bt->tri_verts, 9 * bt->n_slots);
+    if (!rc) rc = dalloc(ctx, &bt->tri_orig, 9 * bt->n_tris);
+    if (!rc) rc = dalloc(ctx, &bt->tri_flip, bt->n_tris);
+    if (!rc) rc = dalloc(ctx, &bt->spheres, bt->n_spheres);
+    if (!rc) rc = dalloc(ctx, &bt->sphere_aux, bt->n_spheres);
+    if (!rc) rc = dalloc(ctx, &bt->sph_shade, bt->n_spheres);
+    if (!rc) rc = dalloc(ctx, &bt->mats, n_mats);
+    if (!rc) rc = dalloc(ctx, &bt->lights, bt->n_lights);
+    for (size_t u = 0; u < U && !rc; ++u) {
+        const bg_scene* s = uniq[u];
+        const size_t ns = s->bvh.slot_prim.size();
+        auto d2d = [&](void* dst, const void* src, size_t b) {
+            if (!rc && b && hipMemcpy(dst, src, b, hipMemcpyDeviceToDevice) != hipSuccess)
+                rc = fail(ctx, BG_ERR_HIP, "batch device copy failed");
+        };
+        d2d(bt->tri_verts + 9 * slot_base[u], s->d_tri_verts, 9 * ns * sizeof(float));
+        d2d(bt->tri_orig + 9 * tri_base[u], s->d_tri_orig, 9 * s->n_tris * sizeof(float));
+        d2d(bt->tri_flip + tri_base[u], s->d_tri_flip, s->n_tris);
+        d2d(bt->spheres + sph_base[u], s->d_spheres, s->n_spheres * sizeof(bg::SphereDev));
+        d2d(bt->sphere_aux + sph_base[u], s->d_sphere_aux, s->n_spheres * sizeof(bg::SphereAux));
+        d2d(bt->sph_shade + sph_base[u], s->d_sph_shade, s->n_spheres * sizeof(bg::SphereShadeDev));
+        d2d(bt->mats + mat_base[u], s->d_mats, s->materials.size() * sizeof(bg::MaterialDev));
+        if (!s->lights.empty() && !rc) {
+            std::vector<bg::LightDev> l(s->lights.size());
+            if (hipMemcpy(l.data(), s->d_lights, l.size() * sizeof(bg::LightDev), hipMemcpyDeviceToHost) != hipSuccess)
+                rc = fail(ctx, BG_ERR_HIP, "batch light copy failed");
+            for (size_t k = 0; k < l.size(); ++k) {
+                l[k].prim += (uint32_t)tri_base[u];  // lights are on triangles
+                lts[light_base[u] + k] = l[k];
+            }
+        }
+    }
+    if (!rc && !lts.empty() && hipMemcpy(bt->lights, lts.data(), lts.size() * sizeof(bg::LightDev), hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(ctx, BG_ERR_HIP, "batch light upload failed");
+    if (!rc) rc = dalloc(ctx, &bt->d_items, n);
+    if (!rc) rc = dalloc(ctx, &bt->film, bt->total_pix);
+    if (!rc) rc = dalloc(ctx, &bt->d_pass, 1);
+    if (!rc) rc = dalloc(ctx, &bt->d_queue, 1);
+    if (rc) {
+        free_batch(bt);
+        delete bt;
+        return rc;
+    }
+    bt->shape = ctx->shape;
+    *out = bt;
+    return BG_OK;
+}
+
+void bg_batch_destroy(bg_batch* bt) {
+    if (!bt) return;
+    free_batch(bt);
+    delete bt;
+}
+
+int bg_batch_film_read(bg_batch* bt, uint32_t item, float* rgbw) {
+    if (!bt || !rgbw || item >= bt->items.size()) return fail(bt ? bt->ctx : nullptr, BG_ERR_INVALID, "bad argument");
+    bg_ctx* ctx = bt->ctx;
+    const uint64_t np = (uint64_t)bt->items[item].camera.width * bt->items[item].camera.height;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipMemcpy(rgbw, bt->film + bt->item_pix_base[item], np * sizeof(float4), hipMemcpyDeviceToHost));
+    return BG_OK;
+}
+
+int bg_batch_launch_config(const bg_batch* bt, bg_launch_config* cfg) {
+    if (!bt || !cfg) return BG_ERR_INVALID;
+    cfg->block = (uint32_t)bt->shape.block;
+    cfg->stack = (uint32_t)bt->shape.stack;
+    cfg->blocks_per_cu = (uint32_t)bt->shape.blocks_per_cu;
+    cfg->variant = (uint32_t)bt->shape.variant;
+    return BG_OK;
+}
+
+int bg_render_batch(bg_ctx* ctx, bg_batch* bt, const bg_batch_desc* d, bg_stats* stats) {
+    if (!ctx || !bt || !d || bt->ctx != ctx) return fail(ctx, BG_ERR_INVALID, "null or foreign argument");
+    if (d->mode > BG_MODE_CORRECTED) return fail(ctx, BG_ERR_INVALID, "unknown mode");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (stats) *stats = bg_stats{};
+    const uint32_t n = (uint32_t)bt->items.size();
+    HIP_TRY(ctx, hipMemsetAsync(bt->film, 0, bt->total_pix * sizeof(float4), ctx->stream));
+    if (d->spp == 0) return BG_OK;
+    // samples per pass: a divisor of spp so that every replay of the captured pass is identical
+    const uint64_t target = d->paths_per_pass ? d->paths_per_pass : (uint64_t)32 << 20;
+    uint32_t spp_pass = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(d->spp, target / bt->total_pix));
+    while (d->spp % spp_pass) --spp_pass;
+    const uint64_t cap = bt->total_pix * spp_pass;
+    int rc = bg::ensure_paths(ctx, cap);
+    if (rc) return rc;
+    if (bt->root_cap < cap) {
+        if (bt->d_root) hipFree(bt->d_root);
+        bt->d_root = nullptr;
+        bt->root_cap = 0;
+        HIP_TRY(ctx, hipMalloc((void**)&bt->d_root, cap * sizeof(uint32_t)));
+        bt->root_cap = cap;
+    }
+    std::vector<bg::BatchItemDev> it(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        bg::camera_setup(bt->items[k].camera, &it[k].cam);
+        it[k].seed = bt->items[k].seed;
+        it[k].path_base = bt->item_pix_base[k] * spp_pass;
+        it[k].pix_base = bt->item_pix_base[k];
+        const uint32_t u = bt->item_scene[k];
+        it[k].root = bt->scene_root[u];
+        it[k].light_base = bt->scene_light_base[u];
+        it[k].n_lights = bt->scene_n_lights[u];
+        it[k].pad = 0;
+    }
+    hipStream_t st = ctx->stream;
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    HIP_TRY(ctx, hipMemcpy(bt->d_items, it.data(), n * sizeof(bg::BatchItemDev), hipMemcpyHostToDevice));
+
+    bg::PathBuffers b = ctx->paths;
+    b.root = bt->d_root;
+    bg::RenderParamsDev p;
+    p.seed = 0;
+    p.x0 = p.y0 = p.x1 = p.y1 = 0;
+    p.first_sample = 0;
+    p.n_samples = spp_pass;
+    p.max_depth = d->max_depth;
+    p.rr = d->rr_threshold;
+    p.mode = d->mode;
+    p.n_paths = cap;
+    p.batch = bt->d_items;
+    p.n_batch = n;
+    p.pass_first = bt->d_pass;
+    std::memset(&p.cam, 0, sizeof(p.cam));
+    bg::ShadeSceneDev sc;
+    sc.tri_orig = bt->tri_orig;
+    sc.tri_flip = bt->tri_flip;
+    sc.prim_mat = bt->prim_mat;
+    sc.prim_light = bt->prim_light;
+    sc.mats = bt->mats;
+    sc.lights = bt->lights;
+    sc.spheres = bt->sph_shade;
+    sc.n_tris = (uint32_t)bt->n_tris;
+    sc.n_lights = bt->n_lights;
+
+    std::vector<bg::LaunchShape> cands;
+    if (d->autotune && !bt->tuned)
+        for (int block : {128, 256})
+            for (int stack : {8, 16, 32})
+                for (int bpc : {4, 8, 16})
+                    for (int variant : {0, 2}) cands.push_back(bg::LaunchShape{block, stack, bpc, variant});
+    // spill rows for the largest grid any launch shape of this call uses
+    uint64_t max_threads = (uint64_t)grid_for(ctx, bt->shape, cap) * bt->shape.block;
+    for (const bg::LaunchShape& c : cands) max_threads = std::max<uint64_t>(max_threads, (uint64_t)grid_for(ctx, c, cap) * c.block);
+    rc = bg::ensure_spill(ctx, (size_t)max_threads * (bt->max_depth + 1) * sizeof(uint32_t));
+    if (rc) return rc;
+
+    if (d->autotune && !bt->tuned) {
+        // time every candidate on the pass's camera rays, then on bounce-1 and shadow rays
+        auto t0 = std::chrono::steady_clock::now();
+        HIP_TRY(ctx, hipMemsetAsync(bt->d_pass, 0, sizeof(uint32_t), st));
+        HIP_TRY(ctx, bg::launch_init_paths(p, b, st));
+        std::vector<float> score(cands.size(), 0.0f);
+        auto time_all = [&](const float4* rays, uint4* hits, bool any) -> int {
+            for (size_t c = 0; c < cands.size(); ++c) {
+                HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+                HIP_TRY(ctx, traverse(bt, ctx, cands[c], rays, hits, cap, b.root, any, st));
+                HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+                HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+                float ms = 0.0f;
+                HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+                score[c] += ms;
+            }
+            return BG_OK;
+        };
+        rc = time_all(b.ray, b.hit, false);
+        if (rc) return rc;
+        HIP_TRY(ctx, bg::launch_shade(p, sc, b, 0, st));
+        rc = time_all(b.ray, b.hit, false);
+        if (!rc && sc.n_lights) rc = time_all(b.sray, b.shit, true);
+        if (rc) return rc;
+        size_t best = 0;
+        for (size_t c = 1; c < cands.size(); ++c)
+            if (score[c] < score[best]) best = c;
+        bt->shape = cands[best];
+        bt->tuned = true;
+        bt->tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    const bg::LaunchShape sh = bt->shape;
+
+    HIP_TRY(ctx, hipMemsetAsync(b.counters, 0, 2 * sizeof(unsigned long long), st));
+    HIP_TRY(ctx, hipMemsetAsync(bt->d_pass, 0, sizeof(uint32_t), st));
+    uint64_t launches = 0;
+    auto enqueue_pass = [&](hipStream_t s) -> hipError_t {
+        hipError_t e;
+        if ((e = bg::launch_init_paths(p, b, s)) != hipSuccess) return e;
+        ++launches;
+        for (uint32_t bounce = 0; bounce <= d->max_depth; ++bounce) {
+            if ((e = traverse(bt, ctx, sh, b.ray, b.hit, cap, b.root, false, s)) != hipSuccess) return e;
+            if ((e = bg::launch_shade(p, sc, b, bounce, s)) != hipSuccess) return e;
+            launches += 2;
+            if (sc.n_lights > 0 && bounce < d->max_depth) {
+                if ((e = traverse(bt, ctx, sh, b.sray, b.shit, cap, b.root, true, s)) != hipSuccess) return e;
+                if ((e = traverse(bt, ctx, sh, b.mray, b.mhit, cap, b.root, false, s)) != hipSuccess) return e;
+                if ((e = bg::launch_nee_resolve(p, sc, b, bounce, s)) != hipSuccess) return e;
+                launches += 3;
+            }
+        }
+        if ((e = bg::launch_batch_film(p, b, bt->total_pix, bt->film, s)) != hipSuccess) return e;
+        if ((e = bg::launch_count_rays(p, b, b.counters, s)) != hipSuccess) return e;
+        if ((e = bg::launch_pass_advance(bt->d_pass, spp_pass, s)) != hipSuccess) return e;
+        launches += 3;
+        return hipSuccess;
+    };
+    const uint32_t passes = d->spp / spp_pass;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+    if (d->use_graph) {
+        HIP_TRY(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+        const hipError_t e = enqueue_pass(st);
+        hipGraph_t g = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(st, &g);
+        graph = g;
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (graph) hipGraphDestroy(graph);
+            return fail(ctx, BG_ERR_HIP, std::string("pass capture failed: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+        }
+        const hipError_t e3 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e3 != hipSuccess) {
+            hipGraphDestroy(graph);
+            return fail(ctx, BG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e3));
+        }
+        for (uint32_t k = 0; k < passes; ++k) {
+            const hipError_t e4 = hipGraphLaunch(exec, st);
+            if (e4 != hipSuccess) {
+                hipGraphExecDestroy(exec);
+                hipGraphDestroy(graph);
+                return fail(ctx, BG_ERR_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e4));
+            }
+        }
+        launches *= passes;  // counted once while capturing; the graph replays them
+    } else {
+        for (uint32_t k = 0; k < passes; ++k) HIP_TRY(ctx, enqueue_pass(st));
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    auto t1 = std::chrono::steady_clock::now();
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    float ms = 0.0f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    unsigned long long nr[2] = {0, 0};
+    HIP_TRY(ctx, hipMemcpy(nr, b.counters, sizeof(nr), hipMemcpyDeviceToHost));
+    if (stats) {
+        stats->kernel_ms = ms;
+        stats->total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        stats->launches = launches;
+        stats->samples = bt->total_pix * d->spp;
+        stats->rays = nr[0];
+        stats->shadow_rays = nr[1];
+    }
+    return BG_OK;
+}
+
+}  // extern "C"

@@ -56,6 +56,8 @@ struct TraverseArgs {
     uint32_t* spill;           // per-thread overflow of the LDS stack
     uint32_t spill_depth;
     unsigned long long* counters;  // [nodes, tris, spheres] when counting
+    const uint32_t* roots = nullptr;   // per ray: BVH root node (scene batches); node 0 when null
+    unsigned long long* queue = nullptr;  // work-queue counter (variant 2), zero at launch
 };
 
 // Default = the best of a sweep over block {64,128,256} x stack {8,16,32} x
@@ -64,7 +66,7 @@ struct LaunchShape {
     int block = 256;
     int stack = 16;
     int blocks_per_cu = 16;
-    int variant = 0;  // 0: one ray per loop round, 1: per-lane refill
+    int variant = 0;  // 0: one ray per loop round, 1: per-lane refill, 2: rays taken from an atomic work queue
 };
 
 // traverse.hip
@@ -143,6 +145,7 @@ struct PathBuffers {
     uint32_t* gslot = nullptr; // per path: its slot in gin/gout this bounce, ~0 when not live
     uint32_t* gcount = nullptr; // live-path count of this bounce (device)
     uint32_t* gblk = nullptr;   // per 256 paths: live count, then its exclusive prefix
+    uint32_t* root = nullptr;   // per path: BVH root of its scene (scene batches)
     float4* rec_in = nullptr;  // 3 per training record: the input the direction was sampled with
     float4* rec_tgt = nullptr; // per record: sampled direction, then (after the pass) its weight in .w
     float4* rec_aux = nullptr; // 2 per record: throughput before scattering, radiance after the vertex
@@ -157,6 +160,17 @@ struct Replay {
     float4* tgt = nullptr;  // direction, weight
     uint32_t* state = nullptr;
     uint32_t cap = 0;
+};
+
+// One scene of a batch rendered in a single wavefront loop.
+struct BatchItemDev {
+    CameraDev cam;        // its camera; its film is cam.width x cam.height
+    uint64_t seed;        // sample-stream seed
+    uint64_t path_base;   // first path of this item in a pass
+    uint64_t pix_base;    // first pixel of its film in the batch film
+    uint32_t root;        // its BVH root in the batch node array
+    uint32_t light_base, n_lights;
+    uint32_t pad;
 };
 
 struct RenderParamsDev {
@@ -174,6 +188,11 @@ struct RenderParamsDev {
     float guide_alpha = 0.0f;
     uint32_t guide_lobes = 3;  // bit 0: guide at diffuse vertices, bit 1: at glossy vertices
     uint32_t rec_stride = 1, rec_paths = 0;
+    // scene batches: items sorted by path_base (and pix_base); the first sample of the pass
+    // is read from pass_first (device) so a captured pass can be replayed
+    const BatchItemDev* batch = nullptr;
+    uint32_t n_batch = 0;
+    const uint32_t* pass_first = nullptr;
 };
 
 // path.hip
@@ -192,8 +211,17 @@ hipError_t launch_replay_append(const PathBuffers& b, uint64_t n_records, const 
 // Minibatch k = 0..n-1 takes ring sample ((first + k) * 2654435761) mod count:
 // a stride-permutation of the ring (the prime exceeds any count).
 hipError_t launch_replay_gather(const Replay& r, uint64_t first, uint64_t n, float4* in, float4* tgt, hipStream_t s);
+hipError_t launch_batch_film(const RenderParamsDev& p, const PathBuffers& b, uint64_t total_pix, float4* film,
+                             hipStream_t s);
+// Adds `spp` to *pass_first (the last node of a captured pass).
+hipError_t launch_pass_advance(uint32_t* pass_first, uint32_t spp, hipStream_t s);
+hipError_t launch_zero_u64(unsigned long long* p, hipStream_t s);
 hipError_t launch_film(const RenderParamsDev& p, const PathBuffers& b, uint32_t width, uint32_t height, float4* film,
                        double4* film64, hipStream_t s);
+
+// render.cpp: (re)allocate the context's path state and traversal spill buffer
+int ensure_paths(bg_ctx* ctx, size_t cap);
+int ensure_spill(bg_ctx* ctx, size_t bytes);
 
 // ---------------------------------------------------------------- guiding MLP
 constexpr int kParamsTotal = 4 * (64 * 64 + 64) + (40 * 64 + 40);  // 19,240

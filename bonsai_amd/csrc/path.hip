@@ -26,7 +26,7 @@ namespace {
 using namespace dev;
 using dev::block_exclusive;
 
-constexpr uint32_t kAlive = 1u, kSpecFlag = 2u;
+constexpr uint32_t kAlive = 1u, kSpecFlag = 2u, kItemShift = 16u;
 // nee_a.w flags
 constexpr uint32_t kHasNee = 1u, kHasLight = 2u, kHasBsdf = 4u;
 
@@ -94,15 +94,48 @@ __device__ __noinline__ float light_point_pdf(const ShadeSceneDev& sc, const Lig
     return pdf;
 }
 
+// Batch item owning path i (items sorted by path_base) or pixel t (by pix_base).
+__device__ __forceinline__ uint32_t item_of_path(const RenderParamsDev& p, uint64_t i) {
+    uint32_t lo = 0, hi = p.n_batch;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (p.batch[mid].path_base <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+__device__ __forceinline__ uint32_t item_of_pixel(const RenderParamsDev& p, uint64_t t) {
+    uint32_t lo = 0, hi = p.n_batch;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (p.batch[mid].pix_base <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 __global__ __launch_bounds__(256) void k_init_paths(RenderParamsDev p, PathBuffers b) {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= p.n_paths) return;
-    const uint32_t ww = p.x1 - p.x0, wh = p.y1 - p.y0;
+    const CameraDev* cam = &p.cam;
+    uint64_t seed = p.seed, li = i;
+    uint32_t x0 = p.x0, y0 = p.y0, x1 = p.x1, y1 = p.y1, item = 0;
+    if (p.n_batch) {  // scene batch: the item's camera, seed and whole film
+        item = item_of_path(p, i);
+        const BatchItemDev& it = p.batch[item];
+        cam = &it.cam;
+        seed = it.seed;
+        x0 = 0; y0 = 0; x1 = it.cam.width; y1 = it.cam.height;
+        li = i - it.path_base;
+        b.root[i] = it.root;
+    }
+    const uint32_t first = p.pass_first ? *p.pass_first : p.first_sample;
+    const uint32_t ww = x1 - x0, wh = y1 - y0;
     const uint64_t npix = (uint64_t)ww * wh;
-    const uint32_t local = (uint32_t)(i % npix);
-    const uint32_t s = p.first_sample + (uint32_t)(i / npix);
-    const uint32_t px = p.x0 + local % ww, py = p.y0 + local / ww;
-    const uint64_t key = stream_key(p.seed, (uint64_t)py * p.cam.width + px, s);
+    const uint32_t local = (uint32_t)(li % npix);
+    const uint32_t s = first + (uint32_t)(li / npix);
+    const uint32_t px = x0 + local % ww, py = y0 + local / ww;
+    const uint64_t key = stream_key(seed, (uint64_t)py * cam->width + px, s);
     uint32_t dim = 0;
     // sampler.rs:14-20: film = pixel + get_2d, lens = get_2d, time = get_1d
     const float fx = (float)px + draw(key, dim);
@@ -110,13 +143,13 @@ __global__ __launch_bounds__(256) void k_init_paths(RenderParamsDev p, PathBuffe
     draw(key, dim);
     draw(key, dim);
     const float tu = draw(key, dim);
-    const CamRay r = generate_ray(p.cam, fx, fy, tu);
+    const CamRay r = generate_ray(*cam, fx, fy, tu);
     b.ray[2 * i] = make_float4(r.ox, r.oy, r.oz, r.t_max);
     b.ray[2 * i + 1] = make_float4(r.dx, r.dy, r.dz, r.time);
     b.beta[i] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(1u));  // .w: rays issued (camera ray)
     b.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     b.key[i] = make_uint2((uint32_t)key, (uint32_t)(key >> 32));
-    b.state[i] = make_uint2(dim, kAlive);
+    b.state[i] = make_uint2(dim, kAlive | (item << kItemShift));  // the batch item rides in the flags
     b.film[i] = make_float2(fx, fy);
 }
 
@@ -218,11 +251,17 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
             gm = reinterpret_cast<const float4*>(b.gout) + kMixRecord * (uint64_t)gslot;
             if (guide_ok(gm)) alpha = p.guide_alpha;
         }
-        if (bs.ncomp(nonspec) > 0 && sc.n_lights > 0) {
+        uint32_t light_base = 0, n_lights = sc.n_lights;
+        if (p.n_batch) {  // the item's own lights
+            const BatchItemDev& it = p.batch[flags >> kItemShift];
+            light_base = it.light_base;
+            n_lights = it.n_lights;
+        }
+        if (bs.ncomp(nonspec) > 0 && n_lights > 0) {
             // integrator.rs:309-328
-            const float nl = (float)sc.n_lights;
+            const float nl = (float)n_lights;
             const float u = draw(key, dim);
-            const uint32_t idx = (uint32_t)rmin(u * nl, nl - 1.0f);
+            const uint32_t idx = light_base + (uint32_t)rmin(u * nl, nl - 1.0f);
             const float light_pdf = 1.0f / nl;
             const float ul0 = draw(key, dim), ul1 = draw(key, dim);
             // the guided estimator has no separate BSDF-sampling branch (its continuation is that branch)
@@ -550,6 +589,47 @@ __global__ __launch_bounds__(1024) void k_live_scan(uint32_t nblk, uint32_t* __r
 // film.rs:232-284 with a radius-0.5 box filter, gathered per pixel: a sample at
 // film point f covers pixels [ceil(f - 1), floor(f) + 1) clipped to its tile's
 // pixel bounds [tile_min - 1, tile_max) (film.rs:121-136) and the film.
+// Adds every sample of the pass whose filter footprint covers pixel (px, py) of a
+// W x H film, in a fixed order: samples s, then rows dy, then columns dx. The
+// pass's paths of this film start at path_base, npix = window pixels per sample.
+__device__ __forceinline__ void gather_pixel(const PathBuffers& b, int px, int py, int x0, int y0, int x1, int y1,
+                                             int W, int H, uint64_t path_base, uint32_t n_samples, float4* acc,
+                                             double4* acc64) {
+    const uint32_t ww = (uint32_t)(x1 - x0);
+    const uint64_t npix = (uint64_t)ww * (uint32_t)(y1 - y0);
+    for (uint32_t s = 0; s < n_samples; ++s) {
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int qy = py + dy;
+            if (qy < y0 || qy >= y1) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int qx = px + dx;
+                if (qx < x0 || qx >= x1) continue;
+                const uint64_t path = path_base + (uint64_t)s * npix + (uint64_t)(qy - y0) * ww + (qx - x0);
+                const float2 f = b.film[path];
+                const float fpx = f.x - 0.5f, fpy = f.y - 0.5f;
+                const int tx0 = (qx / 16) * 16, ty0 = (qy / 16) * 16;
+                const int lx = max((int)ceilf(fpx - 0.5f), max(tx0 - 1, 0));
+                const int ly = max((int)ceilf(fpy - 0.5f), max(ty0 - 1, 0));
+                const int hx = min((int)floorf(fpx + 0.5f) + 1, min(tx0 + 16, W));
+                const int hy = min((int)floorf(fpy + 0.5f) + 1, min(ty0 + 16, H));
+                if (px < lx || px >= hx || py < ly || py >= hy) continue;
+                const float4 l4 = b.L[path];
+                C3 L = C3{l4.x, l4.y, l4.z};
+                // integrator.rs:92-117
+                const float y = luminance(L);
+                if (L.r != L.r || L.g != L.g || L.b != L.b) L = mkc(0.0f);
+                else if (y < -1e-5f) L = mkc(0.0f);
+                else if (__builtin_isinf(y)) L = mkc(0.0f);
+                acc->x += L.r; acc->y += L.g; acc->z += L.b; acc->w += 1.0f;
+                if (acc64) { acc64->x += L.r; acc64->y += L.g; acc64->z += L.b; acc64->w += 1.0; }
+            }
+        }
+    }
+}
+
+// film.rs:232-284 with a radius-0.5 box filter, gathered per pixel: a sample at
+// film point f covers pixels [ceil(f - 1), floor(f) + 1) clipped to its tile's
+// pixel bounds [tile_min - 1, tile_max) (film.rs:121-136) and the film.
 __global__ __launch_bounds__(256) void k_film(RenderParamsDev p, PathBuffers b, uint32_t W, uint32_t H, float4* film,
                                               double4* film64) {
     const uint32_t ww = p.x1 - p.x0, wh = p.y1 - p.y0;
@@ -559,36 +639,30 @@ __global__ __launch_bounds__(256) void k_film(RenderParamsDev p, PathBuffers b, 
     const int px = (int)(p.x0 + t % ww), py = (int)(p.y0 + t / ww);
     float4 acc = film[(uint64_t)py * W + px];
     double4 acc64 = film64 ? film64[(uint64_t)py * W + px] : double4{0, 0, 0, 0};
-    for (uint32_t s = 0; s < p.n_samples; ++s) {
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int qy = py + dy;
-            if (qy < (int)p.y0 || qy >= (int)p.y1) continue;
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int qx = px + dx;
-                if (qx < (int)p.x0 || qx >= (int)p.x1) continue;
-                const uint64_t path = (uint64_t)s * npix + (uint64_t)(qy - (int)p.y0) * ww + (qx - (int)p.x0);
-                const float2 f = b.film[path];
-                const float fpx = f.x - 0.5f, fpy = f.y - 0.5f;
-                const int tx0 = (qx / 16) * 16, ty0 = (qy / 16) * 16;
-                const int lx = max((int)ceilf(fpx - 0.5f), max(tx0 - 1, 0));
-                const int ly = max((int)ceilf(fpy - 0.5f), max(ty0 - 1, 0));
-                const int hx = min((int)floorf(fpx + 0.5f) + 1, min(tx0 + 16, (int)W));
-                const int hy = min((int)floorf(fpy + 0.5f) + 1, min(ty0 + 16, (int)H));
-                if (px < lx || px >= hx || py < ly || py >= hy) continue;
-                const float4 l4 = b.L[path];
-                C3 L = C3{l4.x, l4.y, l4.z};
-                // integrator.rs:92-117
-                const float y = luminance(L);
-                if (L.r != L.r || L.g != L.g || L.b != L.b) L = mkc(0.0f);
-                else if (y < -1e-5f) L = mkc(0.0f);
-                else if (__builtin_isinf(y)) L = mkc(0.0f);
-                acc.x += L.r; acc.y += L.g; acc.z += L.b; acc.w += 1.0f;
-                if (film64) { acc64.x += L.r; acc64.y += L.g; acc64.z += L.b; acc64.w += 1.0; }
-            }
-        }
-    }
+    gather_pixel(b, px, py, (int)p.x0, (int)p.y0, (int)p.x1, (int)p.y1, (int)W, (int)H, 0, p.n_samples, &acc,
+                 film64 ? &acc64 : nullptr);
     film[(uint64_t)py * W + px] = acc;
     if (film64) film64[(uint64_t)py * W + px] = acc64;
+}
+
+// The films of a scene batch, concatenated (item k's pixels from pix_base).
+__global__ __launch_bounds__(256) void k_batch_film(RenderParamsDev p, PathBuffers b, uint64_t total_pix, float4* film) {
+    const uint64_t t = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= total_pix) return;
+    const BatchItemDev& it = p.batch[item_of_pixel(p, t)];
+    const int W = (int)it.cam.width, H = (int)it.cam.height;
+    const uint64_t local = t - it.pix_base;
+    const int px = (int)(local % (uint64_t)W), py = (int)(local / (uint64_t)W);
+    float4 acc = film[t];
+    gather_pixel(b, px, py, 0, 0, W, H, W, H, it.path_base, p.n_samples, &acc, nullptr);
+    film[t] = acc;
+}
+
+__global__ void k_pass_advance(uint32_t* pass_first, uint32_t spp) {
+    if (threadIdx.x == 0) *pass_first += spp;
+}
+__global__ void k_zero_u64(unsigned long long* q) {
+    if (threadIdx.x == 0) *q = 0ull;
 }
 
 }  // namespace
@@ -632,6 +706,20 @@ hipError_t launch_guide_targets(const RenderParamsDev& p, const PathBuffers& b, 
     const uint64_t n = (uint64_t)p.rec_paths * p.max_depth;
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_guide_targets, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s, p, b);
+    return hipGetLastError();
+}
+hipError_t launch_batch_film(const RenderParamsDev& p, const PathBuffers& b, uint64_t total_pix, float4* film,
+                             hipStream_t s) {
+    if (total_pix == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_film, dim3((uint32_t)((total_pix + 255) / 256)), dim3(256), 0, s, p, b, total_pix, film);
+    return hipGetLastError();
+}
+hipError_t launch_pass_advance(uint32_t* pass_first, uint32_t spp, hipStream_t s) {
+    hipLaunchKernelGGL(k_pass_advance, dim3(1), dim3(64), 0, s, pass_first, spp);
+    return hipGetLastError();
+}
+hipError_t launch_zero_u64(unsigned long long* q, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, q);
     return hipGetLastError();
 }
 hipError_t launch_film(const RenderParamsDev& p, const PathBuffers& b, uint32_t W, uint32_t H, float4* film,

@@ -22,8 +22,10 @@ int fail(bg_ctx* ctx, int code, const std::string& msg) {
             return fail((ctx), BG_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+}  // namespace
+
 // Path-state arrays carved out of one allocation, 256-byte aligned each.
-int ensure_paths(bg_ctx* ctx, size_t cap) {
+int bg::ensure_paths(bg_ctx* ctx, size_t cap) {
     bg::PathBuffers& b = ctx->paths;
     if (b.cap >= cap) return BG_OK;
     if (b.mem) hipFree(b.mem);
@@ -60,7 +62,7 @@ int ensure_paths(bg_ctx* ctx, size_t cap) {
     return BG_OK;
 }
 
-int ensure_spill(bg_ctx* ctx, size_t need) {
+int bg::ensure_spill(bg_ctx* ctx, size_t need) {
     if (ctx->spill_bytes >= need) return BG_OK;
     if (ctx->spill) hipFree(ctx->spill);
     ctx->spill = nullptr;
@@ -69,6 +71,8 @@ int ensure_spill(bg_ctx* ctx, size_t need) {
     ctx->spill_bytes = need;
     return BG_OK;
 }
+
+namespace {
 
 // Guiding buffers: network input/output per path and training records.
 int ensure_guide_bufs(bg_ctx* ctx, size_t cap, size_t rec_cap) {
@@ -213,12 +217,12 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
     uint64_t target = d->paths_per_pass ? d->paths_per_pass : (uint64_t)4 << 20;
     uint32_t spp_pass = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(d->spp, target / npix));
     const uint64_t cap = npix * spp_pass;
-    int rc = ensure_paths(ctx, cap);
+    int rc = bg::ensure_paths(ctx, cap);
     if (rc) return rc;
     const int block = ctx->shape.block;
     const uint64_t grid_cap = ctx->shape.blocks_per_cu > 0 ? (uint64_t)ctx->num_cus * ctx->shape.blocks_per_cu : (1u << 20);
     const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((cap + block - 1) / block, std::min<uint64_t>(grid_cap, 1u << 20)));
-    rc = ensure_spill(ctx, (size_t)grid * block * (s->bvh.max_depth + 1) * sizeof(uint32_t));
+    rc = bg::ensure_spill(ctx, (size_t)grid * block * (s->bvh.max_depth + 1) * sizeof(uint32_t));
     if (rc) return rc;
 
     bg::RenderParamsDev p;

@@ -23,7 +23,7 @@ namespace {
 
 using namespace dev;
 
-template <int BLOCK, int STACK, bool ANY, bool COUNT, bool SPH>
+template <int BLOCK, int STACK, bool ANY, bool COUNT, bool SPH, bool QUEUE = false>
 __global__ __launch_bounds__(BLOCK) void k_traverse(TraverseArgs a) {
     __shared__ uint32_t lds[STACK * BLOCK];
     const int tid = threadIdx.x;
@@ -34,7 +34,28 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(TraverseArgs a) {
     constexpr uint32_t kMask = STACK - 1;
     uint64_t c_nodes = 0, c_tris = 0, c_sph = 0;
 
-    for (uint64_t ray = gtid; ray < a.n_rays; ray += stride) {
+    // Rays come in grid-stride order, or (work queue) in chunks of 64 x kChunk that each
+    // wavefront takes from an atomic counter when it has finished its previous chunk.
+    constexpr uint32_t kChunk = 16;
+    const int lane = tid & 63;
+    constexpr bool queued = QUEUE;
+    uint64_t qbase = 0, ray = gtid;
+    uint32_t qk = kChunk;
+    auto next_queued = [&]() {
+        if (qk == kChunk) {
+            unsigned long long b0 = 0;
+            if (lane == 0) b0 = atomicAdd(a.queue, 64ull * kChunk);
+            qbase = __shfl(b0, 0);
+            qk = 0;
+        }
+        ray = qbase + (uint64_t)qk * 64 + lane;
+        ++qk;
+    };
+    if (queued) next_queued();
+    while (queued ? qbase < a.n_rays : ray < a.n_rays) {
+        // (queued: lanes past the end of the last chunk skip the body; every lane of the
+        // wavefront still reaches next_queued() together)
+        if (ray < a.n_rays) {
         const float4 r0 = a.rays[2 * ray];
         const float4 r1 = a.rays[2 * ray + 1];
         RayState rs;
@@ -45,7 +66,7 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(TraverseArgs a) {
         int best = -1;
         float bt = 0.0f, bb1 = 0.0f, bb2 = 0.0f;
         uint32_t sp = 0, base = 0;  // logical stack size, first logical entry held in LDS
-        uint32_t cur = 0;
+        uint32_t cur = a.roots ? a.roots[ray] : 0u;
         for (;;) {
             const float4 n0 = a.nodes[2 * cur];
             const float4 n1 = a.nodes[2 * cur + 1];
@@ -125,6 +146,9 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(TraverseArgs a) {
         out.z = __float_as_uint(bb1);
         out.w = __float_as_uint(bb2);
         a.hits[ray] = out;
+        }
+        if (queued) next_queued();
+        else ray += stride;
     }
     if (COUNT) {
         atomicAdd(&a.counters[0], (unsigned long long)c_nodes);
@@ -162,7 +186,7 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_refill(TraverseArgs a) {
             ray_setup(rs);
             best = -1;
             bt = 0.0f; bb1 = 0.0f; bb2 = 0.0f;
-            sp = 0; base = 0; cur = 0;
+            sp = 0; base = 0; cur = a.roots ? a.roots[r] : 0u;
         };
         start(ray);
         for (;;) {
@@ -260,11 +284,15 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_refill(TraverseArgs a) {
 template <int BLOCK, int STACK, bool ANY, bool COUNT>
 void launch4(const TraverseArgs& a, int variant, int grid, hipStream_t s) {
     // scenes without spheres use instantiations with the sphere code compiled out
+    // variant 2: the work-queue kernel (a.queue set and zeroed by the caller); counting runs use variant 0
+    const bool q = variant == 2 && a.queue != nullptr && !COUNT;
     if (a.spheres != nullptr) {
         if (variant == 1) hipLaunchKernelGGL((k_traverse_refill<BLOCK, STACK, ANY, COUNT, true>), dim3(grid), dim3(BLOCK), 0, s, a);
+        else if (q) hipLaunchKernelGGL((k_traverse<BLOCK, STACK, ANY, false, true, true>), dim3(grid), dim3(BLOCK), 0, s, a);
         else hipLaunchKernelGGL((k_traverse<BLOCK, STACK, ANY, COUNT, true>), dim3(grid), dim3(BLOCK), 0, s, a);
     } else {
         if (variant == 1) hipLaunchKernelGGL((k_traverse_refill<BLOCK, STACK, ANY, COUNT, false>), dim3(grid), dim3(BLOCK), 0, s, a);
+        else if (q) hipLaunchKernelGGL((k_traverse<BLOCK, STACK, ANY, false, false, true>), dim3(grid), dim3(BLOCK), 0, s, a);
         else hipLaunchKernelGGL((k_traverse<BLOCK, STACK, ANY, COUNT, false>), dim3(grid), dim3(BLOCK), 0, s, a);
     }
 }
@@ -296,7 +324,7 @@ hipError_t launch1(const TraverseArgs& a, bool any, bool count, int stack, int v
 bool traverse_shape_supported(const LaunchShape& s) {
     const bool b = s.block == 64 || s.block == 128 || s.block == 256;
     const bool st = s.stack == 8 || s.stack == 16 || s.stack == 32;
-    return b && st && s.blocks_per_cu >= 0 && (s.variant == 0 || s.variant == 1);
+    return b && st && s.blocks_per_cu >= 0 && (s.variant == 0 || s.variant == 1 || s.variant == 2);
 }
 
 hipError_t launch_traverse(const TraverseArgs& a, bool any_hit, bool count, const LaunchShape& shape, int grid,

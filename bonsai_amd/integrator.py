@@ -159,3 +159,51 @@ def rel_mse(img: np.ndarray, ref: np.ndarray) -> float:
     i = img.astype(np.float64)
     r = ref.astype(np.float64)
     return float(np.mean((i - r) ** 2 / (r * r + 0.01)))
+
+
+class BatchRenderer:
+    """Many (scene, camera, seed) items rendered by one wavefront loop with a hipGraph-captured pass
+    and an autotuned traversal launch shape. Each item's film equals what GpuPathIntegrator with the
+    same camera, seed, spp, depth and mode renders (no counterpart in the reference)."""
+
+    def __init__(self, ctx: Context, items: list[tuple[GpuBvh, Camera, int]]):
+        self.ctx = ctx
+        self.items = items
+        arr = (N.bg_batch_item * len(items))()
+        for k, (bvh, cam, seed) in enumerate(items):
+            arr[k].scene = bvh.h
+            arr[k].camera = cam.desc()
+            arr[k].seed = seed
+        h = C.c_void_p()
+        N.check(N.lib().bg_batch_create(ctx.h, arr, len(items), C.byref(h)), ctx.h)
+        self.h = h
+
+    def close(self):
+        if self.h:
+            N.lib().bg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, spp: int, max_depth: int, mode: int = MODE_CORRECTED, rr_threshold: float = 1.0,
+               use_graph: bool = True, autotune: bool = True, paths_per_pass: int = 0) -> N.bg_stats:
+        d = N.bg_batch_desc(spp, max_depth, rr_threshold, mode, paths_per_pass, 1 if use_graph else 0,
+                            1 if autotune else 0)
+        st = N.bg_stats()
+        N.check(N.lib().bg_render_batch(self.ctx.h, self.h, C.byref(d), C.byref(st)), self.ctx.h)
+        return st
+
+    def sums(self, item: int) -> tuple[np.ndarray, np.ndarray]:
+        cam = self.items[item][1]
+        a = np.empty((cam.height, cam.width, 4), np.float32)
+        N.check(N.lib().bg_batch_film_read(self.h, item, a.ctypes.data), self.ctx.h)
+        return a[..., :3], a[..., 3]
+
+    def launch(self) -> tuple[int, int, int, int]:
+        cfg = N.bg_launch_config()
+        N.check(N.lib().bg_batch_launch_config(self.h, C.byref(cfg)), self.ctx.h)
+        return cfg.block, cfg.stack, cfg.blocks_per_cu, cfg.variant

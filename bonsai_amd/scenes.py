@@ -406,6 +406,54 @@ def door_gap_room(width: int = 1920, height: int = 1080, cell: float = 0.0179, s
     return s
 
 
+def batch_scene(index: int, seed: int = 77, scale: float = 1.0) -> SceneData:
+    """Scene `index` of the seeded batch: a triangle budget log-uniform in [1k, 500k] (times `scale`)
+    spent on displaced icospheres (largest level first), a matte ground quad, one double-sided
+    quad light overhead, per object a matte (albedo U[0.1, 0.9]) or Trowbridge-Reitz conductor
+    (alpha U[0.05, 0.8]) material, and a square film of side uniform in [128, 1024]
+    (times sqrt(scale)), viewed from a seeded position."""
+    k = 0
+
+    def u():
+        nonlocal k
+        k += 1
+        return _uniform(seed * 1000 + index, k)
+
+    budget = int(np.exp(np.log(1e3) + (np.log(5e5) - np.log(1e3)) * u()) * scale)
+    side = max(16, int((128 + (1024 - 128) * u()) * np.sqrt(scale)))
+    s = SceneData()
+    s.materials = [matte(0.5, 0.5, 0.5)]
+    tris = [quad((-6, 0, -6), (6, 0, -6), (6, 0, 6), (-6, 0, 6))]
+    mat = [0, 0]
+    j = 0
+    while budget >= 80:
+        lv = 1
+        while lv < 7 and 20 * 4 ** (lv + 1) <= budget:
+            lv += 1
+        r = 0.4 + 0.8 * u()
+        c = (-4 + 8 * u(), r, -4 + 8 * u())
+        m = icosphere(lv, c, r, 0.1, 3.0, seed * 7919 + index * 31 + j)
+        if u() < 0.5:
+            s.materials.append(matte(0.1 + 0.8 * u(), 0.1 + 0.8 * u(), 0.1 + 0.8 * u()))
+        else:
+            s.materials.append(conductor(0.05 + 0.75 * u()))
+        tris.append(m)
+        mat += [len(s.materials) - 1] * m.shape[0]
+        budget -= m.shape[0]
+        j += 1
+    base = sum(t.shape[0] for t in tris)
+    tris.append(quad((-1.5, 7, -1.5), (1.5, 7, -1.5), (1.5, 7, 1.5), (-1.5, 7, 1.5)))
+    s.materials.append(matte(0.0, 0.0, 0.0))
+    mat += [len(s.materials) - 1] * 2
+    s.tri_positions = np.concatenate(tris).astype(np.float32)
+    s.tri_material = np.asarray(mat, np.uint32)
+    s.lights = [area_light(base, (20.0, 20.0, 18.0), True), area_light(base + 1, (20.0, 20.0, 18.0), True)]
+    a = 2 * np.pi * u()
+    s.camera = Camera(pos=(9 * np.cos(a), 4 + 3 * u(), 9 * np.sin(a)), look=(0.0, 0.8, 0.0), up=(0.0, 1.0, 0.0),
+                      fov=45.0, width=side, height=side)
+    return s
+
+
 def intersection_camera(width: int = 4096, height: int = 4096) -> Camera:
     return Camera(pos=(0.0, 0.3, -3.2), look=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=45.0,
                   width=width, height=height)

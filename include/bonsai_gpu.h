@@ -185,7 +185,9 @@ typedef struct bg_launch_config {
     uint32_t stack;       /* LDS stack entries per lane: 8, 16, 32 */
     uint32_t blocks_per_cu; /* persistent blocks per CU; 0 = one block per ray tile */
     uint32_t variant;     /* 0: a lane finishes its ray before the wavefront takes new rays;
-                             1: each lane starts its next ray as soon as its current one ends */
+                             1: each lane starts its next ray as soon as its current one ends;
+                             2: as 0, with rays taken in chunks from an atomic work queue
+                                (scene batches; other calls run it as 0) */
 } bg_launch_config;
 int bg_set_launch_config(bg_ctx* ctx, const bg_launch_config* cfg);
 int bg_get_launch_config(const bg_ctx* ctx, bg_launch_config* cfg);
@@ -251,6 +253,39 @@ int bg_render_film(bg_ctx* ctx, const bg_scene* scene, const bg_render_desc* des
  * (film.rs:156-202: XYZ round trip, divide by weight, clamp at 0) into
  * rgb_out (width*height*3 floats of the full film). */
 int bg_render(bg_ctx* ctx, const bg_scene* scene, const bg_render_desc* desc, float* rgb_out, bg_stats* stats);
+
+/* ----------------------------------------------------------- scene batches */
+
+/* Many scenes, cameras and films in one wavefront loop (the reference renders
+ * one scene per SamplerIntegrator::render call, src/base/integrator.rs:35-133;
+ * a batch replaces a loop of such calls). Items may share a scene. Each item's
+ * film receives exactly what bg_render_film with the same camera, seed, spp,
+ * depth, rr_threshold and mode gives. */
+typedef struct bg_batch bg_batch;
+typedef struct bg_batch_item {
+    const bg_scene* scene;
+    bg_camera_desc camera;
+    uint64_t seed;
+} bg_batch_item;
+/* Concatenates the items' scene data on the device (the scenes stay valid and
+ * owned by the caller). */
+int bg_batch_create(bg_ctx* ctx, const bg_batch_item* items, uint32_t n, bg_batch** out);
+void bg_batch_destroy(bg_batch* batch);
+typedef struct bg_batch_desc {
+    uint32_t spp;            /* samples per pixel of every item */
+    uint32_t max_depth;
+    float rr_threshold;
+    uint32_t mode;           /* BG_MODE_REF / BG_MODE_CORRECTED */
+    uint32_t paths_per_pass; /* 0: about 32M */
+    uint32_t use_graph;      /* nonzero: capture one pass in a hipGraph and replay it */
+    uint32_t autotune;       /* nonzero: time the traversal launch shapes on the first call */
+} bg_batch_desc;
+/* Clears the batch film and renders spp samples of every item into it. */
+int bg_render_batch(bg_ctx* ctx, bg_batch* batch, const bg_batch_desc* desc, bg_stats* stats);
+/* Film sums of one item: width*height*4 floats (r, g, b, weight). */
+int bg_batch_film_read(bg_batch* batch, uint32_t item, float* rgbw);
+/* The traversal launch shape the batch renders with (tuned or the context's). */
+int bg_batch_launch_config(const bg_batch* batch, bg_launch_config* cfg);
 
 /* ------------------------------------------------------------ path guiding */
 

@@ -1,0 +1,83 @@
+"""Scene batches: every item's film from one batched wavefront loop (hipGraph
+replay, autotuned or fixed launch shapes, work-queue traversal) equals what
+the single-scene integrator renders for that item alone, bit for bit."""
+import numpy as np
+import pytest
+
+from bonsai_amd import GpuBvh
+from bonsai_amd.integrator import MODE_CORRECTED, MODE_REF, BatchRenderer, Film, GpuPathIntegrator
+from bonsai_amd.scenes import Camera, batch_scene, cornell_box, path_tracer_scene
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def items(ctx):
+    scenes = [batch_scene(i, scale=0.02) for i in range(4)]
+    scenes.append(cornell_box(width=40, height=30))
+    scenes.append(path_tracer_scene(48, 27, small=True))
+    bvhs = [GpuBvh(ctx, s) for s in scenes]
+    out = [(bvh, s.camera, 100 + k) for k, (bvh, s) in enumerate(zip(bvhs, scenes))]
+    # the same scene twice, from another viewpoint and with another seed
+    out.append((bvhs[0], Camera(pos=(5.0, 6.0, -7.0), look=(0.0, 0.5, 0.0), up=(0.0, 1.0, 0.0), fov=50.0,
+                                width=33, height=21), 999))
+    yield out
+
+
+def per_item(ctx, items, spp, depth, mode):
+    res = []
+    for bvh, cam, seed in items:
+        gi = GpuPathIntegrator(ctx, cam, spp=spp, max_depth=depth, mode=mode, seed=seed)
+        f = Film(ctx, cam.width, cam.height)
+        gi.render_into(bvh, f)
+        res.append(f.sums())
+        f.close()
+    return res
+
+
+@pytest.mark.parametrize("mode", [MODE_REF, MODE_CORRECTED])
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_equals_per_scene(ctx, items, mode, graph):
+    spp, depth = 4, 6
+    ref = per_item(ctx, items, spp, depth, mode)
+    br = BatchRenderer(ctx, items)
+    # paths_per_pass small enough to force several replays of the captured pass
+    total = sum(c.width * c.height for _, c, _ in items)
+    st = br.render(spp, depth, mode=mode, use_graph=graph, autotune=True, paths_per_pass=2 * total)
+    assert st.samples == total * spp and st.rays > 0
+    for k in range(len(items)):
+        rgb, w = br.sums(k)
+        np.testing.assert_array_equal(w, ref[k][1])
+        np.testing.assert_array_equal(rgb, ref[k][0])
+    block, stack, bpc, variant = br.launch()
+    assert block in (128, 256) and stack in (8, 16, 32) and bpc in (4, 8, 16) and variant in (0, 2)
+    br.close()
+
+
+@pytest.mark.parametrize("variant", [0, 2])
+def test_fixed_shapes_and_work_queue(ctx, items, variant):
+    b0, s0, c0, v0 = ctx.launch()
+    ctx.set_launch(128, 8, 4, variant)
+    try:
+        ref = per_item(ctx, items, 2, 4, MODE_CORRECTED)
+        br = BatchRenderer(ctx, items)
+        br.render(2, 4, mode=MODE_CORRECTED, autotune=False)
+        assert br.launch() == (128, 8, 4, variant)
+        for k in range(len(items)):
+            rgb, w = br.sums(k)
+            np.testing.assert_array_equal(rgb, ref[k][0])
+            np.testing.assert_array_equal(w, ref[k][1])
+        br.close()
+    finally:
+        ctx.set_launch(b0, s0, c0, v0)
+
+
+def test_batch_rerender_clears_the_film(ctx, items):
+    br = BatchRenderer(ctx, items[:3])
+    br.render(2, 3, autotune=False)
+    a = [br.sums(k) for k in range(3)]
+    br.render(2, 3, autotune=False)
+    for k in range(3):
+        rgb, w = br.sums(k)
+        np.testing.assert_array_equal(rgb, a[k][0])
+        np.testing.assert_array_equal(w, a[k][1])
