@@ -202,6 +202,41 @@ def batch_leg(ctx, spp: int) -> dict:
     return out
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+MLP_MACS = 4 * 64 * 64 + 64 * 40  # multiply-adds per sample and forward pass
+
+
+def mlp_leg(ctx) -> dict:
+    """Guiding network on device-resident batches: inference over 2^22 samples and Adam steps on
+    2^16-sample batches. Forward = 2 * 18,944 flop per sample; a training step = forward + input
+    and weight gradients = 3 forward's worth. Against the dense BF16 matrix-core peak."""
+    import torch
+    from bonsai_amd.guide import GuideMLP
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import guide_mlp
+
+    n_inf, n_tr = 1 << 22, 1 << 16
+    g = GuideMLP(ctx, guide_mlp.init_params(3).astype(np.float32), n_tr)
+    gen = torch.Generator(device="cpu").manual_seed(5)
+    x = torch.rand((n_inf, 12), generator=gen).to("cuda:0")
+    out = torch.empty((n_inf, 40), dtype=torch.float32, device="cuda:0")
+    tg = torch.nn.functional.normalize(torch.randn((n_tr, 4), generator=gen), dim=1).to("cuda:0")
+    tg[:, 3] = 1.0
+    torch_sync()
+    g.eval_device(x.data_ptr(), n_inf, out.data_ptr())
+    inf_ms = min(g.eval_device(x.data_ptr(), n_inf, out.data_ptr()) for _ in range(5))
+    g.train_device(x.data_ptr(), tg.data_ptr(), n_tr)
+    tr_ms = min(g.train_device(x.data_ptr(), tg.data_ptr(), n_tr) for _ in range(5))
+    inf_tf = 2 * MLP_MACS * n_inf / inf_ms / 1e9
+    tr_tf = 3 * 2 * MLP_MACS * n_tr / tr_ms / 1e9
+    g.close()
+    return {"inference": {"samples": n_inf, "ms": round(inf_ms, 3), "Msamples_s": round(n_inf / inf_ms / 1e3, 1),
+                          "TFLOP_s": round(inf_tf, 1), "mfma_frac": round(inf_tf / MFMA_BF16_PEAK_TFLOPS, 4)},
+            "train_step": {"batch": n_tr, "ms": round(tr_ms, 3), "TFLOP_s": round(tr_tf, 1),
+                           "mfma_frac": round(tr_tf / MFMA_BF16_PEAK_TFLOPS, 4)},
+            "peak_TFLOP_s": MFMA_BF16_PEAK_TFLOPS}
+
+
 def rel_mse_pair(a: np.ndarray, b: np.ndarray) -> float:
     """relMSE estimated from two independent renders of n samples each: per pixel (A - B)^2 / 2 is an
     unbiased estimate of one render's variance, and the squared mean in the denominator is taken
@@ -277,6 +312,7 @@ def main() -> int:
     ap.add_argument("--render-spp", type=int, default=64, help="path-tracer leg samples per pixel (0: skip)")
     ap.add_argument("--guide-spp", type=int, default=32, help="guiding leg samples per render (0: skip)")
     ap.add_argument("--batch-spp", type=int, default=16, help="scene-batch leg samples per pixel (0: skip)")
+    ap.add_argument("--no-mlp", action="store_true", help="skip the guiding-network leg")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=-1)
@@ -393,6 +429,7 @@ def main() -> int:
     pt = path_tracer_leg(ctx, args.render_spp) if args.render_spp > 0 else None
     gl = guiding_leg(ctx, args.guide_spp) if args.guide_spp > 0 else None
     bl = batch_leg(ctx, args.batch_spp) if args.batch_spp > 0 else None
+    ml = mlp_leg(ctx) if not args.no_mlp else None
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -416,6 +453,7 @@ def main() -> int:
         "path_tracer": pt,
         "guiding": gl,
         "batch": bl,
+        "mlp": ml,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -167,6 +167,8 @@ SIGNATURES = {
     "bg_guide_set_bounds": (C.c_int, [P, P, P]),
     "bg_guide_eval": (C.c_int, [P, P, C.c_uint64, P]),
     "bg_guide_train": (C.c_int, [P, P, P, C.c_uint64, C.c_float, C.POINTER(C.c_double), P]),
+    "bg_guide_eval_device": (C.c_int, [P, P, C.c_uint64, P, C.POINTER(C.c_double)]),
+    "bg_guide_train_device": (C.c_int, [P, P, P, C.c_uint64, C.c_float, C.POINTER(C.c_double)]),
     "bg_render_film_guided": (C.c_int, [P, P, C.POINTER(bg_render_desc), P, C.POINTER(bg_guiding_desc), P,
                                         C.POINTER(bg_stats)]),
     "bg_guide_records": (C.c_int, [P, P, P, C.c_uint64, C.POINTER(C.c_uint64)]),

@@ -181,4 +181,38 @@ int bg_guide_train(bg_guide* g, const float* samples, const float* targets, uint
     return BG_OK;
 }
 
+int bg_guide_eval_device(bg_guide* g, const void* d_samples, uint64_t n, void* d_out, double* kernel_ms) {
+    if (!g || (n && (!d_samples || !d_out))) return BG_ERR_INVALID;
+    bg_ctx* ctx = g->ctx;
+    hipStream_t s = ctx->stream;
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
+    HIP_TRY(ctx, bg::guide_infer(g->dev, g->box, (const float4*)d_samples, n, (float*)d_out, grid_of(g), s));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    float ms = 0.0f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (kernel_ms) *kernel_ms = ms;
+    return BG_OK;
+}
+
+int bg_guide_train_device(bg_guide* g, const void* d_samples, const void* d_targets, uint64_t n, float lr,
+                          double* kernel_ms) {
+    if (!g || (n && (!d_samples || !d_targets))) return BG_ERR_INVALID;
+    bg_ctx* ctx = g->ctx;
+    if (n > g->scratch.cap) return fail(ctx, BG_ERR_INVALID, "bg_guide_train_device: n exceeds max_batch");
+    hipStream_t s = ctx->stream;
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
+    if (n) {
+        ++g->step;
+        HIP_TRY(ctx, bg::guide_train_step(g->dev, g->box, (const float4*)d_samples, (const float4*)d_targets, n,
+                                          g->scratch, lr, g->step, grid_of(g), nullptr, s));
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    float ms = 0.0f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (kernel_ms) *kernel_ms = ms;
+    return BG_OK;
+}
+
 }  // extern "C"

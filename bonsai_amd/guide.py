@@ -84,3 +84,15 @@ class GuideMLP:
         N.check(N.lib().bg_guide_train(self.h, samples.ctypes.data, targets.ctypes.data, n, float(lr),
                                        C.byref(loss), grad.ctypes.data if want_grad else None), self.ctx.h)
         return loss.value, grad
+
+    def eval_device(self, d_samples: int, n: int, d_out: int) -> float:
+        """Raw outputs of n device-resident samples into device memory; returns the kernel ms."""
+        ms = C.c_double(0.0)
+        N.check(N.lib().bg_guide_eval_device(self.h, d_samples, n, d_out, C.byref(ms)), self.ctx.h)
+        return ms.value
+
+    def train_device(self, d_samples: int, d_targets: int, n: int, lr: float = 1e-2) -> float:
+        """One Adam step on device-resident samples and targets; returns the kernel ms."""
+        ms = C.c_double(0.0)
+        N.check(N.lib().bg_guide_train_device(self.h, d_samples, d_targets, n, float(lr), C.byref(ms)), self.ctx.h)
+        return ms.value

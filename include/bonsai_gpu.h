@@ -319,6 +319,12 @@ int bg_guide_eval(bg_guide* guide, const float* samples, uint64_t n, float* out)
 int bg_guide_train(bg_guide* guide, const float* samples, const float* targets, uint64_t n, float lr,
                    double* mean_loss, float* grad);
 
+/* The same on device-resident buffers (same layouts), on the context stream;
+ * kernel_ms (optional) receives the HIP-event time of the kernels. */
+int bg_guide_eval_device(bg_guide* guide, const void* d_samples, uint64_t n, void* d_out, double* kernel_ms);
+int bg_guide_train_device(bg_guide* guide, const void* d_samples, const void* d_targets, uint64_t n, float lr,
+                          double* kernel_ms);
+
 /* Guided rendering (BG_MODE_CORRECTED only). At every path vertex the
  * continuation direction comes from the network's mixture with probability
  * select_prob, else from the BSDF, and is weighted by f |cos| / (select_prob *

@@ -196,3 +196,22 @@ def test_batch_bound_is_enforced(ctx):
     with pytest.raises(N.BonsaiError):
         g.eval(pack_samples(p, wo, nrm))
     assert N_PARAMS == G.N_PARAMS
+
+
+def test_device_entry_points_match_host_ones(ctx):
+    from hipmem import DeviceArray
+
+    flat = params(11)
+    p, wo, nrm, w, weight = batch(12, 1000)
+    s = pack_samples(p, wo, nrm)
+    targets = np.concatenate([w, weight[:, None]], 1).astype(np.float32)
+    a = make(ctx, flat, 1024)
+    b = make(ctx, flat, 1024)
+    ds, dt, out = DeviceArray.from_numpy(s), DeviceArray.from_numpy(targets), DeviceArray(1000 * 40 * 4)
+    assert b.eval_device(ds.ptr, 1000, out.ptr) > 0.0
+    np.testing.assert_array_equal(out.to_numpy(np.float32, (1000, 40)), a.eval(s))
+    a.train(s, targets)
+    b.train_device(ds.ptr, dt.ptr, 1000)
+    np.testing.assert_array_equal(a.params()[0], b.params()[0])
+    for d in (ds, dt, out):
+        d.free()
