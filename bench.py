@@ -313,6 +313,8 @@ def main() -> int:
     ap.add_argument("--guide-spp", type=int, default=32, help="guiding leg samples per render (0: skip)")
     ap.add_argument("--batch-spp", type=int, default=16, help="scene-batch leg samples per pixel (0: skip)")
     ap.add_argument("--no-mlp", action="store_true", help="skip the guiding-network leg")
+    ap.add_argument("--skip-intersection", action="store_true",
+                    help="run only the other legs (profiling them alone); value is then null")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=-1)
@@ -348,81 +350,83 @@ def main() -> int:
         b, s, c, v = ctx.launch()
         ctx.set_launch(args.block or b, args.stack or s, c if args.blocks_per_cu < 0 else args.blocks_per_cu,
                        v if args.variant < 0 else args.variant)
-    t0 = time.perf_counter()
-    scene = intersection_scene(LEVEL)
-    gen_s = time.perf_counter() - t0
-    bvh = GpuBvh(ctx, scene)
-    log(f"scene: {scene.n_tris} tris, {bvh.info.n_nodes} nodes, height {bvh.info.max_depth}, "
-        f"build+upload {bvh.info.build_ms:.0f} ms, generation {gen_s * 1e3:.0f} ms")
+    value, wall, breakdown, roofline, cpu, scene = None, 0.0, None, None, None, None
+    queries = []
+    if not args.skip_intersection:
+        t0 = time.perf_counter()
+        scene = intersection_scene(LEVEL)
+        gen_s = time.perf_counter() - t0
+        bvh = GpuBvh(ctx, scene)
+        log(f"scene: {scene.n_tris} tris, {bvh.info.n_nodes} nodes, height {bvh.info.max_depth}, "
+            f"build+upload {bvh.info.build_ms:.0f} ms, generation {gen_s * 1e3:.0f} ms")
 
-    dev = torch.device("cuda:0")
-    cam = intersection_camera(4096, 4096)
-    coh = torch.empty((N_RAYS, 8), dtype=torch.float32, device=dev)
-    ctx.camera_rays_device(cam, coh.data_ptr())
-    lo, hi = scene.bounds()
-    inc_host = incoherent_rays(N_RAYS, lo, hi, 99)
-    inc = torch.from_numpy(inc_host).to(dev)
-    hits = torch.empty((N_RAYS, 4), dtype=torch.int32, device=dev)
-    torch.cuda.synchronize()
-    queries = [("coherent_closest", coh, False), ("coherent_any", coh, True),
-               ("incoherent_closest", inc, False), ("incoherent_any", inc, True)]
+        dev = torch.device("cuda:0")
+        cam = intersection_camera(4096, 4096)
+        coh = torch.empty((N_RAYS, 8), dtype=torch.float32, device=dev)
+        ctx.camera_rays_device(cam, coh.data_ptr())
+        lo, hi = scene.bounds()
+        inc_host = incoherent_rays(N_RAYS, lo, hi, 99)
+        inc = torch.from_numpy(inc_host).to(dev)
+        hits = torch.empty((N_RAYS, 4), dtype=torch.int32, device=dev)
+        torch.cuda.synchronize()
+        queries = [("coherent_closest", coh, False), ("coherent_any", coh, True),
+                   ("incoherent_closest", inc, False), ("incoherent_any", inc, True)]
 
-    # algorithmic bytes per launch from the kernel's counting mode (untimed)
-    counts = {}
-    for name, rays, any_hit in queries:
-        st = bvh.intersect_device(rays.data_ptr(), N_RAYS, hits.data_ptr(), any_hit, count=True)
-        counts[name] = (alg_bytes(N_RAYS, st), st.nodes_visited / N_RAYS, st.tris_tested / N_RAYS)
-        log(f"{name}: {st.nodes_visited / N_RAYS:.1f} box tests/ray, {st.tris_tested / N_RAYS:.1f} tri tests/ray")
-
-    def step(record):
+        # algorithmic bytes per launch from the kernel's counting mode (untimed)
+        counts = {}
         for name, rays, any_hit in queries:
-            st = bvh.intersect_device(rays.data_ptr(), N_RAYS, hits.data_ptr(), any_hit)
-            if record is not None:
-                record.setdefault(name, []).append(st.kernel_ms)
+            st = bvh.intersect_device(rays.data_ptr(), N_RAYS, hits.data_ptr(), any_hit, count=True)
+            counts[name] = (alg_bytes(N_RAYS, st), st.nodes_visited / N_RAYS, st.tris_tested / N_RAYS)
+            log(f"{name}: {st.nodes_visited / N_RAYS:.1f} box tests/ray, {st.tris_tested / N_RAYS:.1f} tri tests/ray")
 
-    for _ in range(args.warmup):
-        step(None)
-    torch.cuda.synchronize()
-    kms: dict = {}
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step(kms)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    total = N_RAYS * len(queries) * args.steps
-    value = total / wall / 1e6
-    breakdown = {}
-    for name, _, _ in queries:
-        ms = float(np.mean(kms[name]))
-        b = counts[name][0]
-        breakdown[name] = {"Mrays_s": round(N_RAYS / ms / 1e3, 1), "kernel_ms": round(ms, 3),
-                           "alg_GB_s": round(b / ms / 1e6, 1), "box_tests_per_ray": round(counts[name][1], 2),
-                           "tri_tests_per_ray": round(counts[name][2], 2)}
-    dom = max(breakdown, key=lambda k: breakdown[k]["kernel_ms"])
-    achieved = breakdown[dom]["alg_GB_s"]
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": (traffic_from(args.traffic_from, dom.endswith("_any"), 1 if dom.startswith("incoherent") else 0)
-                            if args.traffic_from else None),
-                "kernel": f"k_traverse ({dom})",
-                "bytes_per_launch": counts[dom][0]}
+        def step(record):
+            for name, rays, any_hit in queries:
+                st = bvh.intersect_device(rays.data_ptr(), N_RAYS, hits.data_ptr(), any_hit)
+                if record is not None:
+                    record.setdefault(name, []).append(st.kernel_ms)
 
-    cpu = None
-    if not args.no_cpu:
-        sets = [(name, (coh if name.startswith("coherent") else None), a) for name, _, a in queries]
-        coh_host = coh.cpu().numpy()
-        sets = [(n, coh_host if n.startswith("coherent") else inc_host, a) for n, _, a in sets]
-        cpu, samples, outs = cpu_baseline(scene, bvh, sets, args.cpu_seconds)
-        # the sampled GPU results must equal the oracle's
-        for (r, a), ref in zip(samples, outs):
-            g = (bvh.intersect_test(r) if a else bvh.intersect(r)).view(np.uint32).reshape(-1, 4)
-            if not np.array_equal(g[:, 0], ref[:, 0]):
-                log("PARITY FAILURE on the CPU-baseline sample")
-                cpu["parity"] = False
-                break
-        else:
-            cpu["parity"] = True
+        for _ in range(args.warmup):
+            step(None)
+        torch.cuda.synchronize()
+        kms: dict = {}
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step(kms)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        total = N_RAYS * len(queries) * args.steps
+        value = total / wall / 1e6
+        breakdown = {}
+        for name, _, _ in queries:
+            ms = float(np.mean(kms[name]))
+            b = counts[name][0]
+            breakdown[name] = {"Mrays_s": round(N_RAYS / ms / 1e3, 1), "kernel_ms": round(ms, 3),
+                               "alg_GB_s": round(b / ms / 1e6, 1), "box_tests_per_ray": round(counts[name][1], 2),
+                               "tri_tests_per_ray": round(counts[name][2], 2)}
+        dom = max(breakdown, key=lambda k: breakdown[k]["kernel_ms"])
+        achieved = breakdown[dom]["alg_GB_s"]
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": (traffic_from(args.traffic_from, dom.endswith("_any"), 1 if dom.startswith("incoherent") else 0)
+                                if args.traffic_from else None),
+                    "kernel": f"k_traverse ({dom})",
+                    "bytes_per_launch": counts[dom][0]}
 
+        cpu = None
+        if not args.no_cpu:
+            sets = [(name, (coh if name.startswith("coherent") else None), a) for name, _, a in queries]
+            coh_host = coh.cpu().numpy()
+            sets = [(n, coh_host if n.startswith("coherent") else inc_host, a) for n, _, a in sets]
+            cpu, samples, outs = cpu_baseline(scene, bvh, sets, args.cpu_seconds)
+            # the sampled GPU results must equal the oracle's
+            for (r, a), ref in zip(samples, outs):
+                g = (bvh.intersect_test(r) if a else bvh.intersect(r)).view(np.uint32).reshape(-1, 4)
+                if not np.array_equal(g[:, 0], ref[:, 0]):
+                    log("PARITY FAILURE on the CPU-baseline sample")
+                    cpu["parity"] = False
+                    break
+            else:
+                cpu["parity"] = True
     b, s_, c, v_ = ctx.launch()
     hits = coh = inc = None
     torch.cuda.empty_cache()
@@ -432,12 +436,12 @@ def main() -> int:
     ml = mlp_leg(ctx) if not args.no_mlp else None
     out = {
         "metric": METRIC,
-        "value": round(value, 2),
+        "value": round(value, 2) if value is not None else None,
         "unit": "Mrays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "ms_per_step": round(wall / args.steps * 1e3, 3) if value is not None else None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -445,7 +449,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": "intersection microbenchmark: 2^24 coherent + 2^24 incoherent rays, closest-hit "
                                "and any-hit, SAH BVH over a seeded displaced icosphere",
-                   "triangles": scene.n_tris, "rays_per_set": N_RAYS, "queries_per_step": N_RAYS * len(queries),
+                   "triangles": scene.n_tris if scene else None, "rays_per_set": N_RAYS,
+                   "queries_per_step": N_RAYS * len(queries),
                    "launch": {"block": b, "stack": s_, "blocks_per_cu": c, "variant": v_}},
         "breakdown": breakdown,
         "roofline": roofline,
