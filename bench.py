@@ -61,10 +61,13 @@ def traffic_from(prof_dir: str, any_hit: bool, set_index: int):
         files = glob.glob(os.path.join(prof_dir, pattern))
         return list(csv.DictReader(open(files[0]))) if files else []
 
+    import re
+    # timed traversal instantiations: <BLOCK, STACK, ANY, COUNT=false, ...>
+    tag = re.compile(r"k_trav\w*<\d+, \d+, " + ("true" if any_hit else "false") + ", false")
+
     def kernel_values(pattern, counter):
-        tag = f"k_traverse<256, 16, {'true' if any_hit else 'false'}, false"
         vals = [float(r["Counter_Value"]) for r in rows(pattern)
-                if r["Counter_Name"] == counter and tag in r["Kernel_Name"]]
+                if r["Counter_Name"] == counter and tag.search(r["Kernel_Name"])]
         return vals[set_index] if len(vals) > set_index else None
 
     def copy_factor(pattern, counter):

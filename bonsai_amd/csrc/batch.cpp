@@ -2,8 +2,8 @@
 // (no counterpart in the reference, whose SamplerIntegrator::render,
 // base/integrator.rs:35-133, renders one scene per call).
 //
-// The scenes' device data are concatenated once: BVH nodes with their child
-// and leaf offsets rebased, leaf slots mapped to batch-wide primitive ids
+// The scenes' device data are concatenated once: pair records and leaf tables
+// with their references and slot offsets rebased, leaf slots mapped to batch-wide primitive ids
 // (every triangle of every scene, then every sphere), shading records with
 // material and light indices rebased. A path then needs only its item (in its
 // flags) and its BVH root (per path) to run the single-scene kernels. One pass
@@ -53,10 +53,11 @@ struct bg_batch {
     std::vector<uint64_t> item_pix_base;
     uint64_t total_pix = 0;
     // concatenated scene data
-    uint64_t n_tris = 0, n_spheres = 0, n_nodes = 0, n_slots = 0;
+    uint64_t n_tris = 0, n_spheres = 0, n_nodes = 0, n_slots = 0, n_pairs = 0, n_leaves = 0;
     uint32_t n_lights = 0, max_depth = 0;
     std::vector<uint32_t> scene_root, scene_light_base, scene_n_lights;
-    float4* nodes = nullptr;
+    float4* pairs = nullptr;
+    uint2* leaves = nullptr;
     float* tri_verts = nullptr;
     uint32_t* slot_prim = nullptr;
     bg::SphereDev* spheres = nullptr;
@@ -83,7 +84,7 @@ struct bg_batch {
 namespace {
 
 void free_batch(bg_batch* b) {
-    void* ptrs[] = {b->nodes, b->tri_verts, b->slot_prim, b->spheres, b->sphere_aux, b->tri_orig, b->tri_flip,
+    void* ptrs[] = {b->pairs, b->leaves, b->tri_verts, b->slot_prim, b->spheres, b->sphere_aux, b->tri_orig, b->tri_flip,
                     b->prim_mat, b->prim_light, b->mats, b->lights, b->sph_shade, b->d_items, b->film,
                     b->d_pass, b->d_queue, b->d_root};
     for (void* p : ptrs)
@@ -93,7 +94,9 @@ void free_batch(bg_batch* b) {
 bg::TraverseArgs batch_args(const bg_batch* bt, const bg_ctx* ctx, const float4* rays, uint4* hits, uint64_t n,
                             const uint32_t* roots, bool queue) {
     bg::TraverseArgs a;
-    a.nodes = bt->nodes;
+    a.nodes = nullptr;  // batches traverse the pair layout only (variants 0 and 2)
+    a.pairs = bt->pairs;
+    a.leaves = bt->leaves;
     a.tri_verts = bt->tri_verts;
     a.slot_prim = bt->slot_prim;
     a.spheres = bt->spheres;
@@ -123,7 +126,9 @@ hipError_t traverse(const bg_batch* bt, bg_ctx* ctx, const bg::LaunchShape& sh, 
         hipError_t e = bg::launch_zero_u64(bt->d_queue, st);
         if (e != hipSuccess) return e;
     }
-    return bg::launch_traverse(batch_args(bt, ctx, rays, hits, n, roots, queue), any, false, sh, grid_for(ctx, sh, n), st);
+    bg::LaunchShape shape = sh;
+    if (shape.variant != 2) shape.variant = 0;  // the node-record variants need node roots
+    return bg::launch_traverse(batch_args(bt, ctx, rays, hits, n, roots, queue), any, false, shape, grid_for(ctx, shape, n), st);
 }
 
 }  // namespace
@@ -162,9 +167,17 @@ int bg_batch_create(bg_ctx* ctx, const bg_batch_item* items, uint32_t n, bg_batc
     // bases of every unique scene in the concatenated arrays
     const size_t U = uniq.size();
     std::vector<uint64_t> node_base(U), slot_base(U), tri_base(U), sph_base(U), mat_base(U), light_base(U);
+    std::vector<uint64_t> pair_base(U), leaf_base(U);
+    std::vector<std::vector<bg::PairRecord>> spairs(U);
+    std::vector<std::vector<uint32_t>> sleaves(U);
     uint64_t n_mats = 0;
     for (size_t u = 0; u < U; ++u) {
         const bg_scene* s = uniq[u];
+        bg::build_pairs(s->bvh, &spairs[u], &sleaves[u]);
+        pair_base[u] = bt->n_pairs;
+        leaf_base[u] = bt->n_leaves;
+        bt->n_pairs += spairs[u].size();
+        bt->n_leaves += sleaves[u].size() / 2;
         node_base[u] = bt->n_nodes;
         slot_base[u] = bt->n_slots;
         tri_base[u] = bt->n_tris;
@@ -178,25 +191,36 @@ int bg_batch_create(bg_ctx* ctx, const bg_batch_item* items, uint32_t n, bg_batc
         n_mats += s->materials.size();
         bt->n_lights += (uint32_t)s->lights.size();
         bt->max_depth = std::max(bt->max_depth, s->bvh.max_depth);
-        bt->scene_root.push_back((uint32_t)node_base[u]);
+        bt->scene_root.push_back((uint32_t)pair_base[u]);  // the scene's virtual root record
         bt->scene_light_base.push_back((uint32_t)light_base[u]);
         bt->scene_n_lights.push_back((uint32_t)s->lights.size());
     }
-    if (bt->n_nodes >= 0xFFFFFFFFull || bt->n_tris + bt->n_spheres >= 0x7FFFFFFFull) {
+    if (bt->n_pairs >= 0x7FFFFFFFull || bt->n_leaves >= 0x7FFFFFFFull || bt->n_tris + bt->n_spheres >= 0x7FFFFFFFull) {
         delete bt;
         return fail(ctx, BG_ERR_INVALID, "batch too large for 32-bit node and primitive ids");
     }
     const uint64_t np = bt->n_tris + bt->n_spheres;
-    std::vector<bg_bvh_node> nodes(bt->n_nodes);
+    std::vector<bg::PairRecord> pairs(bt->n_pairs);
+    std::vector<uint32_t> leaves(2 * bt->n_leaves);
     std::vector<uint32_t> slots(bt->n_slots), pmat(np);
     std::vector<int32_t> plight(np, -1);
     std::vector<bg::LightDev> lts(bt->n_lights);
     for (size_t u = 0; u < U; ++u) {
         const bg_scene* s = uniq[u];
-        for (size_t k = 0; k < s->bvh.nodes.size(); ++k) {
-            bg_bvh_node nd = s->bvh.nodes[k];
-            nd.offset += (nd.meta >> 3) > 0 ? (uint32_t)slot_base[u] : (uint32_t)node_base[u];
-            nodes[node_base[u] + k] = nd;
+        auto rebase = [&](uint32_t r) -> uint32_t {
+            if (r == bg::kRefNone) return r;
+            if (r & bg::kRefLeaf) return bg::kRefLeaf | ((r & ~bg::kRefLeaf) + (uint32_t)leaf_base[u]);
+            return r + (uint32_t)pair_base[u];
+        };
+        for (size_t k = 0; k < spairs[u].size(); ++k) {
+            bg::PairRecord r = spairs[u][k];
+            r.ref0 = rebase(r.ref0);
+            r.ref1 = rebase(r.ref1);
+            pairs[pair_base[u] + k] = r;
+        }
+        for (size_t k = 0; k < sleaves[u].size() / 2; ++k) {
+            leaves[2 * (leaf_base[u] + k)] = sleaves[u][2 * k] + (uint32_t)slot_base[u];
+            leaves[2 * (leaf_base[u] + k) + 1] = sleaves[u][2 * k + 1];
         }
         const uint64_t nt = s->n_tris;
         auto global = [&](uint64_t p) -> uint64_t {
@@ -217,7 +241,8 @@ int bg_batch_create(bg_ctx* ctx, const bg_batch_item* items, uint32_t n, bg_batc
             rc = fail(ctx, BG_ERR_HIP, "batch upload failed");
         bytes += count * sizeof(**dst);
     };
-    up((bg_bvh_node**)&bt->nodes, nodes.data(), nodes.size());
+    up((bg::PairRecord**)&bt->pairs, pairs.data(), pairs.size());
+    up((uint32_t**)&bt->leaves, leaves.data(), leaves.size());
     up(&bt->slot_prim, slots.data(), slots.size());
     up(&bt->prim_mat, pmat.data(), pmat.size());
     up(&bt->prim_light, plight.data(), plight.size());
@@ -361,13 +386,13 @@ int bg_render_batch(bg_ctx* ctx, bg_batch* bt, const bg_batch_desc* d, bg_stats*
     std::vector<bg::LaunchShape> cands;
     if (d->autotune && !bt->tuned)
         for (int block : {128, 256})
-            for (int stack : {8, 16, 32})
-                for (int bpc : {4, 8, 16})
+            for (int stack : {8, 16})
+                for (int bpc : {8, 16, 32, 64})
                     for (int variant : {0, 2}) cands.push_back(bg::LaunchShape{block, stack, bpc, variant});
     // spill rows for the largest grid any launch shape of this call uses
     uint64_t max_threads = (uint64_t)grid_for(ctx, bt->shape, cap) * bt->shape.block;
     for (const bg::LaunchShape& c : cands) max_threads = std::max<uint64_t>(max_threads, (uint64_t)grid_for(ctx, c, cap) * c.block);
-    rc = bg::ensure_spill(ctx, (size_t)max_threads * (bt->max_depth + 1) * sizeof(uint32_t));
+    rc = bg::ensure_spill(ctx, (size_t)max_threads * (bt->max_depth + 1) * bg::kSpillWords * sizeof(uint32_t));
     if (rc) return rc;
 
     if (d->autotune && !bt->tuned) {

@@ -19,6 +19,20 @@ struct HostBVH {
     uint32_t max_leaf = 0;
 };
 
+// Traversal layout with the boxes of both children in their parent (64 bytes,
+// four 16-byte loads): b[0..5] child 0 (lo, hi), b[6..11] child 1, then the two
+// child references and the split axis (3: no split, child 0 first). A
+// reference is a record index, or kRefLeaf | leaf id into the leaf table
+// (first slot, count << 1 | has_sphere), or kRefNone. Record 0 of a scene is a
+// virtual root whose child 0 is the BVH root (its box is the root's box).
+struct PairRecord {
+    float b[12];
+    uint32_t ref0, ref1, axis, pad;
+};
+constexpr uint32_t kRefLeaf = 0x80000000u, kRefNone = 0xFFFFFFFFu;
+// Words of one traversal stack entry (reference, entry distance) in the spill buffer.
+constexpr size_t kSpillWords = 2;
+
 // Per-primitive world bounds, in scene order (triangles then spheres).
 struct PrimBounds {
     float mn[3], mx[3];
@@ -30,6 +44,8 @@ struct PrimBounds {
 // depend on it.
 void build_bvh(const std::vector<PrimBounds>& bounds, const std::vector<uint8_t>& is_sphere, int threads,
                HostBVH* out);
+// The pair-record layout of a built BVH (same tree, same DFS order).
+void build_pairs(const HostBVH& bvh, std::vector<PairRecord>* pairs, std::vector<uint32_t>* leaves);
 
 // Sphere record in HBM: world_to_object rows 0-2 (the transform is affine),
 // radius, z_min, z_max, phi_max (radians). 64 bytes.
@@ -56,17 +72,21 @@ struct TraverseArgs {
     uint32_t* spill;           // per-thread overflow of the LDS stack
     uint32_t spill_depth;
     unsigned long long* counters;  // [nodes, tris, spheres] when counting
-    const uint32_t* roots = nullptr;   // per ray: BVH root node (scene batches); node 0 when null
+    const float4* pairs = nullptr;     // PairRecord array (4 float4 each)
+    const uint2* leaves = nullptr;     // leaf table of the pair layout
+    const uint32_t* roots = nullptr;   // per ray: root record (scene batches); record/node 0 when null
     unsigned long long* queue = nullptr;  // work-queue counter (variant 2), zero at launch
 };
 
-// Default = the best of a sweep over block {64,128,256} x stack {8,16,32} x
-// blocks/CU {4,8,16} x variant on the 2^24-ray intersection workload.
+// Default = the best of a sweep over block {64,128,256} x stack {8,16} x
+// blocks/CU {8,16,32,64,0} x variant on the 2^24-ray intersection workload.
 struct LaunchShape {
     int block = 256;
-    int stack = 16;
-    int blocks_per_cu = 16;
-    int variant = 0;  // 0: one ray per loop round, 1: per-lane refill, 2: rays taken from an atomic work queue
+    int stack = 8;
+    int blocks_per_cu = 32;
+    // 0: pair records, one ray per loop round; 1: node records, per-lane refill; 2: as 0 with rays
+    // taken from an atomic work queue; 3: node records, one ray per loop round
+    int variant = 0;
 };
 
 // traverse.hip
@@ -283,6 +303,8 @@ struct bg_scene {
     uint64_t n_tris = 0, n_spheres = 0;
     // device
     float4* d_nodes = nullptr;
+    float4* d_pairs = nullptr;
+    uint2* d_leaves = nullptr;
     float* d_tri_verts = nullptr;
     uint32_t* d_slot_prim = nullptr;
     bg::SphereDev* d_spheres = nullptr;

@@ -9,6 +9,7 @@
 // buckets start empty. Float arithmetic is kept in the reference's order so the
 // layout is reproducible on any host.
 #include <algorithm>
+#include <limits>
 #include <cfloat>
 #include <cstring>
 #include <future>
@@ -239,6 +240,53 @@ void build_bvh(const std::vector<PrimBounds>& bounds, const std::vector<uint8_t>
     out->slot_prim = std::move(root.order);
     out->max_depth = root.depth;
     out->max_leaf = root.max_leaf;
+}
+
+
+void build_pairs(const HostBVH& bvh, std::vector<PairRecord>* pairs, std::vector<uint32_t>* leaves) {
+    const size_t n = bvh.nodes.size();
+    std::vector<uint32_t> ref(n);
+    uint32_t n_int = 0, n_leaf = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if ((bvh.nodes[i].meta >> 3) == 0) ref[i] = 1 + n_int++;
+        else ref[i] = kRefLeaf | n_leaf++;
+    }
+    pairs->assign(1 + n_int, PairRecord{});
+    leaves->assign(2 * (size_t)n_leaf, 0u);
+    const float qnan = std::numeric_limits<float>::quiet_NaN();
+    auto put_box = [](PairRecord& r, int k, const bg_bvh_node& nd) {
+        for (int c = 0; c < 3; ++c) {
+            r.b[6 * k + c] = nd.bmin[c];
+            r.b[6 * k + 3 + c] = nd.bmax[c];
+        }
+    };
+    // the virtual root: child 0 is the BVH root, child 1 a NaN box that no ray enters
+    PairRecord& root = (*pairs)[0];
+    for (int c = 0; c < 12; ++c) root.b[c] = qnan;
+    root.ref0 = root.ref1 = kRefNone;
+    root.axis = 3;
+    if (n > 0) {
+        put_box(root, 0, bvh.nodes[0]);
+        root.ref0 = ref[0];
+    }
+    for (size_t i = 0; i < n; ++i) {
+        const bg_bvh_node& nd = bvh.nodes[i];
+        const uint32_t count = nd.meta >> 3;
+        if (count > 0) {
+            const uint32_t id = ref[i] & ~kRefLeaf;
+            (*leaves)[2 * id] = nd.offset;
+            (*leaves)[2 * id + 1] = count << 1 | ((nd.meta >> 2) & 1u);
+            continue;
+        }
+        PairRecord& r = (*pairs)[ref[i]];
+        const size_t c0 = i + 1, c1 = nd.offset;
+        put_box(r, 0, bvh.nodes[c0]);
+        put_box(r, 1, bvh.nodes[c1]);
+        r.ref0 = ref[c0];
+        r.ref1 = ref[c1];
+        r.axis = nd.meta & 3u;
+        r.pad = 0;
+    }
 }
 
 }  // namespace bg

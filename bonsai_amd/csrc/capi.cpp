@@ -79,11 +79,13 @@ int run_traverse(bg_ctx* ctx, const bg_scene* s, const void* d_rays, size_t n, v
     }
     const int grid = traverse_grid(ctx, n);
     const uint32_t depth = s->bvh.max_depth + 1;
-    const size_t spill_need = (size_t)grid * ctx->shape.block * depth * sizeof(uint32_t);
+    const size_t spill_need = (size_t)grid * ctx->shape.block * depth * bg::kSpillWords * sizeof(uint32_t);
     int rc = ensure(ctx, (void**)&ctx->spill, &ctx->spill_bytes, spill_need);
     if (rc) return rc;
     bg::TraverseArgs a;
     a.nodes = s->d_nodes;
+    a.pairs = s->d_pairs;
+    a.leaves = s->d_leaves;
     a.tri_verts = s->d_tri_verts;
     a.slot_prim = s->d_slot_prim;
     a.spheres = s->d_spheres;
@@ -324,6 +326,13 @@ int bg_scene_create(bg_ctx* ctx, const bg_scene_desc* d, bg_scene** out) {
     int rc = BG_OK;
     if (!s->bvh.nodes.empty())
         rc = upload(ctx, (bg_bvh_node**)&s->d_nodes, s->bvh.nodes.data(), s->bvh.nodes.size(), &bytes);
+    if (!s->bvh.nodes.empty()) {
+        std::vector<bg::PairRecord> pairs;
+        std::vector<uint32_t> leaves;
+        bg::build_pairs(s->bvh, &pairs, &leaves);
+        if (!rc) rc = upload(ctx, (bg::PairRecord**)&s->d_pairs, pairs.data(), pairs.size(), &bytes);
+        if (!rc) rc = upload(ctx, (uint32_t**)&s->d_leaves, leaves.data(), leaves.size(), &bytes);
+    }
     if (!rc) rc = upload(ctx, &s->d_tri_verts, verts.data(), verts.size(), &bytes);
     if (!rc) rc = upload(ctx, &s->d_slot_prim, s->bvh.slot_prim.data(), nslot, &bytes);
     if (!rc) rc = upload(ctx, &s->d_spheres, sph.data(), sph.size(), &bytes);
@@ -420,6 +429,8 @@ void bg_scene_destroy(bg_scene* s) {
     if (!s) return;
     if (s->ctx) hipSetDevice(s->ctx->device);
     if (s->d_nodes) hipFree(s->d_nodes);
+    if (s->d_pairs) hipFree(s->d_pairs);
+    if (s->d_leaves) hipFree(s->d_leaves);
     if (s->d_tri_verts) hipFree(s->d_tri_verts);
     if (s->d_slot_prim) hipFree(s->d_slot_prim);
     if (s->d_spheres) hipFree(s->d_spheres);

@@ -103,6 +103,34 @@ __device__ __forceinline__ bool slab(const RayState& r, float lx, float ly, floa
     return (t_min < r.t_max) && (t_max > 0.0f);
 }
 
+// The same test split at its last step: returns whether the box is entered at all
+// (every step but the t_max comparison) and its entry distance t_min (NaN when it
+// is not entered); slab() is then (returned value && t_min < ray t_max). Entry
+// distances stored on the stack give, at pop time and for any smaller t_max,
+// exactly the result a fresh slab() would give.
+__device__ __forceinline__ bool slab_entry(const RayState& r, float lx, float ly, float lz, float hx, float hy, float hz,
+                                           float* t_entry) {
+    float t_min = ((r.neg0 ? hx : lx) - r.ox) * r.ix;
+    float t_max = ((r.neg0 ? lx : hx) - r.ox) * r.ix;
+    const float ty_min = ((r.neg1 ? hy : ly) - r.oy) * r.iy;
+    float ty_max = ((r.neg1 ? ly : hy) - r.oy) * r.iy;
+    t_max *= kSlabPad;
+    ty_max *= kSlabPad;
+    *t_entry = __builtin_nanf("");
+    if (t_min > ty_max || ty_min > t_max) return false;
+    if (ty_min > t_min) t_min = ty_min;
+    if (ty_max < t_max) t_max = ty_max;
+    const float tz_min = ((r.neg2 ? hz : lz) - r.oz) * r.iz;
+    float tz_max = ((r.neg2 ? lz : hz) - r.oz) * r.iz;
+    tz_max *= kSlabPad;
+    if (t_min > tz_max || tz_min > t_max) return false;
+    if (tz_min > t_min) t_min = tz_min;
+    if (tz_max < t_max) t_max = tz_max;
+    if (!(t_max > 0.0f)) return false;
+    *t_entry = t_min;
+    return true;
+}
+
 // f64 cross product rounded to f32, then its f32 squared length (vec3.rs:108-122, 103-105).
 __device__ __forceinline__ float cross64_len2(float ax, float ay, float az, float bx, float by, float bz) {
     const double x = ax, y = ay, z = az, vx = bx, vy = by, vz = bz;

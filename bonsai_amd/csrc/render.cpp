@@ -129,6 +129,8 @@ int ensure_replay(bg_guide* g, uint32_t cap) {
 bg::TraverseArgs traverse_args(const bg_scene* s, bg_ctx* ctx, const float4* rays, uint4* hits, uint64_t n) {
     bg::TraverseArgs a;
     a.nodes = s->d_nodes;
+    a.pairs = s->d_pairs;
+    a.leaves = s->d_leaves;
     a.tri_verts = s->d_tri_verts;
     a.slot_prim = s->d_slot_prim;
     a.spheres = s->d_spheres;
@@ -222,7 +224,7 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
     const int block = ctx->shape.block;
     const uint64_t grid_cap = ctx->shape.blocks_per_cu > 0 ? (uint64_t)ctx->num_cus * ctx->shape.blocks_per_cu : (1u << 20);
     const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((cap + block - 1) / block, std::min<uint64_t>(grid_cap, 1u << 20)));
-    rc = bg::ensure_spill(ctx, (size_t)grid * block * (s->bvh.max_depth + 1) * sizeof(uint32_t));
+    rc = bg::ensure_spill(ctx, (size_t)grid * block * (s->bvh.max_depth + 1) * bg::kSpillWords * sizeof(uint32_t));
     if (rc) return rc;
 
     bg::RenderParamsDev p;

@@ -281,18 +281,179 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_refill(TraverseArgs a) {
     }
 }
 
+// Pair-record traversal (PairRecord): at an interior record both child boxes
+// are tested with one 64-byte fetch; the near child (by the sign of the ray
+// direction on the split axis, bvh.rs:331-339) is entered at once if its box
+// is hit, the far child is pushed with its entry distance and entered when
+// popped if that distance is still below t_max. This is the reference's visit
+// order and its box results, without fetching a far child only to find its box
+// missed. With COUNT, box tests are counted where the reference makes them (the
+// near child on arrival, the far child when popped), so the totals equal the
+// node-record kernels'. The triangle-only any-hit instantiations are held to 64
+// VGPRs (8 waves per SIMD, no spills); the others would spill there and run
+// faster at the register count the compiler picks (5-7 waves).
+template <int BLOCK, int STACK, bool ANY, bool COUNT, bool SPH, bool QUEUE>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(ANY && !SPH ? 8 : 1, 8)))
+void k_trav_pairs(TraverseArgs a) {
+    __shared__ uint2 lds[STACK * BLOCK];
+    const int tid = threadIdx.x;
+    const uint64_t gtid = (uint64_t)blockIdx.x * BLOCK + tid;
+    const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+    uint2* __restrict__ spill = reinterpret_cast<uint2*>(a.spill) + gtid * a.spill_depth;
+    uint2* __restrict__ ring = lds + tid;
+    constexpr uint32_t kMask = STACK - 1;
+    uint64_t c_nodes = 0, c_tris = 0, c_sph = 0;
+
+    constexpr uint32_t kChunk = 16;
+    const int lane = tid & 63;
+    uint64_t qbase = 0, ray = gtid;
+    uint32_t qk = kChunk;
+    auto next_queued = [&]() {
+        if (qk == kChunk) {
+            unsigned long long b0 = 0;
+            if (lane == 0) b0 = atomicAdd(a.queue, 64ull * kChunk);
+            qbase = __shfl(b0, 0);
+            qk = 0;
+        }
+        ray = qbase + (uint64_t)qk * 64 + lane;
+        ++qk;
+    };
+    if (QUEUE) next_queued();
+    while (QUEUE ? qbase < a.n_rays : ray < a.n_rays) {
+        if (ray < a.n_rays) {
+            const float4 r0 = a.rays[2 * ray];
+            const float4 r1 = a.rays[2 * ray + 1];
+            RayState rs;
+            rs.ox = r0.x; rs.oy = r0.y; rs.oz = r0.z; rs.t_max = r0.w;
+            rs.dx = r1.x; rs.dy = r1.y; rs.dz = r1.z;
+            ray_setup(rs);
+
+            int best = -1;
+            float bt = 0.0f, bb1 = 0.0f, bb2 = 0.0f;
+            uint32_t sp = 0, base = 0;
+            uint32_t ref = a.roots ? a.roots[ray] : 0u;  // the scene's virtual root record
+            for (;;) {
+                bool enter = false;
+                if (!(ref & kRefLeaf)) {
+                    const float4* P = a.pairs + 4 * (uint64_t)ref;
+                    const float4 q0 = P[0], q1 = P[1], q2 = P[2];
+                    const uint4 q3 = *reinterpret_cast<const uint4*>(P + 3);
+                    const bool swap = q3.z < 3u && (q3.z == 0 ? rs.neg0 : (q3.z == 1 ? rs.neg1 : rs.neg2));
+                    float t0, t1;
+                    const bool a0 = slab_entry(rs, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, &t0);
+                    const bool a1 = slab_entry(rs, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, &t1);
+                    const uint32_t rn = swap ? q3.y : q3.x, rf = swap ? q3.x : q3.y;
+                    const bool hn = (swap ? a1 : a0) && (swap ? t1 : t0) < rs.t_max;
+                    const bool af = swap ? a0 : a1;
+                    const float tf = swap ? t0 : t1;
+                    if (COUNT) ++c_nodes;  // the near child's box test
+                    if (rf != kRefNone && (COUNT || (af && tf < rs.t_max))) {
+                        if (sp - base == STACK) {
+                            spill[base] = ring[(base & kMask) * BLOCK];
+                            ++base;
+                        }
+                        ring[(sp & kMask) * BLOCK] = make_uint2(rf, __float_as_uint(af ? tf : __builtin_nanf("")));
+                        ++sp;
+                    }
+                    if (hn && rn != kRefNone) {
+                        ref = rn;
+                        enter = true;
+                    }
+                } else {
+                    const uint2 lf = a.leaves[ref & ~kRefLeaf];
+                    const uint32_t off = lf.x, count = lf.y >> 1;
+                    const bool mixed = (lf.y & 1u) != 0;
+                    bool done = false;
+                    for (uint32_t s = off; s < off + count; ++s) {
+                        float t, b1, b2;
+                        bool hit;
+                        uint32_t pid = 0;
+                        if (SPH && mixed && (pid = a.slot_prim[s]) >= a.n_tris) {
+                            if (COUNT) ++c_sph;
+                            const SphereDev& sp_ = a.spheres[pid - a.n_tris];
+                            SphereHit h;
+                            hit = sphere_hit(rs, sp_.w2o, sp_.radius, sp_.z_min, sp_.z_max, sp_.phi_max, &h);
+                            if (hit) {
+                                t = h.t;
+                                b1 = h.phi / sp_.phi_max;
+                                const SphereAux ax = a.sphere_aux[pid - a.n_tris];
+                                float c = h.pz / sp_.radius;
+                                c = c < -1.0f ? -1.0f : (c > 1.0f ? 1.0f : c);
+                                const float theta = acosf(c);
+                                b2 = (theta - ax.theta_min) / (ax.theta_max - ax.theta_min);
+                            }
+                        } else {
+                            if (COUNT) ++c_tris;
+                            hit = tri_hit(rs, a.tri_verts + 9 * (uint64_t)s, &t, &b1, &b2);
+                        }
+                        if (hit) {
+                            best = (int)s;
+                            bt = t;
+                            if (ANY) {
+                                bb1 = 0.0f; bb2 = 0.0f;
+                                done = true;
+                                break;
+                            }
+                            bb1 = b1; bb2 = b2;
+                            rs.t_max = t;
+                        }
+                    }
+                    if (ANY && done) break;
+                }
+                if (enter) continue;
+                // next pushed entry that the current t_max still admits
+                bool found = false;
+                while (sp > 0) {
+                    --sp;
+                    uint2 e;
+                    if (sp >= base) {
+                        e = ring[(sp & kMask) * BLOCK];
+                    } else {
+                        e = spill[sp];
+                        base = sp;
+                    }
+                    if (COUNT) ++c_nodes;  // the far child's box test, where the reference makes it
+                    if (__uint_as_float(e.y) < rs.t_max) {
+                        ref = e.x;
+                        found = true;
+                        break;
+                    }
+                }
+                if (!found) break;
+            }
+            uint4 out;
+            out.x = best >= 0 ? a.slot_prim[best] : 0xFFFFFFFFu;
+            out.y = __float_as_uint(bt);
+            out.z = __float_as_uint(bb1);
+            out.w = __float_as_uint(bb2);
+            a.hits[ray] = out;
+        }
+        if (QUEUE) next_queued();
+        else ray += stride;
+    }
+    if (COUNT) {
+        atomicAdd(&a.counters[0], (unsigned long long)c_nodes);
+        atomicAdd(&a.counters[1], (unsigned long long)c_tris);
+        atomicAdd(&a.counters[2], (unsigned long long)c_sph);
+    }
+}
+
 template <int BLOCK, int STACK, bool ANY, bool COUNT>
 void launch4(const TraverseArgs& a, int variant, int grid, hipStream_t s) {
     // scenes without spheres use instantiations with the sphere code compiled out
-    // variant 2: the work-queue kernel (a.queue set and zeroed by the caller); counting runs use variant 0
+    // variants 0 and 2: pair records (2: work queue, a.queue set and zeroed by the caller; counting
+    // runs take rays in grid-stride order); 1: node records with per-lane refill; 3: node records
     const bool q = variant == 2 && a.queue != nullptr && !COUNT;
+    const bool pairs = (variant == 0 || variant == 2) && a.pairs != nullptr;
     if (a.spheres != nullptr) {
         if (variant == 1) hipLaunchKernelGGL((k_traverse_refill<BLOCK, STACK, ANY, COUNT, true>), dim3(grid), dim3(BLOCK), 0, s, a);
-        else if (q) hipLaunchKernelGGL((k_traverse<BLOCK, STACK, ANY, false, true, true>), dim3(grid), dim3(BLOCK), 0, s, a);
+        else if (pairs && q) hipLaunchKernelGGL((k_trav_pairs<BLOCK, STACK, ANY, false, true, true>), dim3(grid), dim3(BLOCK), 0, s, a);
+        else if (pairs) hipLaunchKernelGGL((k_trav_pairs<BLOCK, STACK, ANY, COUNT, true, false>), dim3(grid), dim3(BLOCK), 0, s, a);
         else hipLaunchKernelGGL((k_traverse<BLOCK, STACK, ANY, COUNT, true>), dim3(grid), dim3(BLOCK), 0, s, a);
     } else {
         if (variant == 1) hipLaunchKernelGGL((k_traverse_refill<BLOCK, STACK, ANY, COUNT, false>), dim3(grid), dim3(BLOCK), 0, s, a);
-        else if (q) hipLaunchKernelGGL((k_traverse<BLOCK, STACK, ANY, false, false, true>), dim3(grid), dim3(BLOCK), 0, s, a);
+        else if (pairs && q) hipLaunchKernelGGL((k_trav_pairs<BLOCK, STACK, ANY, false, false, true>), dim3(grid), dim3(BLOCK), 0, s, a);
+        else if (pairs) hipLaunchKernelGGL((k_trav_pairs<BLOCK, STACK, ANY, COUNT, false, false>), dim3(grid), dim3(BLOCK), 0, s, a);
         else hipLaunchKernelGGL((k_traverse<BLOCK, STACK, ANY, COUNT, false>), dim3(grid), dim3(BLOCK), 0, s, a);
     }
 }
@@ -324,7 +485,7 @@ hipError_t launch1(const TraverseArgs& a, bool any, bool count, int stack, int v
 bool traverse_shape_supported(const LaunchShape& s) {
     const bool b = s.block == 64 || s.block == 128 || s.block == 256;
     const bool st = s.stack == 8 || s.stack == 16 || s.stack == 32;
-    return b && st && s.blocks_per_cu >= 0 && (s.variant == 0 || s.variant == 1 || s.variant == 2);
+    return b && st && s.blocks_per_cu >= 0 && s.variant >= 0 && s.variant <= 3;
 }
 
 hipError_t launch_traverse(const TraverseArgs& a, bool any_hit, bool count, const LaunchShape& shape, int grid,

@@ -50,7 +50,7 @@ def test_batch_equals_per_scene(ctx, items, mode, graph):
         np.testing.assert_array_equal(w, ref[k][1])
         np.testing.assert_array_equal(rgb, ref[k][0])
     block, stack, bpc, variant = br.launch()
-    assert block in (128, 256) and stack in (8, 16, 32) and bpc in (4, 8, 16) and variant in (0, 2)
+    assert block in (128, 256) and stack in (8, 16) and bpc in (8, 16, 32, 64) and variant in (0, 2)
     br.close()
 
 

@@ -160,7 +160,8 @@ def deep_rays(n, seed):
 
 
 @pytest.mark.parametrize("block,stack,variant", [(64, 8, 0), (128, 16, 0), (256, 32, 0), (128, 8, 0), (64, 32, 0),
-                                                  (64, 8, 1), (128, 16, 1), (256, 32, 1)])
+                                                  (64, 8, 1), (128, 16, 1), (256, 32, 1), (64, 8, 3), (256, 16, 3),
+                                                  (128, 16, 2)])
 def test_launch_shapes_do_not_change_results(ctx, block, stack, variant):
     saved = ctx.launch()
     try:
@@ -179,11 +180,15 @@ def test_launch_shapes_do_not_change_results(ctx, block, stack, variant):
         ctx.set_launch(*saved)
 
 
+@pytest.mark.parametrize("variant", [0, 1, 3])
 @pytest.mark.parametrize("any_hit", [False, True])
-def test_visit_counts_equal_oracle(ctx, any_hit):
+def test_visit_counts_equal_oracle(ctx, any_hit, variant):
     """The kernel's box/primitive test counts (the basis of the algorithmic-bytes figure) equal the oracle's
     reference-order counts summed over the ray set, which holds only if every lane replays the reference visit
-    sequence."""
+    sequence: for the pair-record kernel (variant 0, which counts a far child's box test when it pops it,
+    where the reference makes it) and the node-record kernels (1, 3)."""
+    saved = ctx.launch()
+    ctx.set_launch(saved[0], saved[1], saved[2], variant)
     rng = np.random.default_rng(8)
     n = 2000
     c = rng.uniform(-2, 2, (n, 1, 3))
@@ -193,7 +198,10 @@ def test_visit_counts_equal_oracle(ctx, any_hit):
     rays = box_rays(100_000, -3, 3, 19, -2, 2)
     bvh = GpuBvh(ctx, s)
     o = oracle_for(bvh, s)
-    st = bvh.count(rays, any_hit)
+    try:
+        st = bvh.count(rays, any_hit)
+    finally:
+        ctx.set_launch(*saved)
     _, cnt = o.intersect(rays, any_hit=any_hit, counters=True)
     tot = cnt.astype(np.uint64).sum(0)
     assert (st.nodes_visited, st.tris_tested, st.spheres_tested) == tuple(int(x) for x in tot)
