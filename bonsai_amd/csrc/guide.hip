@@ -46,23 +46,10 @@ __device__ __forceinline__ s4 pack4(float a, float b, float c, float d) {
     return r;
 }
 
-// Encoding feature f of a sample (matches oracle/guide_mlp.py encode()).
-__device__ __forceinline__ float feature(int f, const float q[3], const float wo[3], const float n[3]) {
-    if (f < 48) {
-        const int axis = f / 16, k = (f % 16) / 2;
-        const float a = (float)(1 << k) * kPiF * q[axis];
-        return (f & 1) ? cosf(a) : sinf(a);
-    }
-    if (f < 51) return wo[f - 48];
-    if (f < 54) return n[f - 51];
-    if (f < 60) {
-        const int i = (f - 54) / 2;
-        const float a = kPiF * wo[i];
-        return (f & 1) ? cosf(a) : sinf(a);
-    }
-    return 0.0f;
-}
-
+// Encoding features of a sample (oracle/guide_mlp.py encode()): 0-47 sin and
+// cos of 2^k pi q_axis (axis = f / 16, k = (f % 16) / 2), 48-50 wo, 51-53 n,
+// 54-59 sin and cos of pi wo_i, 60-63 zero. sincospif takes the argument in
+// half-turns, so 2^k q is reduced exactly.
 // Weights in LDS: BF16 [layer][64 rows][kLd], rows padded from 64 to 68
 // elements (136 B = 34 banks) so the 32 rows one A-operand read touches spread
 // over the banks instead of all starting on bank 0. A operand of k-step s for
@@ -102,13 +89,25 @@ __device__ __forceinline__ void epilogue(f16v acc[2], const float* __restrict__ 
     }
 }
 
-// B operands of the encoding for the sample of this lane.
+// B operands of the encoding for the sample of this lane: k-step s holds
+// features 8s + 4h .. 8s + 4h + 3, h = lane / 32.
 __device__ __forceinline__ void encode_b(const float q[3], const float wo[3], const float n[3], int lane, s4 xb[8]) {
+    const int h = lane >> 5;
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-        const int f = 8 * s + 4 * (lane >> 5);
-        xb[s] = pack4(feature(f, q, wo, n), feature(f + 1, q, wo, n), feature(f + 2, q, wo, n), feature(f + 3, q, wo, n));
+    for (int s = 0; s < 6; ++s) {
+        const int f = 8 * s + 4 * h;
+        const int axis = f >> 4, k0 = (f & 15) >> 1;  // frequencies k0 and k0 + 1
+        float s0, c0, s1, c1;
+        sincospif((float)(1 << k0) * q[axis], &s0, &c0);
+        sincospif((float)(2 << k0) * q[axis], &s1, &c1);
+        xb[s] = pack4(s0, c0, s1, c1);
     }
+    float sw0, cw0, sw1, cw1, sw2, cw2;
+    sincospif(wo[0], &sw0, &cw0);
+    sincospif(wo[1], &sw1, &cw1);
+    sincospif(wo[2], &sw2, &cw2);
+    xb[6] = h == 0 ? pack4(wo[0], wo[1], wo[2], n[0]) : pack4(n[1], n[2], sw0, cw0);
+    xb[7] = h == 0 ? pack4(sw1, cw1, sw2, cw2) : pack4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 struct Sample {
@@ -143,15 +142,13 @@ __global__ __launch_bounds__(256) void k_guide_infer(GuideDevice g, GuideBox box
                                                      uint64_t n, const uint32_t* __restrict__ n_dev,
                                                      float* __restrict__ out) {
     __shared__ uint16_t wl[kLayers * kLayerLds];
-    __shared__ float tile[4][32][65];
     if (n_dev) n = *n_dev;
     if ((uint64_t)blockIdx.x * 128 >= n) return;  // the whole block is idle (uniform)
     load_weights_lds(g.wbf, wl);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, h = lane >> 5;
     const uint64_t groups = (n + 31) / 32;
-    // every wave of the block runs the same number of rounds (the rounds hold block barriers)
-    for (uint64_t base = (uint64_t)blockIdx.x * 4; base < groups; base += (uint64_t)gridDim.x * 4) {
-        const uint64_t i = (base + wave) * 32 + (lane & 31);
+    for (uint64_t grp = (uint64_t)blockIdx.x * 4 + wave; grp < groups; grp += (uint64_t)gridDim.x * 4) {
+        const uint64_t i = grp * 32 + (lane & 31);
         const bool valid = i < n;
         Sample smp = valid ? load_sample(in, i) : Sample{{0, 0, 0}, {0, 0, 1}, {0, 0, 1}};
         float q[3];
@@ -163,19 +160,27 @@ __global__ __launch_bounds__(256) void k_guide_infer(GuideDevice g, GuideBox box
             layer_mm(wl + l * kLayerLds, xb, lane, acc);
             epilogue(acc, g.bias + l * 64, l < kLayers - 1, lane, xb);
         }
-        for (int mb = 0; mb < 2; ++mb)
-            for (int r = 0; r < 16; ++r) tile[wave][lane & 31][acc_row(mb, r, lane)] = acc[mb][r];
-        __syncthreads();
-        if (lane < 32 && valid) {
-            if (MIX) {
-                float o[kOut];
-                for (int k = 0; k < kOut; ++k) o[k] = tile[wave][lane][k];
-                dev::guide_mixture(o, reinterpret_cast<float4*>(out) + dev::kMixRecord * i);
-            } else {
-                for (int k = 0; k < kOut; ++k) out[i * kOut + k] = tile[wave][lane][k];
-            }
+        // lane l holds rows 8q + 4h .. + 3 (block 0, q < 4) and 32 + 4h .. + 3 (block 1) of sample l % 32
+        if (MIX) {
+            // the other half of the sample's outputs comes from lane l + 32
+            float o[kOut];
+#pragma unroll
+            for (int qq = 0; qq < 5; ++qq)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float mine = qq < 4 ? acc[0][4 * qq + j] : acc[1][j];
+                    const float other = __shfl(mine, (lane & 31) + 32);
+                    o[8 * qq + j] = mine;
+                    o[8 * qq + 4 + j] = other;
+                }
+            if (lane < 32 && valid) dev::guide_mixture(o, reinterpret_cast<float4*>(out) + dev::kMixRecord * i);
+        } else if (valid) {
+            float4* o = reinterpret_cast<float4*>(out + i * kOut);
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq)
+                o[2 * qq + h] = make_float4(acc[0][4 * qq], acc[0][4 * qq + 1], acc[0][4 * qq + 2], acc[0][4 * qq + 3]);
+            o[8 + h] = make_float4(acc[1][0], acc[1][1], acc[1][2], acc[1][3]);
         }
-        __syncthreads();
     }
 }
 
