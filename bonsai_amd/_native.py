@@ -129,6 +129,8 @@ class bg_batch_desc(C.Structure):
 BG_MODE_REF = 0
 BG_MODE_CORRECTED = 1
 BG_FILM_F64 = 1
+BG_OBJ_NORMALS = 1
+BG_OBJ_UVS = 2
 BG_GUIDE_PARAMS = 19240
 BG_GUIDE_OUTPUTS = 40
 
@@ -177,6 +179,7 @@ SIGNATURES = {
     "bg_render_batch": (C.c_int, [P, P, C.POINTER(bg_batch_desc), C.POINTER(bg_stats)]),
     "bg_batch_film_read": (C.c_int, [P, C.c_uint32, P]),
     "bg_batch_launch_config": (C.c_int, [P, C.POINTER(bg_launch_config)]),
+    "bg_obj_read": (C.c_int, [C.c_char_p, P, P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), P, P, P]),
     "bgx_icosphere_tri_count": (C.c_uint64, [C.c_int]),
     "bgx_icosphere": (C.c_int, [C.c_int, P, C.c_float, C.c_float, C.c_float, C.c_uint64, P]),
     "bgx_rays_incoherent": (C.c_int, [C.c_uint64, P, P, C.c_uint64, P]),

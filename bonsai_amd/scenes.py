@@ -129,6 +129,27 @@ class SceneData:
         return d, keep
 
 
+def load_obj(path: str, object_to_world=None, world_to_object=None) -> dict:
+    """OBJ::read (io/obj.rs:16-124) through the library: {"positions": (n, 9) world-space triangles,
+    "normals": (n, 9) or None, "uvs": (n, 6) or None}. object_to_world and its inverse are 4x4
+    row-major (identity by default)."""
+    m = np.eye(4, dtype=np.float32) if object_to_world is None else np.ascontiguousarray(object_to_world, np.float32)
+    mi = (np.linalg.inv(m.astype(np.float64)).astype(np.float32) if world_to_object is None
+          else np.ascontiguousarray(world_to_object, np.float32))
+    L = N.lib()
+    n = C.c_uint64(0)
+    has = C.c_uint32(0)
+    N.check(L.bg_obj_read(str(path).encode(), m.ctypes.data, mi.ctypes.data, C.byref(n), C.byref(has), None, None,
+                          None))
+    pos = np.empty((n.value, 9), np.float32)
+    nrm = np.empty((n.value, 9), np.float32) if has.value & N.BG_OBJ_NORMALS else None
+    uv = np.empty((n.value, 6), np.float32) if has.value & N.BG_OBJ_UVS else None
+    N.check(L.bg_obj_read(str(path).encode(), m.ctypes.data, mi.ctypes.data, C.byref(n), C.byref(has),
+                          pos.ctypes.data, None if nrm is None else nrm.ctypes.data,
+                          None if uv is None else uv.ctypes.data))
+    return {"positions": pos, "normals": nrm, "uvs": uv}
+
+
 def matte(r, g, b) -> N.bg_material:
     m = N.bg_material()
     m.type = N.BG_MAT_MATTE

@@ -171,6 +171,20 @@ int bg_scene_bvh(const bg_scene* scene, bg_bvh_node* nodes, uint32_t* slot_prim)
 int bg_build_bvh(const bg_scene_desc* desc, bg_bvh_node* nodes, uint64_t* n_nodes, uint32_t* slot_prim,
                  uint64_t* n_prims, uint32_t* max_depth);
 
+/* OBJ ingestion: OBJ::read (src/io/obj.rs:16-124) with TriangleMesh::new
+ * (src/shapes/triangle.rs:48-90). Host only. object_to_world and its inverse
+ * are the Transform's m and m_inverse (row-major). Call with positions == NULL
+ * to learn *n_tris and *has (BG_OBJ_NORMALS | BG_OBJ_UVS), then with buffers:
+ * positions 9 floats per triangle (world space), normals 9 (transformed as
+ * normals), uvs 6; normals / uvs may be NULL. Triangles are consecutive
+ * triples of face vertices, as in the reference. Errors: BG_ERR_INVALID for an
+ * unreadable file or a token the reference would fail on, BG_ERR_UNSUPPORTED
+ * when only some face vertices carry normals or uvs. */
+#define BG_OBJ_NORMALS 1u
+#define BG_OBJ_UVS 2u
+int bg_obj_read(const char* path, const float object_to_world[16], const float world_to_object[16], uint64_t* n_tris,
+                uint32_t* has, float* positions, float* normals, float* uvs);
+
 /* Batched queries, host buffers (copied in and out; includes PCIe). */
 int bg_intersect(bg_ctx* ctx, const bg_scene* scene, const bg_ray* rays, size_t n, bg_hit* hits,
                  uint32_t flags, bg_stats* stats);
