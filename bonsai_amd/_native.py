@@ -21,6 +21,7 @@ BG_COUNT = 2
 BG_MISS = 0xFFFFFFFF
 BG_MAT_MATTE = 0
 BG_MAT_CONDUCTOR = 1
+BG_MAT_PLASTIC = 2
 
 
 class bg_ray(C.Structure):

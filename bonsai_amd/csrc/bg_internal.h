@@ -135,6 +135,7 @@ struct ShadeSceneDev {
     const LightDev* lights;
     const SphereShadeDev* spheres;
     uint32_t n_tris, n_lights;
+    uint32_t multi_lobe = 0;  // a material with two lobes (plastic) is present
 };
 
 // Wavefront path state (structure of arrays) for up to `cap` paths.
@@ -327,6 +328,15 @@ struct bg_scene {
     std::vector<float> sphere_o2w, sphere_w2o, sphere_params;
     std::vector<uint8_t> sphere_flags;
 };
+
+namespace bg {
+// a material whose BSDF holds two lobes (plastic) selects the two-lobe shading kernel
+inline uint32_t has_multi_lobe(const std::vector<bg_material>& mats) {
+    for (const bg_material& m : mats)
+        if (m.type == BG_MAT_PLASTIC) return 1u;
+    return 0u;
+}
+}  // namespace bg
 
 struct bg_ctx {
     int device = 0;

@@ -294,6 +294,11 @@ int bg_scene_create(bg_ctx* ctx, const bg_scene_desc* d, bg_scene** out) {
             delete s;
             return fail(ctx, BG_ERR_INVALID, "material index out of range");
         }
+    for (const bg_material& m : s->materials)
+        if (m.type > BG_MAT_PLASTIC) {
+            delete s;
+            return fail(ctx, BG_ERR_INVALID, "unknown material type");
+        }
     s->prim_light.assign(np, -1);
     if (d->n_lights) s->lights.assign(d->lights, d->lights + d->n_lights);
     for (uint32_t i = 0; i < d->n_lights; ++i) s->prim_light[d->lights[i].prim_id] = (int32_t)i;

@@ -295,14 +295,44 @@ __device__ __forceinline__ V tr_sample(bool ref, float ax, float ay, V wo, float
     return flip ? -wh : wh;
 }
 
-// One lobe: Lambertian reflection (lambertian.rs) or microfacet conductor
-// reflection (bxdfs/microfacet.rs:69-147).
+// bxdf.rs:164-187. The reference keeps the sign of cos_i after swapping the
+// indices, so for light arriving from behind the normal the parallel term's
+// denominator passes through zero (cos_i = -0.5547 for 1.5 / 1.0); `fold`
+// (the corrected mode) takes |cos_i| there, as the Fresnel equations do.
+__device__ __forceinline__ float fresnel_dielectric(float cos_i, float eta_i, float eta_t, bool fold) {
+    cos_i = clampf(cos_i, -1.0f, 1.0f);
+    if (!(cos_i > 0.0f)) {
+        const float tmp = eta_i;
+        eta_i = eta_t;
+        eta_t = tmp;
+        if (fold) cos_i = __builtin_fabsf(cos_i);
+    }
+    const float sin_i = __builtin_sqrtf(rmax(0.0f, 1.0f - cos_i * cos_i));
+    const float sin_t = eta_i / eta_t * sin_i;
+    if (sin_t >= 1.0f) return 1.0f;
+    const float cos_t = __builtin_sqrtf(rmax(0.0f, 1.0f - sin_t * sin_t));
+    const float par = ((eta_t * cos_i) - (eta_i * cos_t)) / ((eta_t * cos_i) + (eta_i * cos_t));
+    const float perp = ((eta_i * cos_i) - (eta_t * cos_t)) / ((eta_i * cos_i) + (eta_t * cos_t));
+    return (par * par + perp * perp) / 2.0f;
+}
+
+// microfacet.rs:45-53
+__device__ __forceinline__ float roughness_to_alpha(float roughness) {
+    roughness = rmax(roughness, 1e-3f);
+    const float x = logf(roughness);
+    return 1.62142f + 0.819955f * x + 0.1734f * x * x + 0.0171201f * x * x * x + 0.000640711f * x * x * x * x;
+}
+
+// One lobe: Lambertian reflection (lambertian.rs) or microfacet reflection
+// (bxdfs/microfacet.rs:69-147) with a conductor Fresnel term or, for the
+// plastic material, FresnelDielectric(1.5, 1.0) (plastic.rs:40).
 struct Lobe {
     uint32_t type;  // 0: none
     C3 r;
     float ax, ay;
     C3 eta, k;
     bool ref;
+    bool dielectric;
 };
 
 __device__ __forceinline__ C3 lobe_f(const Lobe& l, V wo, V wi) {
@@ -312,7 +342,8 @@ __device__ __forceinline__ C3 lobe_f(const Lobe& l, V wo, V wi) {
     if (ci == 0.0f || co == 0.0f) return mkc(0.0f);
     if (wh.x == 0.0f && wh.y == 0.0f && wh.z == 0.0f) return mkc(0.0f);
     wh = unit(wh);
-    const C3 F = fresnel_conductor(__builtin_fabsf(dot(wi, face_forward(wh, mkv(0.0f, 0.0f, 1.0f)))), l.eta, l.k);
+    const float cwh = dot(wi, face_forward(wh, mkv(0.0f, 0.0f, 1.0f)));
+    const C3 F = l.dielectric ? mkc(fresnel_dielectric(cwh, 1.5f, 1.0f, !l.ref)) : fresnel_conductor(__builtin_fabsf(cwh), l.eta, l.k);
     return l.r * tr_d(l.ax, l.ay, wh) * tr_g(l.ax, l.ay, wo, wi) * F / (4.0f * ci * co);
 }
 __device__ __forceinline__ float lobe_pdf(const Lobe& l, V wo, V wi) {
@@ -351,53 +382,101 @@ __device__ __forceinline__ BxS lobe_sample(const Lobe& l, V wo, float u0, float 
 }
 __device__ __forceinline__ bool matches(uint32_t type, uint32_t flags) { return (type & flags) == type; }
 
-// bsdf.rs:16-259 with at most one lobe.
-struct Bsdf {
+// bsdf.rs:16-259 with up to NL <= 2 lobes (plastic: Lambertian, then
+// microfacet), held as two named members so that every index is known at
+// compile time; NL = 1 compiles the single-lobe materials' code alone.
+template <int NL>
+struct BsdfN {
     V ns, ng, ss, ts;
-    Lobe lobe;
+    Lobe l0, l1;
+    int n;
+    __device__ const Lobe& lobe(int i) const { return i == 0 ? l0 : l1; }
+    __device__ void add(const Lobe& l) {
+        if (n == 0) l0 = l;
+        else l1 = l;
+        ++n;
+    }
+    __device__ bool any_specular() const {
+        return (n > 0 && (l0.type & kSpecular)) || (NL > 1 && n > 1 && (l1.type & kSpecular));
+    }
     __device__ V local(V v) const { return mkv(dot(v, ss), dot(v, ts), dot(v, ns)); }
     __device__ V world(V v) const {
         return mkv(ss.x * v.x + ts.x * v.y + ns.x * v.z, ss.y * v.x + ts.y * v.y + ns.y * v.z,
                    ss.z * v.x + ts.z * v.y + ns.z * v.z);
     }
-    __device__ int ncomp(uint32_t flags) const { return (lobe.type && matches(lobe.type, flags)) ? 1 : 0; }
+    __device__ int ncomp(uint32_t flags) const {
+        int c = 0;
+#pragma unroll
+        for (int i = 0; i < NL; ++i)
+            if (i < n && matches(lobe(i).type, flags)) ++c;
+        return c;
+    }
+    // bsdf.rs:84-104
     __device__ C3 f(V wo_w, V wi_w, uint32_t flags) const {
         const V wo = local(wo_w), wi = local(wi_w);
         if (wo.z == 0.0f) return mkc(0.0f);
         const bool refl = dot(wi_w, ng) * dot(wo_w, ng) > 0.0f;
         C3 r = mkc(0.0f);
-        if (lobe.type && matches(lobe.type, flags) && ((refl && (lobe.type & kRefl)) || (!refl && (lobe.type & kTrans))))
-            r = r + lobe_f(lobe, wo, wi);
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+            const Lobe& l = lobe(i);
+            if (i < n && matches(l.type, flags) && ((refl && (l.type & kRefl)) || (!refl && (l.type & kTrans))))
+                r = r + lobe_f(l, wo, wi);
+        }
         return r;
     }
+    // bsdf.rs:234-258
     __device__ float pdf(V wo_w, V wi_w, uint32_t flags) const {
-        if (!lobe.type) return 0.0f;
+        if (n == 0) return 0.0f;
         const V wo = local(wo_w), wi = local(wi_w);
         if (wo.z == 0.0f) return 0.0f;
         float ret = 0.0f, comps = 0.0f;
-        if (matches(lobe.type, flags)) { comps += 1.0f; ret += lobe_pdf(lobe, wo, wi); }
+#pragma unroll
+        for (int i = 0; i < NL; ++i)
+            if (i < n && matches(lobe(i).type, flags)) {
+                comps += 1.0f;
+                ret += lobe_pdf(lobe(i), wo, wi);
+            }
         return comps > 0.0f ? ret / comps : 0.0f;
     }
+    // bsdf.rs:140-232: the component-th matching lobe (u0 remapped) is sampled; for a non-specular
+    // pick the pdf averages every matching lobe and f sums them
     __device__ BxS sample(V wo_w, float u0, float u1, uint32_t flags) const {
         BxS out{mkv(0, 0, 0), mkc(0.0f), 0.0f, 0};
         const int m = ncomp(flags);
         if (m == 0) return out;
         uint32_t comp = (uint32_t)floorf(u0 * (float)m);
         if (comp > (uint32_t)m - 1) comp = (uint32_t)m - 1;
+        // the comp-th matching lobe
+        const bool first_matches = n > 0 && matches(l0.type, flags);
+        const int chosen = (NL == 1 || (comp == 0 && first_matches)) ? 0 : 1;
+        const Lobe& b = lobe(chosen);
         const float ur = rmin(u0 * (float)m - (float)comp, kOneMinusEps);
         const V wo = local(wo_w);
         if (wo.z == 0.0f) return out;
-        BxS s = lobe_sample(lobe, wo, ur, u1);
+        BxS s = lobe_sample(b, wo, ur, u1);
         if (s.pdf == 0.0f) return out;
         const V wi_w = world(s.wi);
-        const bool refl = dot(wi_w, ng) * dot(wo_w, ng) > 0.0f;
-        C3 f = mkc(0.0f);
-        if (matches(lobe.type, flags) && ((refl && (lobe.type & kRefl)) || (!refl && (lobe.type & kTrans))))
-            f = f + lobe_f(lobe, wo, s.wi);
+        if (NL > 1 && !(b.type & kSpecular) && m > 1) {
+            const Lobe& o = lobe(1 - chosen);
+            if (matches(o.type, flags)) s.pdf += lobe_pdf(o, wo, s.wi);
+        }
+        if (m > 1) s.pdf /= (float)m;
+        C3 f = s.f;
+        if (!(b.type & kSpecular)) {
+            const bool refl = dot(wi_w, ng) * dot(wo_w, ng) > 0.0f;
+            f = mkc(0.0f);
+#pragma unroll
+            for (int i = 0; i < NL; ++i) {
+                const Lobe& l = lobe(i);
+                if (i < n && matches(l.type, flags) && ((refl && (l.type & kRefl)) || (!refl && (l.type & kTrans))))
+                    f = f + lobe_f(l, wo, s.wi);
+            }
+        }
         out.wi = wi_w;
         out.f = f;
         out.pdf = s.pdf;
-        out.type = lobe.type;
+        out.type = b.type;
         return out;
     }
 };

@@ -153,7 +153,8 @@ __global__ __launch_bounds__(256) void k_init_paths(RenderParamsDev p, PathBuffe
     b.film[i] = make_float2(fx, fy);
 }
 
-template <bool GUIDED>
+// MULTI: the scene has a two-lobe material (plastic)
+template <bool GUIDED, bool MULTI>
 __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev sc, PathBuffers b, uint32_t bounce) {
     const uint64_t i = (uint64_t)blockIdx.x * 128 + threadIdx.x;
     if (i >= p.n_paths) return;
@@ -220,34 +221,61 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
     bool alive = bounce < p.max_depth;
     float cont_pdf = 0.0f;
     if (alive) {
-        // matte.rs:20-41 / conductor; bsdf.rs:33-52
+        // matte.rs:20-41 / conductor / plastic.rs:24-52; bsdf.rs:33-52
         const MaterialDev& m = sc.mats[sc.prim_mat[prim]];
-        Bsdf bs;
+        BsdfN<MULTI ? 2 : 1> bs;
         bs.ns = s.ns;
         bs.ng = s.n;
         bs.ss = unit(s.dpdu);
         bs.ts = cross(bs.ns, bs.ss);
-        bs.lobe.type = 0;
-        bs.lobe.ref = ref;
-        if (m.type == 0) {
+        bs.n = 0;
+        if (m.type == 0 || m.type == 2) {  // Lambertian Kd (clamped at 0), if not black
             const C3 kd = C3{clampf(m.r[0], 0.0f, __builtin_inff()), clampf(m.r[1], 0.0f, __builtin_inff()),
                              clampf(m.r[2], 0.0f, __builtin_inff())};
-            if (!black(kd)) { bs.lobe.type = kRefl | kDiffuse; bs.lobe.r = kd; }
-        } else {
-            bs.lobe.type = kRefl | kGlossy;
-            bs.lobe.r = C3{m.r[0], m.r[1], m.r[2]};
-            bs.lobe.ax = rmax(m.alpha, 0.001f);
-            bs.lobe.ay = rmax(m.alpha, 0.001f);
-            bs.lobe.eta = C3{m.eta[0], m.eta[1], m.eta[2]};
-            bs.lobe.k = C3{m.k[0], m.k[1], m.k[2]};
+            if (!black(kd)) {
+                Lobe l{};
+                l.type = kRefl | kDiffuse;
+                l.r = kd;
+                l.ref = ref;
+                bs.add(l);
+            }
+        }
+        if (MULTI && m.type == 2) {  // plastic: Trowbridge-Reitz Ks with FresnelDielectric(1.5, 1.0), if not black
+            const C3 ks = C3{clampf(m.eta[0], 0.0f, __builtin_inff()), clampf(m.eta[1], 0.0f, __builtin_inff()),
+                             clampf(m.eta[2], 0.0f, __builtin_inff())};
+            if (!black(ks)) {
+                const float rough = m.k[0] != 0.0f ? roughness_to_alpha(m.alpha) : m.alpha;
+                Lobe l{};
+                l.type = kRefl | kGlossy;
+                l.r = ks;
+                l.ax = rmax(rough, 0.001f);
+                l.ay = rmax(rough, 0.001f);
+                l.ref = ref;
+                l.dielectric = true;
+                bs.add(l);
+            }
+        } else if (m.type == 1) {  // conductor
+            Lobe l{};
+            l.type = kRefl | kGlossy;
+            l.r = C3{m.r[0], m.r[1], m.r[2]};
+            l.ax = rmax(m.alpha, 0.001f);
+            l.ay = rmax(m.alpha, 0.001f);
+            l.eta = C3{m.eta[0], m.eta[1], m.eta[2]};
+            l.k = C3{m.k[0], m.k[1], m.k[2]};
+            l.ref = ref;
+            l.dielectric = false;
+            bs.add(l);
         }
         const uint32_t nonspec = kAll & ~kSpecular;
         // network mixture of this vertex (guided estimator with guide_alpha > 0)
         const float4* gm = nullptr;
         float alpha = 0.0f;
-        const bool lobe_ok = (bs.lobe.type & kDiffuse) ? (p.guide_lobes & 1u) : (p.guide_lobes & 2u);
+        // the network is sampled where the material's lobes are of the kinds selected (diffuse / glossy)
+        bool lobe_ok = bs.n > 0;
+        for (int k = 0; k < (MULTI ? 2 : 1); ++k)
+            if (k < bs.n) lobe_ok = lobe_ok && ((bs.lobe(k).type & kDiffuse) ? (p.guide_lobes & 1u) : (p.guide_lobes & 2u));
         const uint32_t gslot = GUIDED ? b.gslot[i] : 0xFFFFFFFFu;
-        if (GUIDED && p.guide_alpha > 0.0f && lobe_ok && !(bs.lobe.type & kSpecular) && gslot != 0xFFFFFFFFu) {
+        if (GUIDED && p.guide_alpha > 0.0f && lobe_ok && !bs.any_specular() && gslot != 0xFFFFFFFFu) {
             gm = reinterpret_cast<const float4*>(b.gout) + kMixRecord * (uint64_t)gslot;
             if (guide_ok(gm)) alpha = p.guide_alpha;
         }
@@ -675,10 +703,14 @@ hipError_t launch_init_paths(const RenderParamsDev& p, const PathBuffers& b, hip
 hipError_t launch_shade(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b, uint32_t bounce,
                         hipStream_t s) {
     if (p.n_paths == 0) return hipSuccess;
-    if (p.guided)
-        hipLaunchKernelGGL(k_shade<true>, dim3((uint32_t)((p.n_paths + 127) / 128)), dim3(128), 0, s, p, sc, b, bounce);
-    else
-        hipLaunchKernelGGL(k_shade<false>, dim3((uint32_t)((p.n_paths + 127) / 128)), dim3(128), 0, s, p, sc, b, bounce);
+    const dim3 grid((uint32_t)((p.n_paths + 127) / 128));
+    if (p.guided) {
+        if (sc.multi_lobe) hipLaunchKernelGGL((k_shade<true, true>), grid, dim3(128), 0, s, p, sc, b, bounce);
+        else hipLaunchKernelGGL((k_shade<true, false>), grid, dim3(128), 0, s, p, sc, b, bounce);
+    } else {
+        if (sc.multi_lobe) hipLaunchKernelGGL((k_shade<false, true>), grid, dim3(128), 0, s, p, sc, b, bounce);
+        else hipLaunchKernelGGL((k_shade<false, false>), grid, dim3(128), 0, s, p, sc, b, bounce);
+    }
     return hipGetLastError();
 }
 hipError_t launch_nee_resolve(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b,

@@ -275,6 +275,7 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
     sc.spheres = s->d_sph_shade;
     sc.n_tris = (uint32_t)s->n_tris;
     sc.n_lights = (uint32_t)s->lights.size();
+    sc.multi_lobe = bg::has_multi_lobe(s->materials);
     const bg::PathBuffers& b = ctx->paths;
     hipStream_t st = ctx->stream;
 

@@ -172,6 +172,18 @@ def conductor(alpha, eta=GOLD_ETA, k=GOLD_K, r=(1.0, 1.0, 1.0)) -> N.bg_material
     return m
 
 
+def plastic(kd, ks, roughness, remap_roughness=True) -> N.bg_material:
+    """PlasticMaterial (materials/plastic.rs:24-52): Lambertian kd + Trowbridge-Reitz ks with a dielectric
+    Fresnel term; roughness is remapped to alpha (microfacet.rs:45-53) when remap_roughness."""
+    m = N.bg_material()
+    m.type = N.BG_MAT_PLASTIC
+    m.reflectance[:] = list(kd)
+    m.eta[:] = list(ks)
+    m.alpha = roughness
+    m.k[:] = [1.0 if remap_roughness else 0.0, 0.0, 0.0]
+    return m
+
+
 def area_light(prim_id, radiance, double_sided=False) -> N.bg_area_light:
     a = N.bg_area_light()
     a.prim_id = prim_id
@@ -226,10 +238,15 @@ def quad(p0, p1, p2, p3) -> np.ndarray:
     return np.array([[*p0, *p1, *p2], [*p0, *p2, *p3]], np.float32)
 
 
-def cornell_box(double_sided_light: bool = False, width: int = 256, height: int = 256) -> SceneData:
+def cornell_box(double_sided_light: bool = False, width: int = 256, height: int = 256,
+                with_plastic: bool = False) -> SceneData:
     """Small Cornell-style box in [0,1]^3: 5 walls as 10 triangles, a 0.25 x 0.25
     quad light (2 triangles, radiance 17) under the ceiling facing down, two
-    r = 0.3 spheres (matte albedo 0.75, conductor alpha 0.2)."""
+    r = 0.3 spheres (matte albedo 0.75, conductor alpha 0.2). `with_plastic`
+    makes the matte sphere plastic (kd (0.5, 0.25, 0.1), ks 0.4, roughness 0.1
+    remapped) and the floor glossy plastic (kd 0.6, ks 0.3, alpha 0.3) with its
+    winding reversed so that its normal faces into the box (the reference's
+    dielectric Fresnel term has a pole for light from behind the normal)."""
     walls = [
         quad((0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1)),        # floor y=0
         quad((0, 1, 0), (0, 1, 1), (1, 1, 1), (1, 1, 0)),        # ceiling y=1
@@ -244,6 +261,12 @@ def cornell_box(double_sided_light: bool = False, width: int = 256, height: int 
     s.materials = [matte(0.73, 0.73, 0.73), matte(0.65, 0.05, 0.05), matte(0.12, 0.45, 0.15),
                    matte(0.0, 0.0, 0.0), matte(0.75, 0.75, 0.75), conductor(0.2)]
     s.tri_material = np.array([0, 0, 0, 0, 0, 0, 1, 1, 2, 2, 3, 3], np.uint32)
+    if with_plastic:
+        tris[0:2] = quad((0, 0, 0), (0, 0, 1), (1, 0, 1), (1, 0, 0))
+        s.tri_positions = tris
+        s.materials[4] = plastic((0.5, 0.25, 0.1), (0.4, 0.4, 0.4), 0.1)
+        s.materials.append(plastic((0.6, 0.6, 0.6), (0.3, 0.3, 0.3), 0.3, remap_roughness=False))
+        s.tri_material[0:2] = 6
     add_spheres(s, [(0.3, 0.3, 0.6), (0.7, 0.3, 0.35)], [0.3, 0.3], [4, 5])
     s.lights = [area_light(10, (17.0, 17.0, 17.0), double_sided_light),
                 area_light(11, (17.0, 17.0, 17.0), double_sided_light)]

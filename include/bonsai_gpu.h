@@ -88,6 +88,10 @@ typedef struct bg_bvh_node {
  * src/base/fresnel.rs:13-46). */
 #define BG_MAT_MATTE 0u
 #define BG_MAT_CONDUCTOR 1u
+/* PlasticMaterial (src/materials/plastic.rs:24-52): Lambertian Kd plus a
+ * Trowbridge-Reitz microfacet lobe Ks with FresnelDielectric(1.5, 1.0). Fields:
+ * reflectance = Kd, eta = Ks, alpha = roughness, k[0] != 0: remap_roughness. */
+#define BG_MAT_PLASTIC 2u
 typedef struct bg_material {
     uint32_t type;
     float reflectance[3]; /* matte Kd / conductor R */

@@ -288,4 +288,21 @@ void orc_mat4_inverse(const float* in, float* out) {
     for (int i = 0; i < 16; ++i) out[i] = r.m[i / 4][i % 4];
 }
 
+// One BSDF probe (integrator.h bsdf_probe). params: reflectance[3], alpha, eta[3], k[3].
+// out: f[3], pdf, sampled wi[3], sampled f[3], sampled pdf, sampled type (as float), component count.
+void orc_bsdf_probe(uint32_t type, const float* params, int mode, const float* wo, const float* wi, float u0, float u1,
+                    float* out) {
+    Material m;
+    m.type = type;
+    m.r = RGB(params[0], params[1], params[2]);
+    m.alpha = params[3];
+    m.eta = RGB(params[4], params[5], params[6]);
+    m.k = RGB(params[7], params[8], params[9]);
+    BsdfProbe r = bsdf_probe(m, (Mode)mode, V3(wo[0], wo[1], wo[2]), V3(wi[0], wi[1], wi[2]), u0, u1);
+    out[0] = r.f[0]; out[1] = r.f[1]; out[2] = r.f[2]; out[3] = r.pdf;
+    out[4] = r.swi.x; out[5] = r.swi.y; out[6] = r.swi.z;
+    out[7] = r.sf[0]; out[8] = r.sf[1]; out[9] = r.sf[2]; out[10] = r.spdf;
+    out[11] = (float)r.stype; out[12] = (float)r.ncomp;
+}
+
 }  // extern "C"

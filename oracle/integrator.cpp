@@ -55,6 +55,31 @@ RGB fresnel_conductor(Float cos_theta_i, const RGB& eta_i, const RGB& eta_t, con
     return (rp + rs) * 0.5f;
 }
 
+// base/bxdf.rs:164-187
+Float fresnel_dielectric(Float cos_theta_i, Float eta_i, Float eta_t, bool fold) {
+    cos_theta_i = clampf(cos_theta_i, -1.0f, 1.0f);
+    if (!(cos_theta_i > 0.0f)) {
+        std::swap(eta_i, eta_t);
+        // the reference keeps the sign here (a pole at cos_i = -0.5547 for 1.5 / 1.0); the
+        // corrected mode folds it, as the Fresnel equations do for light from the other side
+        if (fold) cos_theta_i = std::fabs(cos_theta_i);
+    }
+    Float sin_theta_i = std::sqrt(fmax_r(0.0f, 1.0f - cos_theta_i * cos_theta_i));
+    Float sin_theta_t = eta_i / eta_t * sin_theta_i;
+    if (sin_theta_t >= 1.0f) return 1.0f;
+    Float cos_theta_t = std::sqrt(fmax_r(0.0f, 1.0f - sin_theta_t * sin_theta_t));
+    Float par = ((eta_t * cos_theta_i) - (eta_i * cos_theta_t)) / ((eta_t * cos_theta_i) + (eta_i * cos_theta_t));
+    Float perp = ((eta_i * cos_theta_i) - (eta_t * cos_theta_t)) / ((eta_i * cos_theta_i) + (eta_t * cos_theta_t));
+    return (par * par + perp * perp) / 2.0f;
+}
+
+// microfacet.rs:45-53
+Float roughness_to_alpha(Float roughness) {
+    roughness = fmax_r(roughness, 1e-3f);
+    Float x = std::log(roughness);
+    return 1.62142f + 0.819955f * x + 0.1734f * x * x + 0.0171201f * x * x * x + 0.000640711f * x * x * x * x;
+}
+
 // ---------------------------------------------- Trowbridge-Reitz (microfacet.rs)
 struct TR {
     Float ax, ay;
@@ -143,7 +168,9 @@ struct Lobe {
     uint32_t type = 0;  // BxDF flags; 0 = no lobe
     RGB r;
     TR tr;
-    RGB eta, k;
+    RGB eta, k;               // FresnelConductor (eta_i = 1)
+    bool dielectric = false;  // FresnelDielectric(eta_di, eta_dt) instead (fresnel.rs:19-22, 50-54)
+    Float eta_di = 1.0f, eta_dt = 1.0f;
 };
 
 struct BxSample {
@@ -161,7 +188,9 @@ RGB lobe_f(const Lobe& l, const V3& wo, const V3& wi) {
     if (ci == 0.0f || co == 0.0f) return RGB();
     if (wh.x == 0.0f && wh.y == 0.0f && wh.z == 0.0f) return RGB();
     wh = normalize(wh);
-    RGB F = fresnel_conductor(std::fabs(dot(wi, face_forward(wh, V3(0.0f, 0.0f, 1.0f)))), RGB(1.0f), l.eta, l.k);
+    const Float cwh = dot(wi, face_forward(wh, V3(0.0f, 0.0f, 1.0f)));
+    RGB F = l.dielectric ? RGB(fresnel_dielectric(cwh, l.eta_di, l.eta_dt, l.tr.mode != MODE_REF))
+                         : fresnel_conductor(std::fabs(cwh), RGB(1.0f), l.eta, l.k);
     return l.r * tr_d(l.tr, wh) * tr_g(l.tr, wo, wi) * F / (4.0f * ci * co);
 }
 Float lobe_pdf(const Lobe& l, const V3& wo, const V3& wi) {
@@ -193,54 +222,87 @@ BxSample lobe_sample(const Lobe& l, const V3& wo, Float u0, Float u1) {
 
 inline bool matches(uint32_t type, uint32_t flags) { return (type & flags) == type; }
 
-// bsdf.rs:16-259 specialised to at most one lobe (every material here has 0 or 1).
+// bsdf.rs:16-259 with up to two lobes (plastic: Lambertian, then microfacet).
 struct BSDF {
     V3 ns, ng, ss, ts;
-    Lobe lobe;
+    Lobe lobes[2];
     int n = 0;
     V3 to_local(const V3& v) const { return V3(dot(v, ss), dot(v, ts), dot(v, ns)); }
     V3 to_world(const V3& v) const {
         return V3(ss.x * v.x + ts.x * v.y + ns.x * v.z, ss.y * v.x + ts.y * v.y + ns.y * v.z,
                   ss.z * v.x + ts.z * v.y + ns.z * v.z);
     }
-    int num_components(uint32_t flags) const { return (n && matches(lobe.type, flags)) ? 1 : 0; }
+    void add(const Lobe& l) { lobes[n++] = l; }
+    int num_components(uint32_t flags) const {
+        int c = 0;
+        for (int i = 0; i < n; ++i) c += matches(lobes[i].type, flags) ? 1 : 0;
+        return c;
+    }
+    // bsdf.rs:84-104
     RGB f(const V3& wo_w, const V3& wi_w, uint32_t flags) const {
         V3 wo = to_local(wo_w), wi = to_local(wi_w);
         if (wo.z == 0.0f) return RGB();
         bool refl = dot(wi_w, ng) * dot(wo_w, ng) > 0.0f;
         RGB r;
-        if (n && matches(lobe.type, flags) &&
-            ((refl && (lobe.type & BSDF_REFLECTION)) || (!refl && (lobe.type & BSDF_TRANSMISSION))))
-            r = r + lobe_f(lobe, wo, wi);
+        for (int i = 0; i < n; ++i) {
+            const Lobe& l = lobes[i];
+            if (matches(l.type, flags) &&
+                ((refl && (l.type & BSDF_REFLECTION)) || (!refl && (l.type & BSDF_TRANSMISSION))))
+                r = r + lobe_f(l, wo, wi);
+        }
         return r;
     }
+    // bsdf.rs:234-258
     Float pdf(const V3& wo_w, const V3& wi_w, uint32_t flags) const {
         if (n == 0) return 0.0f;
         V3 wo = to_local(wo_w), wi = to_local(wi_w);
         if (wo.z == 0.0f) return 0.0f;
         Float ret = 0.0f, comps = 0.0f;
-        if (matches(lobe.type, flags)) { comps += 1.0f; ret += lobe_pdf(lobe, wo, wi); }
+        for (int i = 0; i < n; ++i)
+            if (matches(lobes[i].type, flags)) {
+                comps += 1.0f;
+                ret += lobe_pdf(lobes[i], wo, wi);
+            }
         return comps > 0.0f ? ret / comps : 0.0f;
     }
+    // bsdf.rs:140-232: pick the component-th matching lobe with u0, remap u0, sample it; for a
+    // non-specular pick, the pdf averages every matching lobe and f sums them
     BxSample sample(const V3& wo_w, Float u0, Float u1, uint32_t flags) const {
         BxSample out;
         int m = num_components(flags);
         if (m == 0) return out;
         uint32_t comp = (uint32_t)std::floor(u0 * (Float)m);
         if (comp > (uint32_t)m - 1) comp = (uint32_t)m - 1;
+        int chosen = -1;
+        uint32_t count = comp;
+        for (int i = 0; i < n; ++i) {
+            if (matches(lobes[i].type, flags) && count == 0) {
+                chosen = i;
+                break;
+            }
+            --count;
+        }
+        const Lobe& b = lobes[chosen];
         Float ur = fmin_r(u0 * (Float)m - (Float)comp, ONE_MINUS_EPSILON);
         V3 wo = to_local(wo_w);
         if (wo.z == 0.0f) return out;
-        BxSample s = lobe_sample(lobe, wo, ur, u1);
-        uint32_t stype = lobe.type;
+        BxSample s = lobe_sample(b, wo, ur, u1);
+        uint32_t stype = b.type;
         if (s.pdf == 0.0f) return out;
         V3 wi_w = to_world(s.wi);
-        if (!(lobe.type & BSDF_SPECULAR)) {
+        if (!(b.type & BSDF_SPECULAR) && m > 1)
+            for (int i = 0; i < n; ++i)
+                if (i != chosen && matches(lobes[i].type, flags)) s.pdf += lobe_pdf(lobes[i], wo, s.wi);
+        if (m > 1) s.pdf /= (Float)m;
+        if (!(b.type & BSDF_SPECULAR)) {
             bool refl = dot(wi_w, ng) * dot(wo_w, ng) > 0.0f;
             RGB f;
-            if (matches(lobe.type, flags) &&
-                ((refl && (lobe.type & BSDF_REFLECTION)) || (!refl && (lobe.type & BSDF_TRANSMISSION))))
-                f = f + lobe_f(lobe, wo, s.wi);
+            for (int i = 0; i < n; ++i) {
+                const Lobe& l = lobes[i];
+                if (matches(l.type, flags) &&
+                    ((refl && (l.type & BSDF_REFLECTION)) || (!refl && (l.type & BSDF_TRANSMISSION))))
+                    f = f + lobe_f(l, wo, s.wi);
+            }
             s.f = f;
         }
         out.wi = wi_w;
@@ -251,27 +313,47 @@ struct BSDF {
     }
 };
 
-// matte.rs:20-41 (sigma = 0) / conductor; BSDF::new (bsdf.rs:33-52)
+// matte.rs:20-41 (sigma = 0) / conductor / plastic.rs:24-52; BSDF::new (bsdf.rs:33-52)
 BSDF make_bsdf(const Isect& it, const Material& m, Mode mode) {
     BSDF b;
     b.ns = it.ns;
     b.ng = it.n;
     b.ss = normalize(it.dpdu_s);
     b.ts = cross(b.ns, b.ss);
-    if (m.type == MAT_MATTE) {
-        RGB r(clampf(m.r[0], 0.0f, INFINITY), clampf(m.r[1], 0.0f, INFINITY), clampf(m.r[2], 0.0f, INFINITY));
-        if (!is_black(r)) {
-            b.lobe.type = BSDF_REFLECTION | BSDF_DIFFUSE;
-            b.lobe.r = r;
-            b.n = 1;
+    auto clamp0 = [](const RGB& c) {
+        return RGB(clampf(c[0], 0.0f, INFINITY), clampf(c[1], 0.0f, INFINITY), clampf(c[2], 0.0f, INFINITY));
+    };
+    if (m.type == MAT_MATTE || m.type == MAT_PLASTIC) {
+        RGB kd = clamp0(m.r);
+        if (!is_black(kd)) {
+            Lobe l;
+            l.type = BSDF_REFLECTION | BSDF_DIFFUSE;
+            l.r = kd;
+            b.add(l);
         }
-    } else {
-        b.lobe.type = BSDF_REFLECTION | BSDF_GLOSSY;
-        b.lobe.r = m.r;
-        b.lobe.tr = TR{fmax_r(m.alpha, 0.001f), fmax_r(m.alpha, 0.001f), mode};
-        b.lobe.eta = m.eta;
-        b.lobe.k = m.k;
-        b.n = 1;
+    }
+    if (m.type == MAT_PLASTIC) {
+        RGB ks = clamp0(m.eta);
+        if (!is_black(ks)) {
+            Float rough = m.alpha;
+            if (m.k[0] != 0.0f) rough = roughness_to_alpha(rough);
+            Lobe l;
+            l.type = BSDF_REFLECTION | BSDF_GLOSSY;
+            l.r = ks;
+            l.tr = TR{fmax_r(rough, 0.001f), fmax_r(rough, 0.001f), mode};
+            l.dielectric = true;
+            l.eta_di = 1.5f;
+            l.eta_dt = 1.0f;
+            b.add(l);
+        }
+    } else if (m.type == MAT_CONDUCTOR) {
+        Lobe l;
+        l.type = BSDF_REFLECTION | BSDF_GLOSSY;
+        l.r = m.r;
+        l.tr = TR{fmax_r(m.alpha, 0.001f), fmax_r(m.alpha, 0.001f), mode};
+        l.eta = m.eta;
+        l.k = m.k;
+        b.add(l);
     }
     return b;
 }
@@ -550,6 +632,24 @@ void render(const ShadeScene& sc, const PerspCamera& cam, const RenderParams& p,
             }
         }
     }
+}
+
+BsdfProbe bsdf_probe(const Material& m, Mode mode, const V3& wo, const V3& wi, Float u0, Float u1) {
+    Isect it{};
+    it.n = V3(0.0f, 0.0f, 1.0f);
+    it.ns = it.n;
+    it.dpdu_s = V3(1.0f, 0.0f, 0.0f);
+    BSDF b = make_bsdf(it, m, mode);
+    BsdfProbe r;
+    r.f = b.f(wo, wi, BSDF_ALL);
+    r.pdf = b.pdf(wo, wi, BSDF_ALL);
+    BxSample s = b.sample(wo, u0, u1, BSDF_ALL);
+    r.swi = s.wi;
+    r.sf = s.f;
+    r.spdf = s.pdf;
+    r.stype = s.type;
+    r.ncomp = b.num_components(BSDF_ALL);
+    return r;
 }
 
 }  // namespace orc

@@ -41,7 +41,8 @@ inline Float max_component_value(const RGB& a) { return fmax_r(fmax_r(a[0], a[1]
 inline Float luminance(const RGB& a) { return 0.212671f * a[0] + 0.715160f * a[1] + 0.072169f * a[2]; }
 inline RGB rsqrt_rgb(const RGB& a) { return RGB(std::sqrt(a[0]), std::sqrt(a[1]), std::sqrt(a[2])); }
 
-enum MatType { MAT_MATTE = 0, MAT_CONDUCTOR = 1 };
+// MAT_PLASTIC (plastic.rs:24-52): r = Kd, eta = Ks, alpha = roughness, k[0] != 0: remap roughness.
+enum MatType { MAT_MATTE = 0, MAT_CONDUCTOR = 1, MAT_PLASTIC = 2 };
 struct Material {
     uint32_t type = MAT_MATTE;
     RGB r;        // matte Kd / conductor R
@@ -105,5 +106,18 @@ RGB radiance(const ShadeScene& sc, const Ray& camera_ray, SampleStream& st, cons
 // RGB sums and filter-weight sums over the window (film.rs:232-284 box filter).
 void render(const ShadeScene& sc, const PerspCamera& cam, const RenderParams& p, int threads, double* rgb_sum,
             double* weight_sum);
+
+// The BSDF of material m in the frame n = ns = z, dpdu = x: f and pdf of (wo, wi) for every
+// lobe (bsdf.rs f, pdf), and one BSDF::sample with (u0, u1). For tests of the BSDF alone.
+struct BsdfProbe {
+    RGB f;
+    Float pdf;
+    V3 swi;
+    RGB sf;
+    Float spdf;
+    uint32_t stype;
+    int ncomp;
+};
+BsdfProbe bsdf_probe(const Material& m, Mode mode, const V3& wo, const V3& wi, Float u0, Float u1);
 
 }  // namespace orc

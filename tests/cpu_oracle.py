@@ -43,6 +43,7 @@ def lib() -> C.CDLL:
             "orc_scene_set_shading": (None, [P, C.c_uint32, P, P, P, C.c_uint32, P, P, P]),
             "orc_render": (None, [P, P, P] + [C.c_float] * 7 + [C.c_uint32] * 4 + [C.c_float, C.c_uint64] +
                            [C.c_uint32] * 5 + [C.c_int, P, P]),
+            "orc_bsdf_probe": (None, [C.c_uint32, P, C.c_int, P, P, C.c_float, C.c_float, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -178,3 +179,16 @@ def hits_as_words(h) -> np.ndarray:
 
 def ray(o, d, t_max=np.inf, time=0.0) -> np.ndarray:
     return np.array([o[0], o[1], o[2], t_max, d[0], d[1], d[2], time], np.float32)
+
+
+def bsdf_probe(material, mode, wo, wi, u0=0.5, u1=0.5) -> dict:
+    """The oracle BSDF of a bg_material in the frame n = z, dpdu = x: f and pdf of (wo, wi), and one
+    BSDF::sample with (u0, u1)."""
+    params = np.array([*material.reflectance, material.alpha, *material.eta, *material.k], np.float32)
+    a = np.ascontiguousarray(wo, np.float32)
+    b = np.ascontiguousarray(wi, np.float32)
+    out = np.zeros(13, np.float32)
+    lib().orc_bsdf_probe(material.type, params.ctypes.data, mode, a.ctypes.data, b.ctypes.data, u0, u1,
+                         out.ctypes.data)
+    return {"f": out[0:3].copy(), "pdf": float(out[3]), "swi": out[4:7].copy(), "sf": out[7:10].copy(),
+            "spdf": float(out[10]), "stype": int(out[11]), "ncomp": int(out[12])}

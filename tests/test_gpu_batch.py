@@ -15,6 +15,7 @@ pytestmark = pytest.mark.gpu
 def items(ctx):
     scenes = [batch_scene(i, scale=0.02) for i in range(4)]
     scenes.append(cornell_box(width=40, height=30))
+    scenes.append(cornell_box(width=36, height=28, with_plastic=True))  # two-lobe shading for the whole batch
     scenes.append(path_tracer_scene(48, 27, small=True))
     bvhs = [GpuBvh(ctx, s) for s in scenes]
     out = [(bvh, s.camera, 100 + k) for k, (bvh, s) in enumerate(zip(bvhs, scenes))]

@@ -57,8 +57,9 @@ def test_select_prob_zero_ignores_the_network(ctx):
     np.testing.assert_array_equal(films[0][1], films[1][1])
 
 
-def test_guided_agrees_with_plain_on_cornell(ctx):
-    s = cornell_box(width=48, height=48)
+@pytest.mark.parametrize("with_plastic", [False, True])
+def test_guided_agrees_with_plain_on_cornell(ctx, with_plastic):
+    s = cornell_box(width=48, height=48, with_plastic=with_plastic)
     bvh = GpuBvh(ctx, s)
     ref = GpuPathIntegrator(ctx, s.camera, spp=4096, max_depth=5, mode=MODE_CORRECTED, seed=11).render(bvh)
     gi = GuidedPathIntegrator(ctx, s.camera, spp=1024, max_depth=5, guide=new_guide(ctx), seed=12,

@@ -42,6 +42,28 @@ def test_cornell_matches_oracle_same_streams(ctx, mode, double_sided):
 
 
 @pytest.mark.parametrize("mode", [MODE_REF, MODE_CORRECTED])
+def test_plastic_matches_oracle_same_streams(ctx, mode):
+    """Two-lobe plastic (materials/plastic.rs:24-52) on a sphere and the floor, beside single-lobe
+    matte and conductor materials: the GPU's two-lobe shading kernel against the oracle. The light is
+    double-sided so that the reference semantics (emission toward +wi) light the surfaces as well."""
+    s = cornell_box(double_sided_light=True, width=96, height=96, with_plastic=True)
+    bvh = GpuBvh(ctx, s)
+    gi = GpuPathIntegrator(ctx, s.camera, spp=16, max_depth=5, mode=mode, seed=21)
+    film = Film(ctx, 96, 96)
+    gi.render_into(bvh, film)
+    rgb, w = film.sums()
+    img = resolve(rgb.astype(np.float64), w.astype(np.float64))
+    ref, wref = oracle_image(s, 16, 5, mode, seed=21, bvh=bvh)
+    assert np.array_equal(w, wref.astype(np.float32))
+    err = rel_mse(img, ref)
+    assert err < 1e-4, err
+    # the plastic changes the image: the same scene without it differs well beyond that
+    plain = cornell_box(double_sided_light=True, width=96, height=96)
+    img2 = GpuPathIntegrator(ctx, plain.camera, spp=16, max_depth=5, mode=mode, seed=21).render(GpuBvh(ctx, plain))
+    assert rel_mse(img2, ref) > 1e-2
+
+
+@pytest.mark.parametrize("mode", [MODE_REF, MODE_CORRECTED])
 def test_path_tracer_scene_matches_oracle_same_streams(ctx, mode):
     """The plain path-tracer configuration's scene kind (displaced icospheres, 256 spheres, matte and
     conductor materials, double-sided quad lights), three subdivision levels lower, at 80 x 45."""
