@@ -163,6 +163,11 @@ SIGNATURES = {
     "bg_film_read": (C.c_int, [P, P, P]),
     "bg_render_film": (C.c_int, [P, P, C.POINTER(bg_render_desc), P, C.POINTER(bg_stats)]),
     "bg_render": (C.c_int, [P, P, C.POINTER(bg_render_desc), P, C.POINTER(bg_stats)]),
+    "bg_film_load": (C.c_int, [P, P]),
+    "bg_film_write_image": (C.c_int, [P, C.c_float, C.c_float, P, C.c_char_p]),
+    "bg_film_resolve_device": (C.c_int, [P, C.c_float, C.c_float, P, C.POINTER(C.c_double)]),
+    "bg_exr_write": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, P]),
+    "bg_exr_encode": (C.c_int, [C.c_uint32, C.c_uint32, P, P, C.POINTER(C.c_uint64)]),
     "bg_guide_create": (C.c_int, [P, P, C.c_uint64, C.POINTER(P)]),
     "bg_guide_destroy": (None, [P]),
     "bg_guide_set_params": (C.c_int, [P, P]),

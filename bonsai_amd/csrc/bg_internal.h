@@ -275,6 +275,12 @@ hipError_t guide_train_step(const GuideDevice& g, const GuideBox& box, const flo
                             uint64_t n, const GuideScratch& sc, float lr, int t, int grid, float* grad_out,
                             hipStream_t s);
 
+// Film::write_image over n pixel sums (film.hip)
+hipError_t launch_film_resolve(const float4* sums, uint64_t n, float splat_scale, float scale, float* rgb_out,
+                               hipStream_t s);
+// OpenEXR image of width x height RGB f32 pixels (exr.cpp)
+int exr_write_file(const char* path, uint32_t width, uint32_t height, const float* rgb);
+
 }  // namespace bg
 
 struct bg_guide {

@@ -196,6 +196,60 @@ int bg_film_read(bg_film* f, float* rgbw, double* rgbw64) {
     return BG_OK;
 }
 
+int bg_film_load(bg_film* f, const float* rgbw) {
+    if (!f || !rgbw) return fail(f ? f->ctx : nullptr, BG_ERR_INVALID, "null argument");
+    bg_ctx* ctx = f->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const size_t n = (size_t)f->width * f->height;
+    HIP_TRY(ctx, hipMemcpy(f->sums, rgbw, n * sizeof(float4), hipMemcpyHostToDevice));
+    if (f->sums64) {
+        std::vector<double> w64(rgbw, rgbw + 4 * n);
+        HIP_TRY(ctx, hipMemcpy(f->sums64, w64.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    }
+    return BG_OK;
+}
+
+int bg_film_resolve_device(bg_film* f, float splat_scale, float scale, void* d_rgb, double* ms) {
+    if (!f || !d_rgb) return fail(f ? f->ctx : nullptr, BG_ERR_INVALID, "null argument");
+    bg_ctx* ctx = f->ctx;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const uint64_t n = (uint64_t)f->width * f->height;
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    HIP_TRY(ctx, bg::launch_film_resolve(f->sums, n, splat_scale, scale, (float*)d_rgb, ctx->stream));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    if (ms) {
+        float t = 0.0f;
+        HIP_TRY(ctx, hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+        *ms = t;
+    }
+    return BG_OK;
+}
+
+int bg_film_write_image(bg_film* f, float splat_scale, float scale, float* rgb_out, const char* path) {
+    if (!f) return BG_ERR_INVALID;
+    bg_ctx* ctx = f->ctx;
+    if (!rgb_out && !path) return fail(ctx, BG_ERR_INVALID, "neither an output buffer nor a path");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const uint64_t n = (uint64_t)f->width * f->height;
+    float* d_rgb = nullptr;
+    if (hipMalloc((void**)&d_rgb, 3 * n * sizeof(float)) != hipSuccess) return fail(ctx, BG_ERR_NOMEM, "resolve buffer");
+    std::vector<float> host;
+    float* dst = rgb_out;
+    if (!dst) {
+        host.resize(3 * n);
+        dst = host.data();
+    }
+    hipError_t e = bg::launch_film_resolve(f->sums, n, splat_scale, scale, d_rgb, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dst, d_rgb, 3 * n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(d_rgb);
+    if (e != hipSuccess) return fail(ctx, BG_ERR_HIP, hipGetErrorString(e));
+    if (path && bg::exr_write_file(path, f->width, f->height, dst) != BG_OK)
+        return fail(ctx, BG_ERR_INVALID, "cannot write the image file");
+    return BG_OK;
+}
+
 }  // extern "C"
 
 namespace {
@@ -396,26 +450,10 @@ int bg_render(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, float* rg
     int rc = bg_film_create(ctx, d->camera.width, d->camera.height, 0, &film);
     if (rc) return rc;
     rc = bg_render_film(ctx, s, d, film, stats);
-    const size_t n = (size_t)d->camera.width * d->camera.height;
-    std::vector<float> sums(4 * n);
-    if (!rc) rc = bg_film_read(film, sums.data(), nullptr);
+    // integrator.rs:132: film.write_image(1.0) with the film scale 1
+    if (!rc) rc = bg_film_write_image(film, 1.0f, 1.0f, rgb_out, nullptr);
     bg_film_destroy(film);
-    if (rc) return rc;
-    // film.rs:138-202: RGB sums -> XYZ (merge) -> RGB, / weight, clamp >= 0, scale 1
-    for (size_t i = 0; i < n; ++i) {
-        const float r = sums[4 * i], g = sums[4 * i + 1], b = sums[4 * i + 2], w = sums[4 * i + 3];
-        const float X = 0.412453f * r + 0.357580f * g + 0.180423f * b;
-        const float Y = 0.212671f * r + 0.715160f * g + 0.072169f * b;
-        const float Z = 0.019334f * r + 0.119193f * g + 0.950227f * b;
-        float o[3] = {3.240479f * X - 1.537150f * Y - 0.498535f * Z, -0.969256f * X + 1.875991f * Y + 0.041556f * Z,
-                      0.055648f * X - 0.204043f * Y + 1.057311f * Z};
-        if (w != 0.0f) {
-            const float iw = 1.0f / w;
-            for (int k = 0; k < 3; ++k) o[k] = std::max(o[k] * iw, 0.0f);
-        }
-        for (int k = 0; k < 3; ++k) rgb_out[3 * i + k] = o[k];
-    }
-    return BG_OK;
+    return rc;
 }
 
 }  // extern "C"

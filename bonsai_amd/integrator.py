@@ -67,6 +67,21 @@ class Film:
         rgb, w = self.sums()
         return resolve(rgb.astype(np.float64), w.astype(np.float64))
 
+    def load(self, rgb_sum: np.ndarray, weight: np.ndarray) -> None:
+        """Replaces the film's sums (e.g. to resume from sums saved with `sums`)."""
+        a = np.empty((self.height, self.width, 4), np.float32)
+        a[..., :3] = rgb_sum
+        a[..., 3] = weight
+        N.check(N.lib().bg_film_load(self.h, a.ctypes.data), self.ctx.h)
+
+    def write_image(self, path: str | None = None, splat_scale: float = 1.0, scale: float = 1.0) -> np.ndarray:
+        """Film::write_image (film.rs:156-202) resolved on the GPU: the (h, w, 3) float32 image, also
+        written to `path` as OpenEXR (Image::write, io/image.rs:47-53) when given."""
+        out = np.empty((self.height, self.width, 3), np.float32)
+        p = None if path is None else str(path).encode()
+        N.check(N.lib().bg_film_write_image(self.h, float(splat_scale), float(scale), out.ctypes.data, p), self.ctx.h)
+        return out
+
 
 class GpuPathIntegrator:
     """PathIntegrator::new(camera, sampler, max_depth, rr_threshold) with an independent
@@ -149,7 +164,7 @@ class GuidedPathIntegrator(GpuPathIntegrator):
     def render(self, bvh: GpuBvh) -> np.ndarray:
         film = Film(self.ctx, self.camera.width, self.camera.height)
         self.last_stats = self.render_into(bvh, film)
-        img = film.image().astype(np.float32)
+        img = film.write_image()
         film.close()
         return img
 

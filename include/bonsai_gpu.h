@@ -262,15 +262,42 @@ int bg_film_clear(bg_film* film);
 /* rgbw: width*height*4 floats (r, g, b, weight sums); rgbw64 likewise in
  * double (requires BG_FILM_F64). Either may be NULL. */
 int bg_film_read(bg_film* film, float* rgbw, double* rgbw64);
+/* Replaces the film's FP32 sums with rgbw (width*height*4 floats, as
+ * bg_film_read returns them), e.g. to resume a render from saved sums; FP64
+ * sums, if kept, are set to the same values. No counterpart in the reference,
+ * whose film lives only for one render call. */
+int bg_film_load(bg_film* film, const float* rgbw);
 
 /* Renders desc->spp samples per pixel of the window into the film
  * (SamplerIntegrator::render, src/base/integrator.rs:35-133). */
 int bg_render_film(bg_ctx* ctx, const bg_scene* scene, const bg_render_desc* desc, bg_film* film, bg_stats* stats);
 
-/* Convenience: a fresh film, bg_render_film, then Film::write_image
- * (film.rs:156-202: XYZ round trip, divide by weight, clamp at 0) into
- * rgb_out (width*height*3 floats of the full film). */
+/* Convenience: a fresh film, bg_render_film, then bg_film_write_image with
+ * splat_scale 1 and scale 1 (integrator.rs:132) into rgb_out (width*height*3
+ * floats of the full film). */
 int bg_render(bg_ctx* ctx, const bg_scene* scene, const bg_render_desc* desc, float* rgb_out, bg_stats* stats);
+
+/* Film::write_image (src/base/film.rs:156-202), resolved on the GPU: per pixel
+ * the RGB sum goes to XYZ (merge_film_tile, film.rs:138-154) and back to RGB
+ * (spectrum.rs:34-44), is divided by the weight sum and clamped at 0 when
+ * that is non-zero (f32::max: a NaN becomes 0), gets splat_scale times the
+ * (zero) splat added and is multiplied by scale (FilmOptions::scale). rgb_out:
+ * width*height*3 floats on the host (top row first), or NULL. path: the
+ * OpenEXR file Image::write (src/io/image.rs:47-53) would write, or NULL; at
+ * least one of the two must be given. The file is written to path + ".part"
+ * and renamed. */
+int bg_film_write_image(bg_film* film, float splat_scale, float scale, float* rgb_out, const char* path);
+/* The same resolve into device memory (width*height*3 floats) on the
+ * context's stream, synchronised; ms (optional) receives the kernel time. */
+int bg_film_resolve_device(bg_film* film, float splat_scale, float scale, void* d_rgb, double* ms);
+/* Image::write (src/io/image.rs:47-53): width x height RGB f32 pixels (top
+ * row first) as a single-part scanline OpenEXR file with FLOAT channels B, G,
+ * R and no compression. */
+int bg_exr_write(const char* path, uint32_t width, uint32_t height, const float* rgb);
+/* The bytes bg_exr_write would write. With out NULL, *size receives the byte
+ * count; otherwise *size is out's capacity on entry and the byte count on
+ * return (BG_ERR_INVALID if the capacity is short). */
+int bg_exr_encode(uint32_t width, uint32_t height, const float* rgb, unsigned char* out, uint64_t* size);
 
 /* ----------------------------------------------------------- scene batches */
 
