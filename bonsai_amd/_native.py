@@ -118,6 +118,10 @@ class bg_guiding_desc(C.Structure):
                 ("lobes", C.c_uint32)]
 
 
+class bg_direct_desc(C.Structure):
+    _fields_ = [("integrator", C.c_uint32), ("strategy", C.c_uint32)]
+
+
 class bg_batch_item(C.Structure):
     _fields_ = [("scene", C.c_void_p), ("camera", bg_camera_desc), ("seed", C.c_uint64)]
 
@@ -130,6 +134,10 @@ class bg_batch_desc(C.Structure):
 BG_MODE_REF = 0
 BG_MODE_CORRECTED = 1
 BG_FILM_F64 = 1
+BG_INTEGRATOR_DIRECT = 1
+BG_INTEGRATOR_WHITTED = 2
+BG_LIGHT_SAMPLE_ALL = 0
+BG_LIGHT_SAMPLE_ONE = 1
 BG_OBJ_NORMALS = 1
 BG_OBJ_UVS = 2
 BG_GUIDE_PARAMS = 19240
@@ -164,6 +172,8 @@ SIGNATURES = {
     "bg_render_film": (C.c_int, [P, P, C.POINTER(bg_render_desc), P, C.POINTER(bg_stats)]),
     "bg_render": (C.c_int, [P, P, C.POINTER(bg_render_desc), P, C.POINTER(bg_stats)]),
     "bg_film_load": (C.c_int, [P, P]),
+    "bg_render_film_direct": (C.c_int, [P, P, C.POINTER(bg_render_desc), C.POINTER(bg_direct_desc), P,
+                                        C.POINTER(bg_stats)]),
     "bg_film_write_image": (C.c_int, [P, C.c_float, C.c_float, P, C.c_char_p]),
     "bg_film_resolve_device": (C.c_int, [P, C.c_float, C.c_float, P, C.POINTER(C.c_double)]),
     "bg_exr_write": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, P]),

@@ -174,6 +174,20 @@ struct PathBuffers {
     uint64_t last_records = 0; // records of the last trained pass (their weights are final)
 };
 
+// Light slots of the direct-lighting and Whitted integrators, slot-major (slot
+// j of path i at j * n_paths + i); one allocation.
+struct DirectSlots {
+    size_t cap = 0;
+    void* mem = nullptr;
+    float4* sray = nullptr;  // shadow ray, 2 per slot
+    uint4* shit = nullptr;
+    float4* mray = nullptr;  // BSDF-sampling (MIS) ray, 2 per slot
+    uint4* mhit = nullptr;
+    float4* a = nullptr;     // light-branch term rgb, flags
+    float4* b = nullptr;     // BSDF-branch f rgb, MIS weight
+    float2* c = nullptr;     // BSDF-branch pdf, light index
+};
+
 // Replay ring of training samples with nonzero weight, kept by the guide
 // across passes; state = [head, count, base of the last append, last append size].
 struct Replay {
@@ -275,6 +289,12 @@ hipError_t guide_train_step(const GuideDevice& g, const GuideBox& box, const flo
                             uint64_t n, const GuideScratch& sc, float lr, int t, int grid, float* grad_out,
                             hipStream_t s);
 
+// Direct-lighting / Whitted passes (direct.hip); kind 0: all lights, 1: one light, 2: Whitted.
+hipError_t launch_direct_shade(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b,
+                               const DirectSlots& d, int kind, hipStream_t s);
+hipError_t launch_direct_resolve(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b,
+                                 const DirectSlots& d, int kind, hipStream_t s);
+
 // Film::write_image over n pixel sums (film.hip)
 hipError_t launch_film_resolve(const float4* sums, uint64_t n, float splat_scale, float scale, float* rgb_out,
                                hipStream_t s);
@@ -358,4 +378,5 @@ struct bg_ctx {
     void* io = nullptr;
     size_t io_bytes = 0;
     bg::PathBuffers paths;
+    bg::DirectSlots direct;
 };

@@ -225,6 +225,9 @@ void bg_destroy(bg_ctx* ctx) {
     if (ctx->spill) hipFree(ctx->spill);
     if (ctx->counters) hipFree(ctx->counters);
     if (ctx->io) hipFree(ctx->io);
+    if (ctx->paths.mem) hipFree(ctx->paths.mem);
+    if (ctx->paths.guide_mem) hipFree(ctx->paths.guide_mem);
+    if (ctx->direct.mem) hipFree(ctx->direct.mem);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     if (ctx->stream) hipStreamDestroy(ctx->stream);

@@ -17,6 +17,7 @@
 #include "bg_internal.h"
 #include "dev_camera.h"
 #include "dev_guide.h"
+#include "dev_light.h"
 #include "dev_scan.h"
 #include "dev_shade.h"
 
@@ -29,8 +30,6 @@ using dev::block_exclusive;
 constexpr uint32_t kAlive = 1u, kSpecFlag = 2u, kItemShift = 16u;
 // nee_a.w flags
 constexpr uint32_t kHasNee = 1u, kHasLight = 2u, kHasBsdf = 4u;
-
-__device__ __forceinline__ uint64_t key_of(uint2 k) { return (uint64_t)k.x | ((uint64_t)k.y << 32); }
 
 // A path the network is evaluated for at this bounce: alive, with a hit.
 __device__ __forceinline__ bool guide_live(const PathBuffers& b, uint64_t i) {
@@ -45,53 +44,6 @@ __device__ __forceinline__ bool rec_slot(const RenderParamsDev& p, uint64_t i, u
     if (k >= p.rec_paths) return false;
     *r = k * p.max_depth + bounce;
     return true;
-}
-
-__device__ __forceinline__ void write_ray(float4* rays, uint64_t i, V o, float t_max, V d, float time) {
-    rays[2 * i] = make_float4(o.x, o.y, o.z, t_max);
-    rays[2 * i + 1] = make_float4(d.x, d.y, d.z, time);
-}
-__device__ __forceinline__ void kill_ray(float4* rays, uint64_t i) {
-    rays[2 * i] = make_float4(0.0f, 0.0f, 0.0f, -__builtin_inff());
-    rays[2 * i + 1] = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
-}
-
-__device__ __forceinline__ C3 emission(const LightDev& l, V n, V w) {
-    return (l.double_sided || dot(n, w) > 0.0f) ? C3{l.L[0], l.L[1], l.L[2]} : mkc(0.0f);
-}
-
-__device__ __forceinline__ void tri_verts(const float* __restrict__ v, V* p0, V* p1, V* p2) {
-    *p0 = mkv(v[0], v[1], v[2]);
-    *p1 = mkv(v[3], v[4], v[5]);
-    *p2 = mkv(v[6], v[7], v[8]);
-}
-
-// Interaction::spawn_ray (interaction.rs:115-121); t_max 1 - 1e-4 in the reference, +inf corrected.
-__device__ __forceinline__ void spawn(const Surf& s, V d, bool ref, V* o, float* t_max) {
-    *o = offset_ray_origin(s.p, s.perr, s.n, d);
-    *t_max = ref ? 1.0f - 0.0001f : __builtin_inff();
-}
-
-// shape.rs:40-57 on a light's triangle: solid-angle pdf of direction wi.
-__device__ __noinline__ float light_point_pdf(const ShadeSceneDev& sc, const LightDev& l, const Surf& s, V wi, bool ref) {
-    V o;
-    float tm;
-    spawn(s, wi, ref, &o, &tm);
-    RayState rs;
-    rs.ox = o.x; rs.oy = o.y; rs.oz = o.z; rs.t_max = tm;
-    rs.dx = wi.x; rs.dy = wi.y; rs.dz = wi.z;
-    ray_setup(rs);
-    const float* v = sc.tri_orig + 9 * (uint64_t)l.prim;
-    float t, b0, b1, b2;
-    if (!tri_hit_full(rs, v, &t, &b0, &b1, &b2)) return 0.0f;
-    V p0, p1, p2;
-    tri_verts(v, &p0, &p1, &p2);
-    const V hp = b0 * p0 + b1 * p1 + b2 * p2;
-    V hn = unit(cross(p0 - p2, p1 - p2));
-    if (sc.tri_flip[l.prim]) hn = -hn;
-    float pdf = len2(s.p - hp) / (__builtin_fabsf(dot(hn, -wi)) * l.area);
-    if (__builtin_isinf(pdf)) pdf = 0.0f;
-    return pdf;
 }
 
 // Batch item owning path i (items sorted by path_base) or pixel t (by pix_base).
@@ -176,17 +128,9 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
         return;
     }
     const uint32_t prim = h.x;
-    RayState rs;
-    rs.ox = r0.x; rs.oy = r0.y; rs.oz = r0.z; rs.t_max = r0.w;
-    rs.dx = r1.x; rs.dy = r1.y; rs.dz = r1.z;
-    ray_setup(rs);
     Surf s;
     float t;
-    bool ok;
-    if (prim < sc.n_tris) ok = tri_interaction(rs, sc.tri_orig + 9 * (uint64_t)prim, sc.tri_flip[prim] != 0, &s, &t);
-    else ok = sphere_interaction(rs, sc.spheres[prim - sc.n_tris], &s, &t);
-    s.time = r1.w;
-    if (!ok) {  // unreachable for a primitive the traversal accepted on this ray
+    if (!hit_surface(sc, r0, r1, prim, &s, &t)) {  // unreachable for a primitive the traversal accepted on this ray
         b.state[i] = make_uint2(st.x, 0u);
         kill_ray(b.ray, i);
         return;
@@ -224,48 +168,7 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
         // matte.rs:20-41 / conductor / plastic.rs:24-52; bsdf.rs:33-52
         const MaterialDev& m = sc.mats[sc.prim_mat[prim]];
         BsdfN<MULTI ? 2 : 1> bs;
-        bs.ns = s.ns;
-        bs.ng = s.n;
-        bs.ss = unit(s.dpdu);
-        bs.ts = cross(bs.ns, bs.ss);
-        bs.n = 0;
-        if (m.type == 0 || m.type == 2) {  // Lambertian Kd (clamped at 0), if not black
-            const C3 kd = C3{clampf(m.r[0], 0.0f, __builtin_inff()), clampf(m.r[1], 0.0f, __builtin_inff()),
-                             clampf(m.r[2], 0.0f, __builtin_inff())};
-            if (!black(kd)) {
-                Lobe l{};
-                l.type = kRefl | kDiffuse;
-                l.r = kd;
-                l.ref = ref;
-                bs.add(l);
-            }
-        }
-        if (MULTI && m.type == 2) {  // plastic: Trowbridge-Reitz Ks with FresnelDielectric(1.5, 1.0), if not black
-            const C3 ks = C3{clampf(m.eta[0], 0.0f, __builtin_inff()), clampf(m.eta[1], 0.0f, __builtin_inff()),
-                             clampf(m.eta[2], 0.0f, __builtin_inff())};
-            if (!black(ks)) {
-                const float rough = m.k[0] != 0.0f ? roughness_to_alpha(m.alpha) : m.alpha;
-                Lobe l{};
-                l.type = kRefl | kGlossy;
-                l.r = ks;
-                l.ax = rmax(rough, 0.001f);
-                l.ay = rmax(rough, 0.001f);
-                l.ref = ref;
-                l.dielectric = true;
-                bs.add(l);
-            }
-        } else if (m.type == 1) {  // conductor
-            Lobe l{};
-            l.type = kRefl | kGlossy;
-            l.r = C3{m.r[0], m.r[1], m.r[2]};
-            l.ax = rmax(m.alpha, 0.001f);
-            l.ay = rmax(m.alpha, 0.001f);
-            l.eta = C3{m.eta[0], m.eta[1], m.eta[2]};
-            l.k = C3{m.k[0], m.k[1], m.k[2]};
-            l.ref = ref;
-            l.dielectric = false;
-            bs.add(l);
-        }
+        build_bsdf(bs, m, s, ref);
         const uint32_t nonspec = kAll & ~kSpecular;
         // network mixture of this vertex (guided estimator with guide_alpha > 0)
         const float4* gm = nullptr;
@@ -296,45 +199,18 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
             const float us0 = GUIDED ? 0.0f : draw(key, dim), us1 = GUIDED ? 0.0f : draw(key, dim);
             const LightDev& light = sc.lights[idx];
             uint32_t nee = kHasNee;
-            // diffuse.rs:134-167: sample a point on the light's triangle (triangle.rs:522-559)
-            V p0, p1, p2;
-            tri_verts(sc.tri_orig + 9 * (uint64_t)light.prim, &p0, &p1, &p2);
-            const float sq = __builtin_sqrtf(ul0);
-            const float c0 = 1.0f - sq, c1 = ul1 * sq, c2 = 1.0f - c0 - c1;
-            const V lp = c0 * p0 + c1 * p1 + c2 * p2;
-            V ln = unit(cross(p1 - p0, p2 - p0));
-            if (sc.tri_flip[light.prim]) ln = ln * -1.0f;
-            const V labs = vabs(c0 * p0) + vabs(c1 * p1) + vabs(c2 * p2);
-            const V lerr = gamma_n(6.0f) * labs;
-            const float pdf_area = 1.0f / light.area;
-            V wi = lp - s.p;
-            float lpdf;
-            if (len2(wi) == 0.0f) {
-                lpdf = 0.0f;
-            } else {
-                wi = unit(wi);
-                const float conv = len2(s.p - lp) / __builtin_fabsf(dot(ln, -wi));
-                lpdf = ref ? conv : pdf_area * conv;  // shape.rs:26-27 drops 1/area
-                if (__builtin_isinf(lpdf)) lpdf = 0.0f;
-            }
-            C3 lrad = mkc(0.0f);
-            V lwi = mkv(0.0f, 0.0f, 0.0f);
-            if (!(lpdf == 0.0f || len2(lp - s.p) == 0.0f)) {
-                lwi = unit(lp - s.p);
-                lrad = emission(light, ln, ref ? lwi : -lwi);  // diffuse.rs:67 evaluates toward +wi
-            } else {
-                lpdf = 0.0f;
-            }
+            // diffuse.rs:134-167: a point on the light's triangle (triangle.rs:522-559)
+            const LightPoint lpt = sample_light_point(sc, light, s, ul0, ul1, ref);
+            const float lpdf = lpt.pdf;
+            const V lwi = lpt.wi;
+            const C3 lrad = lpt.rad;
             float scattering_pdf = 0.0f;
             C3 term = mkc(0.0f);
             if (lpdf > 0.0f && !black(lrad)) {
                 const C3 f = bs.f(s.wo, lwi, nonspec) * __builtin_fabsf(dot(lwi, bs.ns));
                 scattering_pdf = bs.pdf(s.wo, lwi, nonspec);
                 if (!black(f)) {
-                    // VisibilityTester: spawn_ray_to_it (interaction.rs:132-143)
-                    const V o = offset_ray_origin(s.p, s.perr, s.n, lp - s.p);
-                    const V tgt = offset_ray_origin(lp, lerr, ln, o - lp);
-                    write_ray(b.sray, i, o, 1.0f - 0.0001f, tgt - o, s.time);
+                    write_shadow_ray(b.sray, i, s, lpt);
                     float w;
                     if (GUIDED) {
                         const float pc = (1.0f - alpha) * scattering_pdf + (alpha > 0.0f ? alpha * guide_pdf(gm, lwi) : 0.0f);
@@ -459,12 +335,7 @@ __global__ __launch_bounds__(256) void k_nee_resolve(RenderParamsDev p, ShadeSce
             const float4 dd = b.nee_d[i];
             const bool same = p.mode == 0 ? (li >= 0) : (li == (int32_t)__float_as_uint(dd.y));
             if (same) {
-                V p0, p1, p2;
-                tri_verts(sc.tri_orig + 9 * (uint64_t)hp, &p0, &p1, &p2);
-                V n = unit(cross(p0 - p2, p1 - p2));
-                if (sc.tri_flip[hp]) n = -n;
-                const float4 d = b.mray[2 * i + 1];
-                const C3 Le = emission(sc.lights[li], n, mkv(-d.x, -d.y, -d.z));
+                const C3 Le = emission_at(sc, li, hp, b.mray[2 * i + 1]);
                 if (!black(Le)) {
                     const float4 fw = b.nee_b[i];
                     out = out + C3{fw.x, fw.y, fw.z} * Le * fw.w / dd.x;

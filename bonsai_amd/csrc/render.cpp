@@ -254,8 +254,8 @@ int bg_film_write_image(bg_film* f, float splat_scale, float scale, float* rgb_o
 
 namespace {
 
-int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film* film, bg_stats* stats,
-                bg_guide* guide, const bg_guiding_desc* gd) {
+// Checks a render call's arguments and resolves its pixel window.
+int check_render(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, const bg_film* film, uint32_t win[4]) {
     if (!ctx || !s || !d || !film || s->ctx != ctx || film->ctx != ctx) return fail(ctx, BG_ERR_INVALID, "null or foreign argument");
     const bg_camera_desc& cam = d->camera;
     if (cam.width != film->width || cam.height != film->height) return fail(ctx, BG_ERR_INVALID, "film size differs from camera");
@@ -265,7 +265,33 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
     uint32_t x0 = d->x0, y0 = d->y0, x1 = d->x1, y1 = d->y1;
     if (x1 == 0 && y1 == 0) { x0 = 0; y0 = 0; x1 = cam.width; y1 = cam.height; }
     if (x0 >= x1 || y0 >= y1 || x1 > cam.width || y1 > cam.height) return fail(ctx, BG_ERR_INVALID, "bad window");
+    win[0] = x0; win[1] = y0; win[2] = x1; win[3] = y1;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
+    return BG_OK;
+}
+
+bg::ShadeSceneDev shade_scene(const bg_scene* s) {
+    bg::ShadeSceneDev sc;
+    sc.tri_orig = s->d_tri_orig;
+    sc.tri_flip = s->d_tri_flip;
+    sc.prim_mat = s->d_prim_mat;
+    sc.prim_light = s->d_prim_light;
+    sc.mats = s->d_mats;
+    sc.lights = s->d_lights;
+    sc.spheres = s->d_sph_shade;
+    sc.n_tris = (uint32_t)s->n_tris;
+    sc.n_lights = (uint32_t)s->lights.size();
+    sc.multi_lobe = bg::has_multi_lobe(s->materials);
+    return sc;
+}
+
+int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film* film, bg_stats* stats,
+                bg_guide* guide, const bg_guiding_desc* gd) {
+    uint32_t win[4];
+    int rc = check_render(ctx, s, d, film, win);
+    if (rc) return rc;
+    const bg_camera_desc& cam = d->camera;
+    const uint32_t x0 = win[0], y0 = win[1], x1 = win[2], y1 = win[3];
     if (stats) *stats = bg_stats{};
     if (d->spp == 0 || s->bvh.nodes.empty()) return BG_OK;
 
@@ -273,7 +299,7 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
     uint64_t target = d->paths_per_pass ? d->paths_per_pass : (uint64_t)4 << 20;
     uint32_t spp_pass = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(d->spp, target / npix));
     const uint64_t cap = npix * spp_pass;
-    int rc = bg::ensure_paths(ctx, cap);
+    rc = bg::ensure_paths(ctx, cap);
     if (rc) return rc;
     const int block = ctx->shape.block;
     const uint64_t grid_cap = ctx->shape.blocks_per_cu > 0 ? (uint64_t)ctx->num_cus * ctx->shape.blocks_per_cu : (1u << 20);
@@ -319,17 +345,7 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
     }
     const int guide_grid = 2 * std::max(ctx->num_cus, 1);
     uint64_t train_steps = 0;
-    bg::ShadeSceneDev sc;
-    sc.tri_orig = s->d_tri_orig;
-    sc.tri_flip = s->d_tri_flip;
-    sc.prim_mat = s->d_prim_mat;
-    sc.prim_light = s->d_prim_light;
-    sc.mats = s->d_mats;
-    sc.lights = s->d_lights;
-    sc.spheres = s->d_sph_shade;
-    sc.n_tris = (uint32_t)s->n_tris;
-    sc.n_lights = (uint32_t)s->lights.size();
-    sc.multi_lobe = bg::has_multi_lobe(s->materials);
+    const bg::ShadeSceneDev sc = shade_scene(s);
     const bg::PathBuffers& b = ctx->paths;
     hipStream_t st = ctx->stream;
 
@@ -418,12 +434,141 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
     return BG_OK;
 }
 
+// Light-slot arrays of the direct-lighting and Whitted integrators, one allocation.
+int ensure_direct(bg_ctx* ctx, size_t slots) {
+    bg::DirectSlots& d = ctx->direct;
+    if (d.cap >= slots) return BG_OK;
+    if (d.mem) hipFree(d.mem);
+    d = bg::DirectSlots{};
+    const size_t per_slot[] = {32, 16, 32, 16, 16, 16, 8};
+    size_t total = 0;
+    for (size_t b : per_slot) total += ((b * slots + 255) / 256) * 256;
+    void* mem = nullptr;
+    HIP_TRY(ctx, hipMalloc(&mem, total));
+    char* p = (char*)mem;
+    auto take = [&](size_t bytes) {
+        void* r = p;
+        p += ((bytes + 255) / 256) * 256;
+        return r;
+    };
+    d.mem = mem;
+    d.sray = (float4*)take(32 * slots);
+    d.shit = (uint4*)take(16 * slots);
+    d.mray = (float4*)take(32 * slots);
+    d.mhit = (uint4*)take(16 * slots);
+    d.a = (float4*)take(16 * slots);
+    d.b = (float4*)take(16 * slots);
+    d.c = (float2*)take(8 * slots);
+    d.cap = slots;
+    return BG_OK;
+}
+
+// SamplerIntegrator::render (integrator.rs:35-133) with DirectLightingIntegrator
+// or WhittedIntegrator radiance: camera rays, their closest hits, the light
+// slots, the shadow and MIS queries of every slot, the per-path sums, the film.
+int direct_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, const bg_direct_desc* dd, bg_film* film,
+                bg_stats* stats) {
+    uint32_t win[4];
+    int rc = check_render(ctx, s, d, film, win);
+    if (rc) return rc;
+    if (!dd) return fail(ctx, BG_ERR_INVALID, "null integrator descriptor");
+    if (dd->integrator != BG_INTEGRATOR_DIRECT && dd->integrator != BG_INTEGRATOR_WHITTED)
+        return fail(ctx, BG_ERR_INVALID, "unknown integrator");
+    if (dd->integrator == BG_INTEGRATOR_DIRECT && dd->strategy > BG_LIGHT_SAMPLE_ONE)
+        return fail(ctx, BG_ERR_INVALID, "unknown light strategy");
+    const int kind = dd->integrator == BG_INTEGRATOR_WHITTED ? 2 : (dd->strategy == BG_LIGHT_SAMPLE_ONE ? 1 : 0);
+    const uint32_t n_lights = (uint32_t)s->lights.size();
+    // per path: the camera query plus up to two per slot, counted in 16 bits
+    if (kind != 1 && n_lights > 16383) return fail(ctx, BG_ERR_UNSUPPORTED, "more than 16383 lights with a slot per light");
+    if (stats) *stats = bg_stats{};
+    if (d->spp == 0 || s->bvh.nodes.empty()) return BG_OK;
+    const uint32_t x0 = win[0], y0 = win[1], x1 = win[2], y1 = win[3];
+    const uint64_t npix = (uint64_t)(x1 - x0) * (y1 - y0);
+    const uint64_t n_slots = kind == 1 ? 1 : std::max<uint32_t>(n_lights, 1u);
+    // a pass holds whole samples of the window; its slots stay within a budget when they can
+    const uint64_t target = d->paths_per_pass ? d->paths_per_pass : (uint64_t)4 << 20;
+    const uint32_t spp_pass =
+        (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(d->spp, std::max<uint64_t>(target, 1) / (npix * n_slots)));
+    const uint64_t cap = npix * spp_pass;
+    rc = bg::ensure_paths(ctx, cap);
+    if (!rc) rc = ensure_direct(ctx, cap * n_slots);
+    if (rc) return rc;
+    const int block = ctx->shape.block;
+    const uint64_t grid_cap = ctx->shape.blocks_per_cu > 0 ? (uint64_t)ctx->num_cus * ctx->shape.blocks_per_cu : (1u << 20);
+    const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((cap * n_slots + block - 1) / block,
+                                                                    std::min<uint64_t>(grid_cap, 1u << 20)));
+    rc = bg::ensure_spill(ctx, (size_t)grid * block * (s->bvh.max_depth + 1) * bg::kSpillWords * sizeof(uint32_t));
+    if (rc) return rc;
+
+    bg::RenderParamsDev p;
+    bg::camera_setup(d->camera, &p.cam);
+    p.seed = d->seed;
+    p.x0 = x0; p.y0 = y0; p.x1 = x1; p.y1 = y1;
+    p.max_depth = d->max_depth;
+    p.rr = d->rr_threshold;
+    p.mode = d->mode;
+    const bg::ShadeSceneDev sc = shade_scene(s);
+    const bg::PathBuffers& b = ctx->paths;
+    const bg::DirectSlots& ds = ctx->direct;
+    hipStream_t st = ctx->stream;
+    HIP_TRY(ctx, hipMemsetAsync(b.counters, 0, 2 * sizeof(unsigned long long), st));
+    auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+    uint64_t launches = 0;
+    for (uint32_t s0 = 0; s0 < d->spp; s0 += spp_pass) {
+        const uint32_t ns = std::min(spp_pass, d->spp - s0);
+        p.first_sample = d->first_sample + s0;
+        p.n_samples = ns;
+        p.n_paths = npix * ns;
+        const uint64_t nq = p.n_paths * (n_lights ? n_slots : 0);
+        const int g = (int)std::min<uint64_t>(grid, (p.n_paths + block - 1) / block);
+        const int gq = (int)std::max<uint64_t>(1, std::min<uint64_t>(grid, (nq + block - 1) / block));
+        HIP_TRY(ctx, bg::launch_init_paths(p, b, st));
+        HIP_TRY(ctx, bg::launch_traverse(traverse_args(s, ctx, b.ray, b.hit, p.n_paths), false, false, ctx->shape, g, st));
+        HIP_TRY(ctx, bg::launch_direct_shade(p, sc, b, ds, kind, st));
+        launches += 3;
+        if (nq) {
+            HIP_TRY(ctx, bg::launch_traverse(traverse_args(s, ctx, ds.sray, ds.shit, nq), true, false, ctx->shape, gq, st));
+            ++launches;
+            if (kind != 2) {
+                HIP_TRY(ctx, bg::launch_traverse(traverse_args(s, ctx, ds.mray, ds.mhit, nq), false, false, ctx->shape, gq, st));
+                ++launches;
+            }
+        }
+        HIP_TRY(ctx, bg::launch_direct_resolve(p, sc, b, ds, kind, st));
+        HIP_TRY(ctx, bg::launch_film(p, b, film->width, film->height, film->sums, film->sums64, st));
+        HIP_TRY(ctx, bg::launch_count_rays(p, b, b.counters, st));
+        launches += 3;
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    auto t1 = std::chrono::steady_clock::now();
+    float ms = 0.0f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    unsigned long long nr[2] = {0, 0};
+    HIP_TRY(ctx, hipMemcpy(nr, b.counters, sizeof(nr), hipMemcpyDeviceToHost));
+    if (stats) {
+        stats->rays = nr[0];
+        stats->shadow_rays = nr[1];
+        stats->kernel_ms = ms;
+        stats->total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        stats->launches = launches;
+        stats->samples = npix * d->spp;
+    }
+    return BG_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
 int bg_render_film(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film* film, bg_stats* stats) {
     return render_impl(ctx, s, d, film, stats, nullptr, nullptr);
+}
+
+int bg_render_film_direct(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, const bg_direct_desc* dd,
+                          bg_film* film, bg_stats* stats) {
+    return direct_impl(ctx, s, d, dd, film, stats);
 }
 
 int bg_render_film_guided(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_guide* guide,

@@ -128,6 +128,60 @@ class GpuPathIntegrator:
         return out
 
 
+UNIFORM_SAMPLE_ALL = N.BG_LIGHT_SAMPLE_ALL
+UNIFORM_SAMPLE_ONE = N.BG_LIGHT_SAMPLE_ONE
+
+
+class _FilmRenderer(GpuPathIntegrator):
+    """A SamplerIntegrator whose render call fills a Film (render_into) and resolves it (render)."""
+
+    def _render_film(self, bvh, d, film, st):
+        raise NotImplementedError
+
+    def render_into(self, bvh: GpuBvh, film: Film, spp=None, first_sample=0, window=None,
+                    paths_per_pass=0) -> N.bg_stats:
+        d = self.desc(spp, first_sample, window, paths_per_pass)
+        st = N.bg_stats()
+        self._render_film(bvh, d, film, st)
+        return st
+
+    def render(self, bvh: GpuBvh) -> np.ndarray:
+        film = Film(self.ctx, self.camera.width, self.camera.height)
+        self.last_stats = self.render_into(bvh, film)
+        img = film.write_image()
+        film.close()
+        return img
+
+
+class DirectLightingIntegrator(_FilmRenderer):
+    """DirectLightingIntegrator::new(camera, sampler, scene, max_depth, strategy) (direct.rs:28-53)
+    on the GPU: emission plus estimate_direct for every light (UNIFORM_SAMPLE_ALL) or one chosen
+    light (UNIFORM_SAMPLE_ONE) at the camera ray's hit."""
+
+    def __init__(self, ctx: Context, camera: Camera, spp: int, max_depth: int = 5,
+                 strategy: int = UNIFORM_SAMPLE_ALL, mode: int = MODE_REF, seed: int = 1):
+        super().__init__(ctx, camera, spp, max_depth, 1.0, mode, seed)
+        self.direct = N.bg_direct_desc(N.BG_INTEGRATOR_DIRECT, strategy)
+
+    def _render_film(self, bvh, d, film, st):
+        N.check(N.lib().bg_render_film_direct(self.ctx.h, bvh.h, C.byref(d), C.byref(self.direct), film.h,
+                                              C.byref(st)), self.ctx.h)
+
+
+class WhittedIntegrator(_FilmRenderer):
+    """WhittedIntegrator::new(camera, sampler, max_depth) (whitted.rs:16-24) on the GPU: emission
+    plus one light-sampled, shadow-tested contribution per light at the camera ray's hit."""
+
+    def __init__(self, ctx: Context, camera: Camera, spp: int, max_depth: int = 5, mode: int = MODE_REF,
+                 seed: int = 1):
+        super().__init__(ctx, camera, spp, max_depth, 1.0, mode, seed)
+        self.direct = N.bg_direct_desc(N.BG_INTEGRATOR_WHITTED, 0)
+
+    def _render_film(self, bvh, d, film, st):
+        N.check(N.lib().bg_render_film_direct(self.ctx.h, bvh.h, C.byref(d), C.byref(self.direct), film.h,
+                                              C.byref(st)), self.ctx.h)
+
+
 class GuidedPathIntegrator(GpuPathIntegrator):
     """The corrected path integrator with a learned directional proposal: at every vertex the
     continuation is drawn from the guiding network's vMF mixture with probability ``select_prob``,

@@ -272,6 +272,28 @@ int bg_film_load(bg_film* film, const float* rgbw);
  * (SamplerIntegrator::render, src/base/integrator.rs:35-133). */
 int bg_render_film(bg_ctx* ctx, const bg_scene* scene, const bg_render_desc* desc, bg_film* film, bg_stats* stats);
 
+/* The reference's other SamplerIntegrators on the GPU path:
+ * DirectLightingIntegrator (src/integrators/direct.rs:72-113, with
+ * uniform_sample_all_lights / uniform_sample_one_light and estimate_direct,
+ * src/base/integrator.rs:274-437) and WhittedIntegrator
+ * (src/integrators/whitted.rs:41-93). desc->mode selects the reference or
+ * corrected semantics as for the path integrator; max_depth bounds only the
+ * specular recursion, which adds nothing for the supported materials (none
+ * has a specular lobe); rr_threshold is unused. Light slots (one per light
+ * for BG_LIGHT_SAMPLE_ALL and Whitted, one for BG_LIGHT_SAMPLE_ONE) take 136
+ * bytes each per camera sample of a pass; with a slot per light at most 16383
+ * lights are supported. */
+#define BG_INTEGRATOR_DIRECT 1u
+#define BG_INTEGRATOR_WHITTED 2u
+#define BG_LIGHT_SAMPLE_ALL 0u /* LightStrategy::UniformSampleAll */
+#define BG_LIGHT_SAMPLE_ONE 1u /* LightStrategy::UniformSampleOne */
+typedef struct bg_direct_desc {
+    uint32_t integrator; /* BG_INTEGRATOR_DIRECT / BG_INTEGRATOR_WHITTED */
+    uint32_t strategy;   /* direct lighting only */
+} bg_direct_desc;
+int bg_render_film_direct(bg_ctx* ctx, const bg_scene* scene, const bg_render_desc* desc, const bg_direct_desc* direct,
+                          bg_film* film, bg_stats* stats);
+
 /* Convenience: a fresh film, bg_render_film, then bg_film_write_image with
  * splat_scale 1 and scale 1 (integrator.rs:132) into rgb_out (width*height*3
  * floats of the full film). */

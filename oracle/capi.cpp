@@ -259,13 +259,15 @@ void orc_scene_set_shading(void* h, uint32_t n_mat, const uint32_t* mat_type, co
 void orc_render(void* h, const float* c2w, const float* c2w_inv, float fov, float near_clip, float far_clip,
                 float lens_radius, float focal_distance, float shutter_open, float shutter_close, uint32_t width,
                 uint32_t height, uint32_t spp, uint32_t max_depth, float rr, uint64_t seed, uint32_t mode,
-                uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1, int threads, double* rgb_sum, double* weight_sum) {
+                uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1, uint32_t integrator, uint32_t strategy, int threads,
+                double* rgb_sum, double* weight_sum) {
     OrcScene* s = static_cast<OrcScene*>(h);
     s->shade.geom = &s->scene;
     s->shade.bvh = &s->bvh;
     PerspCamera cam = make_camera(mat4_from(c2w), mat4_from(c2w_inv), fov, near_clip, far_clip, shutter_open,
                                   shutter_close, lens_radius, focal_distance, (Float)width, (Float)height);
-    RenderParams p{width, height, spp, max_depth, rr, seed, mode == 0 ? MODE_REF : MODE_CORRECTED, x0, y0, x1, y1};
+    RenderParams p{width, height, spp, max_depth, rr, seed, mode == 0 ? MODE_REF : MODE_CORRECTED, x0, y0, x1, y1,
+                   integrator, strategy};
     render(s->shade, cam, p, threads, rgb_sum, weight_sum);
 }
 

@@ -556,6 +556,73 @@ RGB radiance(const ShadeScene& sc, const Ray& camera_ray, SampleStream& st, cons
     return L;
 }
 
+// direct.rs:72-113. Every primitive has a material, so the BSDF always exists (the
+// reference's bsdf.is_none() continuation never runs); area lights return no
+// radiance for escaping rays (light.rs:32-34); the supported materials have no
+// specular lobes, so specular_reflect / specular_transmit (integrator.rs:143-271)
+// add nothing and draw only samples no later step uses.
+RGB radiance_direct(const ShadeScene& sc, const Ray& camera_ray, SampleStream& st, const RenderParams& p) {
+    const Mode mode = p.mode;
+    Ray ray = camera_ray;
+    Isect it;
+    uint32_t prim = 0xFFFFFFFFu;
+    if (!scene_intersect(sc, &ray, &it, &prim)) return RGB();
+    const Material& m = sc.materials[sc.prim_material[prim]];
+    BSDF bsdf = make_bsdf(it, m, mode);
+    RGB L;
+    // it.emitted_radiance(&it.direction)
+    int32_t hit_light = sc.prim_light[prim];
+    if (hit_light >= 0) L = L + emission(sc.lights[hit_light], it.n, it.wo);
+    if (!sc.lights.empty()) {
+        if (p.strategy == SAMPLE_ALL_LIGHTS) {
+            // integrator.rs:274-306 with Light::sample_count() = 1 (light.rs:52-54)
+            RGB all;
+            for (uint32_t i = 0; i < (uint32_t)sc.lights.size(); ++i) {
+                Float ul0 = get_1d(st), ul1 = get_1d(st);
+                Float us0 = get_1d(st), us1 = get_1d(st);
+                RGB direct = RGB() + estimate_direct(sc, it, bsdf, sc.lights[i], i, us0, us1, ul0, ul1, mode);
+                all = all + direct / 1.0f;
+            }
+            L = L + all;
+        } else {
+            // integrator.rs:309-328
+            Float n = (Float)sc.lights.size();
+            Float u = get_1d(st);
+            uint32_t idx = (uint32_t)fmin_r(u * n, n - 1.0f);
+            Float light_pdf = 1.0f / n;
+            Float ul0 = get_1d(st), ul1 = get_1d(st);
+            Float us0 = get_1d(st), us1 = get_1d(st);
+            L = L + estimate_direct(sc, it, bsdf, sc.lights[idx], idx, us0, us1, ul0, ul1, mode) / light_pdf;
+        }
+    }
+    return L;
+}
+
+// whitted.rs:41-93, with the same remarks as radiance_direct.
+RGB radiance_whitted(const ShadeScene& sc, const Ray& camera_ray, SampleStream& st, const RenderParams& p) {
+    const Mode mode = p.mode;
+    Ray ray = camera_ray;
+    Isect it;
+    uint32_t prim = 0xFFFFFFFFu;
+    if (!scene_intersect(sc, &ray, &it, &prim)) return RGB();
+    const Material& m = sc.materials[sc.prim_material[prim]];
+    BSDF bsdf = make_bsdf(it, m, mode);
+    RGB L;
+    int32_t hit_light = sc.prim_light[prim];
+    if (hit_light >= 0) L = L + emission(sc.lights[hit_light], it.n, it.wo);
+    for (uint32_t i = 0; i < (uint32_t)sc.lights.size(); ++i) {
+        Float u0 = get_1d(st), u1 = get_1d(st);
+        LightSample ls = light_sample_point(sc, sc.lights[i], it, u0, u1, mode);
+        if (is_black(ls.radiance) || ls.pdf == 0.0f) continue;
+        RGB f = bsdf.f(it.wo, ls.wi, BSDF_ALL);
+        if (ls.vis && !is_black(f)) {
+            Ray r = spawn_ray_to(it.p, it.perr, it.n, it.time, ls.lp, ls.lerr, ls.ln);
+            if (!scene_occluded(sc, r)) L = L + f * ls.radiance * abs_dot(ls.wi, bsdf.ns) / ls.pdf;
+        }
+    }
+    return L;
+}
+
 void render(const ShadeScene& sc, const PerspCamera& cam, const RenderParams& p, int threads, double* rgb_sum,
             double* weight_sum) {
     const int TILE = 16;
@@ -588,7 +655,9 @@ void render(const ShadeScene& sc, const PerspCamera& cam, const RenderParams& p,
                             Float lu = get_1d(st), lv = get_1d(st);
                             Float tu = get_1d(st);
                             Ray r = camera_generate_ray(cam, fx, fy, lu, lv, tu);
-                            RGB L = radiance(sc, r, st, p);
+                            RGB L = p.integrator == INTEGRATOR_DIRECT    ? radiance_direct(sc, r, st, p)
+                                    : p.integrator == INTEGRATOR_WHITTED ? radiance_whitted(sc, r, st, p)
+                                                                         : radiance(sc, r, st, p);
                             // integrator.rs:92-117
                             Float y = luminance(L);
                             if (L[0] != L[0] || L[1] != L[1] || L[2] != L[2]) L = RGB();

@@ -91,16 +91,27 @@ inline Float get_1d(SampleStream& s) {
     return (Float)(uint32_t)(h >> 40) * (1.0f / 16777216.0f);
 }
 
+// The integrator a render runs: PathIntegrator (path.rs), DirectLightingIntegrator
+// (direct.rs) with LightStrategy::UniformSampleAll / UniformSampleOne, WhittedIntegrator (whitted.rs).
+enum Integrator : uint32_t { INTEGRATOR_PATH = 0, INTEGRATOR_DIRECT = 1, INTEGRATOR_WHITTED = 2 };
+enum LightStrategy : uint32_t { SAMPLE_ALL_LIGHTS = 0, SAMPLE_ONE_LIGHT = 1 };
+
 struct RenderParams {
     uint32_t width, height, spp, max_depth;
     Float rr_threshold;
     uint64_t seed;
     Mode mode;
     uint32_t x0, y0, x1, y1;  // pixel window to render (film is still width x height)
+    uint32_t integrator = INTEGRATOR_PATH;
+    uint32_t strategy = SAMPLE_ALL_LIGHTS;
 };
 
 // Radiance of one camera sample (path.rs:51-146 / base/integrator.rs:309-437).
 RGB radiance(const ShadeScene& sc, const Ray& camera_ray, SampleStream& st, const RenderParams& p);
+// DirectLightingIntegrator::radiance (direct.rs:72-113, integrator.rs:274-328).
+RGB radiance_direct(const ShadeScene& sc, const Ray& camera_ray, SampleStream& st, const RenderParams& p);
+// WhittedIntegrator::radiance (whitted.rs:41-93).
+RGB radiance_whitted(const ShadeScene& sc, const Ray& camera_ray, SampleStream& st, const RenderParams& p);
 
 // SamplerIntegrator::render (integrator.rs:35-133): returns per-pixel double
 // RGB sums and filter-weight sums over the window (film.rs:232-284 box filter).

@@ -42,7 +42,7 @@ def lib() -> C.CDLL:
             "orc_mat4_inverse": (None, [P, P]),
             "orc_scene_set_shading": (None, [P, C.c_uint32, P, P, P, C.c_uint32, P, P, P]),
             "orc_render": (None, [P, P, P] + [C.c_float] * 7 + [C.c_uint32] * 4 + [C.c_float, C.c_uint64] +
-                           [C.c_uint32] * 5 + [C.c_int, P, P]),
+                           [C.c_uint32] * 7 + [C.c_int, P, P]),
             "orc_bsdf_probe": (None, [C.c_uint32, P, C.c_int, P, P, C.c_float, C.c_float, P]),
         }
         for name, (res, args) in sig.items():
@@ -136,8 +136,9 @@ class OracleScene:
                                     lp.ctypes.data, lL.ctypes.data, lds.ctypes.data)
 
     def render(self, scene, spp: int, max_depth: int, mode: int = 0, seed: int = 1, rr: float = 1.0,
-               window=None, threads: int | None = None):
-        """Returns (rgb_sum (h, w, 3) float64, weight_sum (h, w)) for the window (x0, y0, x1, y1)."""
+               window=None, threads: int | None = None, integrator: int = 0, strategy: int = 0):
+        """Returns (rgb_sum (h, w, 3) float64, weight_sum (h, w)) for the window (x0, y0, x1, y1).
+        integrator: 0 path, 1 direct lighting (strategy 0 all lights, 1 one light), 2 Whitted."""
         cam = scene.camera
         m, mi = cam.matrices()
         x0, y0, x1, y1 = window or (0, 0, cam.width, cam.height)
@@ -146,7 +147,8 @@ class OracleScene:
         th = threads if threads is not None else len(os.sched_getaffinity(0))
         lib().orc_render(self.h, m.ctypes.data, mi.ctypes.data, cam.fov, cam.near, cam.far, cam.lens_radius,
                          cam.focal_distance, cam.shutter_open, cam.shutter_close, cam.width, cam.height, spp,
-                         max_depth, rr, seed, mode, x0, y0, x1, y1, th, rgb.ctypes.data, w.ctypes.data)
+                         max_depth, rr, seed, mode, x0, y0, x1, y1, integrator, strategy, th, rgb.ctypes.data,
+                         w.ctypes.data)
         return rgb, w
 
     def prim(self, prim_id: int, ray, kind: int = 0):
