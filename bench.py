@@ -25,6 +25,7 @@ rendered by one hipGraph-replayed wavefront loop against one call per scene).
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -141,6 +142,85 @@ def path_tracer_leg(ctx, spp: int) -> dict:
                      "kernel_ms": round(st.kernel_ms, 1), "image_mean": round(float(film.image().mean()), 5)}
         log(f"path tracer ({name}): {spp} spp in {wall:.2f} s, {st.rays / wall / 1e6:.0f} Mrays/s")
         film.close()
+    bvh.close()
+    return out
+
+
+def integrators_leg(ctx, spp: int) -> dict:
+    """The reference's other integrators and the two-lobe material on the path-tracer scene at
+    1920 x 1080: DirectLightingIntegrator with UniformSampleAll and UniformSampleOne, Whitted, and
+    the path tracer with every non-black matte material turned plastic (ks 0.25, roughness 0.2),
+    all with corrected semantics; then the film's GPU resolve (Film::write_image) and the OpenEXR
+    write of the result."""
+    import tempfile
+
+    from bonsai_amd import GpuBvh
+    from bonsai_amd import _native as N
+    from bonsai_amd.integrator import (MODE_CORRECTED, UNIFORM_SAMPLE_ALL, UNIFORM_SAMPLE_ONE,
+                                       DirectLightingIntegrator, Film, GpuPathIntegrator, WhittedIntegrator)
+    from bonsai_amd.scenes import path_tracer_scene, plastic
+
+    scene = path_tracer_scene()
+    cam = scene.camera
+    out = {"workload": "path-tracer scene 1920x1080 (2,048,000 triangles + 256 spheres, 8 light triangles)",
+           "spp": spp}
+    bvh = GpuBvh(ctx, scene)
+    runs = [("direct_all_lights", DirectLightingIntegrator(ctx, cam, spp, strategy=UNIFORM_SAMPLE_ALL,
+                                                           mode=MODE_CORRECTED, seed=5)),
+            ("direct_one_light", DirectLightingIntegrator(ctx, cam, spp, strategy=UNIFORM_SAMPLE_ONE,
+                                                          mode=MODE_CORRECTED, seed=5)),
+            ("whitted", WhittedIntegrator(ctx, cam, spp, mode=MODE_CORRECTED, seed=5))]
+    film = Film(ctx, cam.width, cam.height)
+    for name, integ in runs:
+        integ.render_into(bvh, film, spp=1)
+        film.clear()
+        t0 = time.perf_counter()
+        st = integ.render_into(bvh, film)
+        wall = time.perf_counter() - t0
+        out[name] = {"seconds": round(wall, 3), "Msamples_s": round(st.samples / wall / 1e6, 2),
+                     "rays": int(st.rays), "Mrays_s": round(st.rays / wall / 1e6, 1),
+                     "shadow_rays": int(st.shadow_rays)}
+        log(f"{name}: {spp} spp in {wall:.2f} s, {st.rays / wall / 1e6:.0f} Mrays/s")
+    # Film::write_image on the GPU and the OpenEXR file
+    d_ms = []
+    import torch
+
+    buf = torch.empty(cam.width * cam.height * 3, dtype=torch.float32, device="cuda:0")
+    for _ in range(5):
+        ms = C.c_double(0.0)
+        N.check(N.lib().bg_film_resolve_device(film.h, 1.0, 1.0, C.c_void_p(buf.data_ptr()), C.byref(ms)), ctx.h)
+        d_ms.append(ms.value)
+    with tempfile.TemporaryDirectory() as td:
+        t0 = time.perf_counter()
+        film.write_image(os.path.join(td, "result.exr"))
+        exr_s = time.perf_counter() - t0
+        size = os.path.getsize(os.path.join(td, "result.exr"))
+    npx = cam.width * cam.height
+    res_ms = float(np.median(d_ms[1:]))
+    out["film"] = {"resolve_kernel_ms": round(res_ms, 4), "resolve_GB_s": round(28 * npx / res_ms / 1e6, 1),
+                   "write_image_exr_s": round(exr_s, 4), "exr_bytes": size}
+    log(f"film resolve {res_ms * 1e3:.1f} us, write_image + EXR {exr_s * 1e3:.1f} ms ({size} bytes)")
+    film.close()
+    bvh.close()
+    # the two-lobe material: non-black mattes turned plastic
+    mats = []
+    for m in scene.materials:
+        if m.type == N.BG_MAT_MATTE and any(m.reflectance[k] > 0 for k in range(3)):
+            m = plastic(tuple(m.reflectance), (0.25, 0.25, 0.25), 0.2)
+        mats.append(m)
+    scene.materials = mats
+    bvh = GpuBvh(ctx, scene)
+    gi = GpuPathIntegrator(ctx, cam, spp=spp, max_depth=8, mode=MODE_CORRECTED, seed=5)
+    film = Film(ctx, cam.width, cam.height)
+    gi.render_into(bvh, film, spp=1)
+    film.clear()
+    t0 = time.perf_counter()
+    st = gi.render_into(bvh, film)
+    wall = time.perf_counter() - t0
+    out["path_plastic"] = {"seconds": round(wall, 3), "Msamples_s": round(st.samples / wall / 1e6, 2),
+                           "rays": int(st.rays), "Mrays_s": round(st.rays / wall / 1e6, 1)}
+    log(f"path tracer, plastic: {spp} spp in {wall:.2f} s, {st.rays / wall / 1e6:.0f} Mrays/s")
+    film.close()
     bvh.close()
     return out
 
@@ -316,6 +396,8 @@ def main() -> int:
     ap.add_argument("--guide-spp", type=int, default=32, help="guiding leg samples per render (0: skip)")
     ap.add_argument("--batch-spp", type=int, default=16, help="scene-batch leg samples per pixel (0: skip)")
     ap.add_argument("--no-mlp", action="store_true", help="skip the guiding-network leg")
+    ap.add_argument("--integrators-spp", type=int, default=16,
+                    help="direct / Whitted / plastic leg samples per pixel (0: skip)")
     ap.add_argument("--skip-intersection", action="store_true",
                     help="run only the other legs (profiling them alone); value is then null")
     ap.add_argument("--block", type=int, default=0)
@@ -437,6 +519,7 @@ def main() -> int:
     gl = guiding_leg(ctx, args.guide_spp) if args.guide_spp > 0 else None
     bl = batch_leg(ctx, args.batch_spp) if args.batch_spp > 0 else None
     ml = mlp_leg(ctx) if not args.no_mlp else None
+    il = integrators_leg(ctx, args.integrators_spp) if args.integrators_spp > 0 else None
     out = {
         "metric": METRIC,
         "value": round(value, 2) if value is not None else None,
@@ -462,6 +545,7 @@ def main() -> int:
         "guiding": gl,
         "batch": bl,
         "mlp": ml,
+        "integrators": il,
     }
     print(json.dumps(out), flush=True)
     return 0
