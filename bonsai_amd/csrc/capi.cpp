@@ -9,6 +9,7 @@
 #include <thread>
 
 #include "bg_internal.h"
+#include "tri_degenerate.h"
 
 using bg::HostBVH;
 
@@ -323,12 +324,20 @@ int bg_scene_create(bg_ctx* ctx, const bg_scene_desc* d, bg_scene** out) {
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
     bg::build_bvh(bounds, is_sphere, (int)std::min(hw, 16u), &s->bvh);
 
-    // Slot-ordered triangle vertices.
+    // Slot-ordered triangle vertices for traversal. A degenerate triangle (rejected by every
+    // intersection test, whatever the ray) is stored as its first vertex three times: its edge
+    // functions are then exactly zero and the test rejects it at det == 0, so the traversal
+    // kernels skip the per-test degeneracy check.
     const size_t nslot = s->bvh.slot_prim.size();
     std::vector<float> verts(9 * nslot, 0.0f);
     for (size_t k = 0; k < nslot; ++k) {
         const uint32_t id = s->bvh.slot_prim[k];
-        if (id < nt) std::memcpy(&verts[9 * k], d->tri_positions + 9 * (uint64_t)id, 9 * sizeof(float));
+        if (id >= nt) continue;
+        const float* v = d->tri_positions + 9 * (uint64_t)id;
+        float* o = &verts[9 * k];
+        std::memcpy(o, v, 9 * sizeof(float));
+        if (bg::tri_degenerate(v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]))
+            for (int c = 3; c < 9; ++c) o[c] = v[c % 3];
     }
     uint64_t bytes = 0;
     int rc = BG_OK;

@@ -14,6 +14,8 @@
 
 #include <cstdint>
 
+#include "tri_degenerate.h"
+
 namespace bg {
 namespace dev {
 
@@ -131,30 +133,8 @@ __device__ __forceinline__ bool slab_entry(const RayState& r, float lx, float ly
     return true;
 }
 
-// f64 cross product rounded to f32, then its f32 squared length (vec3.rs:108-122, 103-105).
-__device__ __forceinline__ float cross64_len2(float ax, float ay, float az, float bx, float by, float bz) {
-    const double x = ax, y = ay, z = az, vx = bx, vy = by, vz = bz;
-    const float cx = (float)((y * vz) - (z * vy));
-    const float cy = (float)((z * vx) - (x * vz));
-    const float cz = (float)((x * vy) - (y * vx));
-    return cx * cx + cy * cy + cz * cz;
-}
-
-// triangle.rs:241-268 / 493-517 with the default uvs (0,0), (1,0), (1,1):
-// dpdu = (-1*dp02 - (-1)*dp12) * 1, dpdv = (-0*dp02 + (-1)*dp12) * 1.
-// Returns true when the triangle is degenerate (the intersection is rejected).
-__device__ __forceinline__ bool tri_degenerate(float p0x, float p0y, float p0z, float p1x, float p1y, float p1z,
-                                               float p2x, float p2y, float p2z) {
-    const float d02x = p0x - p2x, d02y = p0y - p2y, d02z = p0z - p2z;
-    const float d12x = p1x - p2x, d12y = p1y - p2y, d12z = p1z - p2z;
-    const float m1 = -1.0f, nz = -0.0f;
-    const float ux = (m1 * d02x - m1 * d12x) * 1.0f, uy = (m1 * d02y - m1 * d12y) * 1.0f,
-                uz = (m1 * d02z - m1 * d12z) * 1.0f;
-    const float vx = (nz * d02x + m1 * d12x) * 1.0f, vy = (nz * d02y + m1 * d12y) * 1.0f,
-                vz = (nz * d02z + m1 * d12z) * 1.0f;
-    if (cross64_len2(ux, uy, uz, vx, vy, vz) != 0.0f) return false;
-    return cross64_len2(p2x - p0x, p2y - p0y, p2z - p0z, p1x - p0x, p1y - p0y, p1z - p0z) == 0.0f;
-}
+using bg::cross64_len2;
+using bg::tri_degenerate;
 
 // triangle.rs:179-190 (operand order as written there).
 __device__ __forceinline__ void edges_f64(float p0x, float p0y, float p1x, float p1y, float p2x, float p2y, float* e0,
@@ -171,7 +151,10 @@ __device__ __forceinline__ void edges_f64(float p0x, float p0y, float p1x, float
 }
 
 // triangle.rs:135-240 (+ the degenerate exit): on success returns t and the
-// barycentrics b0, b1, b2.
+// barycentrics b0, b1, b2. DEGEN = false leaves out the degenerate exit, for
+// vertex buffers in which every degenerate triangle has been collapsed to one
+// point (its edge functions are then all zero and det == 0 rejects it first).
+template <bool DEGEN = true>
 __device__ __forceinline__ bool tri_hit_full(const RayState& r, const float* __restrict__ v, float* t_out,
                                              float* b0_out, float* b1_out, float* b2_out) {
     const float p0x = v[0], p0y = v[1], p0z = v[2];
@@ -214,7 +197,7 @@ __device__ __forceinline__ bool tri_hit_full(const RayState& r, const float* __r
     const float delta_t =
         3.0f * (kGamma3 * max_e * max_zt + delta_e * max_zt + delta_z * max_e) * __builtin_fabsf(inv_det);
     if (t <= delta_t) return false;
-    if (tri_degenerate(p0x, p0y, p0z, p1x, p1y, p1z, p2x, p2y, p2z)) return false;
+    if (DEGEN && tri_degenerate(p0x, p0y, p0z, p1x, p1y, p1z, p2x, p2y, p2z)) return false;
     *t_out = t;
     *b0_out = e0 * inv_det;
     *b1_out = e1 * inv_det;
@@ -222,10 +205,11 @@ __device__ __forceinline__ bool tri_hit_full(const RayState& r, const float* __r
     return true;
 }
 
+// The traversal's test, on the slot-ordered vertices (degenerate triangles collapsed).
 __device__ __forceinline__ bool tri_hit(const RayState& r, const float* __restrict__ v, float* t_out, float* b1_out,
                                         float* b2_out) {
     float b0;
-    return tri_hit_full(r, v, t_out, &b0, b1_out, b2_out);
+    return tri_hit_full<false>(r, v, t_out, &b0, b1_out, b2_out);
 }
 
 // ---------------------------------------------------------------- spheres

@@ -62,8 +62,11 @@ __device__ __forceinline__ bool hit_surface(const ShadeSceneDev& sc, float4 r0, 
 
 // MatteMaterial (matte.rs:20-41, sigma 0), the conductor, PlasticMaterial
 // (plastic.rs:24-52); BSDF::new (bsdf.rs:33-52). NL = 1 leaves plastic out.
+// Returned by value: filling a caller's BSDF through a reference made the
+// compiler keep more values in scratch across the shading kernels' calls.
 template <int NL>
-__device__ __forceinline__ void build_bsdf(BsdfN<NL>& bs, const MaterialDev& m, const Surf& s, bool ref) {
+__device__ __forceinline__ BsdfN<NL> make_bsdf(const MaterialDev& m, const Surf& s, bool ref) {
+    BsdfN<NL> bs;
     bs.ns = s.ns;
     bs.ng = s.n;
     bs.ss = unit(s.dpdu);
@@ -106,6 +109,7 @@ __device__ __forceinline__ void build_bsdf(BsdfN<NL>& bs, const MaterialDev& m, 
         l.dielectric = false;
         bs.add(l);
     }
+    return bs;
 }
 
 // A point sampled on an area light as seen from a surface point; pdf 0 means

@@ -101,8 +101,7 @@ __global__ __launch_bounds__(128) void k_direct_shade(RenderParamsDev p, ShadeSc
     C3 L = mkc(0.0f);
     const int32_t hit_light = sc.prim_light[prim];
     if (hit_light >= 0) L = L + emission(sc.lights[hit_light], s.n, s.wo);  // it.emitted_radiance(&it.direction)
-    BsdfN<MULTI ? 2 : 1> bs;
-    build_bsdf(bs, sc.mats[sc.prim_mat[prim]], s, ref);
+    const BsdfN<MULTI ? 2 : 1> bs = make_bsdf<MULTI ? 2 : 1>(sc.mats[sc.prim_mat[prim]], s, ref);
     const uint32_t nl = sc.n_lights;
     if (KIND == 0) {  // integrator.rs:274-306, one sample per light (light.rs:52-54)
         for (uint32_t j = 0; j < nl; ++j) {

@@ -167,8 +167,7 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
     if (alive) {
         // matte.rs:20-41 / conductor / plastic.rs:24-52; bsdf.rs:33-52
         const MaterialDev& m = sc.mats[sc.prim_mat[prim]];
-        BsdfN<MULTI ? 2 : 1> bs;
-        build_bsdf(bs, m, s, ref);
+        const BsdfN<MULTI ? 2 : 1> bs = make_bsdf<MULTI ? 2 : 1>(m, s, ref);
         const uint32_t nonspec = kAll & ~kSpecular;
         // network mixture of this vertex (guided estimator with guide_alpha > 0)
         const float4* gm = nullptr;

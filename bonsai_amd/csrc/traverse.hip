@@ -287,9 +287,15 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_refill(TraverseArgs a) {
 // is hit, the far child is pushed with its entry distance and entered when
 // popped if that distance is still below t_max. This is the reference's visit
 // order and its box results, without fetching a far child only to find its box
-// missed. With COUNT, box tests are counted where the reference makes them (the
-// near child on arrival, the far child when popped), so the totals equal the
-// node-record kernels'. The triangle-only any-hit instantiations are held to 64
+// missed. Leaves are processed in wavefront-wide leaf steps once at least as
+// many lanes wait at a leaf as remain at interior records (the kernel is bound
+// by VALU issue under divergence: without that gate nearly every step of a
+// 64-lane wavefront would run both the box and the triangle code). With COUNT,
+// box tests are counted where the reference makes them (the near child on
+// arrival, the far child when popped), so the totals equal the node-record
+// kernels'. Degenerate triangles are collapsed in the slot vertices at scene
+// build, so the leaf test skips the reference's per-test degeneracy check. The
+// triangle-only any-hit instantiations are held to 64
 // VGPRs (8 waves per SIMD, no spills); the others would spill there and run
 // faster at the register count the compiler picks (5-7 waves).
 template <int BLOCK, int STACK, bool ANY, bool COUNT, bool SPH, bool QUEUE>
@@ -332,9 +338,62 @@ void k_trav_pairs(TraverseArgs a) {
             float bt = 0.0f, bb1 = 0.0f, bb2 = 0.0f;
             uint32_t sp = 0, base = 0;
             uint32_t ref = a.roots ? a.roots[ray] : 0u;  // the scene's virtual root record
+            bool live = true;
             for (;;) {
-                bool enter = false;
-                if (!(ref & kRefLeaf)) {
+                // Every step advances the lanes at an interior record; the lanes at a leaf
+                // wait until they are at least as many as the interior lanes (for the whole
+                // wavefront, from two ballots), so the triangle code runs for many lanes at once
+                // instead of in nearly every step. Each lane still takes its own nodes and
+                // leaves in the reference's order, so hits and visit counts are unchanged.
+                const bool at_leaf = live && (ref & kRefLeaf) != 0;
+                const bool at_int = live && !(ref & kRefLeaf);
+                const uint64_t m_leaf = __ballot(at_leaf), m_int = __ballot(at_int);
+                if ((m_leaf | m_int) == 0) break;
+                const bool leaf_step = __popcll(m_leaf) >= __popcll(m_int);
+                bool next = false;  // this lane needs the next pushed entry
+                if (leaf_step) {
+                    if (at_leaf) {
+                        const uint2 lf = a.leaves[ref & ~kRefLeaf];
+                        const uint32_t off = lf.x, count = lf.y >> 1;
+                        const bool mixed = (lf.y & 1u) != 0;
+                        for (uint32_t s = off; s < off + count; ++s) {
+                            float t, b1, b2;
+                            bool hit;
+                            uint32_t pid = 0;
+                            if (SPH && mixed && (pid = a.slot_prim[s]) >= a.n_tris) {
+                                if (COUNT) ++c_sph;
+                                const SphereDev& sp_ = a.spheres[pid - a.n_tris];
+                                SphereHit h;
+                                hit = sphere_hit(rs, sp_.w2o, sp_.radius, sp_.z_min, sp_.z_max, sp_.phi_max, &h);
+                                if (hit) {
+                                    t = h.t;
+                                    b1 = h.phi / sp_.phi_max;
+                                    const SphereAux ax = a.sphere_aux[pid - a.n_tris];
+                                    float c = h.pz / sp_.radius;
+                                    c = c < -1.0f ? -1.0f : (c > 1.0f ? 1.0f : c);
+                                    const float theta = acosf(c);
+                                    b2 = (theta - ax.theta_min) / (ax.theta_max - ax.theta_min);
+                                }
+                            } else {
+                                if (COUNT) ++c_tris;
+                                hit = tri_hit(rs, a.tri_verts + 9 * (uint64_t)s, &t, &b1, &b2);
+                            }
+                            if (hit) {
+                                best = (int)s;
+                                bt = t;
+                                if (ANY) {
+                                    bb1 = 0.0f; bb2 = 0.0f;
+                                    live = false;
+                                    break;
+                                }
+                                bb1 = b1; bb2 = b2;
+                                rs.t_max = t;
+                            }
+                        }
+                        next = live;
+                    }
+                }
+                if (at_int) {
                     const float4* P = a.pairs + 4 * (uint64_t)ref;
                     const float4 q0 = P[0], q1 = P[1], q2 = P[2];
                     const uint4 q3 = *reinterpret_cast<const uint4*>(P + 3);
@@ -355,71 +414,30 @@ void k_trav_pairs(TraverseArgs a) {
                         ring[(sp & kMask) * BLOCK] = make_uint2(rf, __float_as_uint(af ? tf : __builtin_nanf("")));
                         ++sp;
                     }
-                    if (hn && rn != kRefNone) {
-                        ref = rn;
-                        enter = true;
-                    }
-                } else {
-                    const uint2 lf = a.leaves[ref & ~kRefLeaf];
-                    const uint32_t off = lf.x, count = lf.y >> 1;
-                    const bool mixed = (lf.y & 1u) != 0;
-                    bool done = false;
-                    for (uint32_t s = off; s < off + count; ++s) {
-                        float t, b1, b2;
-                        bool hit;
-                        uint32_t pid = 0;
-                        if (SPH && mixed && (pid = a.slot_prim[s]) >= a.n_tris) {
-                            if (COUNT) ++c_sph;
-                            const SphereDev& sp_ = a.spheres[pid - a.n_tris];
-                            SphereHit h;
-                            hit = sphere_hit(rs, sp_.w2o, sp_.radius, sp_.z_min, sp_.z_max, sp_.phi_max, &h);
-                            if (hit) {
-                                t = h.t;
-                                b1 = h.phi / sp_.phi_max;
-                                const SphereAux ax = a.sphere_aux[pid - a.n_tris];
-                                float c = h.pz / sp_.radius;
-                                c = c < -1.0f ? -1.0f : (c > 1.0f ? 1.0f : c);
-                                const float theta = acosf(c);
-                                b2 = (theta - ax.theta_min) / (ax.theta_max - ax.theta_min);
-                            }
+                    if (hn && rn != kRefNone) ref = rn;
+                    else next = true;
+                }
+                if (next) {
+                    // the next pushed entry that the current t_max still admits
+                    bool found = false;
+                    while (sp > 0) {
+                        --sp;
+                        uint2 e;
+                        if (sp >= base) {
+                            e = ring[(sp & kMask) * BLOCK];
                         } else {
-                            if (COUNT) ++c_tris;
-                            hit = tri_hit(rs, a.tri_verts + 9 * (uint64_t)s, &t, &b1, &b2);
+                            e = spill[sp];
+                            base = sp;
                         }
-                        if (hit) {
-                            best = (int)s;
-                            bt = t;
-                            if (ANY) {
-                                bb1 = 0.0f; bb2 = 0.0f;
-                                done = true;
-                                break;
-                            }
-                            bb1 = b1; bb2 = b2;
-                            rs.t_max = t;
+                        if (COUNT) ++c_nodes;  // the far child's box test, where the reference makes it
+                        if (__uint_as_float(e.y) < rs.t_max) {
+                            ref = e.x;
+                            found = true;
+                            break;
                         }
                     }
-                    if (ANY && done) break;
+                    if (!found) live = false;
                 }
-                if (enter) continue;
-                // next pushed entry that the current t_max still admits
-                bool found = false;
-                while (sp > 0) {
-                    --sp;
-                    uint2 e;
-                    if (sp >= base) {
-                        e = ring[(sp & kMask) * BLOCK];
-                    } else {
-                        e = spill[sp];
-                        base = sp;
-                    }
-                    if (COUNT) ++c_nodes;  // the far child's box test, where the reference makes it
-                    if (__uint_as_float(e.y) < rs.t_max) {
-                        ref = e.x;
-                        found = true;
-                        break;
-                    }
-                }
-                if (!found) break;
             }
             uint4 out;
             out.x = best >= 0 ? a.slot_prim[best] : 0xFFFFFFFFu;

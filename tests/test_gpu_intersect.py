@@ -207,6 +207,34 @@ def test_visit_counts_equal_oracle(ctx, any_hit, variant):
     assert (st.nodes_visited, st.tris_tested, st.spheres_tested) == tuple(int(x) for x in tot)
 
 
+def test_degenerate_triangles_are_never_hit(ctx):
+    """The reference rejects every hit on a degenerate triangle (triangle.rs:241-268); the traversal gets
+    the same result from the collapsed slot vertices the scene build stores for it. Collinear, repeated-
+    vertex and tiny triangles (whose squared cross product underflows, so the check calls them degenerate
+    although the ray meets them) sit in front of a backdrop; every ray aimed at them must reach the
+    backdrop, with visit counts equal to the oracle's."""
+    back = np.array([[-5, -5, 5, 5, -5, 5, 5, 5, 5], [-5, -5, 5, 5, 5, 5, -5, 5, 5]], np.float32)
+    degen = np.array([[0, 0, 0, 1, 1, 0, 2, 2, 0],            # collinear
+                      [0, 1, 0, 0, 1, 0, 1, 0, 0],            # repeated vertex
+                      [1e-20, 0, 0, 0, 1e-20, 0, 0, 0, 0],    # tiny: |cross|^2 underflows in f32
+                      [0.5, 0.5, 0, 0.5, 0.5, 0, 0.5, 0.5, 0]], np.float32)  # a point
+    s = SceneData(tri_positions=np.concatenate([back, degen]))
+    aims = np.array([[1.0, 1.0], [0.2, 0.6], [3e-21, 3e-21], [0.5, 0.5], [1.5, 1.5], [0.1, 0.9]], np.float32)
+    rng = np.random.default_rng(12)
+    aims = np.concatenate([aims, rng.uniform(-0.5, 2.5, (500, 2)).astype(np.float32)])
+    rays = np.zeros((len(aims), 8), np.float32)
+    rays[:, 0:2] = aims
+    rays[:, 2] = -1.0
+    rays[:, 3] = np.inf
+    rays[:, 6] = 1.0
+    bvh, o = run_both(ctx, s, rays)
+    hits = O.hits_as_words(bvh.intersect(rays))
+    assert set(hits[:, 0].tolist()) <= {0, 1}  # the backdrop only
+    st = bvh.count(rays, False)
+    _, cnt = o.intersect(rays, any_hit=False, counters=True)
+    assert (st.nodes_visited, st.tris_tested) == tuple(int(x) for x in cnt.astype(np.uint64).sum(0)[:2])
+
+
 def test_deep_tree_really_is_deep(ctx):
     bvh = GpuBvh(ctx, deep_scene())
     assert bvh.info.max_depth > 20
