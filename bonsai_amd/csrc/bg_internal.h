@@ -283,6 +283,9 @@ struct GuideScratch {
 hipError_t guide_refresh(const GuideDevice& g, hipStream_t s);
 // n_dev (optional) holds the sample count on the device; n then only bounds the grid.
 // mixture: write 10-float4 mixture records (dev_guide.h) instead of the 40 raw outputs.
+// Blocks per CU of the inference kernel: it is held to 3 waves per SIMD, so 3
+// blocks of 4 waves fill every SIMD.
+constexpr int kInferBlocksPerCu = 3;
 hipError_t guide_infer(const GuideDevice& g, const GuideBox& box, const float4* in, uint64_t n, float* out, int grid,
                        hipStream_t s, const uint32_t* n_dev = nullptr, bool mixture = false);
 hipError_t guide_train_step(const GuideDevice& g, const GuideBox& box, const float4* in, const float4* target,

@@ -21,34 +21,51 @@ namespace dev {
 
 constexpr int kMixRecord = 10;  // float4 per sample
 
-__device__ __forceinline__ void guide_mixture(const float* __restrict__ o, float4* __restrict__ rec) {
+// Half of a sample's mixture record, computed by the two lanes that hold the
+// sample's outputs (lanes l and l + 32 of a wavefront): this lane has the
+// outputs of lobes 4h .. 4h + 3 (o[5j + c] for lobe 4h + j, c = logit,
+// direction 3-vector, concentration parameter). The lanes exchange the logit
+// maximum, the exp sum and finiteness; lane h writes lobes 4h .. 4h + 3 and
+// the mixture weights float4 8 + h. All 64 lanes must call this (it shuffles);
+// only lanes with `write` store. A sample with a non-finite output gets a
+// zero record with pi_0 = -1 (guide_ok then fails).
+__device__ __forceinline__ void guide_mixture_half(const float* __restrict__ o, int h, bool write,
+                                                   float4* __restrict__ rec) {
     bool finite = true;
     float amax = -3.0e38f;
-    for (int j = 0; j < 8; ++j) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
         amax = fmaxf(amax, o[5 * j]);
+#pragma unroll
         for (int k = 0; k < 5; ++k) finite = finite && __builtin_isfinite(o[5 * j + k]);
     }
-    if (!finite) {
-        for (int k = 0; k < kMixRecord; ++k) rec[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        rec[8].x = -1.0f;
-        return;
-    }
-    float pi[8], sum = 0.0f;
-    for (int j = 0; j < 8; ++j) {
+    amax = fmaxf(amax, __shfl_xor(amax, 32));
+    finite = (__shfl_xor((int)finite, 32) != 0) && finite;
+    float pi[4], sum = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
         pi[j] = __expf(o[5 * j] - amax);
         sum += pi[j];
     }
+    sum += __shfl_xor(sum, 32);
+    if (!write) return;
+    if (!finite) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) rec[4 * h + j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        rec[8 + h] = make_float4(h == 0 ? -1.0f : 0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
     const float inv = 1.0f / sum;
-    for (int j = 0; j < 8; ++j) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
         pi[j] *= inv;
         const V m = mkv(o[5 * j + 1], o[5 * j + 2], o[5 * j + 3]);
         const V mu = m * (1.0f / __builtin_sqrtf(len2(m) + 1e-12f));
         const float k = expf(1.5f + 5.5f * tanhf((o[5 * j + 4] - 1.5f) / 5.5f));
         const float norm = k / (2.0f * kPi * -expm1f(-2.0f * k));
-        rec[j] = make_float4(k * mu.x, k * mu.y, k * mu.z, logf(pi[j] * norm) - k);
+        rec[4 * h + j] = make_float4(k * mu.x, k * mu.y, k * mu.z, logf(pi[j] * norm) - k);
     }
-    rec[8] = make_float4(pi[0], pi[1], pi[2], pi[3]);
-    rec[9] = make_float4(pi[4], pi[5], pi[6], pi[7]);
+    rec[8 + h] = make_float4(pi[0], pi[1], pi[2], pi[3]);
 }
 
 __device__ __forceinline__ bool guide_ok(const float4* __restrict__ rec) { return rec[8].x >= 0.0f; }

@@ -26,6 +26,8 @@ namespace {
 
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f16v __attribute__((ext_vector_type(16)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf4v __attribute__((ext_vector_type(4)));
 
 constexpr int kOut = 40, kLayers = 5;
 // log kappa = C + R tanh((s - C) / R): kappa in (e^-4, e^7) with a gradient everywhere
@@ -40,10 +42,12 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
+// Four floats rounded to BF16 (nearest even) with gfx950's packed conversion
+// (v_cvt_pk_bf16_f32, two values per instruction); finite values round exactly
+// as f2bf does.
 __device__ __forceinline__ s4 pack4(float a, float b, float c, float d) {
-    s4 r;
-    r.x = (short)f2bf(a); r.y = (short)f2bf(b); r.z = (short)f2bf(c); r.w = (short)f2bf(d);
-    return r;
+    const bf4v v = __builtin_convertvector((f4v){a, b, c, d}, bf4v);
+    return __builtin_bit_cast(s4, v);
 }
 
 // Encoding features of a sample (oracle/guide_mlp.py encode()): 0-47 sin and
@@ -74,12 +78,26 @@ __device__ __forceinline__ void layer_mm(const uint16_t* __restrict__ w, const s
 // Row held by accumulator register r of lane `lane` in block mb.
 __device__ __forceinline__ int acc_row(int mb, int r, int lane) { return 32 * mb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
 
+// Output row of the last layer that accumulator row p holds in the inference
+// kernel: lanes 0-31 of a wavefront hold accumulator rows 8q + j and 32 + j
+// (q, j < 4), lanes 32-63 those rows + 4; ordering the outputs so that the first
+// set is outputs 0-19 (lobes 0-3) and the second outputs 20-39 (lobes 4-7) gives
+// each half-wavefront four whole lobes.
+__device__ __forceinline__ int out_row(int p) {
+    if (p >= 40) return p;
+    const int h = (p >> 2) & 1, idx = p < 32 ? 4 * (p >> 3) + (p & 3) : 16 + (p & 3);
+    return 20 * h + idx;
+}
+
 // Bias + optional ReLU, then repack the accumulators as the next layer's B operands.
-__device__ __forceinline__ void epilogue(f16v acc[2], const float* __restrict__ bias, bool relu, int lane, s4 xb[8]) {
+// permute_out: the accumulator rows hold the outputs out_row(row) (last layer, inference).
+__device__ __forceinline__ void epilogue(f16v acc[2], const float* __restrict__ bias, bool relu, int lane, s4 xb[8],
+                                         bool permute_out = false) {
     for (int mb = 0; mb < 2; ++mb) {
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            float v = acc[mb][r] + bias[acc_row(mb, r, lane)];
+            const int row = acc_row(mb, r, lane);
+            float v = acc[mb][r] + bias[permute_out ? out_row(row) : row];
             if (relu) v = v > 0.0f ? v : 0.0f;
             acc[mb][r] = v;
         }
@@ -90,17 +108,31 @@ __device__ __forceinline__ void epilogue(f16v acc[2], const float* __restrict__ 
 }
 
 // B operands of the encoding for the sample of this lane: k-step s holds
-// features 8s + 4h .. 8s + 4h + 3, h = lane / 32.
+// features 8s + 4h .. 8s + 4h + 3, h = lane / 32. On each axis a lane needs
+// the frequencies b, b + 1, b + 4, b + 5 (b = 2h): one sincospif at 2^b and
+// double-angle steps (sin 2x = 2 sin x cos x, cos 2x = 1 - 2 sin^2 x) for the
+// rest; five doublings cost a few ulps, far below the BF16 rounding that follows.
+__device__ __forceinline__ void dbl(float& sn, float& cs) {
+    const float s2 = 2.0f * sn * cs;
+    cs = 1.0f - 2.0f * sn * sn;
+    sn = s2;
+}
 __device__ __forceinline__ void encode_b(const float q[3], const float wo[3], const float n[3], int lane, s4 xb[8]) {
     const int h = lane >> 5;
 #pragma unroll
-    for (int s = 0; s < 6; ++s) {
-        const int f = 8 * s + 4 * h;
-        const int axis = f >> 4, k0 = (f & 15) >> 1;  // frequencies k0 and k0 + 1
-        float s0, c0, s1, c1;
-        sincospif((float)(1 << k0) * q[axis], &s0, &c0);
-        sincospif((float)(2 << k0) * q[axis], &s1, &c1);
-        xb[s] = pack4(s0, c0, s1, c1);
+    for (int axis = 0; axis < 3; ++axis) {
+        float s0, c0;
+        sincospif((float)(1 << (2 * h)) * q[axis], &s0, &c0);  // frequency 2h
+        float s1 = s0, c1 = c0;
+        dbl(s1, c1);                                            // 2h + 1
+        float s4v = s1, c4v = c1;
+        dbl(s4v, c4v);
+        dbl(s4v, c4v);
+        dbl(s4v, c4v);                                          // 2h + 4
+        float s5 = s4v, c5 = c4v;
+        dbl(s5, c5);                                            // 2h + 5
+        xb[2 * axis] = pack4(s0, c0, s1, c1);
+        xb[2 * axis + 1] = pack4(s4v, c4v, s5, c5);
     }
     float sw0, cw0, sw1, cw1, sw2, cw2;
     sincospif(wo[0], &sw0, &cw0);
@@ -123,12 +155,14 @@ __device__ __forceinline__ void normalise_pos(const Sample& s, const GuideBox& b
 }
 
 // Copies the 5 BF16 weight matrices (dense rows of 64) into padded LDS rows,
-// 8 bytes per thread and step.
-__device__ void load_weights_lds(const uint16_t* __restrict__ wg, uint16_t* __restrict__ lds) {
+// 8 bytes per thread and step; with permute_out the last layer's accumulator
+// row p takes output row out_row(p).
+__device__ void load_weights_lds(const uint16_t* __restrict__ wg, uint16_t* __restrict__ lds, bool permute_out = false) {
     const uint2* src = reinterpret_cast<const uint2*>(wg);
     for (int i = threadIdx.x; i < kLayers * 64 * 16; i += blockDim.x) {
         const int row = i / 16, q = i % 16;  // row over all layers, 4-element quad in the row
-        *reinterpret_cast<uint2*>(lds + row * kLd + 4 * q) = src[i];
+        const int from = permute_out && row >= (kLayers - 1) * 64 ? (kLayers - 1) * 64 + out_row(row % 64) : row;
+        *reinterpret_cast<uint2*>(lds + row * kLd + 4 * q) = src[from * 16 + q];
     }
     __syncthreads();
 }
@@ -137,14 +171,16 @@ __device__ void load_weights_lds(const uint16_t* __restrict__ wg, uint16_t* __re
 // in: 3 float4 per sample (p, wo, n); out: 40 floats per sample.
 // MIX: write the 10-float4 mixture record of each sample (dev_guide.h) instead
 // of its 40 raw outputs.
+// Held to 3 waves per SIMD (156 VGPRs, the accumulators in VGPRs): left alone
+// the compiler settles at 2.
 template <bool MIX>
-__global__ __launch_bounds__(256) void k_guide_infer(GuideDevice g, GuideBox box, const float4* __restrict__ in,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_guide_infer(GuideDevice g, GuideBox box, const float4* __restrict__ in,
                                                      uint64_t n, const uint32_t* __restrict__ n_dev,
                                                      float* __restrict__ out) {
     __shared__ uint16_t wl[kLayers * kLayerLds];
     if (n_dev) n = *n_dev;
     if ((uint64_t)blockIdx.x * 128 >= n) return;  // the whole block is idle (uniform)
-    load_weights_lds(g.wbf, wl);
+    load_weights_lds(g.wbf, wl, true);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, h = lane >> 5;
     const uint64_t groups = (n + 31) / 32;
     for (uint64_t grp = (uint64_t)blockIdx.x * 4 + wave; grp < groups; grp += (uint64_t)gridDim.x * 4) {
@@ -158,28 +194,21 @@ __global__ __launch_bounds__(256) void k_guide_infer(GuideDevice g, GuideBox box
         f16v acc[2];
         for (int l = 0; l < kLayers; ++l) {
             layer_mm(wl + l * kLayerLds, xb, lane, acc);
-            epilogue(acc, g.bias + l * 64, l < kLayers - 1, lane, xb);
+            epilogue(acc, g.bias + l * 64, l < kLayers - 1, lane, xb, l == kLayers - 1);
         }
-        // lane l holds rows 8q + 4h .. + 3 (block 0, q < 4) and 32 + 4h .. + 3 (block 1) of sample l % 32
+        // this lane holds outputs 20h .. 20h + 19 of sample l % 32 (lobes 4h .. 4h + 3): accumulators
+        // 0-15 of block 0, then 0-3 of block 1 (out_row)
+        float o[20];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) o[k] = acc[0][k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) o[16 + k] = acc[1][k];
         if (MIX) {
-            // the other half of the sample's outputs comes from lane l + 32
-            float o[kOut];
-#pragma unroll
-            for (int qq = 0; qq < 5; ++qq)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const float mine = qq < 4 ? acc[0][4 * qq + j] : acc[1][j];
-                    const float other = __shfl(mine, (lane & 31) + 32);
-                    o[8 * qq + j] = mine;
-                    o[8 * qq + 4 + j] = other;
-                }
-            if (lane < 32 && valid) dev::guide_mixture(o, reinterpret_cast<float4*>(out) + dev::kMixRecord * i);
+            dev::guide_mixture_half(o, h, valid, reinterpret_cast<float4*>(out) + dev::kMixRecord * i);
         } else if (valid) {
-            float4* o = reinterpret_cast<float4*>(out + i * kOut);
+            float4* dst = reinterpret_cast<float4*>(out + i * kOut + 20 * h);
 #pragma unroll
-            for (int qq = 0; qq < 4; ++qq)
-                o[2 * qq + h] = make_float4(acc[0][4 * qq], acc[0][4 * qq + 1], acc[0][4 * qq + 2], acc[0][4 * qq + 3]);
-            o[8 + h] = make_float4(acc[1][0], acc[1][1], acc[1][2], acc[1][3]);
+            for (int qq = 0; qq < 5; ++qq) dst[qq] = make_float4(o[4 * qq], o[4 * qq + 1], o[4 * qq + 2], o[4 * qq + 3]);
         }
     }
 }

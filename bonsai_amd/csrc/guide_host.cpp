@@ -61,6 +61,7 @@ GuideHostBufs carve(bg_guide* g, size_t* total_out) {
 }
 
 int grid_of(const bg_guide* g) { return 2 * (g->ctx->num_cus > 0 ? g->ctx->num_cus : 1); }
+int infer_grid_of(const bg_guide* g) { return bg::kInferBlocksPerCu * (g->ctx->num_cus > 0 ? g->ctx->num_cus : 1); }
 
 }  // namespace
 
@@ -147,7 +148,7 @@ int bg_guide_eval(bg_guide* g, const float* samples, uint64_t n, float* out) {
     GuideHostBufs b = carve(g, nullptr);
     hipStream_t s = ctx->stream;
     HIP_TRY(ctx, hipMemcpyAsync(b.in, samples, 48 * n, hipMemcpyHostToDevice, s));
-    HIP_TRY(ctx, bg::guide_infer(g->dev, g->box, b.in, n, b.out, grid_of(g), s));
+    HIP_TRY(ctx, bg::guide_infer(g->dev, g->box, b.in, n, b.out, infer_grid_of(g), s));
     HIP_TRY(ctx, hipMemcpyAsync(out, b.out, 4 * BG_GUIDE_OUTPUTS * n, hipMemcpyDeviceToHost, s));
     HIP_TRY(ctx, hipStreamSynchronize(s));
     return BG_OK;
@@ -186,7 +187,7 @@ int bg_guide_eval_device(bg_guide* g, const void* d_samples, uint64_t n, void* d
     bg_ctx* ctx = g->ctx;
     hipStream_t s = ctx->stream;
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, s));
-    HIP_TRY(ctx, bg::guide_infer(g->dev, g->box, (const float4*)d_samples, n, (float*)d_out, grid_of(g), s));
+    HIP_TRY(ctx, bg::guide_infer(g->dev, g->box, (const float4*)d_samples, n, (float*)d_out, infer_grid_of(g), s));
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, s));
     HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
     float ms = 0.0f;

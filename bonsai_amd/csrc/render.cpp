@@ -344,6 +344,7 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
         }
     }
     const int guide_grid = 2 * std::max(ctx->num_cus, 1);
+    const int infer_grid = bg::kInferBlocksPerCu * std::max(ctx->num_cus, 1);
     uint64_t train_steps = 0;
     const bg::ShadeSceneDev sc = shade_scene(s);
     const bg::PathBuffers& b = ctx->paths;
@@ -371,7 +372,7 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
                 HIP_TRY(ctx, bg::launch_guide_inputs(p, sc, b, st));  // the records copy these inputs
                 launches += 3;
                 if (p.guide_alpha > 0.0f) {  // the live paths only (count on the device)
-                    HIP_TRY(ctx, bg::guide_infer(guide->dev, guide->box, b.gin, p.n_paths, b.gout, guide_grid, st,
+                    HIP_TRY(ctx, bg::guide_infer(guide->dev, guide->box, b.gin, p.n_paths, b.gout, infer_grid, st,
                                                  b.gcount, true));
                     ++launches;
                 }
