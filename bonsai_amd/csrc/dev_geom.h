@@ -105,32 +105,32 @@ __device__ __forceinline__ bool slab(const RayState& r, float lx, float ly, floa
     return (t_min < r.t_max) && (t_max > 0.0f);
 }
 
-// The same test split at its last step: returns whether the box is entered at all
-// (every step but the t_max comparison) and its entry distance t_min (NaN when it
-// is not entered); slab() is then (returned value && t_min < ray t_max). Entry
+// The same test split at its last step: the entry distance t_min of the box
+// when the box is entered at all (every step but the t_max comparison), NaN
+// otherwise; slab() is then (t_entry < ray t_max), NaN comparing false. Entry
 // distances stored on the stack give, at pop time and for any smaller t_max,
-// exactly the result a fresh slab() would give.
-__device__ __forceinline__ bool slab_entry(const RayState& r, float lx, float ly, float lz, float hx, float hy, float hz,
-                                           float* t_entry) {
+// exactly the result a fresh slab() would give. All six planes are computed
+// before the reference's early exits are combined (they only decide that the
+// result is "not entered", so the values are the same), which keeps the record
+// loads ahead of any branch and the lanes of a wavefront on one path.
+__device__ __forceinline__ float slab_entry(const RayState& r, float lx, float ly, float lz, float hx, float hy,
+                                            float hz) {
     float t_min = ((r.neg0 ? hx : lx) - r.ox) * r.ix;
     float t_max = ((r.neg0 ? lx : hx) - r.ox) * r.ix;
     const float ty_min = ((r.neg1 ? hy : ly) - r.oy) * r.iy;
     float ty_max = ((r.neg1 ? ly : hy) - r.oy) * r.iy;
-    t_max *= kSlabPad;
-    ty_max *= kSlabPad;
-    *t_entry = __builtin_nanf("");
-    if (t_min > ty_max || ty_min > t_max) return false;
-    if (ty_min > t_min) t_min = ty_min;
-    if (ty_max < t_max) t_max = ty_max;
     const float tz_min = ((r.neg2 ? hz : lz) - r.oz) * r.iz;
     float tz_max = ((r.neg2 ? lz : hz) - r.oz) * r.iz;
+    t_max *= kSlabPad;
+    ty_max *= kSlabPad;
     tz_max *= kSlabPad;
-    if (t_min > tz_max || tz_min > t_max) return false;
-    if (tz_min > t_min) t_min = tz_min;
-    if (tz_max < t_max) t_max = tz_max;
-    if (!(t_max > 0.0f)) return false;
-    *t_entry = t_min;
-    return true;
+    const bool out_xy = t_min > ty_max || ty_min > t_max;
+    t_min = ty_min > t_min ? ty_min : t_min;
+    t_max = ty_max < t_max ? ty_max : t_max;
+    const bool out_z = t_min > tz_max || tz_min > t_max;
+    t_min = tz_min > t_min ? tz_min : t_min;
+    t_max = tz_max < t_max ? tz_max : t_max;
+    return (!out_xy && !out_z && t_max > 0.0f) ? t_min : __builtin_nanf("");
 }
 
 using bg::cross64_len2;

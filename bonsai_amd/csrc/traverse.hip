@@ -398,23 +398,21 @@ void k_trav_pairs(TraverseArgs a) {
                     const float4 q0 = P[0], q1 = P[1], q2 = P[2];
                     const uint4 q3 = *reinterpret_cast<const uint4*>(P + 3);
                     const bool swap = q3.z < 3u && (q3.z == 0 ? rs.neg0 : (q3.z == 1 ? rs.neg1 : rs.neg2));
-                    float t0, t1;
-                    const bool a0 = slab_entry(rs, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, &t0);
-                    const bool a1 = slab_entry(rs, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, &t1);
+                    // entry distances, NaN for a box the ray does not enter (NaN < t_max is false)
+                    const float t0 = slab_entry(rs, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y);
+                    const float t1 = slab_entry(rs, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w);
                     const uint32_t rn = swap ? q3.y : q3.x, rf = swap ? q3.x : q3.y;
-                    const bool hn = (swap ? a1 : a0) && (swap ? t1 : t0) < rs.t_max;
-                    const bool af = swap ? a0 : a1;
-                    const float tf = swap ? t0 : t1;
+                    const float tn = swap ? t1 : t0, tf = swap ? t0 : t1;
                     if (COUNT) ++c_nodes;  // the near child's box test
-                    if (rf != kRefNone && (COUNT || (af && tf < rs.t_max))) {
+                    if (rf != kRefNone && (COUNT || tf < rs.t_max)) {
                         if (sp - base == STACK) {
                             spill[base] = ring[(base & kMask) * BLOCK];
                             ++base;
                         }
-                        ring[(sp & kMask) * BLOCK] = make_uint2(rf, __float_as_uint(af ? tf : __builtin_nanf("")));
+                        ring[(sp & kMask) * BLOCK] = make_uint2(rf, __float_as_uint(tf));
                         ++sp;
                     }
-                    if (hn && rn != kRefNone) ref = rn;
+                    if (tn < rs.t_max && rn != kRefNone) ref = rn;
                     else next = true;
                 }
                 if (next) {
