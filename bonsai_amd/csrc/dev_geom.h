@@ -56,6 +56,7 @@ struct RayState {
     float t_max;
     float ix, iy, iz;       // 1 / d
     int neg0, neg1, neg2;   // inv < 0
+    uint32_t negbits;       // neg0 | neg1 << 1 | neg2 << 2 (bit 3, a record's "no axis", is 0)
     int kx, ky, kz;         // triangle.rs:148-156 permutation
     float okx, oky, okz;    // origin components in permuted order
     float sx, sy, sz;       // triangle.rs:163-165 shear
@@ -68,6 +69,7 @@ __device__ __forceinline__ void ray_setup(RayState& r) {
     r.neg0 = r.ix < 0.0f;
     r.neg1 = r.iy < 0.0f;
     r.neg2 = r.iz < 0.0f;
+    r.negbits = (uint32_t)r.neg0 | (uint32_t)r.neg1 << 1 | (uint32_t)r.neg2 << 2;
     const float ax = __builtin_fabsf(r.dx), ay = __builtin_fabsf(r.dy), az = __builtin_fabsf(r.dz);
     // vec3.rs:159-171 max_dimension
     int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);

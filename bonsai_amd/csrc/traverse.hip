@@ -397,7 +397,7 @@ void k_trav_pairs(TraverseArgs a) {
                     const float4* P = a.pairs + 4 * (uint64_t)ref;
                     const float4 q0 = P[0], q1 = P[1], q2 = P[2];
                     const uint4 q3 = *reinterpret_cast<const uint4*>(P + 3);
-                    const bool swap = q3.z < 3u && (q3.z == 0 ? rs.neg0 : (q3.z == 1 ? rs.neg1 : rs.neg2));
+                    const bool swap = (rs.negbits >> q3.z) & 1u;  // near child by the split axis (3: none)
                     // entry distances, NaN for a box the ray does not enter (NaN < t_max is false)
                     const float t0 = slab_entry(rs, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y);
                     const float t1 = slab_entry(rs, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w);
