@@ -329,12 +329,18 @@ def rel_mse_pair(a: np.ndarray, b: np.ndarray) -> float:
     return float(np.mean(0.5 * (a - b) ** 2 / (m * m + 0.01)))
 
 
+# Vertices per path that the guided estimators guide (and evaluate the network at): a sweep on
+# this workload (1, 2, 3, 4, 6, all) put the lowest predicted time to relMSE 0.02 at 3.
+GUIDE_BOUNCES = 3
+
+
 def guiding_leg(ctx, spp: int) -> dict:
     """Door-gap interior at 1920 x 1080, depth 10: per-sample relative variance v (from two
     independent renders of `spp` samples each) and seconds per sample of each estimator; the
     predicted time to relMSE 0.02 is v * seconds_per_spp / 0.02 (Monte Carlo error falls as
-    1 / samples). The guided estimator trains its network from scratch during the first render
-    and keeps training during the second."""
+    1 / samples), plus any training warm-up. "guided_training" trains its network from scratch
+    during both measured renders; "guided" trains it during a warm-up render and measures with
+    the network frozen. Both guide the first GUIDE_BOUNCES vertices of each path."""
     from bonsai_amd import GpuBvh
     from bonsai_amd.guide import GuideMLP
     from bonsai_amd.integrator import MODE_CORRECTED, Film, GpuPathIntegrator, GuidedPathIntegrator
@@ -346,34 +352,50 @@ def guiding_leg(ctx, spp: int) -> dict:
     bvh = GpuBvh(ctx, scene)
     W, H = scene.camera.width, scene.camera.height
     out = {"workload": "door-gap interior 1920x1080, max depth 10, 1 light behind a 5 cm gap",
-           "triangles": scene.n_tris, "spp_per_render": spp}
+           "triangles": scene.n_tris, "spp_per_render": spp, "guided_bounces": GUIDE_BOUNCES}
     guide = GuideMLP(ctx, guide_mlp.init_params(3).astype(np.float32), 65536)
-    variants = [("plain", None, 0.0), ("no_network", guide, 0.0), ("guided", guide, 0.5)]
-    for name, g, sel in variants:
-        imgs, secs, rays = [], 0.0, 0
+
+    def render(g, sel, train, seed, first=0):
+        if g is None:
+            gi = GpuPathIntegrator(ctx, scene.camera, spp=spp, max_depth=10, mode=MODE_CORRECTED, seed=seed)
+        else:
+            gi = GuidedPathIntegrator(ctx, scene.camera, spp=spp, max_depth=10, guide=g, seed=seed,
+                                      select_prob=sel, steps=8, train=train, bounces=GUIDE_BOUNCES)
+        film = Film(ctx, W, H)
+        rays = 0
+        torch_sync()
+        t0 = time.perf_counter()
+        for k in range(0, spp, 2):  # passes of 2 samples per pixel (4.1M paths)
+            st = gi.render_into(bvh, film, spp=min(2, spp - k), first_sample=first + k)
+            rays += st.rays
+        secs = time.perf_counter() - t0
+        img = film.image()
+        film.close()
+        return img, secs, rays
+
+    # plain and network-free estimators; the guided one trained throughout both renders (from
+    # scratch); and the guided one with its network trained during a warm-up render of `spp`
+    # samples and then frozen (steady state; the warm-up time counts as a fixed cost)
+    variants = [("plain", None, 0.0, False, False), ("no_network", guide, 0.0, False, False),
+                ("guided_training", guide, 0.5, True, False), ("guided", guide, 0.5, False, True)]
+    for name, g, sel, train, warm in variants:
         if g is not None:
             g.set_params(guide_mlp.init_params(3).astype(np.float32))
-        for seed in (31, 32):
-            if g is None:
-                gi = GpuPathIntegrator(ctx, scene.camera, spp=spp, max_depth=10, mode=MODE_CORRECTED, seed=seed)
-            else:
-                gi = GuidedPathIntegrator(ctx, scene.camera, spp=spp, max_depth=10, guide=g, seed=seed,
-                                          select_prob=sel, steps=8, train=sel > 0)
-            film = Film(ctx, W, H)
-            torch_sync()
-            t0 = time.perf_counter()
-            for k in range(0, spp, 2):  # passes of 2 samples per pixel (4.1M paths)
-                st = gi.render_into(bvh, film, spp=min(2, spp - k), first_sample=k)
-                rays += st.rays
-            secs += time.perf_counter() - t0
-            imgs.append(film.image())
-            film.close()
-        v = rel_mse_pair(imgs[0], imgs[1]) * spp
+        warm_s = 0.0
+        if warm:
+            _, warm_s, _ = render(g, sel, True, 30, first=1 << 20)
+        res = [render(g, sel, train, seed) for seed in (31, 32)]
+        secs = sum(r[1] for r in res)
+        rays = sum(r[2] for r in res)
+        v = rel_mse_pair(res[0][0], res[1][0]) * spp
         t_spp = secs / (2 * spp)
         out[name] = {"v_relmse_x_spp": round(v, 3), "seconds_per_spp": round(t_spp, 4),
                      "Mrays_s": round(rays / secs / 1e6, 1),
-                     "predicted_seconds_to_relmse_0.02": round(v * t_spp / 0.02, 1)}
-        log(f"guiding leg {name}: v {v:.2f}, {t_spp * 1e3:.1f} ms/spp, {rays / secs / 1e6:.0f} Mrays/s")
+                     "predicted_seconds_to_relmse_0.02": round(warm_s + v * t_spp / 0.02, 1)}
+        if warm:
+            out[name]["warmup_training_seconds"] = round(warm_s, 3)
+        log(f"guiding leg {name}: v {v:.2f}, {t_spp * 1e3:.1f} ms/spp, {rays / secs / 1e6:.0f} Mrays/s"
+            + (f", warm-up {warm_s:.2f} s" if warm else ""))
     p, gd = out["plain"]["predicted_seconds_to_relmse_0.02"], out["guided"]["predicted_seconds_to_relmse_0.02"]
     out["speedup_guided_vs_plain"] = round(p / gd, 3) if gd > 0 else None
     bvh.close()

@@ -176,7 +176,8 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
         bool lobe_ok = bs.n > 0;
         for (int k = 0; k < (MULTI ? 2 : 1); ++k)
             if (k < bs.n) lobe_ok = lobe_ok && ((bs.lobe(k).type & kDiffuse) ? (p.guide_lobes & 1u) : (p.guide_lobes & 2u));
-        const uint32_t gslot = GUIDED ? b.gslot[i] : 0xFFFFFFFFu;
+        // past the guided bounces there is no network output (and no training record) for this vertex
+        const uint32_t gslot = GUIDED && bounce < p.guide_bounces ? b.gslot[i] : 0xFFFFFFFFu;
         if (GUIDED && p.guide_alpha > 0.0f && lobe_ok && !bs.any_specular() && gslot != 0xFFFFFFFFu) {
             gm = reinterpret_cast<const float4*>(b.gout) + kMixRecord * (uint64_t)gslot;
             if (guide_ok(gm)) alpha = p.guide_alpha;

@@ -325,6 +325,7 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
         p.guided = 1;
         p.guide_alpha = gd->select_prob;
         p.guide_lobes = gd->lobes ? gd->lobes : 3u;
+        p.guide_bounces = gd->bounces ? gd->bounces : 0xFFFFFFFFu;
         if (gd->train) {
             train_paths = gd->train_paths ? gd->train_paths : (1u << 20);
             batch = std::min<uint64_t>(gd->batch ? gd->batch : 16384, guide->scratch.cap);
@@ -368,7 +369,7 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
         HIP_TRY(ctx, bg::launch_init_paths(p, b, st));
         for (uint32_t bounce = 0; bounce <= d->max_depth; ++bounce) {
             HIP_TRY(ctx, bg::launch_traverse(traverse_args(s, ctx, b.ray, b.hit, p.n_paths), false, false, ctx->shape, g, st));
-            if (guided && bounce < d->max_depth && (p.guide_alpha > 0.0f || train_paths)) {
+            if (guided && bounce < d->max_depth && bounce < p.guide_bounces && (p.guide_alpha > 0.0f || train_paths)) {
                 HIP_TRY(ctx, bg::launch_guide_inputs(p, sc, b, st));  // the records copy these inputs
                 launches += 3;
                 if (p.guide_alpha > 0.0f) {  // the live paths only (count on the device)

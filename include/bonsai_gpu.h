@@ -411,6 +411,9 @@ typedef struct bg_guiding_desc {
     float lr;              /* Adam learning rate (0: 1e-2) */
     uint32_t replay;       /* replay ring capacity in samples (0: 2^20) */
     uint32_t lobes;        /* where the network is sampled: BG_GUIDE_DIFFUSE | BG_GUIDE_GLOSSY (0: both) */
+    uint32_t bounces;      /* guide (and evaluate the network) at the first `bounces` vertices of a
+                              path only; later vertices use the same estimator with select_prob 0
+                              and are not recorded for training (0: every vertex) */
 } bg_guiding_desc;
 #define BG_GUIDE_DIFFUSE 1u
 #define BG_GUIDE_GLOSSY 2u

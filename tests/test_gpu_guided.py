@@ -24,13 +24,14 @@ def new_guide(ctx, seed=3, cap=65536):
     return GuideMLP(ctx, G.init_params(seed).astype(np.float32), cap)
 
 
-@pytest.mark.parametrize("select_prob,train", [(0.5, True), (1.0, False)])
-def test_guided_furnace_closed_form(ctx, select_prob, train):
+@pytest.mark.parametrize("select_prob,train,bounces", [(0.5, True, 0), (1.0, False, 0), (1.0, True, 2)])
+def test_guided_furnace_closed_form(ctx, select_prob, train, bounces):
+    """Unbiased for any network state, whether every vertex is guided or only the first `bounces`."""
     s = furnace(res=32)
     bvh = GpuBvh(ctx, s)
     for depth in (1, 5):
         gi = GuidedPathIntegrator(ctx, s.camera, spp=64, max_depth=depth, guide=new_guide(ctx), select_prob=select_prob,
-                                  train=train, train_paths=4096)
+                                  train=train, train_paths=4096, bounces=bounces)
         film = Film(ctx, 32, 32)
         for k in range(8):  # 8 passes of 8 spp: the network changes between passes when training
             gi.render_into(bvh, film, spp=8, first_sample=8 * k)
