@@ -111,28 +111,38 @@ __device__ __forceinline__ bool slab(const RayState& r, float lx, float ly, floa
 // when the box is entered at all (every step but the t_max comparison), NaN
 // otherwise; slab() is then (t_entry < ray t_max), NaN comparing false. Entry
 // distances stored on the stack give, at pop time and for any smaller t_max,
-// exactly the result a fresh slab() would give. All six planes are computed
-// before the reference's early exits are combined (they only decide that the
-// result is "not entered", so the values are the same), which keeps the record
-// loads ahead of any branch and the lanes of a wavefront on one path.
+// exactly the result a fresh slab() would give.
+//
+// The six plane distances are the reference's (same selection by direction
+// sign, same operations, same padding); their combination is an identity of
+// the reference's sequence, in three max/min instructions and three compares:
+// - a NaN x distance always ends in "not entered" there (a NaN t_min or t_max
+//   survives every update and fails the last comparison);
+// - with x ordered, every NaN y or z distance takes part in no comparison and
+//   no update there (each test involving it is false), as IEEE maxNum/minNum
+//   ignore a NaN operand;
+// - the early exits there compare each entry with each other axis's exit;
+//   entry <= exit also holds on one axis (the sign selection orders the two
+//   planes and the padding only raises a non-negative exit), except where that
+//   exit is negative, and then the final exit > 0 fails in both forms.
+// So entered <=> max(entries) <= min(exits) && min(exits) > 0, with the entry
+// distance max(entries) (equal to the reference's up to the sign of a zero,
+// which no comparison sees).
 __device__ __forceinline__ float slab_entry(const RayState& r, float lx, float ly, float lz, float hx, float hy,
                                             float hz) {
-    float t_min = ((r.neg0 ? hx : lx) - r.ox) * r.ix;
-    float t_max = ((r.neg0 ? lx : hx) - r.ox) * r.ix;
+    const float tx_min = ((r.neg0 ? hx : lx) - r.ox) * r.ix;
+    float tx_max = ((r.neg0 ? lx : hx) - r.ox) * r.ix;
     const float ty_min = ((r.neg1 ? hy : ly) - r.oy) * r.iy;
     float ty_max = ((r.neg1 ? ly : hy) - r.oy) * r.iy;
     const float tz_min = ((r.neg2 ? hz : lz) - r.oz) * r.iz;
     float tz_max = ((r.neg2 ? lz : hz) - r.oz) * r.iz;
-    t_max *= kSlabPad;
+    tx_max *= kSlabPad;
     ty_max *= kSlabPad;
     tz_max *= kSlabPad;
-    const bool out_xy = t_min > ty_max || ty_min > t_max;
-    t_min = ty_min > t_min ? ty_min : t_min;
-    t_max = ty_max < t_max ? ty_max : t_max;
-    const bool out_z = t_min > tz_max || tz_min > t_max;
-    t_min = tz_min > t_min ? tz_min : t_min;
-    t_max = tz_max < t_max ? tz_max : t_max;
-    return (!out_xy && !out_z && t_max > 0.0f) ? t_min : __builtin_nanf("");
+    const float t_in = __builtin_fmaxf(__builtin_fmaxf(tx_min, ty_min), tz_min);
+    const float t_out = __builtin_fminf(__builtin_fminf(tx_max, ty_max), tz_max);
+    const bool entered = !__builtin_isunordered(tx_min, tx_max) && t_in <= t_out && t_out > 0.0f;
+    return entered ? t_in : __builtin_nanf("");
 }
 
 using bg::cross64_len2;

@@ -337,19 +337,24 @@ void k_trav_pairs(TraverseArgs a) {
             int best = -1;
             float bt = 0.0f, bb1 = 0.0f, bb2 = 0.0f;
             uint32_t sp = 0, base = 0;
+            // the record or leaf the lane is at; kRefNone once its traversal has ended (a leaf
+            // ref is below it: primitive counts are held under 2^31 - 1 at scene build)
             uint32_t ref = a.roots ? a.roots[ray] : 0u;  // the scene's virtual root record
-            bool live = true;
             for (;;) {
                 // Every step advances the lanes at an interior record; the lanes at a leaf
                 // wait until they are at least as many as the interior lanes (for the whole
                 // wavefront, from two ballots), so the triangle code runs for many lanes at once
                 // instead of in nearly every step. Each lane still takes its own nodes and
                 // leaves in the reference's order, so hits and visit counts are unchanged.
-                const bool at_leaf = live && (ref & kRefLeaf) != 0;
-                const bool at_int = live && !(ref & kRefLeaf);
-                const uint64_t m_leaf = __ballot(at_leaf), m_int = __ballot(at_int);
+                // (single compares, so the ballots and the count comparison stay scalar)
+                const bool at_leaf = (int32_t)ref < (int32_t)kRefNone;
+                const bool at_int = (int32_t)ref >= 0;
+                const uint64_t m_leaf = __builtin_amdgcn_ballot_w64(at_leaf);
+                const uint64_t m_int = __builtin_amdgcn_ballot_w64(at_int);
                 if ((m_leaf | m_int) == 0) break;
-                const bool leaf_step = __popcll(m_leaf) >= __popcll(m_int);
+                const uint32_t n_leaf = __builtin_popcount((uint32_t)m_leaf) + __builtin_popcount((uint32_t)(m_leaf >> 32));
+                const uint32_t n_int = __builtin_popcount((uint32_t)m_int) + __builtin_popcount((uint32_t)(m_int >> 32));
+                const bool leaf_step = n_leaf >= n_int;
                 bool next = false;  // this lane needs the next pushed entry
                 if (leaf_step) {
                     if (at_leaf) {
@@ -383,14 +388,14 @@ void k_trav_pairs(TraverseArgs a) {
                                 bt = t;
                                 if (ANY) {
                                     bb1 = 0.0f; bb2 = 0.0f;
-                                    live = false;
+                                    ref = kRefNone;
                                     break;
                                 }
                                 bb1 = b1; bb2 = b2;
                                 rs.t_max = t;
                             }
                         }
-                        next = live;
+                        next = ref != kRefNone;
                     }
                 }
                 if (at_int) {
@@ -420,13 +425,11 @@ void k_trav_pairs(TraverseArgs a) {
                     bool found = false;
                     while (sp > 0) {
                         --sp;
-                        uint2 e;
-                        if (sp >= base) {
-                            e = ring[(sp & kMask) * BLOCK];
-                        } else {
-                            e = spill[sp];
+                        if (sp < base) {  // the ring is empty: the entry comes back from the spill row
+                            ring[(sp & kMask) * BLOCK] = spill[sp];
                             base = sp;
                         }
+                        const uint2 e = ring[(sp & kMask) * BLOCK];
                         if (COUNT) ++c_nodes;  // the far child's box test, where the reference makes it
                         if (__uint_as_float(e.y) < rs.t_max) {
                             ref = e.x;
@@ -434,7 +437,7 @@ void k_trav_pairs(TraverseArgs a) {
                             break;
                         }
                     }
-                    if (!found) live = false;
+                    if (!found) ref = kRefNone;
                 }
             }
             uint4 out;

@@ -207,6 +207,45 @@ def test_visit_counts_equal_oracle(ctx, any_hit, variant):
     assert (st.nodes_visited, st.tris_tested, st.spheres_tested) == tuple(int(x) for x in tot)
 
 
+def grid_scene_and_rays(n_tris, n_rays, seed):
+    """Triangles with vertices on a 0.5 grid (so every box plane is a grid value) and rays whose origins
+    are grid points and whose direction components come from {+0, -0, +-0.5, +-1, +-2}: plane
+    distances (plane - origin) * (1 / 0) are 0 * inf = NaN or +-inf on most box tests, the cases
+    where the slab test's comparisons decide by NaN ordering."""
+    rng = np.random.default_rng(seed)
+    c = rng.integers(-4, 5, (n_tris, 1, 3)) * 0.5
+    tris = (c + rng.integers(-1, 2, (n_tris, 3, 3)) * 0.5).astype(np.float32).reshape(n_tris, 9)
+    r = np.zeros((n_rays, 8), np.float32)
+    r[:, 0:3] = rng.integers(-6, 7, (n_rays, 3)) * 0.5
+    comps = np.array([0.0, -0.0, 0.5, -0.5, 1.0, -1.0, 2.0, -2.0], np.float32)
+    d = comps[rng.integers(0, len(comps), (n_rays, 3))]
+    d[np.all(d == 0, axis=1), 2] = 1.0
+    r[:, 4:7] = d
+    r[:, 3] = np.inf
+    return SceneData(tri_positions=tris), r
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_axis_aligned_rays_on_box_planes(ctx, variant):
+    s, rays = grid_scene_and_rays(3000, 100_000, 29)
+    saved = ctx.launch()
+    ctx.set_launch(saved[0], saved[1], saved[2], variant)
+    try:
+        bvh = GpuBvh(ctx, s)
+        o = oracle_for(bvh, s)
+        for any_hit in (False, True):
+            g = bvh.intersect_test(rays) if any_hit else bvh.intersect(rays)
+            ref = o.intersect(rays, any_hit=any_hit)
+            assert_hits_equal(g, ref, s.n_tris, any_hit)
+            st = bvh.count(rays, any_hit)
+            _, cnt = o.intersect(rays, any_hit=any_hit, counters=True)
+            assert (st.nodes_visited, st.tris_tested) == tuple(int(x) for x in cnt.astype(np.uint64).sum(0)[:2])
+        hit = O.hits_as_words(bvh.intersect(rays))[:, 0] != MISS
+        assert 0.05 < hit.mean() < 0.95
+    finally:
+        ctx.set_launch(*saved)
+
+
 def test_degenerate_triangles_are_never_hit(ctx):
     """The reference rejects every hit on a degenerate triangle (triangle.rs:241-268); the traversal gets
     the same result from the collapsed slot vertices the scene build stores for it. Collinear, repeated-
