@@ -95,7 +95,7 @@ void free_batch(bg_batch* b) {
 bg::TraverseArgs batch_args(const bg_batch* bt, const bg_ctx* ctx, const float4* rays, uint4* hits, uint64_t n,
                             const uint32_t* roots, bool queue) {
     bg::TraverseArgs a;
-    a.nodes = nullptr;  // batches traverse the pair layout only (variants 0 and 2)
+    a.nodes = nullptr;  // batches traverse the pair layout only (variants 0, 2 and 4)
     a.pairs = bt->pairs;
     a.leaves = bt->leaves;
     a.tri_verts = bt->tri_verts;
@@ -128,7 +128,7 @@ hipError_t traverse(const bg_batch* bt, bg_ctx* ctx, const bg::LaunchShape& sh, 
         if (e != hipSuccess) return e;
     }
     bg::LaunchShape shape = sh;
-    if (shape.variant != 2) shape.variant = 0;  // the node-record variants need node roots
+    if (shape.variant != 2 && shape.variant != 4) shape.variant = 0;  // the node-record variants need node roots
     return bg::launch_traverse(batch_args(bt, ctx, rays, hits, n, roots, queue), any, false, shape, grid_for(ctx, shape, n), st);
 }
 
@@ -391,7 +391,7 @@ int bg_render_batch(bg_ctx* ctx, bg_batch* bt, const bg_batch_desc* d, bg_stats*
         for (int block : {128, 256})
             for (int stack : {8, 16})
                 for (int bpc : {8, 16, 32, 64})
-                    for (int variant : {0, 2}) cands.push_back(bg::LaunchShape{block, stack, bpc, variant});
+                    for (int variant : {0, 2, 4}) cands.push_back(bg::LaunchShape{block, stack, bpc, variant});
     // spill rows for the largest grid any launch shape of this call uses
     uint64_t max_threads = (uint64_t)grid_for(ctx, bt->shape, cap) * bt->shape.block;
     for (const bg::LaunchShape& c : cands) max_threads = std::max<uint64_t>(max_threads, (uint64_t)grid_for(ctx, c, cap) * c.block);

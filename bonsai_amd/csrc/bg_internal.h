@@ -76,17 +76,21 @@ struct TraverseArgs {
     const uint2* leaves = nullptr;     // leaf table of the pair layout
     const uint32_t* roots = nullptr;   // per ray: root record (scene batches); record/node 0 when null
     unsigned long long* queue = nullptr;  // work-queue counter (variant 2), zero at launch
+    uint32_t chunk_shift = 9;             // variant 4: log2 of the rays per chunk (set at launch)
+    uint32_t refill_min = 16;             // variant 4: finished lanes that trigger a refill
 };
 
 // Default = the best of a sweep over block {64,128,256} x stack {8,16} x
-// blocks/CU {8,16,32,64,0} x variant on the 2^24-ray intersection workload.
+// blocks/CU {8,16,32,64,0} x variant (and, for variant 4, refill_min
+// {8,16,24,32,48}) on the 2^24-ray intersection workload.
 struct LaunchShape {
     int block = 256;
     int stack = 8;
-    int blocks_per_cu = 32;
+    int blocks_per_cu = 16;
     // 0: pair records, one ray per loop round; 1: node records, per-lane refill; 2: as 0 with rays
-    // taken from an atomic work queue; 3: node records, one ray per loop round
-    int variant = 0;
+    // taken from an atomic work queue; 3: node records, one ray per loop round; 4: pair records,
+    // batched refill of finished lanes from per-wavefront ray sequences
+    int variant = 4;
 };
 
 // traverse.hip

@@ -281,23 +281,171 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_refill(TraverseArgs a) {
     }
 }
 
+// Per-lane state of the pair-record traversal.
+struct PairLane {
+    RayState rs;
+    // the record or leaf the lane is at; kRefNone once its traversal has ended (a leaf ref is
+    // below it: primitive counts are held under 2^31 - 1 at scene build)
+    uint32_t ref;
+    uint32_t sp, base;  // stack top, and the first entry still in the LDS ring (older ones spilled)
+    int best;           // slot of the accepted primitive, -1 for none
+    float bt, bb1, bb2;
+};
+
+struct PairCounts {
+    uint64_t nodes = 0, tris = 0, sph = 0;
+};
+
+// Starts ray `ray` on this lane at the scene's virtual root record.
+__device__ __forceinline__ void pair_start(const TraverseArgs& a, uint64_t ray, PairLane& L) {
+    const float4 r0 = a.rays[2 * ray];
+    const float4 r1 = a.rays[2 * ray + 1];
+    L.rs.ox = r0.x; L.rs.oy = r0.y; L.rs.oz = r0.z; L.rs.t_max = r0.w;
+    L.rs.dx = r1.x; L.rs.dy = r1.y; L.rs.dz = r1.z;
+    ray_setup(L.rs);
+    L.best = -1;
+    L.bt = 0.0f; L.bb1 = 0.0f; L.bb2 = 0.0f;
+    L.sp = 0; L.base = 0;
+    L.ref = a.roots ? a.roots[ray] : 0u;
+}
+
+__device__ __forceinline__ void pair_store(const TraverseArgs& a, uint64_t ray, const PairLane& L) {
+    uint4 out;
+    out.x = L.best >= 0 ? a.slot_prim[L.best] : 0xFFFFFFFFu;
+    out.y = __float_as_uint(L.bt);
+    out.z = __float_as_uint(L.bb1);
+    out.w = __float_as_uint(L.bb2);
+    a.hits[ray] = out;
+}
+
+__device__ __forceinline__ uint32_t popc64(uint64_t m) {
+    // (two 32-bit counts keep the comparisons of counts in scalar registers)
+    return __builtin_popcount((uint32_t)m) + __builtin_popcount((uint32_t)(m >> 32));
+}
+
+// One wavefront-wide step. Every step advances the lanes at an interior record; the lanes at a
+// leaf (m_leaf, from the caller's ballot, as m_int) wait until they are at least as many as the
+// interior lanes, so the triangle code runs for many lanes at once instead of in nearly every
+// step. Each lane still takes its own nodes and leaves in the reference's order, so hits and
+// visit counts are unchanged.
+template <int BLOCK, int STACK, bool ANY, bool COUNT, bool SPH>
+__device__ __forceinline__ void pair_step(const TraverseArgs& a, PairLane& L, uint2* __restrict__ ring,
+                                          uint2* __restrict__ spill, uint64_t m_leaf, uint64_t m_int,
+                                          PairCounts& c) {
+    constexpr uint32_t kMask = STACK - 1;
+    const bool at_leaf = (int32_t)L.ref < (int32_t)kRefNone;
+    const bool at_int = (int32_t)L.ref >= 0;
+    const bool leaf_step = popc64(m_leaf) >= popc64(m_int);
+    bool next = false;  // this lane needs the next pushed entry
+    if (leaf_step) {
+        if (at_leaf) {
+            const uint2 lf = a.leaves[L.ref & ~kRefLeaf];
+            const uint32_t off = lf.x, count = lf.y >> 1;
+            const bool mixed = (lf.y & 1u) != 0;
+            for (uint32_t s = off; s < off + count; ++s) {
+                float t, b1, b2;
+                bool hit;
+                uint32_t pid = 0;
+                if (SPH && mixed && (pid = a.slot_prim[s]) >= a.n_tris) {
+                    if (COUNT) ++c.sph;
+                    const SphereDev& sp_ = a.spheres[pid - a.n_tris];
+                    SphereHit h;
+                    hit = sphere_hit(L.rs, sp_.w2o, sp_.radius, sp_.z_min, sp_.z_max, sp_.phi_max, &h);
+                    if (hit) {
+                        t = h.t;
+                        b1 = h.phi / sp_.phi_max;
+                        const SphereAux ax = a.sphere_aux[pid - a.n_tris];
+                        float cz = h.pz / sp_.radius;
+                        cz = cz < -1.0f ? -1.0f : (cz > 1.0f ? 1.0f : cz);
+                        const float theta = acosf(cz);
+                        b2 = (theta - ax.theta_min) / (ax.theta_max - ax.theta_min);
+                    }
+                } else {
+                    if (COUNT) ++c.tris;
+                    hit = tri_hit(L.rs, a.tri_verts + 9 * (uint64_t)s, &t, &b1, &b2);
+                }
+                if (hit) {
+                    L.best = (int)s;
+                    L.bt = t;
+                    if (ANY) {
+                        L.bb1 = 0.0f; L.bb2 = 0.0f;
+                        L.ref = kRefNone;
+                        break;
+                    }
+                    L.bb1 = b1; L.bb2 = b2;
+                    L.rs.t_max = t;
+                }
+            }
+            next = L.ref != kRefNone;
+        }
+    }
+    if (at_int) {
+        const float4* P = a.pairs + 4 * (uint64_t)L.ref;
+        const float4 q0 = P[0], q1 = P[1], q2 = P[2];
+        const uint4 q3 = *reinterpret_cast<const uint4*>(P + 3);
+        const bool swap = (L.rs.negbits >> q3.z) & 1u;  // near child by the split axis (3: none)
+        // entry distances, NaN for a box the ray does not enter (NaN < t_max is false)
+        const float t0 = slab_entry(L.rs, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y);
+        const float t1 = slab_entry(L.rs, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w);
+        const uint32_t rn = swap ? q3.y : q3.x, rf = swap ? q3.x : q3.y;
+        const float tn = swap ? t1 : t0, tf = swap ? t0 : t1;
+        if (COUNT) ++c.nodes;  // the near child's box test
+        if (rf != kRefNone && (COUNT || tf < L.rs.t_max)) {
+            if (L.sp - L.base == STACK) {
+                spill[L.base] = ring[(L.base & kMask) * BLOCK];
+                ++L.base;
+            }
+            ring[(L.sp & kMask) * BLOCK] = make_uint2(rf, __float_as_uint(tf));
+            ++L.sp;
+        }
+        if (tn < L.rs.t_max && rn != kRefNone) L.ref = rn;
+        else next = true;
+    }
+    if (next) {
+        // the next pushed entry that the current t_max still admits
+        bool found = false;
+        while (L.sp > 0) {
+            --L.sp;
+            if (L.sp < L.base) {  // the ring is empty: the entry comes back from the spill row
+                ring[(L.sp & kMask) * BLOCK] = spill[L.sp];
+                L.base = L.sp;
+            }
+            const uint2 e = ring[(L.sp & kMask) * BLOCK];
+            if (COUNT) ++c.nodes;  // the far child's box test, where the reference makes it
+            if (__uint_as_float(e.y) < L.rs.t_max) {
+                L.ref = e.x;
+                found = true;
+                break;
+            }
+        }
+        if (!found) L.ref = kRefNone;
+    }
+}
+
+template <bool COUNT>
+__device__ __forceinline__ void flush_counts(const TraverseArgs& a, const PairCounts& c) {
+    if (COUNT) {
+        atomicAdd(&a.counters[0], (unsigned long long)c.nodes);
+        atomicAdd(&a.counters[1], (unsigned long long)c.tris);
+        atomicAdd(&a.counters[2], (unsigned long long)c.sph);
+    }
+}
+
 // Pair-record traversal (PairRecord): at an interior record both child boxes
 // are tested with one 64-byte fetch; the near child (by the sign of the ray
 // direction on the split axis, bvh.rs:331-339) is entered at once if its box
 // is hit, the far child is pushed with its entry distance and entered when
 // popped if that distance is still below t_max. This is the reference's visit
 // order and its box results, without fetching a far child only to find its box
-// missed. Leaves are processed in wavefront-wide leaf steps once at least as
-// many lanes wait at a leaf as remain at interior records (the kernel is bound
-// by VALU issue under divergence: without that gate nearly every step of a
-// 64-lane wavefront would run both the box and the triangle code). With COUNT,
-// box tests are counted where the reference makes them (the near child on
-// arrival, the far child when popped), so the totals equal the node-record
+// missed. Leaves are processed in wavefront-wide leaf steps (pair_step). With
+// COUNT, box tests are counted where the reference makes them (the near child
+// on arrival, the far child when popped), so the totals equal the node-record
 // kernels'. Degenerate triangles are collapsed in the slot vertices at scene
 // build, so the leaf test skips the reference's per-test degeneracy check. The
-// triangle-only any-hit instantiations are held to 64
-// VGPRs (8 waves per SIMD, no spills); the others would spill there and run
-// faster at the register count the compiler picks (5-7 waves).
+// triangle-only any-hit instantiations are held to 64 VGPRs (8 waves per SIMD,
+// no spills); the others would spill there and run faster at the register
+// count the compiler picks (5-7 waves). Rays are taken in grid-stride order, or
+// (QUEUE) in chunks of 64 x 16 that each wavefront takes from an atomic counter.
 template <int BLOCK, int STACK, bool ANY, bool COUNT, bool SPH, bool QUEUE>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(ANY && !SPH ? 8 : 1, 8)))
 void k_trav_pairs(TraverseArgs a) {
@@ -307,8 +455,7 @@ void k_trav_pairs(TraverseArgs a) {
     const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
     uint2* __restrict__ spill = reinterpret_cast<uint2*>(a.spill) + gtid * a.spill_depth;
     uint2* __restrict__ ring = lds + tid;
-    constexpr uint32_t kMask = STACK - 1;
-    uint64_t c_nodes = 0, c_tris = 0, c_sph = 0;
+    PairCounts cnt;
 
     constexpr uint32_t kChunk = 16;
     const int lane = tid & 63;
@@ -327,150 +474,110 @@ void k_trav_pairs(TraverseArgs a) {
     if (QUEUE) next_queued();
     while (QUEUE ? qbase < a.n_rays : ray < a.n_rays) {
         if (ray < a.n_rays) {
-            const float4 r0 = a.rays[2 * ray];
-            const float4 r1 = a.rays[2 * ray + 1];
-            RayState rs;
-            rs.ox = r0.x; rs.oy = r0.y; rs.oz = r0.z; rs.t_max = r0.w;
-            rs.dx = r1.x; rs.dy = r1.y; rs.dz = r1.z;
-            ray_setup(rs);
-
-            int best = -1;
-            float bt = 0.0f, bb1 = 0.0f, bb2 = 0.0f;
-            uint32_t sp = 0, base = 0;
-            // the record or leaf the lane is at; kRefNone once its traversal has ended (a leaf
-            // ref is below it: primitive counts are held under 2^31 - 1 at scene build)
-            uint32_t ref = a.roots ? a.roots[ray] : 0u;  // the scene's virtual root record
+            PairLane L;
+            pair_start(a, ray, L);
             for (;;) {
-                // Every step advances the lanes at an interior record; the lanes at a leaf
-                // wait until they are at least as many as the interior lanes (for the whole
-                // wavefront, from two ballots), so the triangle code runs for many lanes at once
-                // instead of in nearly every step. Each lane still takes its own nodes and
-                // leaves in the reference's order, so hits and visit counts are unchanged.
-                // (single compares, so the ballots and the count comparison stay scalar)
-                const bool at_leaf = (int32_t)ref < (int32_t)kRefNone;
-                const bool at_int = (int32_t)ref >= 0;
-                const uint64_t m_leaf = __builtin_amdgcn_ballot_w64(at_leaf);
-                const uint64_t m_int = __builtin_amdgcn_ballot_w64(at_int);
+                // (single compares, so the ballots stay scalar)
+                const uint64_t m_leaf = __builtin_amdgcn_ballot_w64((int32_t)L.ref < (int32_t)kRefNone);
+                const uint64_t m_int = __builtin_amdgcn_ballot_w64((int32_t)L.ref >= 0);
                 if ((m_leaf | m_int) == 0) break;
-                const uint32_t n_leaf = __builtin_popcount((uint32_t)m_leaf) + __builtin_popcount((uint32_t)(m_leaf >> 32));
-                const uint32_t n_int = __builtin_popcount((uint32_t)m_int) + __builtin_popcount((uint32_t)(m_int >> 32));
-                const bool leaf_step = n_leaf >= n_int;
-                bool next = false;  // this lane needs the next pushed entry
-                if (leaf_step) {
-                    if (at_leaf) {
-                        const uint2 lf = a.leaves[ref & ~kRefLeaf];
-                        const uint32_t off = lf.x, count = lf.y >> 1;
-                        const bool mixed = (lf.y & 1u) != 0;
-                        for (uint32_t s = off; s < off + count; ++s) {
-                            float t, b1, b2;
-                            bool hit;
-                            uint32_t pid = 0;
-                            if (SPH && mixed && (pid = a.slot_prim[s]) >= a.n_tris) {
-                                if (COUNT) ++c_sph;
-                                const SphereDev& sp_ = a.spheres[pid - a.n_tris];
-                                SphereHit h;
-                                hit = sphere_hit(rs, sp_.w2o, sp_.radius, sp_.z_min, sp_.z_max, sp_.phi_max, &h);
-                                if (hit) {
-                                    t = h.t;
-                                    b1 = h.phi / sp_.phi_max;
-                                    const SphereAux ax = a.sphere_aux[pid - a.n_tris];
-                                    float c = h.pz / sp_.radius;
-                                    c = c < -1.0f ? -1.0f : (c > 1.0f ? 1.0f : c);
-                                    const float theta = acosf(c);
-                                    b2 = (theta - ax.theta_min) / (ax.theta_max - ax.theta_min);
-                                }
-                            } else {
-                                if (COUNT) ++c_tris;
-                                hit = tri_hit(rs, a.tri_verts + 9 * (uint64_t)s, &t, &b1, &b2);
-                            }
-                            if (hit) {
-                                best = (int)s;
-                                bt = t;
-                                if (ANY) {
-                                    bb1 = 0.0f; bb2 = 0.0f;
-                                    ref = kRefNone;
-                                    break;
-                                }
-                                bb1 = b1; bb2 = b2;
-                                rs.t_max = t;
-                            }
-                        }
-                        next = ref != kRefNone;
-                    }
-                }
-                if (at_int) {
-                    const float4* P = a.pairs + 4 * (uint64_t)ref;
-                    const float4 q0 = P[0], q1 = P[1], q2 = P[2];
-                    const uint4 q3 = *reinterpret_cast<const uint4*>(P + 3);
-                    const bool swap = (rs.negbits >> q3.z) & 1u;  // near child by the split axis (3: none)
-                    // entry distances, NaN for a box the ray does not enter (NaN < t_max is false)
-                    const float t0 = slab_entry(rs, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y);
-                    const float t1 = slab_entry(rs, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w);
-                    const uint32_t rn = swap ? q3.y : q3.x, rf = swap ? q3.x : q3.y;
-                    const float tn = swap ? t1 : t0, tf = swap ? t0 : t1;
-                    if (COUNT) ++c_nodes;  // the near child's box test
-                    if (rf != kRefNone && (COUNT || tf < rs.t_max)) {
-                        if (sp - base == STACK) {
-                            spill[base] = ring[(base & kMask) * BLOCK];
-                            ++base;
-                        }
-                        ring[(sp & kMask) * BLOCK] = make_uint2(rf, __float_as_uint(tf));
-                        ++sp;
-                    }
-                    if (tn < rs.t_max && rn != kRefNone) ref = rn;
-                    else next = true;
-                }
-                if (next) {
-                    // the next pushed entry that the current t_max still admits
-                    bool found = false;
-                    while (sp > 0) {
-                        --sp;
-                        if (sp < base) {  // the ring is empty: the entry comes back from the spill row
-                            ring[(sp & kMask) * BLOCK] = spill[sp];
-                            base = sp;
-                        }
-                        const uint2 e = ring[(sp & kMask) * BLOCK];
-                        if (COUNT) ++c_nodes;  // the far child's box test, where the reference makes it
-                        if (__uint_as_float(e.y) < rs.t_max) {
-                            ref = e.x;
-                            found = true;
-                            break;
-                        }
-                    }
-                    if (!found) ref = kRefNone;
-                }
+                pair_step<BLOCK, STACK, ANY, COUNT, SPH>(a, L, ring, spill, m_leaf, m_int, cnt);
             }
-            uint4 out;
-            out.x = best >= 0 ? a.slot_prim[best] : 0xFFFFFFFFu;
-            out.y = __float_as_uint(bt);
-            out.z = __float_as_uint(bb1);
-            out.w = __float_as_uint(bb2);
-            a.hits[ray] = out;
+            pair_store(a, ray, L);
         }
         if (QUEUE) next_queued();
         else ray += stride;
     }
-    if (COUNT) {
-        atomicAdd(&a.counters[0], (unsigned long long)c_nodes);
-        atomicAdd(&a.counters[1], (unsigned long long)c_tris);
-        atomicAdd(&a.counters[2], (unsigned long long)c_sph);
+    flush_counts<COUNT>(a, cnt);
+}
+
+// The same traversal with batched refill: each wavefront owns a sequence of
+// rays (chunks of 2^chunk_shift consecutive rays, chunk c belonging to
+// wavefront c mod wavefronts), and once at least refill_min of its lanes have
+// finished their rays (or all have), those lanes store their hits and take the
+// next rays of the sequence in lane order. Lanes therefore do not idle from
+// their ray's end until the slowest lane of the wavefront ends; neighbouring
+// rays stay together in a wavefront (consecutive rays of a chunk). Each lane
+// still traces each ray exactly as above, so hits and visit counts are the
+// same; only which lane traces which ray, and when, differ. The triangle-only
+// instantiations are held to 64 VGPRs (8 waves per SIMD, no spills).
+template <int BLOCK, int STACK, bool ANY, bool COUNT, bool SPH>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(!SPH ? 8 : 1, 8)))
+void k_trav_pairs_refill(TraverseArgs a) {
+    __shared__ uint2 lds[STACK * BLOCK];
+    const int tid = threadIdx.x;
+    const uint64_t gtid = (uint64_t)blockIdx.x * BLOCK + tid;
+    uint2* __restrict__ spill = reinterpret_cast<uint2*>(a.spill) + gtid * a.spill_depth;
+    uint2* __restrict__ ring = lds + tid;
+    PairCounts cnt;
+
+    const uint32_t shift = a.chunk_shift;
+    const uint64_t chunk = 1ull << shift;
+    const uint64_t n_waves = (uint64_t)gridDim.x * (BLOCK / 64);
+    const uint64_t wave = (uint64_t)blockIdx.x * (BLOCK / 64) + (uint32_t)__builtin_amdgcn_readfirstlane(tid >> 6);
+    // position s of this wavefront's sequence -> ray id (increasing in s)
+    auto seq_ray = [&](uint64_t s) { return (((s >> shift) * n_waves + wave) << shift) + (s & (chunk - 1)); };
+    uint64_t pos = 0;               // next sequence position (the same in every lane)
+    uint64_t ray = ~0ull;           // this lane's ray, ~0 for none
+    PairLane L;
+    L.ref = kRefNone;
+    for (;;) {
+        const uint64_t m_leaf = __builtin_amdgcn_ballot_w64((int32_t)L.ref < (int32_t)kRefNone);
+        const uint64_t m_int = __builtin_amdgcn_ballot_w64((int32_t)L.ref >= 0);
+        const uint64_t m_free = ~(m_leaf | m_int);
+        const uint32_t n_free = popc64(m_free);
+        if (n_free != 0 && (n_free >= a.refill_min || n_free == 64) && seq_ray(pos) < a.n_rays) {
+            if (L.ref == kRefNone) {
+                if (ray != ~0ull) pair_store(a, ray, L);
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m_free >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)m_free, 0u));
+                const uint64_t r = seq_ray(pos + rank);
+                ray = ~0ull;
+                if (r < a.n_rays) {
+                    ray = r;
+                    pair_start(a, ray, L);
+                }
+            }
+            pos += n_free;
+            continue;
+        }
+        if ((m_leaf | m_int) == 0) break;
+        pair_step<BLOCK, STACK, ANY, COUNT, SPH>(a, L, ring, spill, m_leaf, m_int, cnt);
     }
+    if (ray != ~0ull) pair_store(a, ray, L);
+    flush_counts<COUNT>(a, cnt);
+}
+
+// Chunk of the refill kernel's ray sequences: rays / wavefronts rounded down to a
+// power of two, within [64, 512], so that every wavefront has rays and a
+// wavefront's consecutive rays come from one stretch of the ray array.
+uint32_t refill_chunk_shift(uint64_t n_rays, uint64_t n_waves) {
+    const uint64_t per = n_rays / (n_waves ? n_waves : 1);
+    uint32_t sh = 6;
+    while (sh < 9 && (2ull << sh) <= per) ++sh;
+    return sh;
 }
 
 template <int BLOCK, int STACK, bool ANY, bool COUNT>
-void launch4(const TraverseArgs& a, int variant, int grid, hipStream_t s) {
+void launch4(const TraverseArgs& a_in, int variant, int grid, hipStream_t s) {
     // scenes without spheres use instantiations with the sphere code compiled out
     // variants 0 and 2: pair records (2: work queue, a.queue set and zeroed by the caller; counting
-    // runs take rays in grid-stride order); 1: node records with per-lane refill; 3: node records
+    // runs take rays in grid-stride order); 4: pair records with batched refill; 1: node records
+    // with per-lane refill; 3: node records
+    TraverseArgs a = a_in;
     const bool q = variant == 2 && a.queue != nullptr && !COUNT;
-    const bool pairs = (variant == 0 || variant == 2) && a.pairs != nullptr;
+    const bool pairs = (variant == 0 || variant == 2 || variant == 4) && a.pairs != nullptr;
+    const bool refill = variant == 4 && pairs;
+    if (refill) a.chunk_shift = refill_chunk_shift(a.n_rays, (uint64_t)grid * (BLOCK / 64));
     if (a.spheres != nullptr) {
-        if (variant == 1) hipLaunchKernelGGL((k_traverse_refill<BLOCK, STACK, ANY, COUNT, true>), dim3(grid), dim3(BLOCK), 0, s, a);
+        if (refill) hipLaunchKernelGGL((k_trav_pairs_refill<BLOCK, STACK, ANY, COUNT, true>), dim3(grid), dim3(BLOCK), 0, s, a);
+        else if (variant == 1) hipLaunchKernelGGL((k_traverse_refill<BLOCK, STACK, ANY, COUNT, true>), dim3(grid), dim3(BLOCK), 0, s, a);
         else if (pairs && q) hipLaunchKernelGGL((k_trav_pairs<BLOCK, STACK, ANY, false, true, true>), dim3(grid), dim3(BLOCK), 0, s, a);
         else if (pairs) hipLaunchKernelGGL((k_trav_pairs<BLOCK, STACK, ANY, COUNT, true, false>), dim3(grid), dim3(BLOCK), 0, s, a);
         else hipLaunchKernelGGL((k_traverse<BLOCK, STACK, ANY, COUNT, true>), dim3(grid), dim3(BLOCK), 0, s, a);
     } else {
-        if (variant == 1) hipLaunchKernelGGL((k_traverse_refill<BLOCK, STACK, ANY, COUNT, false>), dim3(grid), dim3(BLOCK), 0, s, a);
+        if (refill) hipLaunchKernelGGL((k_trav_pairs_refill<BLOCK, STACK, ANY, COUNT, false>), dim3(grid), dim3(BLOCK), 0, s, a);
+        else if (variant == 1) hipLaunchKernelGGL((k_traverse_refill<BLOCK, STACK, ANY, COUNT, false>), dim3(grid), dim3(BLOCK), 0, s, a);
         else if (pairs && q) hipLaunchKernelGGL((k_trav_pairs<BLOCK, STACK, ANY, false, false, true>), dim3(grid), dim3(BLOCK), 0, s, a);
         else if (pairs) hipLaunchKernelGGL((k_trav_pairs<BLOCK, STACK, ANY, COUNT, false, false>), dim3(grid), dim3(BLOCK), 0, s, a);
         else hipLaunchKernelGGL((k_traverse<BLOCK, STACK, ANY, COUNT, false>), dim3(grid), dim3(BLOCK), 0, s, a);
@@ -504,7 +611,7 @@ hipError_t launch1(const TraverseArgs& a, bool any, bool count, int stack, int v
 bool traverse_shape_supported(const LaunchShape& s) {
     const bool b = s.block == 64 || s.block == 128 || s.block == 256;
     const bool st = s.stack == 8 || s.stack == 16 || s.stack == 32;
-    return b && st && s.blocks_per_cu >= 0 && s.variant >= 0 && s.variant <= 3;
+    return b && st && s.blocks_per_cu >= 0 && s.variant >= 0 && s.variant <= 4;
 }
 
 hipError_t launch_traverse(const TraverseArgs& a, bool any_hit, bool count, const LaunchShape& shape, int grid,

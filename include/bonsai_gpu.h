@@ -203,9 +203,14 @@ typedef struct bg_launch_config {
     uint32_t stack;       /* LDS stack entries per lane: 8, 16, 32 */
     uint32_t blocks_per_cu; /* persistent blocks per CU; 0 = one block per ray tile */
     uint32_t variant;     /* 0: a lane finishes its ray before the wavefront takes new rays;
-                             1: each lane starts its next ray as soon as its current one ends;
+                             1: each lane starts its next ray as soon as its current one ends
+                                (node records; scene batches run it as 0);
                              2: as 0, with rays taken in chunks from an atomic work queue
-                                (scene batches; other calls run it as 0) */
+                                (scene batches; other calls run it as 0);
+                             3: as 0 on node records (scene batches run it as 0);
+                             4 (default): once 16 lanes of a wavefront have finished their
+                                rays, they take the next rays of the wavefront's sequence
+                                (runs of consecutive rays) together */
 } bg_launch_config;
 int bg_set_launch_config(bg_ctx* ctx, const bg_launch_config* cfg);
 int bg_get_launch_config(const bg_ctx* ctx, bg_launch_config* cfg);

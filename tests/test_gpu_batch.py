@@ -51,11 +51,11 @@ def test_batch_equals_per_scene(ctx, items, mode, graph):
         np.testing.assert_array_equal(w, ref[k][1])
         np.testing.assert_array_equal(rgb, ref[k][0])
     block, stack, bpc, variant = br.launch()
-    assert block in (128, 256) and stack in (8, 16) and bpc in (8, 16, 32, 64) and variant in (0, 2)
+    assert block in (128, 256) and stack in (8, 16) and bpc in (8, 16, 32, 64) and variant in (0, 2, 4)
     br.close()
 
 
-@pytest.mark.parametrize("variant", [0, 2])
+@pytest.mark.parametrize("variant", [0, 2, 4])
 def test_fixed_shapes_and_work_queue(ctx, items, variant):
     b0, s0, c0, v0 = ctx.launch()
     ctx.set_launch(128, 8, 4, variant)

@@ -161,7 +161,7 @@ def deep_rays(n, seed):
 
 @pytest.mark.parametrize("block,stack,variant", [(64, 8, 0), (128, 16, 0), (256, 32, 0), (128, 8, 0), (64, 32, 0),
                                                   (64, 8, 1), (128, 16, 1), (256, 32, 1), (64, 8, 3), (256, 16, 3),
-                                                  (128, 16, 2)])
+                                                  (128, 16, 2), (64, 8, 4), (256, 8, 4), (128, 32, 4)])
 def test_launch_shapes_do_not_change_results(ctx, block, stack, variant):
     saved = ctx.launch()
     try:
@@ -180,7 +180,35 @@ def test_launch_shapes_do_not_change_results(ctx, block, stack, variant):
         ctx.set_launch(*saved)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 3])
+@pytest.mark.parametrize("block,stack", [(64, 8), (256, 8), (128, 16)])
+def test_refill_variant_with_many_rays_per_lane(ctx, block, stack):
+    """Variant 4 refills finished lanes from each wavefront's ray sequence; with one block per CU
+    every lane traces several rays (and the ray count is not a multiple of a chunk), so refills run
+    mid-wavefront. Hits and visit counts equal the oracle's."""
+    saved = ctx.launch()
+    ctx.set_launch(block, stack, 1, 4)
+    try:
+        s = deep_scene()
+        rays = np.concatenate([deep_rays(50_001, 5), box_rays(60_000, -0.5, 1.5, 6, 0, 1)])
+        rng = np.random.default_rng(3)
+        rays = rays[rng.permutation(len(rays))]
+        bvh = GpuBvh(ctx, s)
+        o = oracle_for(bvh, s)
+        for any_hit in (False, True):
+            g = bvh.intersect_test(rays) if any_hit else bvh.intersect(rays)
+            assert_hits_equal(g, o.intersect(rays, any_hit=any_hit), s.n_tris, any_hit)
+            st = bvh.count(rays, any_hit)
+            _, cnt = o.intersect(rays, any_hit=any_hit, counters=True)
+            assert (st.nodes_visited, st.tris_tested) == tuple(int(x) for x in cnt.astype(np.uint64).sum(0)[:2])
+        s2 = SceneData(tri_positions=icosphere(4, (0.0, 0.0, 0.0), 1.0, 0.15, 4.0, 3))
+        add_spheres(s2, rng.uniform(-1, 1, (50, 3)), rng.uniform(0.01, 0.2, 50))
+        lo, hi = s2.bounds()
+        run_both(ctx, s2, incoherent_rays(150_003, lo, hi, 2))
+    finally:
+        ctx.set_launch(*saved)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 3, 4])
 @pytest.mark.parametrize("any_hit", [False, True])
 def test_visit_counts_equal_oracle(ctx, any_hit, variant):
     """The kernel's box/primitive test counts (the basis of the algorithmic-bytes figure) equal the oracle's
@@ -225,7 +253,7 @@ def grid_scene_and_rays(n_tris, n_rays, seed):
     return SceneData(tri_positions=tris), r
 
 
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 4])
 def test_axis_aligned_rays_on_box_planes(ctx, variant):
     s, rays = grid_scene_and_rays(3000, 100_000, 29)
     saved = ctx.launch()
