@@ -86,7 +86,7 @@ struct TraverseArgs {
 struct LaunchShape {
     int block = 256;
     int stack = 8;
-    int blocks_per_cu = 16;
+    int blocks_per_cu = 32;
     // 0: pair records, one ray per loop round; 1: node records, per-lane refill; 2: as 0 with rays
     // taken from an atomic work queue; 3: node records, one ray per loop round; 4: pair records,
     // batched refill of finished lanes from per-wavefront ray sequences

@@ -210,7 +210,8 @@ typedef struct bg_launch_config {
                              3: as 0 on node records (scene batches run it as 0);
                              4 (default): once 16 lanes of a wavefront have finished their
                                 rays, they take the next rays of the wavefront's sequence
-                                (runs of consecutive rays) together */
+                                (runs of consecutive rays) together; launches of fewer than
+                                four rays per lane, and scenes with spheres, run it as 0 */
 } bg_launch_config;
 int bg_set_launch_config(bg_ctx* ctx, const bg_launch_config* cfg);
 int bg_get_launch_config(const bg_ctx* ctx, bg_launch_config* cfg);

@@ -182,14 +182,16 @@ def test_launch_shapes_do_not_change_results(ctx, block, stack, variant):
 
 @pytest.mark.parametrize("block,stack", [(64, 8), (256, 8), (128, 16)])
 def test_refill_variant_with_many_rays_per_lane(ctx, block, stack):
-    """Variant 4 refills finished lanes from each wavefront's ray sequence; with one block per CU
-    every lane traces several rays (and the ray count is not a multiple of a chunk), so refills run
-    mid-wavefront. Hits and visit counts equal the oracle's."""
+    """Variant 4 refills finished lanes from each wavefront's ray sequence once every lane has at
+    least four rays (triangle scenes); with one block per CU every lane traces several rays (and
+    the ray count is not a multiple of a chunk), so refills run mid-wavefront. Hits and visit counts
+    equal the oracle's. (Sized for up to 256 CUs: 300,003 rays give each of 1,024 wavefronts at
+    least 256.)"""
     saved = ctx.launch()
     ctx.set_launch(block, stack, 1, 4)
     try:
         s = deep_scene()
-        rays = np.concatenate([deep_rays(50_001, 5), box_rays(60_000, -0.5, 1.5, 6, 0, 1)])
+        rays = np.concatenate([deep_rays(150_001, 5), box_rays(150_002, -0.5, 1.5, 6, 0, 1)])
         rng = np.random.default_rng(3)
         rays = rays[rng.permutation(len(rays))]
         bvh = GpuBvh(ctx, s)
@@ -201,9 +203,11 @@ def test_refill_variant_with_many_rays_per_lane(ctx, block, stack):
             _, cnt = o.intersect(rays, any_hit=any_hit, counters=True)
             assert (st.nodes_visited, st.tris_tested) == tuple(int(x) for x in cnt.astype(np.uint64).sum(0)[:2])
         s2 = SceneData(tri_positions=icosphere(4, (0.0, 0.0, 0.0), 1.0, 0.15, 4.0, 3))
-        add_spheres(s2, rng.uniform(-1, 1, (50, 3)), rng.uniform(0.01, 0.2, 50))
         lo, hi = s2.bounds()
-        run_both(ctx, s2, incoherent_rays(150_003, lo, hi, 2))
+        run_both(ctx, s2, incoherent_rays(300_003, lo, hi, 2))
+        # a scene with spheres runs variant 4 as 0: the same results
+        add_spheres(s2, rng.uniform(-1, 1, (50, 3)), rng.uniform(0.01, 0.2, 50))
+        run_both(ctx, s2, incoherent_rays(100_000, lo, hi, 4))
     finally:
         ctx.set_launch(*saved)
 
