@@ -82,3 +82,29 @@ def test_batch_rerender_clears_the_film(ctx, items):
         rgb, w = br.sums(k)
         np.testing.assert_array_equal(rgb, a[k][0])
         np.testing.assert_array_equal(w, a[k][1])
+
+
+def test_refill_traversal_with_per_ray_roots(ctx):
+    """Triangle-only scenes batched with one block per CU and 64-thread blocks: the first bounces
+    give every lane at least four rays, so the traversal runs variant 4's refill kernel with per-ray
+    root records. Each item's film equals the single-scene render (grid-stride traversal)."""
+    import dataclasses
+    scenes = [batch_scene(i, scale=0.02) for i in range(3)]
+    bvhs = [GpuBvh(ctx, s) for s in scenes]
+    items = [(b, dataclasses.replace(s.camera, width=160, height=160), 200 + k)
+             for k, (b, s) in enumerate(zip(bvhs, scenes))]
+    ref = per_item(ctx, items, 2, 4, MODE_CORRECTED)
+    saved = ctx.launch()
+    ctx.set_launch(64, 8, 1, 4)
+    try:
+        br = BatchRenderer(ctx, items)
+        total = sum(c.width * c.height for _, c, _ in items)
+        br.render(2, 4, mode=MODE_CORRECTED, autotune=False, paths_per_pass=2 * total)
+        assert br.launch() == (64, 8, 1, 4)
+        for k in range(len(items)):
+            rgb, w = br.sums(k)
+            np.testing.assert_array_equal(w, ref[k][1])
+            np.testing.assert_array_equal(rgb, ref[k][0])
+        br.close()
+    finally:
+        ctx.set_launch(*saved)
