@@ -128,8 +128,8 @@ __device__ __forceinline__ bool slab(const RayState& r, float lx, float ly, floa
 // So entered <=> max(entries) <= min(exits) && min(exits) > 0, with the entry
 // distance max(entries) (equal to the reference's up to the sign of a zero,
 // which no comparison sees).
-__device__ __forceinline__ float slab_entry(const RayState& r, float lx, float ly, float lz, float hx, float hy,
-                                            float hz) {
+__host__ __device__ __forceinline__ float slab_entry(const RayState& r, float lx, float ly, float lz, float hx,
+                                                     float hy, float hz) {
     const float tx_min = ((r.neg0 ? hx : lx) - r.ox) * r.ix;
     float tx_max = ((r.neg0 ? lx : hx) - r.ox) * r.ix;
     const float ty_min = ((r.neg1 ? hy : ly) - r.oy) * r.iy;
