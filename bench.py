@@ -329,18 +329,32 @@ def rel_mse_pair(a: np.ndarray, b: np.ndarray) -> float:
     return float(np.mean(0.5 * (a - b) ** 2 / (m * m + 0.01)))
 
 
-# Vertices per path that the guided estimators guide (and evaluate the network at): a sweep on
-# this workload (1, 2, 3, 4, 6, all) put the lowest predicted time to relMSE 0.02 at 3.
-GUIDE_BOUNCES = 3
+# Settings of the guided estimators on the door-gap workload, from sweeps there (predicted time to
+# relMSE 0.02, warm-up included): Russian roulette off (with the reference's rule, q = 1 - max(beta)
+# after three bounces, roulette removes the deep low-throughput paths that reach the light through
+# the gap and the survivors' 1/(1-q) boosts dominate the error: guided 91 s with it, 45 s without);
+# the network guides the first 4 vertices (3: 29 s, 4: 26 s, 5: 26 s, 6: 50 s, all: 52 s); it is
+# trained during a warm-up render of 4x the measured samples with 32 Adam steps per pass (32 spp and
+# 8 steps: 45 s; 64 and 16: 32 s; 128 and 16: 26 s; 128 and 32: 23 s; 256 and 32: 26 s).
+GUIDE_BOUNCES = 4
+GUIDE_RR = 0.0
+GUIDE_STEPS = 32
+GUIDE_WARMUP_FACTOR = 4
+# The plain path's best roulette threshold on the same workload, from {1 (the reference's default),
+# 0.25, 0.1, 0.05, 0.02, 0}: 114, 107, 98, 89, 91 and 92 s predicted.
+PLAIN_BEST_RR = 0.05
 
 
 def guiding_leg(ctx, spp: int) -> dict:
     """Door-gap interior at 1920 x 1080, depth 10: per-sample relative variance v (from two
     independent renders of `spp` samples each) and seconds per sample of each estimator; the
     predicted time to relMSE 0.02 is v * seconds_per_spp / 0.02 (Monte Carlo error falls as
-    1 / samples), plus any training warm-up. "guided_training" trains its network from scratch
-    during both measured renders; "guided" trains it during a warm-up render and measures with
-    the network frozen. Both guide the first GUIDE_BOUNCES vertices of each path."""
+    1 / samples), plus any training warm-up. "plain" is the plain path with the reference's default
+    Russian roulette, "plain_best_rr" with its best roulette threshold on this workload;
+    "no_network" is the guided estimator's structure with the network left out; "guided_training"
+    trains its network from scratch during both measured renders; "guided" trains it during a
+    warm-up render and measures with the network frozen. The guided estimators use the settings
+    above (GUIDE_*)."""
     from bonsai_amd import GpuBvh
     from bonsai_amd.guide import GuideMLP
     from bonsai_amd.integrator import MODE_CORRECTED, Film, GpuPathIntegrator, GuidedPathIntegrator
@@ -352,52 +366,55 @@ def guiding_leg(ctx, spp: int) -> dict:
     bvh = GpuBvh(ctx, scene)
     W, H = scene.camera.width, scene.camera.height
     out = {"workload": "door-gap interior 1920x1080, max depth 10, 1 light behind a 5 cm gap",
-           "triangles": scene.n_tris, "spp_per_render": spp, "guided_bounces": GUIDE_BOUNCES}
+           "triangles": scene.n_tris, "spp_per_render": spp, "guided_bounces": GUIDE_BOUNCES,
+           "guided_rr_threshold": GUIDE_RR, "guided_adam_steps_per_pass": GUIDE_STEPS,
+           "warmup_spp": GUIDE_WARMUP_FACTOR * spp, "plain_best_rr_threshold": PLAIN_BEST_RR}
     guide = GuideMLP(ctx, guide_mlp.init_params(3).astype(np.float32), 65536)
 
-    def render(g, sel, train, seed, first=0):
+    def render(g, sel, train, seed, rr, n=spp, first=0):
         if g is None:
-            gi = GpuPathIntegrator(ctx, scene.camera, spp=spp, max_depth=10, mode=MODE_CORRECTED, seed=seed)
+            gi = GpuPathIntegrator(ctx, scene.camera, spp=n, max_depth=10, mode=MODE_CORRECTED, seed=seed,
+                                   rr_threshold=rr)
         else:
-            gi = GuidedPathIntegrator(ctx, scene.camera, spp=spp, max_depth=10, guide=g, seed=seed,
-                                      select_prob=sel, steps=8, train=train, bounces=GUIDE_BOUNCES)
+            gi = GuidedPathIntegrator(ctx, scene.camera, spp=n, max_depth=10, guide=g, seed=seed, select_prob=sel,
+                                      steps=GUIDE_STEPS, train=train, bounces=GUIDE_BOUNCES, rr_threshold=rr)
         film = Film(ctx, W, H)
         rays = 0
         torch_sync()
         t0 = time.perf_counter()
-        for k in range(0, spp, 2):  # passes of 2 samples per pixel (4.1M paths)
-            st = gi.render_into(bvh, film, spp=min(2, spp - k), first_sample=first + k)
+        for k in range(0, n, 2):  # passes of 2 samples per pixel (4.1M paths)
+            st = gi.render_into(bvh, film, spp=min(2, n - k), first_sample=first + k)
             rays += st.rays
         secs = time.perf_counter() - t0
         img = film.image()
         film.close()
         return img, secs, rays
 
-    # plain and network-free estimators; the guided one trained throughout both renders (from
-    # scratch); and the guided one with its network trained during a warm-up render of `spp`
-    # samples and then frozen (steady state; the warm-up time counts as a fixed cost)
-    variants = [("plain", None, 0.0, False, False), ("no_network", guide, 0.0, False, False),
-                ("guided_training", guide, 0.5, True, False), ("guided", guide, 0.5, False, True)]
-    for name, g, sel, train, warm in variants:
+    variants = [("plain", None, 0.0, False, False, 1.0), ("plain_best_rr", None, 0.0, False, False, PLAIN_BEST_RR),
+                ("no_network", guide, 0.0, False, False, GUIDE_RR),
+                ("guided_training", guide, 0.5, True, False, GUIDE_RR), ("guided", guide, 0.5, False, True, GUIDE_RR)]
+    for name, g, sel, train, warm, rr in variants:
         if g is not None:
             g.set_params(guide_mlp.init_params(3).astype(np.float32))
         warm_s = 0.0
         if warm:
-            _, warm_s, _ = render(g, sel, True, 30, first=1 << 20)
-        res = [render(g, sel, train, seed) for seed in (31, 32)]
+            _, warm_s, _ = render(g, sel, True, 30, rr, n=GUIDE_WARMUP_FACTOR * spp, first=1 << 20)
+        res = [render(g, sel, train, seed, rr) for seed in (31, 32)]
         secs = sum(r[1] for r in res)
         rays = sum(r[2] for r in res)
         v = rel_mse_pair(res[0][0], res[1][0]) * spp
         t_spp = secs / (2 * spp)
-        out[name] = {"v_relmse_x_spp": round(v, 3), "seconds_per_spp": round(t_spp, 4),
+        out[name] = {"rr_threshold": rr, "v_relmse_x_spp": round(v, 3), "seconds_per_spp": round(t_spp, 4),
                      "Mrays_s": round(rays / secs / 1e6, 1),
                      "predicted_seconds_to_relmse_0.02": round(warm_s + v * t_spp / 0.02, 1)}
         if warm:
             out[name]["warmup_training_seconds"] = round(warm_s, 3)
         log(f"guiding leg {name}: v {v:.2f}, {t_spp * 1e3:.1f} ms/spp, {rays / secs / 1e6:.0f} Mrays/s"
             + (f", warm-up {warm_s:.2f} s" if warm else ""))
-    p, gd = out["plain"]["predicted_seconds_to_relmse_0.02"], out["guided"]["predicted_seconds_to_relmse_0.02"]
-    out["speedup_guided_vs_plain"] = round(p / gd, 3) if gd > 0 else None
+    gd = out["guided"]["predicted_seconds_to_relmse_0.02"]
+    for key, ref in (("speedup_guided_vs_plain", "plain"), ("speedup_guided_vs_plain_best_rr", "plain_best_rr")):
+        p = out[ref]["predicted_seconds_to_relmse_0.02"]
+        out[key] = round(p / gd, 3) if gd > 0 else None
     bvh.close()
     return out
 
