@@ -372,6 +372,8 @@ def guiding_leg(ctx, spp: int) -> dict:
     guide = GuideMLP(ctx, guide_mlp.init_params(3).astype(np.float32), 65536)
 
     def render(g, sel, train, seed, rr, n=spp, first=0):
+        # training renders call once per 2 samples (the network trains after every call); the
+        # others render all samples in one call, in the library's passes (about 16M paths)
         if g is None:
             gi = GpuPathIntegrator(ctx, scene.camera, spp=n, max_depth=10, mode=MODE_CORRECTED, seed=seed,
                                    rr_threshold=rr)
@@ -382,8 +384,9 @@ def guiding_leg(ctx, spp: int) -> dict:
         rays = 0
         torch_sync()
         t0 = time.perf_counter()
-        for k in range(0, n, 2):  # passes of 2 samples per pixel (4.1M paths)
-            st = gi.render_into(bvh, film, spp=min(2, n - k), first_sample=first + k)
+        step = 2 if train else n
+        for k in range(0, n, step):
+            st = gi.render_into(bvh, film, spp=min(step, n - k), first_sample=first + k)
             rays += st.rays
         secs = time.perf_counter() - t0
         img = film.image()

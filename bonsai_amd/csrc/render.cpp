@@ -296,7 +296,9 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
     if (d->spp == 0 || s->bvh.nodes.empty()) return BG_OK;
 
     const uint64_t npix = (uint64_t)(x1 - x0) * (y1 - y0);
-    uint64_t target = d->paths_per_pass ? d->paths_per_pass : (uint64_t)4 << 20;
+    // default: about 16M paths per pass (264 bytes each, 476 guided: 4.4 / 8 GB), enough for the
+    // refill traversal on the first bounces and few launches per sample
+    uint64_t target = d->paths_per_pass ? d->paths_per_pass : (uint64_t)16 << 20;
     uint32_t spp_pass = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(d->spp, target / npix));
     const uint64_t cap = npix * spp_pass;
     rc = bg::ensure_paths(ctx, cap);

@@ -256,7 +256,8 @@ typedef struct bg_render_desc {
     uint32_t mode;          /* BG_MODE_REF / BG_MODE_CORRECTED */
     uint64_t seed;
     uint32_t x0, y0, x1, y1; /* pixel window; x1 == 0 and y1 == 0 means the full film */
-    uint32_t paths_per_pass; /* 0 = choose automatically */
+    uint32_t paths_per_pass; /* 0: about 16M for the path integrators (whole samples per pixel,
+                                at least one), about 4M for direct lighting and Whitted */
 } bg_render_desc;
 
 /* Film: per-pixel RGB sums and filter-weight sums (src/base/film.rs). */
