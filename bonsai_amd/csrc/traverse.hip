@@ -569,8 +569,8 @@ void launch4(const TraverseArgs& a_in, int variant, int grid, hipStream_t s) {
     const bool pairs = (variant == 0 || variant == 2 || variant == 4) && a.pairs != nullptr;
     // Variant 4 refills only where lanes have rays to refill from: at least four rays per lane
     // and no spheres (measured: 1.13x on the 2^24-ray triangle intersection sets; on launches of
-    // about one ray per lane, and on the sphere-and-triangle workloads, the grid-stride kernel
-    // was 4-18% faster, so those run as 0).
+    // about one ray per lane, and on the sphere-and-triangle workloads even at eight rays per
+    // lane, the grid-stride kernel was 3-18% faster, so those run as 0).
     const uint64_t n_waves = (uint64_t)grid * (BLOCK / 64);
     const bool refill = variant == 4 && pairs && a.spheres == nullptr && a.n_rays >= 256 * n_waves;
     if (refill) a.chunk_shift = refill_chunk_shift(a.n_rays, n_waves);
