@@ -32,6 +32,11 @@ class bg_hit(C.Structure):
     _fields_ = [("prim_id", C.c_uint32), ("t", C.c_float), ("b1", C.c_float), ("b2", C.c_float)]
 
 
+class bg_surface(C.Structure):
+    _fields_ = [("p", C.c_float * 3), ("t", C.c_float), ("p_error", C.c_float * 3), ("n", C.c_float * 3),
+                ("ns", C.c_float * 3), ("dpdu", C.c_float * 3), ("uv", C.c_float * 2), ("pad", C.c_float * 2)]
+
+
 class bg_bvh_node(C.Structure):
     _fields_ = [("bmin", C.c_float * 3), ("bmax", C.c_float * 3), ("offset", C.c_uint32), ("meta", C.c_uint32)]
 
@@ -159,6 +164,7 @@ SIGNATURES = {
     "bg_build_bvh": (C.c_int, [C.POINTER(bg_scene_desc), P, C.POINTER(C.c_uint64), P, C.POINTER(C.c_uint64),
                                C.POINTER(C.c_uint32)]),
     "bg_intersect": (C.c_int, [P, P, P, C.c_size_t, P, C.c_uint32, C.POINTER(bg_stats)]),
+    "bg_surface_interactions": (C.c_int, [P, P, P, P, C.c_size_t, P]),
     "bg_intersect_device": (C.c_int, [P, P, P, C.c_size_t, P, C.c_uint32, C.POINTER(bg_stats)]),
     "bg_set_launch_config": (C.c_int, [P, C.POINTER(bg_launch_config)]),
     "bg_get_launch_config": (C.c_int, [P, C.POINTER(bg_launch_config)]),

@@ -304,6 +304,8 @@ hipError_t launch_direct_resolve(const RenderParamsDev& p, const ShadeSceneDev& 
                                  const DirectSlots& d, int kind, hipStream_t s);
 
 // Film::write_image over n pixel sums (film.hip)
+hipError_t launch_surface(const ShadeSceneDev& sc, const float4* rays, const uint4* hits, uint64_t n, float4* out,
+                          hipStream_t s);
 hipError_t launch_film_resolve(const float4* sums, uint64_t n, float splat_scale, float scale, float* rgb_out,
                                hipStream_t s);
 // OpenEXR image of width x height RGB f32 pixels (exr.cpp)

@@ -511,6 +511,33 @@ int bg_intersect(bg_ctx* ctx, const bg_scene* s, const bg_ray* rays, size_t n, b
     return BG_OK;
 }
 
+int bg_surface_interactions(bg_ctx* ctx, const bg_scene* s, const bg_ray* rays, const bg_hit* hits, size_t n,
+                            bg_surface* out) {
+    if (!ctx || !s) return fail(ctx, BG_ERR_INVALID, "null argument");
+    if (n > 0 && (!rays || !hits || !out)) return fail(ctx, BG_ERR_INVALID, "null ray, hit or output buffer");
+    if (n == 0) return BG_OK;
+    for (size_t i = 0; i < n; ++i)
+        if (hits[i].prim_id != BG_MISS && hits[i].prim_id >= s->n_tris + s->n_spheres)
+            return fail(ctx, BG_ERR_INVALID, "hit names a primitive the scene does not have");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const size_t rb = n * sizeof(bg_ray), hb = n * sizeof(bg_hit), ob = n * sizeof(bg_surface);
+    int rc = ensure(ctx, &ctx->io, &ctx->io_bytes, rb + hb + ob);
+    if (rc) return rc;
+    char* base = (char*)ctx->io;
+    HIP_TRY(ctx, hipMemcpyAsync(base, rays, rb, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(base + rb, hits, hb, hipMemcpyHostToDevice, ctx->stream));
+    bg::ShadeSceneDev sc;
+    sc.tri_orig = s->d_tri_orig;
+    sc.tri_flip = s->d_tri_flip;
+    sc.spheres = s->d_sph_shade;
+    sc.n_tris = (uint32_t)s->n_tris;
+    HIP_TRY(ctx, bg::launch_surface(sc, (const float4*)base, (const uint4*)(base + rb), n, (float4*)(base + rb + hb),
+                                    ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(out, base + rb + hb, ob, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return BG_OK;
+}
+
 int bg_camera_rays_device(bg_ctx* ctx, const bg_camera_desc* cam, float fx, float fy, void* d_rays) {
     if (!ctx || !cam || !d_rays) return fail(ctx, BG_ERR_INVALID, "null argument");
     if (cam->width == 0 || cam->height == 0) return fail(ctx, BG_ERR_INVALID, "empty film");

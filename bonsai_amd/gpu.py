@@ -16,6 +16,8 @@ from . import _native as N
 from .scenes import Camera, SceneData
 
 HIT_DTYPE = np.dtype([("prim_id", "<u4"), ("t", "<f4"), ("b1", "<f4"), ("b2", "<f4")])
+SURFACE_DTYPE = np.dtype([("p", "<f4", 3), ("t", "<f4"), ("p_error", "<f4", 3), ("n", "<f4", 3), ("ns", "<f4", 3),
+                          ("dpdu", "<f4", 3), ("uv", "<f4", 2), ("pad", "<f4", 2)])
 
 
 class Context:
@@ -119,6 +121,18 @@ class GpuBvh:
     def intersect(self, rays: np.ndarray) -> np.ndarray:
         """Closest hit per ray (prim_id = BG_MISS on a miss)."""
         return self._query(rays, N.BG_CLOSEST)[0]
+
+    def surfaces(self, rays: np.ndarray, hits: np.ndarray) -> np.ndarray:
+        """The surface interaction of each hit (as GeometricPrimitive::intersect fills it): point, its
+        error bounds, t, geometric and shading normals, shading dpdu and uv; NaN for misses."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        hits = np.ascontiguousarray(hits).view(HIT_DTYPE).reshape(-1)
+        if hits.shape[0] != rays.shape[0]:
+            raise ValueError("one hit per ray")
+        out = np.empty(rays.shape[0], SURFACE_DTYPE)
+        N.check(N.lib().bg_surface_interactions(self.ctx.h, self.h, rays.ctypes.data, hits.ctypes.data, rays.shape[0],
+                                                out.ctypes.data), self.ctx.h)
+        return out
 
     def intersect_test(self, rays: np.ndarray) -> np.ndarray:
         """Any hit per ray: prim_id of the first primitive that reported a hit."""

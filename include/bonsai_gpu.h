@@ -197,6 +197,25 @@ int bg_intersect(bg_ctx* ctx, const bg_scene* scene, const bg_ray* rays, size_t 
 int bg_intersect_device(bg_ctx* ctx, const bg_scene* scene, const void* d_rays, size_t n, void* d_hits,
                         uint32_t flags, bg_stats* stats);
 
+/* The surface interaction of each hit (the geometry GeometricPrimitive::intersect
+ * fills into an Interaction, src/primitives/geometric.rs:27-38 over
+ * shapes/triangle.rs:135-388 and shapes/sphere.rs): for ray i and hits[i]
+ * (from bg_intersect on the same rays) the point, its error bounds, the hit t,
+ * the geometric and shading normals, the shading dpdu and the uv. All fields
+ * are NaN for a miss. BG_ERR_INVALID if a hit names no primitive of the scene. */
+typedef struct bg_surface {
+    float p[3];
+    float t;
+    float p_error[3];
+    float n[3];
+    float ns[3];
+    float dpdu[3];
+    float uv[2];
+    float pad[2];
+} bg_surface;
+int bg_surface_interactions(bg_ctx* ctx, const bg_scene* scene, const bg_ray* rays, const bg_hit* hits, size_t n,
+                            bg_surface* out);
+
 /* Traversal launch shape (chosen by the autotuner or the caller). */
 typedef struct bg_launch_config {
     uint32_t block;       /* threads per block: 64, 128, 256 */

@@ -213,6 +213,27 @@ int orc_prim_intersect(void* h, uint32_t id, const float* ray, uint32_t kind, fl
     return ok ? 1 : 0;
 }
 
+// The interaction GeometricPrimitive::intersect (geometric.rs:27-38) fills for
+// primitive id on a ray (triangle.rs:135-388 / sphere.rs through Isect).
+// Returns 1 on hit and fills out[0..17] = t, p, p_error, n, ns, shading dpdu, u, v.
+int orc_surface(void* h, uint32_t id, const float* ray, float* out) {
+    OrcScene* s = static_cast<OrcScene*>(h);
+    Ray r = load_ray(ray);
+    Float t = 0, b[3] = {0, 0, 0};
+    Isect it;
+    bool ok;
+    if (id < s->scene.tris.size()) {
+        ok = tri_intersect(s->scene.tris[id], r, s->scene.tri_flip[id] != 0, &t, b, &it);
+    } else {
+        ok = sphere_intersect(s->scene.spheres[id - s->scene.tris.size()], r, &t, b, &it);
+    }
+    if (!ok) return 0;
+    const Float vals[18] = {t, it.p.x, it.p.y, it.p.z, it.perr.x, it.perr.y, it.perr.z, it.n.x, it.n.y, it.n.z,
+                            it.ns.x, it.ns.y, it.ns.z, it.dpdu_s.x, it.dpdu_s.y, it.dpdu_s.z, it.u, it.v};
+    for (int k = 0; k < 18; ++k) out[k] = vals[k];
+    return 1;
+}
+
 // Camera rays for a pinhole/thin-lens camera (perspective.rs:49-175); see camera.h.
 void orc_camera_rays(const float* c2w, const float* c2w_inv, float fov, float near_clip, float far_clip,
                      uint32_t width, uint32_t height, float fx, float fy, float* rays_out) {

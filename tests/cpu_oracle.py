@@ -32,6 +32,7 @@ def lib() -> C.CDLL:
             "orc_intersect": (None, [P, P, C.c_uint64, C.c_uint32, P, P, C.c_int]),
             "orc_intersect_brute": (None, [P, P, C.c_uint64, P, P]),
             "orc_prim_intersect": (C.c_int, [P, C.c_uint32, P, C.c_uint32, P]),
+            "orc_surface": (C.c_int, [P, C.c_uint32, P, P]),
             "orc_camera_rays": (None, [P, P, C.c_float, C.c_float, C.c_float, C.c_uint32, C.c_uint32,
                                        C.c_float, C.c_float, P]),
             "orc_gamma": (C.c_float, [C.c_float]),
@@ -150,6 +151,13 @@ class OracleScene:
                          max_depth, rr, seed, mode, x0, y0, x1, y1, integrator, strategy, th, rgb.ctypes.data,
                          w.ctypes.data)
         return rgb, w
+
+    def surface(self, prim_id: int, ray):
+        """(ok, 18 floats: t, p, p_error, n, ns, shading dpdu, u, v) of primitive prim_id on a ray."""
+        r = np.ascontiguousarray(ray, np.float32).reshape(8)
+        out = np.full(18, np.nan, np.float32)
+        ok = lib().orc_surface(self.h, prim_id, r.ctypes.data, out.ctypes.data)
+        return bool(ok), out
 
     def prim(self, prim_id: int, ray, kind: int = 0):
         r = np.ascontiguousarray(ray, np.float32).reshape(8)
