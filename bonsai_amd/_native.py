@@ -81,7 +81,8 @@ class bg_stats(C.Structure):
 
 
 class bg_launch_config(C.Structure):
-    _fields_ = [("block", C.c_uint32), ("stack", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("variant", C.c_uint32)]
+    _fields_ = [("block", C.c_uint32), ("stack", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("variant", C.c_uint32),
+                ("compact", C.c_uint32)]
 
 
 class bg_camera_desc(C.Structure):

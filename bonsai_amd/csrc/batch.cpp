@@ -318,6 +318,7 @@ int bg_batch_launch_config(const bg_batch* bt, bg_launch_config* cfg) {
     cfg->stack = (uint32_t)bt->shape.stack;
     cfg->blocks_per_cu = (uint32_t)bt->shape.blocks_per_cu;
     cfg->variant = (uint32_t)bt->shape.variant;
+    cfg->compact = 0;  // a batch's graph-replayed passes trace every path slot
     return BG_OK;
 }
 

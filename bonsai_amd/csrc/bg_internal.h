@@ -78,6 +78,10 @@ struct TraverseArgs {
     unsigned long long* queue = nullptr;  // work-queue counter (variant 2), zero at launch
     uint32_t chunk_shift = 9;             // variant 4: log2 of the rays per chunk (set at launch)
     uint32_t refill_min = 16;             // variant 4: finished lanes that trigger a refill
+    // Compacted launches: logical ray j is ray idx[j] (read from rays, written to hits), for
+    // j < *n_idx (a count on the device); n_rays is then only the upper bound for the grid.
+    const uint32_t* idx = nullptr;
+    const uint32_t* n_idx = nullptr;
 };
 
 // Default = the best of a sweep over block {64,128,256} x stack {8,16} x
@@ -91,6 +95,9 @@ struct LaunchShape {
     // taken from an atomic work queue; 3: node records, one ray per loop round; 4: pair records,
     // batched refill of finished lanes from per-wavefront ray sequences
     int variant = 4;
+    // render passes: trace only the rays that can hit (t_max > 0), compacted into an index list
+    // (the others' hits are written as misses), instead of launching over every path slot
+    int compact = 1;
 };
 
 // traverse.hip
@@ -167,6 +174,9 @@ struct PathBuffers {
     void* guide_mem = nullptr;
     float4* gin = nullptr;     // 3 per live path (compacted): network input (p, wo, n) at the current hit
     float* gout = nullptr;     // 40 per live path (compacted): network output
+    uint32_t* ridx = nullptr;   // compacted launches: indices of the rays that can hit
+    uint32_t* rblk = nullptr;   // per 256 rays: count, then its exclusive prefix
+    uint32_t* rcount = nullptr; // their number (device)
     uint32_t* gslot = nullptr; // per path: its slot in gin/gout this bounce, ~0 when not live
     uint32_t* gcount = nullptr; // live-path count of this bounce (device)
     uint32_t* gblk = nullptr;   // per 256 paths: live count, then its exclusive prefix
@@ -243,6 +253,13 @@ hipError_t launch_nee_resolve(const RenderParamsDev& p, const ShadeSceneDev& sc,
                               uint32_t bounce, hipStream_t s);
 hipError_t launch_count_rays(const RenderParamsDev& p, const PathBuffers& b, unsigned long long* counters,
                              hipStream_t s);
+// Index list of the rays among n with t_max > 0 (ascending), their count in *count,
+// and a miss written to hits[i] of every other ray.
+// Whether launch_traverse with this shape and grid runs the refill kernel (variant 4 where it
+// refills: four or more rays per lane, no spheres, no index list).
+bool traverse_refills(const TraverseArgs& a, const LaunchShape& shape, int grid);
+hipError_t launch_ray_compact(const float4* rays, uint64_t n, uint32_t* blk, uint32_t* idx, uint32_t* count,
+                              uint4* hits, hipStream_t s);
 hipError_t launch_guide_inputs(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b, hipStream_t s);
 hipError_t launch_guide_targets(const RenderParamsDev& p, const PathBuffers& b, hipStream_t s);
 // Appends the pass's nonzero-weight records to the ring (order-preserving,

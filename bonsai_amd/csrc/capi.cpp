@@ -246,6 +246,7 @@ int bg_set_launch_config(bg_ctx* ctx, const bg_launch_config* cfg) {
     s.stack = (int)cfg->stack;
     s.blocks_per_cu = (int)cfg->blocks_per_cu;
     s.variant = (int)cfg->variant;
+    s.compact = cfg->compact != 0;
     if (!bg::traverse_shape_supported(s)) return fail(ctx, BG_ERR_INVALID, "unsupported launch config");
     ctx->shape = s;
     return BG_OK;
@@ -257,6 +258,7 @@ int bg_get_launch_config(const bg_ctx* ctx, bg_launch_config* cfg) {
     cfg->stack = (uint32_t)ctx->shape.stack;
     cfg->blocks_per_cu = (uint32_t)ctx->shape.blocks_per_cu;
     cfg->variant = (uint32_t)ctx->shape.variant;
+    cfg->compact = ctx->shape.compact ? 1u : 0u;
     return BG_OK;
 }
 

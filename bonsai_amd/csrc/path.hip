@@ -450,6 +450,33 @@ __global__ __launch_bounds__(256) void k_count_rays(uint64_t n, const float4* __
 // network: per-block counts, one-block exclusive scan, then k_guide_inputs
 // writes each live path at its block's offset plus its rank in the block.
 
+// Rays that can report a hit: t_max > 0 (every primitive test needs 0 < t < t_max,
+// so a ray with t_max <= 0 or NaN, such as a killed path's, can only miss).
+__device__ __forceinline__ bool ray_can_hit(const float4* __restrict__ rays, uint64_t i) { return rays[2 * i].w > 0.0f; }
+
+__global__ __launch_bounds__(256) void k_ray_count(const float4* __restrict__ rays, uint64_t n,
+                                                   uint32_t* __restrict__ blk) {
+    __shared__ uint32_t wt[4];
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    uint32_t total;
+    block_exclusive(i < n && ray_can_hit(rays, i) ? 1u : 0u, wt, &total);
+    if (threadIdx.x == 0) blk[blockIdx.x] = total;
+}
+
+// After k_live_scan over blk: each ray that can hit at its block's offset plus its rank in the
+// block (ascending ray order); every other ray gets the miss the traversal would report.
+__global__ __launch_bounds__(256) void k_ray_compact(const float4* __restrict__ rays, uint64_t n,
+                                                     const uint32_t* __restrict__ blk, uint32_t* __restrict__ idx,
+                                                     uint4* __restrict__ hits) {
+    __shared__ uint32_t wt[4];
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool live = i < n && ray_can_hit(rays, i);
+    uint32_t total;
+    const uint32_t rank = block_exclusive(live ? 1u : 0u, wt, &total);
+    if (live) idx[blk[blockIdx.x] + rank] = (uint32_t)i;
+    else if (i < n) hits[i] = make_uint4(0xFFFFFFFFu, 0u, 0u, 0u);
+}
+
 __global__ __launch_bounds__(256) void k_live_count(uint64_t n, PathBuffers b) {
     __shared__ uint32_t wt[4];
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
@@ -595,6 +622,15 @@ hipError_t launch_count_rays(const RenderParamsDev& p, const PathBuffers& b, uns
     if (p.n_paths == 0) return hipSuccess;
     const uint64_t blocks = std::min<uint64_t>((p.n_paths + 255) / 256, 1024);
     hipLaunchKernelGGL(k_count_rays, dim3((uint32_t)blocks), dim3(256), 0, s, p.n_paths, b.beta, counters);
+    return hipGetLastError();
+}
+hipError_t launch_ray_compact(const float4* rays, uint64_t n, uint32_t* blk, uint32_t* idx, uint32_t* count,
+                              uint4* hits, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const uint32_t nblk = (uint32_t)((n + 255) / 256);
+    hipLaunchKernelGGL(k_ray_count, dim3(nblk), dim3(256), 0, s, rays, n, blk);
+    hipLaunchKernelGGL(k_live_scan, dim3(1), dim3(1024), 0, s, nblk, blk, count);
+    hipLaunchKernelGGL(k_ray_compact, dim3(nblk), dim3(256), 0, s, rays, n, blk, idx, hits);
     return hipGetLastError();
 }
 hipError_t launch_guide_inputs(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b, hipStream_t s) {

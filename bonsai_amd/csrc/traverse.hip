@@ -445,7 +445,8 @@ __device__ __forceinline__ void flush_counts(const TraverseArgs& a, const PairCo
 // triangle-only any-hit instantiations are held to 64 VGPRs (8 waves per SIMD,
 // no spills); the others would spill there and run faster at the register
 // count the compiler picks (5-7 waves). Rays are taken in grid-stride order, or
-// (QUEUE) in chunks of 64 x 16 that each wavefront takes from an atomic counter.
+// (QUEUE) in chunks of 64 x 16 that each wavefront takes from an atomic counter;
+// with an index list (a.idx) logical ray j is ray a.idx[j], for j < *a.n_idx.
 template <int BLOCK, int STACK, bool ANY, bool COUNT, bool SPH, bool QUEUE>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(ANY && !SPH ? 8 : 1, 8)))
 void k_trav_pairs(TraverseArgs a) {
@@ -471,11 +472,13 @@ void k_trav_pairs(TraverseArgs a) {
         ray = qbase + (uint64_t)qk * 64 + lane;
         ++qk;
     };
+    const uint64_t n_rays = a.n_idx ? (uint64_t)*a.n_idx : a.n_rays;
     if (QUEUE) next_queued();
-    while (QUEUE ? qbase < a.n_rays : ray < a.n_rays) {
-        if (ray < a.n_rays) {
+    while (QUEUE ? qbase < n_rays : ray < n_rays) {
+        if (ray < n_rays) {
+            const uint64_t rid = a.idx ? (uint64_t)a.idx[ray] : ray;
             PairLane L;
-            pair_start(a, ray, L);
+            pair_start(a, rid, L);
             for (;;) {
                 // (single compares, so the ballots stay scalar)
                 const uint64_t m_leaf = __builtin_amdgcn_ballot_w64((int32_t)L.ref < (int32_t)kRefNone);
@@ -483,7 +486,7 @@ void k_trav_pairs(TraverseArgs a) {
                 if ((m_leaf | m_int) == 0) break;
                 pair_step<BLOCK, STACK, ANY, COUNT, SPH>(a, L, ring, spill, m_leaf, m_int, cnt);
             }
-            pair_store(a, ray, L);
+            pair_store(a, rid, L);
         }
         if (QUEUE) next_queued();
         else ray += stride;
@@ -558,6 +561,10 @@ uint32_t refill_chunk_shift(uint64_t n_rays, uint64_t n_waves) {
     return sh;
 }
 
+bool refill_applies(const TraverseArgs& a, uint64_t n_waves) {
+    return a.idx == nullptr && a.spheres == nullptr && a.n_rays >= 256 * n_waves;
+}
+
 template <int BLOCK, int STACK, bool ANY, bool COUNT>
 void launch4(const TraverseArgs& a_in, int variant, int grid, hipStream_t s) {
     // scenes without spheres use instantiations with the sphere code compiled out
@@ -565,6 +572,8 @@ void launch4(const TraverseArgs& a_in, int variant, int grid, hipStream_t s) {
     // runs take rays in grid-stride order); 4: pair records with batched refill; 1: node records
     // with per-lane refill; 3: node records
     TraverseArgs a = a_in;
+    // compacted launches (an index list) run the grid-stride or queue pair kernel
+    if (a.idx != nullptr && a.pairs != nullptr && variant != 2) variant = 0;
     const bool q = variant == 2 && a.queue != nullptr && !COUNT;
     const bool pairs = (variant == 0 || variant == 2 || variant == 4) && a.pairs != nullptr;
     // Variant 4 refills only where lanes have rays to refill from: at least four rays per lane
@@ -572,7 +581,7 @@ void launch4(const TraverseArgs& a_in, int variant, int grid, hipStream_t s) {
     // about one ray per lane, and on the sphere-and-triangle workloads even at eight rays per
     // lane, the grid-stride kernel was 3-18% faster, so those run as 0).
     const uint64_t n_waves = (uint64_t)grid * (BLOCK / 64);
-    const bool refill = variant == 4 && pairs && a.spheres == nullptr && a.n_rays >= 256 * n_waves;
+    const bool refill = variant == 4 && pairs && refill_applies(a, n_waves);
     if (refill) a.chunk_shift = refill_chunk_shift(a.n_rays, n_waves);
     if (a.spheres != nullptr) {
         if (refill) hipLaunchKernelGGL((k_trav_pairs_refill<BLOCK, STACK, ANY, COUNT, true>), dim3(grid), dim3(BLOCK), 0, s, a);
@@ -612,6 +621,10 @@ hipError_t launch1(const TraverseArgs& a, bool any, bool count, int stack, int v
 }
 
 }  // namespace
+
+bool traverse_refills(const TraverseArgs& a, const LaunchShape& shape, int grid) {
+    return shape.variant == 4 && a.pairs != nullptr && refill_applies(a, (uint64_t)grid * (shape.block / 64));
+}
 
 bool traverse_shape_supported(const LaunchShape& s) {
     const bool b = s.block == 64 || s.block == 128 || s.block == 256;

@@ -48,9 +48,18 @@ class Context:
     def stream(self) -> int:
         return int(N.lib().bg_stream(self.h) or 0)
 
-    def set_launch(self, block: int, stack: int, blocks_per_cu: int, variant: int = 0) -> None:
-        cfg = N.bg_launch_config(block, stack, blocks_per_cu, variant)
+    def set_launch(self, block: int, stack: int, blocks_per_cu: int, variant: int = 0, compact=None) -> None:
+        """Traversal launch shape; ``compact`` (None: unchanged) sets whether path-integrator passes
+        trace only the rays that can hit, compacted on the device."""
+        if compact is None:
+            compact = self.compaction()
+        cfg = N.bg_launch_config(block, stack, blocks_per_cu, variant, 1 if compact else 0)
         N.check(N.lib().bg_set_launch_config(self.h, C.byref(cfg)), self.h)
+
+    def compaction(self) -> bool:
+        cfg = N.bg_launch_config()
+        N.check(N.lib().bg_get_launch_config(self.h, C.byref(cfg)), self.h)
+        return bool(cfg.compact)
 
     def launch(self) -> tuple[int, int, int, int]:
         cfg = N.bg_launch_config()

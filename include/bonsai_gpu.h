@@ -231,6 +231,9 @@ typedef struct bg_launch_config {
                                 rays, they take the next rays of the wavefront's sequence
                                 (runs of consecutive rays) together; launches of fewer than
                                 four rays per lane, and scenes with spheres, run it as 0 */
+    uint32_t compact;     /* path-integrator passes: nonzero (default) traces only the rays that
+                             can hit (t_max > 0), compacted on the device into an index list,
+                             instead of every path slot; the hits are the same */
 } bg_launch_config;
 int bg_set_launch_config(bg_ctx* ctx, const bg_launch_config* cfg);
 int bg_get_launch_config(const bg_ctx* ctx, bg_launch_config* cfg);

@@ -145,3 +145,43 @@ def test_converges_to_the_oracle_at_4096_spp(ctx, mode):
           f"means {img.mean():.4f} / {ref.mean():.4f}")
     assert err < 1e-3
     assert abs(img.mean() / ref.mean() - 1.0) < 0.01
+
+
+@pytest.mark.parametrize("mode", [MODE_REF, MODE_CORRECTED])
+def test_compaction_does_not_change_the_film(ctx, mode):
+    """Tracing only the rays that can hit (compacted on the device) instead of every path slot gives
+    bit-identical film sums: on the path-tracer scene kind (spheres and triangles, 8 light
+    triangles, roulette ending paths) and, in the corrected semantics, for the guided estimator
+    with a frozen network."""
+    import sys
+    from pathlib import Path
+    from bonsai_amd.guide import GuideMLP
+    from bonsai_amd.integrator import GuidedPathIntegrator
+    sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "oracle"))
+    import guide_mlp as G
+
+    s = path_tracer_scene(80, 45, small=True)
+    bvh = GpuBvh(ctx, s)
+    guide = GuideMLP(ctx, G.init_params(5).astype(np.float32), 65536)
+    saved, saved_c = ctx.launch(), ctx.compaction()
+    out = {}
+    try:
+        for compact in (False, True):
+            ctx.set_launch(*saved, compact=compact)
+            integs = [GpuPathIntegrator(ctx, s.camera, spp=8, max_depth=8, mode=mode, seed=19)]
+            if mode == MODE_CORRECTED:
+                integs.append(GuidedPathIntegrator(ctx, s.camera, spp=8, max_depth=8, guide=guide, seed=23,
+                                                   train=False, rr_threshold=0.0))
+            sums = []
+            for gi in integs:
+                film = Film(ctx, 80, 45)
+                st = gi.render_into(bvh, film)
+                sums.append(film.sums() + (st.rays,))
+                film.close()
+            out[compact] = sums
+    finally:
+        ctx.set_launch(*saved, compact=saved_c)
+    for (rgb0, w0, r0), (rgb1, w1, r1) in zip(out[False], out[True]):
+        np.testing.assert_array_equal(rgb1, rgb0)
+        np.testing.assert_array_equal(w1, w0)
+        assert r1 == r0
