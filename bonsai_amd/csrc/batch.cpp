@@ -129,7 +129,17 @@ hipError_t traverse(const bg_batch* bt, bg_ctx* ctx, const bg::LaunchShape& sh, 
     }
     bg::LaunchShape shape = sh;
     if (shape.variant != 2 && shape.variant != 4) shape.variant = 0;  // the node-record variants need node roots
-    return bg::launch_traverse(batch_args(bt, ctx, rays, hits, n, roots, queue), any, false, shape, grid_for(ctx, shape, n), st);
+    bg::TraverseArgs a = batch_args(bt, ctx, rays, hits, n, roots, queue);
+    const int grid = grid_for(ctx, shape, n);
+    if (bg::traverse_compacts(a, shape, grid)) {
+        // only the rays that can hit, through a device index list (the others' hits become misses)
+        const bg::PathBuffers& b = ctx->paths;
+        const hipError_t e = bg::launch_ray_compact(rays, n, b.rblk, b.ridx, b.rcount, hits, st);
+        if (e != hipSuccess) return e;
+        a.idx = b.ridx;
+        a.n_idx = b.rcount;
+    }
+    return bg::launch_traverse(a, any, false, shape, grid, st);
 }
 
 }  // namespace
@@ -318,7 +328,7 @@ int bg_batch_launch_config(const bg_batch* bt, bg_launch_config* cfg) {
     cfg->stack = (uint32_t)bt->shape.stack;
     cfg->blocks_per_cu = (uint32_t)bt->shape.blocks_per_cu;
     cfg->variant = (uint32_t)bt->shape.variant;
-    cfg->compact = 0;  // a batch's graph-replayed passes trace every path slot
+    cfg->compact = bt->shape.compact ? 1u : 0u;
     return BG_OK;
 }
 
@@ -392,7 +402,8 @@ int bg_render_batch(bg_ctx* ctx, bg_batch* bt, const bg_batch_desc* d, bg_stats*
         for (int block : {128, 256})
             for (int stack : {8, 16})
                 for (int bpc : {8, 16, 32, 64})
-                    for (int variant : {0, 2, 4}) cands.push_back(bg::LaunchShape{block, stack, bpc, variant});
+                    for (int variant : {0, 2, 4})
+                        for (int compact : {0, 1}) cands.push_back(bg::LaunchShape{block, stack, bpc, variant, compact});
     // spill rows for the largest grid any launch shape of this call uses
     uint64_t max_threads = (uint64_t)grid_for(ctx, bt->shape, cap) * bt->shape.block;
     for (const bg::LaunchShape& c : cands) max_threads = std::max<uint64_t>(max_threads, (uint64_t)grid_for(ctx, c, cap) * c.block);
@@ -422,6 +433,14 @@ int bg_render_batch(bg_ctx* ctx, bg_batch* bt, const bg_batch_desc* d, bg_stats*
         HIP_TRY(ctx, bg::launch_shade(p, sc, b, 0, st));
         rc = time_all(b.ray, b.hit, false);
         if (!rc && sc.n_lights) rc = time_all(b.sray, b.shit, true);
+        if (rc) return rc;
+        // and a deep bounce, where ended paths leave most slots empty (what compaction is for)
+        const uint32_t deep = std::min<uint32_t>(4, d->max_depth);
+        for (uint32_t bounce = 1; bounce < deep; ++bounce) {
+            HIP_TRY(ctx, traverse(bt, ctx, bt->shape, b.ray, b.hit, cap, b.root, false, st));
+            HIP_TRY(ctx, bg::launch_shade(p, sc, b, bounce, st));
+        }
+        if (deep > 1) rc = time_all(b.ray, b.hit, false);
         if (rc) return rc;
         size_t best = 0;
         for (size_t c = 1; c < cands.size(); ++c)

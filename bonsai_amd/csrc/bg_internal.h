@@ -258,6 +258,9 @@ hipError_t launch_count_rays(const RenderParamsDev& p, const PathBuffers& b, uns
 // Whether launch_traverse with this shape and grid runs the refill kernel (variant 4 where it
 // refills: four or more rays per lane, no spheres, no index list).
 bool traverse_refills(const TraverseArgs& a, const LaunchShape& shape, int grid);
+// Whether a render pass compacts this launch's rays first: the shape asks for it, the launch does
+// not refill, and it covers at least 1M ray slots.
+bool traverse_compacts(const TraverseArgs& a, const LaunchShape& shape, int grid);
 hipError_t launch_ray_compact(const float4* rays, uint64_t n, uint32_t* blk, uint32_t* idx, uint32_t* count,
                               uint4* hits, hipStream_t s);
 hipError_t launch_guide_inputs(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b, hipStream_t s);

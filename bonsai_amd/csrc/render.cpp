@@ -153,11 +153,11 @@ bg::TraverseArgs traverse_args(const bg_scene* s, bg_ctx* ctx, const float4* ray
 // list built on the device, and the others' hits are written as misses: the hits are the same.
 // Launches that refill finished lanes skip it: there a lane at an ended path takes the next ray at
 // once (measured: compacting them cost 10% on a triangle scene, where it saved 11-14% on the
-// sphere-and-triangle path-tracer scene).
+// sphere-and-triangle path-tracer scene); so do launches of fewer than 1M slots.
 hipError_t trace_pass(const bg_scene* s, bg_ctx* ctx, const float4* rays, uint4* hits, uint64_t n, bool any, int grid,
                       hipStream_t st) {
     bg::TraverseArgs a = traverse_args(s, ctx, rays, hits, n);
-    if (ctx->shape.compact && s->d_pairs != nullptr && !bg::traverse_refills(a, ctx->shape, grid)) {
+    if (bg::traverse_compacts(a, ctx->shape, grid)) {
         const bg::PathBuffers& b = ctx->paths;
         const hipError_t e = bg::launch_ray_compact(rays, n, b.rblk, b.ridx, b.rcount, hits, st);
         if (e != hipSuccess) return e;

@@ -626,6 +626,11 @@ bool traverse_refills(const TraverseArgs& a, const LaunchShape& shape, int grid)
     return shape.variant == 4 && a.pairs != nullptr && refill_applies(a, (uint64_t)grid * (shape.block / 64));
 }
 
+bool traverse_compacts(const TraverseArgs& a, const LaunchShape& shape, int grid) {
+    // (below about 1M slots the three compaction launches cost more than the empty slots)
+    return shape.compact && a.pairs != nullptr && a.n_rays >= (1ull << 20) && !traverse_refills(a, shape, grid);
+}
+
 bool traverse_shape_supported(const LaunchShape& s) {
     const bool b = s.block == 64 || s.block == 128 || s.block == 256;
     const bool st = s.stack == 8 || s.stack == 16 || s.stack == 32;

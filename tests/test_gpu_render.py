@@ -149,8 +149,9 @@ def test_converges_to_the_oracle_at_4096_spp(ctx, mode):
 
 @pytest.mark.parametrize("mode", [MODE_REF, MODE_CORRECTED])
 def test_compaction_does_not_change_the_film(ctx, mode):
-    """Tracing only the rays that can hit (compacted on the device) instead of every path slot gives
-    bit-identical film sums: on the path-tracer scene kind (spheres and triangles, 8 light
+    """Tracing only the rays that can hit (compacted on the device, for launches of 1M slots or
+    more that do not refill) instead of every path slot gives bit-identical film sums: on the
+    path-tracer scene kind (spheres and triangles, 8 light
     triangles, roulette ending paths) and, in the corrected semantics, for the guided estimator
     with a frozen network."""
     import sys
@@ -160,7 +161,7 @@ def test_compaction_does_not_change_the_film(ctx, mode):
     sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "oracle"))
     import guide_mlp as G
 
-    s = path_tracer_scene(80, 45, small=True)
+    s = path_tracer_scene(512, 256, small=True)  # 512 x 256 x 8 spp = 2^20 path slots: compacted launches
     bvh = GpuBvh(ctx, s)
     guide = GuideMLP(ctx, G.init_params(5).astype(np.float32), 65536)
     saved, saved_c = ctx.launch(), ctx.compaction()
@@ -174,7 +175,7 @@ def test_compaction_does_not_change_the_film(ctx, mode):
                                                    train=False, rr_threshold=0.0))
             sums = []
             for gi in integs:
-                film = Film(ctx, 80, 45)
+                film = Film(ctx, 512, 256)
                 st = gi.render_into(bvh, film)
                 sums.append(film.sums() + (st.rays,))
                 film.close()
