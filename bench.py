@@ -334,12 +334,13 @@ def rel_mse_pair(a: np.ndarray, b: np.ndarray) -> float:
 # after three bounces, roulette removes the deep low-throughput paths that reach the light through
 # the gap and the survivors' 1/(1-q) boosts dominate the error: guided 91 s with it, 45 s without);
 # the network guides the first 4 vertices (3: 29 s, 4: 26 s, 5: 26 s, 6: 50 s, all: 52 s); it is
-# trained during a warm-up render of 4x the measured samples with 32 Adam steps per pass (32 spp and
-# 8 steps: 45 s; 64 and 16: 32 s; 128 and 16: 26 s; 128 and 32: 23 s; 256 and 32: 26 s).
+# trained during a warm-up render of 3x the measured samples with 48 Adam steps per 2-spp pass
+# (warm-up spp and steps: 32 and 8: 45 s; 64 and 16: 32 s; 64 and 32: 23 s; 128 and 16: 26 s;
+# 128 and 32: 19.4 s; 64 and 64: 18.6 s; 96 and 48: 18.2 s; 48 and 96: 19.8 s; 256 and 32: 26 s).
 GUIDE_BOUNCES = 4
 GUIDE_RR = 0.0
-GUIDE_STEPS = 32
-GUIDE_WARMUP_FACTOR = 4
+GUIDE_STEPS = 48
+GUIDE_WARMUP_FACTOR = 3
 # The plain path's best roulette threshold on the same workload, from {1 (the reference's default),
 # 0.25, 0.1, 0.05, 0.02, 0}: 114, 107, 98, 89, 91 and 92 s predicted.
 PLAIN_BEST_RR = 0.05
