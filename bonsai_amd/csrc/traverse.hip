@@ -442,13 +442,13 @@ __device__ __forceinline__ void flush_counts(const TraverseArgs& a, const PairCo
 // on arrival, the far child when popped), so the totals equal the node-record
 // kernels'. Degenerate triangles are collapsed in the slot vertices at scene
 // build, so the leaf test skips the reference's per-test degeneracy check. The
-// triangle-only any-hit instantiations are held to 64 VGPRs (8 waves per SIMD,
-// no spills); the others would spill there and run faster at the register
-// count the compiler picks (5-7 waves). Rays are taken in grid-stride order, or
+// triangle-only instantiations (not counting, not queued) are held to 64 VGPRs
+// (8 waves per SIMD, no spills); the others would spill there and run faster at
+// the register count the compiler picks (5-7 waves). Rays are taken in grid-stride order, or
 // (QUEUE) in chunks of 64 x 16 that each wavefront takes from an atomic counter;
 // with an index list (a.idx) logical ray j is ray a.idx[j], for j < *a.n_idx.
 template <int BLOCK, int STACK, bool ANY, bool COUNT, bool SPH, bool QUEUE>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(ANY && !SPH ? 8 : 1, 8)))
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(!SPH && !COUNT && !QUEUE ? 8 : 1, 8)))
 void k_trav_pairs(TraverseArgs a) {
     __shared__ uint2 lds[STACK * BLOCK];
     const int tid = threadIdx.x;
