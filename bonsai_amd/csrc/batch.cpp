@@ -134,7 +134,7 @@ hipError_t traverse(const bg_batch* bt, bg_ctx* ctx, const bg::LaunchShape& sh, 
     if (bg::traverse_compacts(a, shape, grid)) {
         // only the rays that can hit, through a device index list (the others' hits become misses)
         const bg::PathBuffers& b = ctx->paths;
-        const hipError_t e = bg::launch_ray_compact(rays, n, b.rblk, b.ridx, b.rcount, hits, st);
+        const hipError_t e = bg::launch_ray_compact(rays, n, b.rblk, b.rmask, b.ridx, b.rcount, hits, st);
         if (e != hipSuccess) return e;
         a.idx = b.ridx;
         a.n_idx = b.rcount;

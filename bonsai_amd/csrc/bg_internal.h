@@ -176,6 +176,7 @@ struct PathBuffers {
     float* gout = nullptr;     // 40 per live path (compacted): network output
     uint32_t* ridx = nullptr;   // compacted launches: indices of the rays that can hit
     uint32_t* rblk = nullptr;   // per 256 rays: count, then its exclusive prefix
+    uint64_t* rmask = nullptr;  // per 64 rays: which of them can hit (bit l for ray 64 k + l)
     uint32_t* rcount = nullptr; // their number (device)
     uint32_t* gslot = nullptr; // per path: its slot in gin/gout this bounce, ~0 when not live
     uint32_t* gcount = nullptr; // live-path count of this bounce (device)
@@ -261,8 +262,10 @@ bool traverse_refills(const TraverseArgs& a, const LaunchShape& shape, int grid)
 // Whether a render pass compacts this launch's rays first: the shape asks for it, the launch does
 // not refill, and it covers at least 1M ray slots.
 bool traverse_compacts(const TraverseArgs& a, const LaunchShape& shape, int grid);
-hipError_t launch_ray_compact(const float4* rays, uint64_t n, uint32_t* blk, uint32_t* idx, uint32_t* count,
-                              uint4* hits, hipStream_t s);
+// Writes idx (ascending indices of the rays with t_max > 0), their number to *count and misses
+// for the other rays; blk holds ceil(n / 256) words and mask 4 * ceil(n / 256).
+hipError_t launch_ray_compact(const float4* rays, uint64_t n, uint32_t* blk, uint64_t* mask, uint32_t* idx,
+                              uint32_t* count, uint4* hits, hipStream_t s);
 hipError_t launch_guide_inputs(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b, hipStream_t s);
 hipError_t launch_guide_targets(const RenderParamsDev& p, const PathBuffers& b, hipStream_t s);
 // Appends the pass's nonzero-weight records to the ring (order-preserving,

@@ -455,25 +455,31 @@ __global__ __launch_bounds__(256) void k_count_rays(uint64_t n, const float4* __
 __device__ __forceinline__ bool ray_can_hit(const float4* __restrict__ rays, uint64_t i) { return rays[2 * i].w > 0.0f; }
 
 __global__ __launch_bounds__(256) void k_ray_count(const float4* __restrict__ rays, uint64_t n,
-                                                   uint32_t* __restrict__ blk) {
+                                                   uint32_t* __restrict__ blk, uint64_t* __restrict__ mask) {
     __shared__ uint32_t wt[4];
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool live = i < n && ray_can_hit(rays, i);
+    const uint64_t m = __ballot(live);
+    if ((threadIdx.x & 63) == 0) mask[i >> 6] = m;  // one word per wavefront: bit l for ray i + l
     uint32_t total;
-    block_exclusive(i < n && ray_can_hit(rays, i) ? 1u : 0u, wt, &total);
+    block_exclusive(live ? 1u : 0u, wt, &total);
     if (threadIdx.x == 0) blk[blockIdx.x] = total;
 }
 
 // After k_live_scan over blk: each ray that can hit at its block's offset plus its rank in the
-// block (ascending ray order); every other ray gets the miss the traversal would report.
-__global__ __launch_bounds__(256) void k_ray_compact(const float4* __restrict__ rays, uint64_t n,
-                                                     const uint32_t* __restrict__ blk, uint32_t* __restrict__ idx,
+// block (ascending ray order); every other ray gets the miss the traversal would report. The
+// ranks come from the wavefront masks k_ray_count stored (bits past n are zero), so the rays
+// are not read again.
+__global__ __launch_bounds__(256) void k_ray_compact(uint64_t n, const uint32_t* __restrict__ blk,
+                                                     const uint64_t* __restrict__ mask, uint32_t* __restrict__ idx,
                                                      uint4* __restrict__ hits) {
-    __shared__ uint32_t wt[4];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    const bool live = i < n && ray_can_hit(rays, i);
-    uint32_t total;
-    const uint32_t rank = block_exclusive(live ? 1u : 0u, wt, &total);
-    if (live) idx[blk[blockIdx.x] + rank] = (uint32_t)i;
+    const uint64_t* bm = mask + (uint64_t)blockIdx.x * 4;
+    uint32_t off = blk[blockIdx.x];
+    for (uint32_t w = 0; w < wave; ++w) off += (uint32_t)__popcll(bm[w]);
+    const uint64_t m = bm[wave];
+    if ((m >> lane) & 1ull) idx[off + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)i;
     else if (i < n) hits[i] = make_uint4(0xFFFFFFFFu, 0u, 0u, 0u);
 }
 
@@ -485,27 +491,50 @@ __global__ __launch_bounds__(256) void k_live_count(uint64_t n, PathBuffers b) {
     if (threadIdx.x == 0) b.gblk[blockIdx.x] = total;
 }
 
+// Exclusive prefix of nblk counts in place, and their sum, in one block: each thread sums a
+// run of consecutive counts (loads issued eight at a time), a shuffle scan within each
+// wavefront and one over the 16 wavefront totals give each run's offset, and each thread then
+// rewrites its run.
 __global__ __launch_bounds__(1024) void k_live_scan(uint32_t nblk, uint32_t* __restrict__ blk, uint32_t* __restrict__ total) {
-    __shared__ uint32_t part[1024];
-    const uint32_t t = threadIdx.x;
+    __shared__ uint32_t wsum[16];
+    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const uint32_t per = (nblk + 1023) / 1024;
-    const uint32_t b0 = t * per, b1 = min(b0 + per, nblk);
-    uint32_t sum = 0;
-    for (uint32_t k = b0; k < b1; ++k) sum += blk[k];
-    part[t] = sum;
-    __syncthreads();
-    if (t == 0) {
-        uint32_t run = 0;
-        for (int k = 0; k < 1024; ++k) {
-            const uint32_t v = part[k];
-            part[k] = run;
-            run += v;
-        }
-        *total = run;
+    const uint32_t b0 = min(t * per, nblk), b1 = min(b0 + per, nblk);
+    uint32_t sum = 0, k = b0;
+    for (; k + 8 <= b1; k += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = blk[k + j];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) sum += v[j];
     }
+    for (; k < b1; ++k) sum += blk[k];
+    uint32_t x = sum;  // inclusive scan over the wavefront
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
     __syncthreads();
-    uint32_t run = part[t];
-    for (uint32_t k = b0; k < b1; ++k) {
+    uint32_t run = x - sum, all = 0;
+    for (uint32_t w = 0; w < 16; ++w) {
+        const uint32_t v = wsum[w];
+        if (w < wave) run += v;
+        all += v;
+    }
+    if (t == 0) *total = all;
+    for (k = b0; k + 8 <= b1; k += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = blk[k + j];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            blk[k + j] = run;
+            run += v[j];
+        }
+    }
+    for (; k < b1; ++k) {
         const uint32_t v = blk[k];
         blk[k] = run;
         run += v;
@@ -624,13 +653,13 @@ hipError_t launch_count_rays(const RenderParamsDev& p, const PathBuffers& b, uns
     hipLaunchKernelGGL(k_count_rays, dim3((uint32_t)blocks), dim3(256), 0, s, p.n_paths, b.beta, counters);
     return hipGetLastError();
 }
-hipError_t launch_ray_compact(const float4* rays, uint64_t n, uint32_t* blk, uint32_t* idx, uint32_t* count,
-                              uint4* hits, hipStream_t s) {
+hipError_t launch_ray_compact(const float4* rays, uint64_t n, uint32_t* blk, uint64_t* mask, uint32_t* idx,
+                              uint32_t* count, uint4* hits, hipStream_t s) {
     if (n == 0) return hipSuccess;
     const uint32_t nblk = (uint32_t)((n + 255) / 256);
-    hipLaunchKernelGGL(k_ray_count, dim3(nblk), dim3(256), 0, s, rays, n, blk);
+    hipLaunchKernelGGL(k_ray_count, dim3(nblk), dim3(256), 0, s, rays, n, blk, mask);
     hipLaunchKernelGGL(k_live_scan, dim3(1), dim3(1024), 0, s, nblk, blk, count);
-    hipLaunchKernelGGL(k_ray_compact, dim3(nblk), dim3(256), 0, s, rays, n, blk, idx, hits);
+    hipLaunchKernelGGL(k_ray_compact, dim3(nblk), dim3(256), 0, s, n, blk, mask, idx, hits);
     return hipGetLastError();
 }
 hipError_t launch_guide_inputs(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b, hipStream_t s) {

@@ -32,6 +32,7 @@ int bg::ensure_paths(bg_ctx* ctx, size_t cap) {
     b = bg::PathBuffers{};
     const size_t per_path[] = {32, 16, 16, 16, 8, 8, 8, 32, 16, 32, 16, 16, 16, 16, 16, 4};
     size_t total = 256 + ((4 * (cap / 256 + 1) + 255) / 256) * 256 + 256;  // + rblk, rcount
+    total += ((32 * (cap / 256 + 1) + 255) / 256) * 256;                    // + rmask
     for (size_t s : per_path) total += ((s * cap + 255) / 256) * 256;
     void* mem = nullptr;
     HIP_TRY(ctx, hipMalloc(&mem, total));
@@ -59,6 +60,7 @@ int bg::ensure_paths(bg_ctx* ctx, size_t cap) {
     b.nee_d = (float4*)take(16 * cap);
     b.ridx = (uint32_t*)take(4 * cap);
     b.rblk = (uint32_t*)take(4 * (cap / 256 + 1));
+    b.rmask = (uint64_t*)take(32 * (cap / 256 + 1));
     b.rcount = (uint32_t*)take(256);
     b.counters = (unsigned long long*)take(256);  // the 256 spare bytes of `total`
     b.cap = cap;
@@ -159,7 +161,7 @@ hipError_t trace_pass(const bg_scene* s, bg_ctx* ctx, const float4* rays, uint4*
     bg::TraverseArgs a = traverse_args(s, ctx, rays, hits, n);
     if (bg::traverse_compacts(a, ctx->shape, grid)) {
         const bg::PathBuffers& b = ctx->paths;
-        const hipError_t e = bg::launch_ray_compact(rays, n, b.rblk, b.ridx, b.rcount, hits, st);
+        const hipError_t e = bg::launch_ray_compact(rays, n, b.rblk, b.rmask, b.ridx, b.rcount, hits, st);
         if (e != hipSuccess) return e;
         a.idx = b.ridx;
         a.n_idx = b.rcount;

@@ -147,13 +147,15 @@ def test_converges_to_the_oracle_at_4096_spp(ctx, mode):
     assert abs(img.mean() / ref.mean() - 1.0) < 0.01
 
 
+@pytest.mark.parametrize("size", [(512, 256), (1024, 600)])
 @pytest.mark.parametrize("mode", [MODE_REF, MODE_CORRECTED])
-def test_compaction_does_not_change_the_film(ctx, mode):
+def test_compaction_does_not_change_the_film(ctx, mode, size):
     """Tracing only the rays that can hit (compacted on the device, for launches of 1M slots or
     more that do not refill) instead of every path slot gives bit-identical film sums: on the
     path-tracer scene kind (spheres and triangles, 8 light
     triangles, roulette ending paths) and, in the corrected semantics, for the guided estimator
-    with a frozen network."""
+    with a frozen network. 1024 x 600 x 8 spp = 4,915,200 slots gives 19,200 block counts, so
+    the one-block scan runs both its eight-wide and its single-count loops."""
     import sys
     from pathlib import Path
     from bonsai_amd.guide import GuideMLP
@@ -161,7 +163,8 @@ def test_compaction_does_not_change_the_film(ctx, mode):
     sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "oracle"))
     import guide_mlp as G
 
-    s = path_tracer_scene(512, 256, small=True)  # 512 x 256 x 8 spp = 2^20 path slots: compacted launches
+    w, h = size
+    s = path_tracer_scene(w, h, small=True)  # 512 x 256 x 8 spp = 2^20 path slots: compacted launches
     bvh = GpuBvh(ctx, s)
     guide = GuideMLP(ctx, G.init_params(5).astype(np.float32), 65536)
     saved, saved_c = ctx.launch(), ctx.compaction()
@@ -175,7 +178,7 @@ def test_compaction_does_not_change_the_film(ctx, mode):
                                                    train=False, rr_threshold=0.0))
             sums = []
             for gi in integs:
-                film = Film(ctx, 512, 256)
+                film = Film(ctx, w, h)
                 st = gi.render_into(bvh, film)
                 sums.append(film.sums() + (st.rays,))
                 film.close()
