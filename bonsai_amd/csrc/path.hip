@@ -421,8 +421,6 @@ __global__ __launch_bounds__(256) void k_guide_targets(RenderParamsDev p, PathBu
 }
 
 // Sums the per-path ray counts of a pass into counters[0] (all queries) and
-// counters[1] (any-hit queries): one atomic per block.
-// Sums the per-path ray counts of a pass into counters[0] (all queries) and
 // counters[1] (any-hit queries): grid-stride partial sums, two atomics per block.
 __global__ __launch_bounds__(256) void k_count_rays(uint64_t n, const float4* __restrict__ beta,
                                                     unsigned long long* __restrict__ counters) {

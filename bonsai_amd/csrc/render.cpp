@@ -29,7 +29,23 @@ int bg::ensure_paths(bg_ctx* ctx, size_t cap) {
     bg::PathBuffers& b = ctx->paths;
     if (b.cap >= cap) return BG_OK;
     if (b.mem) hipFree(b.mem);
+    // only the path-state allocation is replaced: the guiding buffers (a separate allocation,
+    // sized by ensure_guide_bufs) keep their memory and pointers
+    bg::PathBuffers kept = b;
     b = bg::PathBuffers{};
+    b.guide_cap = kept.guide_cap;
+    b.rec_cap = kept.rec_cap;
+    b.guide_mem = kept.guide_mem;
+    b.gin = kept.gin;
+    b.gout = kept.gout;
+    b.gslot = kept.gslot;
+    b.gcount = kept.gcount;
+    b.gblk = kept.gblk;
+    b.rec_in = kept.rec_in;
+    b.rec_tgt = kept.rec_tgt;
+    b.rec_aux = kept.rec_aux;
+    b.rec_blk = kept.rec_blk;
+    b.last_records = kept.last_records;
     const size_t per_path[] = {32, 16, 16, 16, 8, 8, 8, 32, 16, 32, 16, 16, 16, 16, 16, 4};
     size_t total = 256 + ((4 * (cap / 256 + 1) + 255) / 256) * 256 + 256;  // + rblk, rcount
     total += ((32 * (cap / 256 + 1) + 255) / 256) * 256;                    // + rmask

@@ -15,7 +15,16 @@ def hip():
         _hip.hipFree.argtypes = [C.c_void_p]
         _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         _hip.hipDeviceSynchronize.argtypes = []
+        _hip.hipMemGetInfo.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
     return _hip
+
+
+def free_bytes() -> int:
+    """Free device memory of the current device (hipMemGetInfo)."""
+    free, total = C.c_size_t(0), C.c_size_t(0)
+    if hip().hipMemGetInfo(C.byref(free), C.byref(total)) != 0:
+        raise RuntimeError("hipMemGetInfo failed")
+    return free.value
 
 
 H2D, D2H = 1, 2

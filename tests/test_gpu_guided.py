@@ -153,3 +153,24 @@ def test_trained_guide_cuts_variance_without_roulette(ctx):
           f"without network {v_free:.1f}, guided {v_guided:.1f} (ratio {v_free / v_guided:.2f})")
     assert v_guided * 2.0 < v_free
     assert v_guided < v_plain
+
+
+def test_growing_path_buffers_keeps_the_guiding_buffers(ctx):
+    """Growing the path-state buffers (a plain render with more path slots than any call before)
+    keeps the guiding buffers a guided render allocated: the next guided render of the same size
+    allocates nothing more on the device."""
+    from hipmem import free_bytes
+    s = cornell_box(width=64, height=64)
+    bvh = GpuBvh(ctx, s)
+    guide = new_guide(ctx)
+    gi = GuidedPathIntegrator(ctx, s.camera, spp=4, max_depth=4, guide=guide, train=True, train_paths=4096)
+    film = Film(ctx, 64, 64)
+    gi.render_into(bvh, film)
+    # more path slots than any render before: 768 x 768 x 64 spp passes hold the 16M-slot default
+    big = cornell_box(width=768, height=768)
+    GpuPathIntegrator(ctx, big.camera, spp=64, max_depth=1, mode=MODE_CORRECTED).render_into(
+        GpuBvh(ctx, big), Film(ctx, 768, 768), spp=64)
+    before = free_bytes()
+    gi.render_into(bvh, film, first_sample=4)
+    after = free_bytes()
+    assert after == before, (before, after)
