@@ -18,6 +18,7 @@ BG_ERR_UNSUPPORTED = -4
 BG_CLOSEST = 0
 BG_ANY = 1
 BG_COUNT = 2
+BG_NOALLOC_SCRATCH_BYTES = 128 << 20
 BG_MISS = 0xFFFFFFFF
 BG_MAT_MATTE = 0
 BG_MAT_CONDUCTOR = 1
@@ -167,6 +168,7 @@ SIGNATURES = {
     "bg_intersect": (C.c_int, [P, P, P, C.c_size_t, P, C.c_uint32, C.POINTER(bg_stats)]),
     "bg_surface_interactions": (C.c_int, [P, P, P, P, C.c_size_t, P]),
     "bg_intersect_device": (C.c_int, [P, P, P, C.c_size_t, P, C.c_uint32, C.POINTER(bg_stats)]),
+    "bg_intersect_device_noalloc": (C.c_int, [P, P, P, C.c_size_t, P, C.c_uint32, C.POINTER(bg_stats)]),
     "bg_set_launch_config": (C.c_int, [P, C.POINTER(bg_launch_config)]),
     "bg_get_launch_config": (C.c_int, [P, C.POINTER(bg_launch_config)]),
     "bg_camera_rays_device": (C.c_int, [P, C.POINTER(bg_camera_desc), C.c_float, C.c_float, P]),

@@ -282,6 +282,14 @@ hipError_t launch_zero_u64(unsigned long long* p, hipStream_t s);
 hipError_t launch_film(const RenderParamsDev& p, const PathBuffers& b, uint32_t width, uint32_t height, float4* film,
                        double4* film64, hipStream_t s);
 
+// noalloc.hip: the static scratch of bg_intersect_device_noalloc. init takes the
+// array's address on `device` (which must be current) once; scratch returns it
+// (nullptr before init).
+hipError_t noalloc_scratch_init(int device);
+void* noalloc_scratch(int device);
+hipError_t noalloc_zero_counters(unsigned long long* c, uint32_t n, hipStream_t s);
+hipError_t noalloc_fill_miss(void* hits, uint64_t n, hipStream_t s);
+
 // render.cpp: (re)allocate the context's path state and traversal spill buffer
 int ensure_paths(bg_ctx* ctx, size_t cap);
 int ensure_spill(bg_ctx* ctx, size_t bytes);

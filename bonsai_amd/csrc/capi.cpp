@@ -72,17 +72,40 @@ int traverse_grid(const bg_ctx* ctx, size_t n) {
     return (int)std::max<uint64_t>(1, std::min(tiles, cap));
 }
 
+// Device memory a traversal call works in beyond its rays and hits: the spill
+// rows of the LDS stacks and the three BG_COUNT counters. Null members: the
+// context's buffers, grown on demand.
+struct TraverseScratch {
+    uint32_t* spill = nullptr;
+    size_t spill_bytes = 0;
+    unsigned long long* counters = nullptr;
+};
+
 int run_traverse(bg_ctx* ctx, const bg_scene* s, const void* d_rays, size_t n, void* d_hits, uint32_t flags,
-                 bg_stats* stats) {
+                 bg_stats* stats, const TraverseScratch* fixed = nullptr) {
     if (n == 0) {
         if (stats) *stats = bg_stats{};
         return BG_OK;
     }
-    const int grid = traverse_grid(ctx, n);
+    int grid = traverse_grid(ctx, n);
     const uint32_t depth = s->bvh.max_depth + 1;
-    const size_t spill_need = (size_t)grid * ctx->shape.block * depth * bg::kSpillWords * sizeof(uint32_t);
-    int rc = ensure(ctx, (void**)&ctx->spill, &ctx->spill_bytes, spill_need);
-    if (rc) return rc;
+    const size_t row_bytes = (size_t)depth * bg::kSpillWords * sizeof(uint32_t);
+    const size_t block_bytes = (size_t)ctx->shape.block * row_bytes;
+    if (fixed) {
+        // the grid is narrowed to the spill rows the scratch holds; traversal results do not
+        // depend on the grid (each ray is traced the same way by whichever lane takes it)
+        const size_t fit = fixed->spill_bytes / block_bytes;
+        if (fit == 0)
+            return fail(ctx, BG_ERR_INVALID,
+                        "bg_intersect_device_noalloc: one block of " + std::to_string(ctx->shape.block) +
+                            " lanes needs " + std::to_string(block_bytes) + " bytes of stack spill (BVH depth " +
+                            std::to_string(depth) + "), the static scratch holds " +
+                            std::to_string(fixed->spill_bytes));
+        grid = (int)std::min<size_t>((size_t)grid, fit);
+    } else {
+        int rc = ensure(ctx, (void**)&ctx->spill, &ctx->spill_bytes, (size_t)grid * block_bytes);
+        if (rc) return rc;
+    }
     bg::TraverseArgs a;
     a.nodes = s->d_nodes;
     a.pairs = s->d_pairs;
@@ -95,13 +118,16 @@ int run_traverse(bg_ctx* ctx, const bg_scene* s, const void* d_rays, size_t n, v
     a.rays = (const float4*)d_rays;
     a.hits = (uint4*)d_hits;
     a.n_rays = n;
-    a.spill = ctx->spill;
+    a.spill = fixed ? fixed->spill : ctx->spill;
     a.spill_depth = depth;
     a.counters = nullptr;
     const bool any = (flags & BG_ANY) != 0;
     const bool count = (flags & BG_COUNT) != 0;
     const unsigned long long zeros[3] = {0, 0, 0};
-    if (count) {
+    if (count && fixed) {
+        HIP_TRY(ctx, bg::noalloc_zero_counters(fixed->counters, 3, ctx->stream));
+        a.counters = fixed->counters;
+    } else if (count) {
         if (!ctx->counters) HIP_TRY(ctx, hipMalloc((void**)&ctx->counters, sizeof(zeros)));
         HIP_TRY(ctx, hipMemcpyAsync(ctx->counters, zeros, sizeof(zeros), hipMemcpyHostToDevice, ctx->stream));
         a.counters = ctx->counters;
@@ -115,7 +141,7 @@ int run_traverse(bg_ctx* ctx, const bg_scene* s, const void* d_rays, size_t n, v
     float ms = 0.0f;
     HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     unsigned long long cnt[3] = {0, 0, 0};
-    if (count) HIP_TRY(ctx, hipMemcpy(cnt, ctx->counters, sizeof(cnt), hipMemcpyDeviceToHost));
+    if (count) HIP_TRY(ctx, hipMemcpy(cnt, a.counters, sizeof(cnt), hipMemcpyDeviceToHost));
     if (stats) {
         *stats = bg_stats{};
         stats->kernel_ms = ms;
@@ -216,6 +242,14 @@ int bg_create(int device, bg_ctx** out) {
         return BG_ERR_HIP;
     }
     ctx->num_cus = prop.multiProcessorCount;
+    // the static scratch of bg_intersect_device_noalloc: its address on this device, taken once
+    if (bg::noalloc_scratch_init(device) != hipSuccess) {
+        hipEventDestroy(ctx->ev0);
+        hipEventDestroy(ctx->ev1);
+        hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return BG_ERR_HIP;
+    }
     *out = ctx;
     return BG_OK;
 }
@@ -490,6 +524,29 @@ int bg_intersect_device(bg_ctx* ctx, const bg_scene* s, const void* d_rays, size
         return BG_OK;
     }
     return run_traverse(ctx, s, d_rays, n, d_hits, flags, stats);
+}
+
+int bg_intersect_device_noalloc(bg_ctx* ctx, const bg_scene* s, const void* d_rays, size_t n, void* d_hits,
+                                uint32_t flags, bg_stats* stats) {
+    if (!ctx || !s || s->ctx != ctx) return fail(ctx, BG_ERR_INVALID, "scene does not belong to this context");
+    if (n > 0 && (!d_rays || !d_hits)) return fail(ctx, BG_ERR_INVALID, "null ray or hit buffer");
+    if (flags & ~3u) return fail(ctx, BG_ERR_INVALID, "unknown flags");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    char* base = static_cast<char*>(bg::noalloc_scratch(ctx->device));
+    if (!base) return fail(ctx, BG_ERR_HIP, "bg_intersect_device_noalloc: static scratch has no address on this device");
+    if (s->bvh.nodes.empty()) {
+        // Empty scene: every query misses (bvh.rs:286, 359).
+        HIP_TRY(ctx, bg::noalloc_fill_miss(d_hits, n, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        if (stats) *stats = bg_stats{0.0, 0.0, n, 0};
+        return BG_OK;
+    }
+    // layout: counters in the first 256 bytes, spill rows after them
+    TraverseScratch sc;
+    sc.counters = reinterpret_cast<unsigned long long*>(base);
+    sc.spill = reinterpret_cast<uint32_t*>(base + 256);
+    sc.spill_bytes = (size_t)BG_NOALLOC_SCRATCH_BYTES - 256;
+    return run_traverse(ctx, s, d_rays, n, d_hits, flags, stats, &sc);
 }
 
 int bg_intersect(bg_ctx* ctx, const bg_scene* s, const bg_ray* rays, size_t n, bg_hit* hits, uint32_t flags,

@@ -197,6 +197,22 @@ int bg_intersect(bg_ctx* ctx, const bg_scene* scene, const bg_ray* rays, size_t 
 int bg_intersect_device(bg_ctx* ctx, const bg_scene* scene, const void* d_rays, size_t n, void* d_hits,
                         uint32_t flags, bg_stats* stats);
 
+/* bg_intersect_device without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). No counterpart in the reference. The traversal's stack
+ * spill rows and the BG_COUNT counters live in a static device array of
+ * BG_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends) instead of
+ * context buffers; the grid is narrowed so that its spill rows fit, which
+ * changes no result. Returns BG_ERR_INVALID when one block of the context's
+ * launch shape needs more spill than the array holds (a BVH deeper than
+ * BG_NOALLOC_SCRATCH_BYTES / (8 * block) - 1 pending entries). Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (two streams, two
+ * contexts or two threads at once) race and must not be issued. Results are
+ * those of bg_intersect_device with the same arguments. */
+#define BG_NOALLOC_SCRATCH_BYTES (128u << 20)
+int bg_intersect_device_noalloc(bg_ctx* ctx, const bg_scene* scene, const void* d_rays, size_t n, void* d_hits,
+                                uint32_t flags, bg_stats* stats);
+
 /* The surface interaction of each hit (the geometry GeometricPrimitive::intersect
  * fills into an Interaction, src/primitives/geometric.rs:27-38 over
  * shapes/triangle.rs:135-388 and shapes/sphere.rs): for ray i and hits[i]
