@@ -328,4 +328,62 @@ void orc_bsdf_probe(uint32_t type, const float* params, int mode, const float* w
     out[11] = (float)r.stype; out[12] = (float)r.ncomp;
 }
 
+// ------------------------------------------------ single-function probes
+// n inputs each; arrays are packed float triples.
+void orc_fresnel_conductor(uint64_t n, const float* cos_i, const float* eta, const float* k, float* out) {
+    for (uint64_t i = 0; i < n; ++i) {
+        RGB r = probe_fresnel_conductor(cos_i[i], RGB(eta[3 * i], eta[3 * i + 1], eta[3 * i + 2]),
+                                        RGB(k[3 * i], k[3 * i + 1], k[3 * i + 2]));
+        out[3 * i] = r[0]; out[3 * i + 1] = r[1]; out[3 * i + 2] = r[2];
+    }
+}
+
+// out: 4 floats per (wo, wh) pair: D(wh), Lambda(wo), G1(wo), pdf(wo, wh).
+void orc_tr_eval(int mode, float alpha, uint64_t n, const float* wo, const float* wh, float* out) {
+    for (uint64_t i = 0; i < n; ++i)
+        probe_tr((Mode)mode, alpha, V3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]),
+                 V3(wh[3 * i], wh[3 * i + 1], wh[3 * i + 2]), out + 4 * i);
+}
+
+void orc_sample11(int mode, uint64_t n, const float* cos_theta, const float* u1, const float* u2, float* out) {
+    for (uint64_t i = 0; i < n; ++i) probe_sample11((Mode)mode, cos_theta[i], u1[i], u2[i], out + 2 * i);
+}
+
+// One outgoing direction wo, n uniform pairs (u1, u2): n sampled microfacet normals.
+void orc_tr_sample(int mode, float alpha, const float* wo, uint64_t n, const float* u1, const float* u2, float* out) {
+    const V3 w(wo[0], wo[1], wo[2]);
+    for (uint64_t i = 0; i < n; ++i) {
+        V3 h = probe_tr_sample((Mode)mode, alpha, w, u1[i], u2[i]);
+        out[3 * i] = h.x; out[3 * i + 1] = h.y; out[3 * i + 2] = h.z;
+    }
+}
+
+// out: 8 floats per ray (o, t_max, d, time).
+void orc_spawn_ray(int mode, uint64_t n, const float* p, const float* perr, const float* nrm, const float* d,
+                   float* out) {
+    for (uint64_t i = 0; i < n; ++i) {
+        Ray r = probe_spawn_ray(V3(p[3 * i], p[3 * i + 1], p[3 * i + 2]),
+                                V3(perr[3 * i], perr[3 * i + 1], perr[3 * i + 2]),
+                                V3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]), 0.25f,
+                                V3(d[3 * i], d[3 * i + 1], d[3 * i + 2]), (Mode)mode);
+        float* o = out + 8 * i;
+        o[0] = r.o.x; o[1] = r.o.y; o[2] = r.o.z; o[3] = r.t_max;
+        o[4] = r.d.x; o[5] = r.d.y; o[6] = r.d.z; o[7] = r.time;
+    }
+}
+
+// out: 11 floats per ray (o, t_max, d, time, then the offset target point).
+void orc_spawn_ray_to(uint64_t n, const float* p0, const float* e0, const float* n0, const float* p1, const float* e1,
+                      const float* n1, float* out) {
+    auto v = [](const float* a, uint64_t i) { return V3(a[3 * i], a[3 * i + 1], a[3 * i + 2]); };
+    for (uint64_t i = 0; i < n; ++i) {
+        V3 t;
+        Ray r = probe_spawn_ray_to(v(p0, i), v(e0, i), v(n0, i), 0.25f, v(p1, i), v(e1, i), v(n1, i), &t);
+        float* o = out + 11 * i;
+        o[0] = r.o.x; o[1] = r.o.y; o[2] = r.o.z; o[3] = r.t_max;
+        o[4] = r.d.x; o[5] = r.d.y; o[6] = r.d.z; o[7] = r.time;
+        o[8] = t.x; o[9] = t.y; o[10] = t.z;
+    }
+}
+
 }  // extern "C"

@@ -44,9 +44,9 @@ RGB fresnel_conductor(Float cos_theta_i, const RGB& eta_i, const RGB& eta_t, con
     RGB eta2 = eta * eta;
     RGB etak2 = eta_k * eta_k;
     RGB t0 = eta2 - etak2 - s2;
-    RGB a2b2 = rsqrt_rgb(t0 * t0 + eta2 * etak2 * 4.0f);
+    RGB a2b2 = sqrt_rgb(t0 * t0 + eta2 * etak2 * 4.0f);
     RGB t1 = a2b2 + c2;
-    RGB a = rsqrt_rgb((a2b2 + t0) * 0.5f);
+    RGB a = sqrt_rgb((a2b2 + t0) * 0.5f);
     RGB t2 = c * a * 2.0f;
     RGB rs = (t1 - t2) / (t1 + t2);
     RGB t3 = c2 * a2b2 + s2 * s2;
@@ -371,9 +371,10 @@ inline Ray spawn_ray(const Isect& it, const V3& d, Mode mode) {
 }
 // interaction.rs:132-143
 inline Ray spawn_ray_to(const V3& p0, const V3& e0, const V3& n0, Float time, const V3& p1, const V3& e1,
-                        const V3& n1) {
+                        const V3& n1, V3* target = nullptr) {
     V3 o = offset_ray_origin(p0, e0, n0, p1 - p0);
     V3 t = offset_ray_origin(p1, e1, n1, o - p1);
+    if (target) *target = t;
     return Ray{o, t - o, 1.0f - 0.0001f, time};
 }
 
@@ -719,6 +720,41 @@ BsdfProbe bsdf_probe(const Material& m, Mode mode, const V3& wo, const V3& wi, F
     r.stype = s.type;
     r.ncomp = b.num_components(BSDF_ALL);
     return r;
+}
+
+RGB probe_fresnel_conductor(Float cos_theta_i, const RGB& eta, const RGB& k) {
+    return fresnel_conductor(cos_theta_i, RGB(1.0f), eta, k);
+}
+
+void probe_tr(Mode mode, Float alpha, const V3& wo, const V3& wh, Float out[4]) {
+    TR d{fmax_r(alpha, 0.001f), fmax_r(alpha, 0.001f), mode};
+    out[0] = tr_d(d, wh);
+    out[1] = tr_lambda(d, wo);
+    out[2] = tr_g1(d, wo);
+    out[3] = tr_pdf(d, wo, wh);
+}
+
+void probe_sample11(Mode mode, Float cos_theta, Float u1, Float u2, Float out[2]) {
+    sample11(mode, cos_theta, u1, u2, &out[0], &out[1]);
+}
+
+V3 probe_tr_sample(Mode mode, Float alpha, const V3& wo, Float u1, Float u2) {
+    TR d{fmax_r(alpha, 0.001f), fmax_r(alpha, 0.001f), mode};
+    return tr_sample(d, wo, u1, u2);
+}
+
+Ray probe_spawn_ray(const V3& p, const V3& perr, const V3& n, Float time, const V3& d, Mode mode) {
+    Isect it{};
+    it.p = p;
+    it.perr = perr;
+    it.n = n;
+    it.time = time;
+    return spawn_ray(it, d, mode);
+}
+
+Ray probe_spawn_ray_to(const V3& p0, const V3& e0, const V3& n0, Float time, const V3& p1, const V3& e1,
+                       const V3& n1, V3* target) {
+    return spawn_ray_to(p0, e0, n0, time, p1, e1, n1, target);
 }
 
 }  // namespace orc

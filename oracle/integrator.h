@@ -39,7 +39,7 @@ inline bool is_black(const RGB& a) { return a[0] == 0.0f && a[1] == 0.0f && a[2]
 // rgb.rs:97-104
 inline Float max_component_value(const RGB& a) { return fmax_r(fmax_r(a[0], a[1]), a[2]); }
 inline Float luminance(const RGB& a) { return 0.212671f * a[0] + 0.715160f * a[1] + 0.072169f * a[2]; }
-inline RGB rsqrt_rgb(const RGB& a) { return RGB(std::sqrt(a[0]), std::sqrt(a[1]), std::sqrt(a[2])); }
+inline RGB sqrt_rgb(const RGB& a) { return RGB(std::sqrt(a[0]), std::sqrt(a[1]), std::sqrt(a[2])); }
 
 // MAT_PLASTIC (plastic.rs:24-52): r = Kd, eta = Ks, alpha = roughness, k[0] != 0: remap roughness.
 enum MatType { MAT_MATTE = 0, MAT_CONDUCTOR = 1, MAT_PLASTIC = 2 };
@@ -130,5 +130,20 @@ struct BsdfProbe {
     int ncomp;
 };
 BsdfProbe bsdf_probe(const Material& m, Mode mode, const V3& wo, const V3& wi, Float u0, Float u1);
+
+// Single functions of the restatement, for the tests that pin them.
+// fresnel_conductor (bxdf.rs:190-217) with eta_i = 1.
+RGB probe_fresnel_conductor(Float cos_theta_i, const RGB& eta, const RGB& k);
+// Isotropic Trowbridge-Reitz (microfacet.rs) of alpha: out = D(wh), Lambda(wo), G1(wo), pdf(wo, wh).
+void probe_tr(Mode mode, Float alpha, const V3& wo, const V3& wh, Float out[4]);
+// TrowbridgeReitzDistribution::sample11 (microfacet.rs:55-97): (slope_x, slope_y).
+void probe_sample11(Mode mode, Float cos_theta, Float u1, Float u2, Float out[2]);
+// MicrofacetDistribution::sample (microfacet.rs:144-153): the sampled microfacet normal.
+V3 probe_tr_sample(Mode mode, Float alpha, const V3& wo, Float u1, Float u2);
+// Interaction::spawn_ray (interaction.rs:115-121) from (p, p_error, n, time) in direction d.
+Ray probe_spawn_ray(const V3& p, const V3& perr, const V3& n, Float time, const V3& d, Mode mode);
+// Interaction::spawn_ray_to_it (interaction.rs:132-143); *target receives the offset target point.
+Ray probe_spawn_ray_to(const V3& p0, const V3& e0, const V3& n0, Float time, const V3& p1, const V3& e1,
+                       const V3& n1, V3* target);
 
 }  // namespace orc
