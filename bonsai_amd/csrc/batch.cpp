@@ -285,8 +285,11 @@ int bg_batch_create(bg_ctx* ctx, const bg_batch_item* items, uint32_t n, bg_batc
             std::vector<bg::LightDev> l(s->lights.size());
             if (hipMemcpy(l.data(), s->d_lights, l.size() * sizeof(bg::LightDev), hipMemcpyDeviceToHost) != hipSuccess)
                 rc = fail(ctx, BG_ERR_HIP, "batch light copy failed");
+            const uint64_t ntu = s->n_tris;
             for (size_t k = 0; k < l.size(); ++k) {
-                l[k].prim += (uint32_t)tri_base[u];  // lights are on triangles
+                // batch-wide primitive ids: every scene's triangles first, then every scene's spheres
+                const uint64_t pr = l[k].prim;
+                l[k].prim = (uint32_t)(pr < ntu ? tri_base[u] + pr : bt->n_tris + sph_base[u] + (pr - ntu));
                 lts[light_base[u] + k] = l[k];
             }
         }

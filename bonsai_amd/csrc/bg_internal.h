@@ -135,7 +135,8 @@ struct SphereShadeDev {
     float o2w[12];
     float w2o[12];
     float radius, z_min, z_max, phi_max, theta_min, theta_max;
-    uint32_t flip, pad;
+    uint32_t flip;     // the normal's flip: reverse_orientation xor the transform swaps handedness
+    uint32_t reverse;  // reverse_orientation alone (Sphere::sample, sphere.rs:333-335)
 };
 struct ShadeSceneDev {
     const float* tri_orig;      // 9 floats per triangle, scene order

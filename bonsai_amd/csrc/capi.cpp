@@ -312,7 +312,7 @@ int bg_scene_create(bg_ctx* ctx, const bg_scene_desc* d, bg_scene** out) {
         if (!(d->sphere_params[4 * i] > 0.0f)) return fail(ctx, BG_ERR_INVALID, "sphere radius must be > 0");
     }
     for (uint32_t i = 0; i < d->n_lights; ++i)
-        if (d->lights[i].prim_id >= nt) return fail(ctx, BG_ERR_INVALID, "area lights must be on triangles");
+        if (d->lights[i].prim_id >= np) return fail(ctx, BG_ERR_INVALID, "area light on a primitive the scene does not have");
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     auto t0 = std::chrono::steady_clock::now();
 
@@ -405,6 +405,13 @@ int bg_scene_create(bg_ctx* ctx, const bg_scene_desc* d, bg_scene** out) {
     std::vector<bg::LightDev> lts(s->lights.size());
     for (size_t i = 0; i < lts.size(); ++i) {
         const bg_area_light& l = s->lights[i];
+        if (l.prim_id >= nt) {
+            // sphere.rs:375-377: phi_max * radius * (z_max - z_min)
+            const bg::SphereDev& sp = sph[l.prim_id - nt];
+            lts[i] = bg::LightDev{l.prim_id, {l.radiance[0], l.radiance[1], l.radiance[2]}, l.double_sided ? 1u : 0u,
+                                  sp.phi_max * sp.radius * (sp.z_max - sp.z_min), {0u, 0u}};
+            continue;
+        }
         const float* v = d->tri_positions + 9 * (uint64_t)l.prim_id;
         // triangle.rs:561-566: 0.5 * |(p1 - p0) x (p2 - p0)| with the f64 cross of vec3.rs:108-122
         const double ax = v[3] - v[0], ay = v[4] - v[1], az = v[5] - v[2];
@@ -432,7 +439,7 @@ int bg_scene_create(bg_ctx* ctx, const bg_scene_desc* d, bg_scene** out) {
         const float det = m[0] * (m[5] * m[10] - m[6] * m[9]) - m[1] * (m[4] * m[10] - m[6] * m[8]) +
                           m[2] * (m[4] * m[9] - m[5] * m[8]);
         ssh[i].flip = ((s->sphere_flags[i] != 0) != (det < 0.0f)) ? 1u : 0u;
-        ssh[i].pad = 0;
+        ssh[i].reverse = s->sphere_flags[i] != 0 ? 1u : 0u;
     }
     if (!rc) rc = upload(ctx, &s->d_sph_shade, ssh.data(), ssh.size(), &bytes);
     if (rc) {

@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) void k_direct_resolve(RenderParamsDev p, Shade
                 // integrator.rs:412-419: the reference accepts any area light, pbrt only the sampled one
                 const bool same = p.mode == 0 ? (li >= 0) : (li == (int32_t)__float_as_uint(c.y));
                 if (same) {
-                    const C3 Le = emission_at(sc, li, hp, d.mray[2 * slot + 1]);
+                    const C3 Le = emission_at(sc, li, hp, d.mray[2 * slot], d.mray[2 * slot + 1]);
                     if (!black(Le)) {
                         const float4 fw = d.b[slot];
                         ed = ed + C3{fw.x, fw.y, fw.z} * Le * fw.w / c.x;

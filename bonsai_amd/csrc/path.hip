@@ -157,7 +157,7 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
             const float cosl = __builtin_fabsf(dot(s.n, rd));
             const float pc = l4.w;  // continuation pdf of the previous vertex
             if (cosl > 0.0f && pc > 0.0f) {
-                const float pl = t * t / (cosl * lt.area) / (float)sc.n_lights;
+                const float pl = light_pdf_of_hit(sc, lt, r0, t, cosl) / (float)sc.n_lights;
                 L = L + beta * emission(lt, s.n, -rd) * power_heuristic(pc, pl);
             }
         }
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(256) void k_nee_resolve(RenderParamsDev p, ShadeSce
             const float4 dd = b.nee_d[i];
             const bool same = p.mode == 0 ? (li >= 0) : (li == (int32_t)__float_as_uint(dd.y));
             if (same) {
-                const C3 Le = emission_at(sc, li, hp, b.mray[2 * i + 1]);
+                const C3 Le = emission_at(sc, li, hp, b.mray[2 * i], b.mray[2 * i + 1]);
                 if (!black(Le)) {
                     const float4 fw = b.nee_b[i];
                     out = out + C3{fw.x, fw.y, fw.z} * Le * fw.w / dd.x;

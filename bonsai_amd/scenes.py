@@ -239,14 +239,16 @@ def quad(p0, p1, p2, p3) -> np.ndarray:
 
 
 def cornell_box(double_sided_light: bool = False, width: int = 256, height: int = 256,
-                with_plastic: bool = False) -> SceneData:
+                with_plastic: bool = False, sphere_light: bool = False) -> SceneData:
     """Small Cornell-style box in [0,1]^3: 5 walls as 10 triangles, a 0.25 x 0.25
     quad light (2 triangles, radiance 17) under the ceiling facing down, two
     r = 0.3 spheres (matte albedo 0.75, conductor alpha 0.2). `with_plastic`
     makes the matte sphere plastic (kd (0.5, 0.25, 0.1), ks 0.4, roughness 0.1
     remapped) and the floor glossy plastic (kd 0.6, ks 0.3, alpha 0.3) with its
     winding reversed so that its normal faces into the box (the reference's
-    dielectric Fresnel term has a pole for light from behind the normal)."""
+    dielectric Fresnel term has a pole for light from behind the normal).
+    `sphere_light` adds a third area light: a black r = 0.08 sphere at
+    (0.5, 0.78, 0.45) emitting (8, 6, 4), with the quad light's sidedness."""
     walls = [
         quad((0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1)),        # floor y=0
         quad((0, 1, 0), (0, 1, 1), (1, 1, 1), (1, 1, 0)),        # ceiling y=1
@@ -270,6 +272,9 @@ def cornell_box(double_sided_light: bool = False, width: int = 256, height: int 
     add_spheres(s, [(0.3, 0.3, 0.6), (0.7, 0.3, 0.35)], [0.3, 0.3], [4, 5])
     s.lights = [area_light(10, (17.0, 17.0, 17.0), double_sided_light),
                 area_light(11, (17.0, 17.0, 17.0), double_sided_light)]
+    if sphere_light:
+        add_spheres(s, [(0.5, 0.78, 0.45)], [0.08], [3])
+        s.lights.append(area_light(s.n_tris + 2, (8.0, 6.0, 4.0), double_sided_light))
     s.camera = Camera(pos=(0.5, 0.5, -1.4), look=(0.5, 0.5, 0.5), up=(0.0, 1.0, 0.0), fov=40.0,
                       width=width, height=height)
     return s

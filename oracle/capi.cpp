@@ -328,6 +328,31 @@ void orc_bsdf_probe(uint32_t type, const float* params, int mode, const float* w
     out[11] = (float)r.stype; out[12] = (float)r.ncomp;
 }
 
+// Light sampling of light `light` of a scene with shading set, from n points (p, perr, n: float
+// triples): out 10 floats per point (probe_light_sample); light_pdf: one pdf per (point, wi).
+void orc_light_sample(void* h, uint32_t light, int mode, uint64_t n, const float* p, const float* perr,
+                      const float* nrm, const float* u0, const float* u1, float* out) {
+    OrcScene* s = static_cast<OrcScene*>(h);
+    s->shade.geom = &s->scene;
+    s->shade.bvh = &s->bvh;
+    for (uint64_t i = 0; i < n; ++i)
+        probe_light_sample(s->shade, light, (Mode)mode, V3(p[3 * i], p[3 * i + 1], p[3 * i + 2]),
+                           V3(perr[3 * i], perr[3 * i + 1], perr[3 * i + 2]),
+                           V3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]), u0[i], u1[i], out + 10 * i);
+}
+
+void orc_light_pdf(void* h, uint32_t light, int mode, uint64_t n, const float* p, const float* perr, const float* nrm,
+                   const float* wi, float* out) {
+    OrcScene* s = static_cast<OrcScene*>(h);
+    s->shade.geom = &s->scene;
+    s->shade.bvh = &s->bvh;
+    for (uint64_t i = 0; i < n; ++i)
+        out[i] = probe_light_pdf(s->shade, light, (Mode)mode, V3(p[3 * i], p[3 * i + 1], p[3 * i + 2]),
+                                 V3(perr[3 * i], perr[3 * i + 1], perr[3 * i + 2]),
+                                 V3(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]),
+                                 V3(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]));
+}
+
 // ------------------------------------------------ single-function probes
 // n inputs each; arrays are packed float triples.
 void orc_fresnel_conductor(uint64_t n, const float* cos_i, const float* eta, const float* k, float* out) {

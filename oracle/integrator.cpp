@@ -391,23 +391,102 @@ struct LightSample {
     bool vis = false;
 };
 
-// diffuse.rs:134-167 with shape.rs:17-34 and triangle.rs:522-559
+// sampling.rs:211-216
+inline V3 uniform_sample_sphere(Float u0, Float u1) {
+    Float z = 1.0f - 2.0f * u0;
+    Float r = std::sqrt(fmax_r(1.0f - z * z, 0.0f));
+    Float phi = 2.0f * PI * u1;
+    return V3(r * std::cos(phi), r * std::sin(phi), z);
+}
+
+// Sphere::sample (sphere.rs:328-354): a uniform point on the whole sphere (also a partial one).
+// The reference transforms the normal of the unreprojected point without normalising it, so in
+// REF its length is the radius (for a transform without scale); CORRECTED normalises it (pbrt-v3).
+void sphere_sample_area(const Sphere& sp, Float u0, Float u1, Mode mode, V3* p, V3* perr, V3* n) {
+    V3 obj = V3(0.0f, 0.0f, 0.0f) + sp.radius * uniform_sample_sphere(u0, u1);
+    V3 nn = xform_normal(sp.o2w, obj);
+    if (mode == MODE_CORRECTED) nn = normalize(nn);
+    if (sp.reverse_orientation) nn = nn * -1.0f;
+    obj = obj * (sp.radius / distance(obj, V3(0.0f, 0.0f, 0.0f)));
+    V3 oerr = gamma(5.0f) * vabs(obj);
+    *p = xform_point_perr(sp.o2w, obj, oerr, perr);
+    *n = nn;
+}
+
+// The centre test of sphere.rs:356-363: whether the origin a ray from `it` toward the centre would
+// have lies inside the sphere.
+inline bool sphere_contains_origin(const Sphere& sp, const Isect& it, V3* center) {
+    *center = xform_point(sp.o2w, V3(0.0f, 0.0f, 0.0f));
+    V3 o = offset_ray_origin(it.p, it.perr, it.n, *center - it.p);
+    return distance_squared(o, *center) <= sp.radius * sp.radius;
+}
+
+// sampling.rs:229-231
+inline Float uniform_cone_pdf(Float cos_max) { return 1.0f / (2.0f * PI * (1.0f - cos_max)); }
+
+// CORRECTED (pbrt-v3 Sphere::Sample(ref, u)) from outside: a direction uniform in the cone the
+// sphere subtends, and the point of the sphere it sees first. Spheres are taken unscaled.
+void sphere_sample_cone(const Sphere& sp, const V3& c, const V3& ref_p, Float u0, Float u1, V3* p, V3* perr, V3* n,
+                        Float* pdf) {
+    Float dc = distance(ref_p, c);
+    Float inv_dc = 1.0f / dc;
+    V3 wc = (c - ref_p) * inv_dc;
+    V3 wcx, wcy;
+    coordinate_system(wc, &wcx, &wcy);
+    Float sin_max = sp.radius * inv_dc;
+    Float sin_max2 = sin_max * sin_max;
+    Float inv_sin_max = 1.0f / sin_max;
+    Float cos_max = std::sqrt(fmax_r(0.0f, 1.0f - sin_max2));
+    Float cos_t = (cos_max - 1.0f) * u0 + 1.0f;
+    Float sin_t2 = 1.0f - cos_t * cos_t;
+    if (sin_max2 < 0.00068523f) {  // sin^2(1.5 deg): small cones in the sin^2 parameterisation
+        sin_t2 = sin_max2 * u0;
+        cos_t = std::sqrt(1.0f - sin_t2);
+    }
+    // angle alpha at the centre between -wc and the point the sampled direction meets
+    Float cos_a = sin_t2 * inv_sin_max + cos_t * std::sqrt(fmax_r(0.0f, 1.0f - sin_t2 * inv_sin_max * inv_sin_max));
+    Float sin_a = std::sqrt(fmax_r(0.0f, 1.0f - cos_a * cos_a));
+    Float phi = u1 * 2.0f * PI;
+    V3 nw = (sin_a * std::cos(phi)) * (-wcx) + (sin_a * std::sin(phi)) * (-wcy) + cos_a * (-wc);
+    *p = c + sp.radius * nw;
+    *perr = gamma(5.0f) * vabs(*p);
+    *n = sp.reverse_orientation ? nw * -1.0f : nw;
+    *pdf = uniform_cone_pdf(cos_max);
+}
+
+// diffuse.rs:134-167 with shape.rs:17-34 over triangle.rs:522-559 or sphere.rs:328-354
+// (CORRECTED spheres seen from outside: the cone sampling above)
 LightSample light_sample_point(const ShadeScene& sc, const AreaLight& l, const Isect& it, Float u0, Float u1,
                                Mode mode) {
     LightSample ls;
-    const TriVerts& tv = sc.geom->tris[l.prim];
     V3 p, perr, n;
     Float pdf;
-    tri_sample(tv, sc.geom->tri_flip[l.prim] != 0, u0, u1, &p, &perr, &n, &pdf);
-    V3 wi = p - it.p;
-    if (length_squared(wi) == 0.0f) {
-        pdf = 0.0f;
+    bool solid_angle = false;  // pdf already per solid angle
+    if (sc.geom->is_sphere(l.prim)) {
+        const Sphere& sp = sc.geom->spheres[l.prim - sc.geom->tris.size()];
+        V3 c;
+        if (mode == MODE_CORRECTED && !sphere_contains_origin(sp, it, &c)) {
+            sphere_sample_cone(sp, c, it.p, u0, u1, &p, &perr, &n, &pdf);
+            solid_angle = true;
+        } else {
+            sphere_sample_area(sp, u0, u1, mode, &p, &perr, &n);
+            pdf = 1.0f / sphere_area(sp);
+        }
     } else {
-        wi = normalize(wi);
-        // Q3: the reference replaces the area pdf by the solid-angle factor
-        Float conv = distance_squared(it.p, p) / abs_dot(n, -wi);
-        pdf = (mode == MODE_REF) ? conv : pdf * conv;
-        if (std::isinf(pdf)) pdf = 0.0f;
+        const TriVerts& tv = sc.geom->tris[l.prim];
+        tri_sample(tv, sc.geom->tri_flip[l.prim] != 0, u0, u1, &p, &perr, &n, &pdf);
+    }
+    if (!solid_angle) {
+        V3 wi = p - it.p;
+        if (length_squared(wi) == 0.0f) {
+            pdf = 0.0f;
+        } else {
+            wi = normalize(wi);
+            // Q3: the reference replaces the area pdf by the solid-angle factor
+            Float conv = distance_squared(it.p, p) / abs_dot(n, -wi);
+            pdf = (mode == MODE_REF) ? conv : pdf * conv;
+            if (std::isinf(pdf)) pdf = 0.0f;
+        }
     }
     if (pdf == 0.0f || length_squared(p - it.p) == 0.0f) return ls;
     ls.wi = normalize(p - it.p);
@@ -419,8 +498,25 @@ LightSample light_sample_point(const ShadeScene& sc, const AreaLight& l, const I
     return ls;
 }
 
-// shape.rs:40-57 for the light's triangle
+// shape.rs:40-57 on the light's triangle; sphere.rs:356-373 on a sphere (the cone pdf from
+// outside, whatever wi, as the reference and pbrt-v3 have it)
 Float light_point_pdf(const ShadeScene& sc, const AreaLight& l, const Isect& it, const V3& wi, Mode mode) {
+    if (sc.geom->is_sphere(l.prim)) {
+        const Sphere& sp = sc.geom->spheres[l.prim - sc.geom->tris.size()];
+        V3 c;
+        if (!sphere_contains_origin(sp, it, &c)) {
+            Float sin2 = sp.radius * sp.radius / distance_squared(it.p, c);
+            Float cos_max = std::sqrt(fmax_r(0.0f, 1.0f - sin2));
+            return uniform_cone_pdf(cos_max);
+        }
+        Ray r = spawn_ray(it, wi, mode);
+        Float t;
+        Isect hit;
+        if (!sphere_intersect(sp, r, &t, nullptr, &hit)) return 0.0f;
+        Float pdf = distance_squared(it.p, hit.p) / (abs_dot(hit.n, -wi) * sphere_area(sp));
+        if (std::isinf(pdf)) pdf = 0.0f;
+        return pdf;
+    }
     Ray r = spawn_ray(it, wi, mode);
     const TriVerts& tv = sc.geom->tris[l.prim];
     Float t;
@@ -755,6 +851,30 @@ Ray probe_spawn_ray(const V3& p, const V3& perr, const V3& n, Float time, const 
 Ray probe_spawn_ray_to(const V3& p0, const V3& e0, const V3& n0, Float time, const V3& p1, const V3& e1,
                        const V3& n1, V3* target) {
     return spawn_ray_to(p0, e0, n0, time, p1, e1, n1, target);
+}
+
+void probe_light_sample(const ShadeScene& sc, uint32_t light, Mode mode, const V3& p, const V3& perr, const V3& n,
+                        Float u0, Float u1, Float out[10]) {
+    Isect it{};
+    it.p = p;
+    it.perr = perr;
+    it.n = n;
+    it.ns = n;
+    LightSample ls = light_sample_point(sc, sc.lights[light], it, u0, u1, mode);
+    out[0] = ls.lp.x; out[1] = ls.lp.y; out[2] = ls.lp.z;
+    out[3] = ls.ln.x; out[4] = ls.ln.y; out[5] = ls.ln.z;
+    out[6] = ls.pdf;
+    out[7] = ls.radiance[0]; out[8] = ls.radiance[1]; out[9] = ls.radiance[2];
+}
+
+Float probe_light_pdf(const ShadeScene& sc, uint32_t light, Mode mode, const V3& p, const V3& perr, const V3& n,
+                      const V3& wi) {
+    Isect it{};
+    it.p = p;
+    it.perr = perr;
+    it.n = n;
+    it.ns = n;
+    return light_point_pdf(sc, sc.lights[light], it, wi, mode);
 }
 
 }  // namespace orc

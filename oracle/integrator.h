@@ -142,6 +142,13 @@ void probe_sample11(Mode mode, Float cos_theta, Float u1, Float u2, Float out[2]
 V3 probe_tr_sample(Mode mode, Float alpha, const V3& wo, Float u1, Float u2);
 // Interaction::spawn_ray (interaction.rs:115-121) from (p, p_error, n, time) in direction d.
 Ray probe_spawn_ray(const V3& p, const V3& perr, const V3& n, Float time, const V3& d, Mode mode);
+// DiffuseAreaLight::sample_point (diffuse.rs:134-167) of light `light` seen from (p, perr, n):
+// out = point[3], normal[3], pdf (solid angle), radiance toward the point[3]; pdf 0: no sample.
+void probe_light_sample(const ShadeScene& sc, uint32_t light, Mode mode, const V3& p, const V3& perr, const V3& n,
+                        Float u0, Float u1, Float out[10]);
+// The light's point_pdf (shape.rs:40-57, sphere.rs:356-373) of direction wi from (p, perr, n).
+Float probe_light_pdf(const ShadeScene& sc, uint32_t light, Mode mode, const V3& p, const V3& perr, const V3& n,
+                      const V3& wi);
 // Interaction::spawn_ray_to_it (interaction.rs:132-143); *target receives the offset target point.
 Ray probe_spawn_ray_to(const V3& p0, const V3& e0, const V3& n0, Float time, const V3& p1, const V3& e1,
                        const V3& n1, V3* target);

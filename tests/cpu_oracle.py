@@ -45,6 +45,8 @@ def lib() -> C.CDLL:
             "orc_render": (None, [P, P, P] + [C.c_float] * 7 + [C.c_uint32] * 4 + [C.c_float, C.c_uint64] +
                            [C.c_uint32] * 7 + [C.c_int, P, P]),
             "orc_bsdf_probe": (None, [C.c_uint32, P, C.c_int, P, P, C.c_float, C.c_float, P]),
+            "orc_light_sample": (None, [P, C.c_uint32, C.c_int, C.c_uint64, P, P, P, P, P, P]),
+            "orc_light_pdf": (None, [P, C.c_uint32, C.c_int, C.c_uint64, P, P, P, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -151,6 +153,22 @@ class OracleScene:
                          max_depth, rr, seed, mode, x0, y0, x1, y1, integrator, strategy, th, rgb.ctypes.data,
                          w.ctypes.data)
         return rgb, w
+
+    def light_sample(self, light: int, mode: int, p, perr, n, u0, u1) -> np.ndarray:
+        """DiffuseAreaLight::sample_point from points (p, perr, n): (m, 10) = light point, its normal,
+        solid-angle pdf, radiance toward the point (set_shading first)."""
+        a = [np.ascontiguousarray(x, np.float32) for x in (p, perr, n, u0, u1)]
+        m = a[3].size
+        out = np.empty((m, 10), np.float32)
+        self._keep += a
+        lib().orc_light_sample(self.h, light, mode, m, *[x.ctypes.data for x in a], out.ctypes.data)
+        return out
+
+    def light_pdf(self, light: int, mode: int, p, perr, n, wi) -> np.ndarray:
+        a = [np.ascontiguousarray(x, np.float32).reshape(-1, 3) for x in (p, perr, n, wi)]
+        out = np.empty(a[0].shape[0], np.float32)
+        lib().orc_light_pdf(self.h, light, mode, a[0].shape[0], *[x.ctypes.data for x in a], out.ctypes.data)
+        return out
 
     def surface(self, prim_id: int, ray):
         """(ok, 18 floats: t, p, p_error, n, ns, shading dpdu, u, v) of primitive prim_id on a ray."""
