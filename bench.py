@@ -294,12 +294,10 @@ def mlp_leg(ctx) -> dict:
     2^16-sample batches. Forward = 2 * 18,944 flop per sample; a training step = forward + input
     and weight gradients = 3 forward's worth. Against the dense BF16 matrix-core peak."""
     import torch
-    from bonsai_amd.guide import GuideMLP
-    sys.path.insert(0, str(ROOT / "oracle"))
-    import guide_mlp
+    from bonsai_amd.guide import GuideMLP, init_params
 
     n_inf, n_tr = 1 << 22, 1 << 16
-    g = GuideMLP(ctx, guide_mlp.init_params(3).astype(np.float32), n_tr)
+    g = GuideMLP(ctx, init_params(3), n_tr)
     gen = torch.Generator(device="cpu").manual_seed(5)
     x = torch.rand((n_inf, 12), generator=gen).to("cuda:0")
     out = torch.empty((n_inf, 40), dtype=torch.float32, device="cuda:0")
@@ -357,11 +355,9 @@ def guiding_leg(ctx, spp: int) -> dict:
     warm-up render and measures with the network frozen. The guided estimators use the settings
     above (GUIDE_*)."""
     from bonsai_amd import GpuBvh
-    from bonsai_amd.guide import GuideMLP
+    from bonsai_amd.guide import GuideMLP, init_params
     from bonsai_amd.integrator import MODE_CORRECTED, Film, GpuPathIntegrator, GuidedPathIntegrator
     from bonsai_amd.scenes import door_gap_room
-    sys.path.insert(0, str(ROOT / "oracle"))
-    import guide_mlp  # initial parameters only (He-uniform, seeded)
 
     scene = door_gap_room()
     bvh = GpuBvh(ctx, scene)
@@ -370,7 +366,7 @@ def guiding_leg(ctx, spp: int) -> dict:
            "triangles": scene.n_tris, "spp_per_render": spp, "guided_bounces": GUIDE_BOUNCES,
            "guided_rr_threshold": GUIDE_RR, "guided_adam_steps_per_pass": GUIDE_STEPS,
            "warmup_spp": GUIDE_WARMUP_FACTOR * spp, "plain_best_rr_threshold": PLAIN_BEST_RR}
-    guide = GuideMLP(ctx, guide_mlp.init_params(3).astype(np.float32), 65536)
+    guide = GuideMLP(ctx, init_params(3), 65536)
 
     def render(g, sel, train, seed, rr, n=spp, first=0):
         # training renders call once per 2 samples (the network trains after every call); the
@@ -399,7 +395,7 @@ def guiding_leg(ctx, spp: int) -> dict:
                 ("guided_training", guide, 0.5, True, False, GUIDE_RR), ("guided", guide, 0.5, False, True, GUIDE_RR)]
     for name, g, sel, train, warm, rr in variants:
         if g is not None:
-            g.set_params(guide_mlp.init_params(3).astype(np.float32))
+            g.set_params(init_params(3))
         warm_s = 0.0
         if warm:
             _, warm_s, _ = render(g, sel, True, 30, rr, n=GUIDE_WARMUP_FACTOR * spp, first=1 << 20)

@@ -192,6 +192,9 @@ SIGNATURES = {
     "bg_guide_set_params": (C.c_int, [P, P]),
     "bg_guide_get_params": (C.c_int, [P, P, P, P]),
     "bg_guide_set_bounds": (C.c_int, [P, P, P]),
+    "bg_guide_init_params": (C.c_int, [C.c_uint64, P]),
+    "bg_guide_snapshot": (C.c_int, [P, P, C.POINTER(C.c_uint64)]),
+    "bg_guide_restore": (C.c_int, [P, P, C.c_uint64]),
     "bg_guide_eval": (C.c_int, [P, P, C.c_uint64, P]),
     "bg_guide_train": (C.c_int, [P, P, P, C.c_uint64, C.c_float, C.POINTER(C.c_double), P]),
     "bg_guide_eval_device": (C.c_int, [P, P, C.c_uint64, P, C.POINTER(C.c_double)]),

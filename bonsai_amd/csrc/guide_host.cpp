@@ -1,12 +1,14 @@
 // Host side of the path-guiding MLP: parameter upload, batched evaluation and
 // one Adam step per call (kernels in guide.hip). The reference crate has no
 // path guiding; these entry points have no counterpart there.
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <string>
 #include <vector>
 
 #include "bg_internal.h"
+#include "../../include/bonsai_scenes.h"
 
 namespace {
 
@@ -136,6 +138,113 @@ int bg_guide_set_bounds(bg_guide* g, const float lo[3], const float hi[3]) {
     for (int k = 0; k < 3; ++k) {
         g->box.lo[k] = lo[k];
         g->box.hi[k] = hi[k];
+    }
+    return BG_OK;
+}
+
+int bg_guide_init_params(uint64_t seed, float* params) {
+    if (!params) return BG_ERR_INVALID;
+    // layers 64 -> 64 (x4) and 64 -> 40: weights (out x in, row-major), then biases
+    const int outs[5] = {64, 64, 64, 64, 40};
+    uint64_t k = 0;
+    for (int l = 0; l < 5; ++l) {
+        const float lim = std::sqrt(6.0f / 64.0f);
+        for (int i = 0; i < outs[l] * 64; ++i, ++k) params[k] = (2.0f * bgx_uniform(seed, k) - 1.0f) * lim;
+        for (int i = 0; i < outs[l]; ++i, ++k) params[k] = 0.0f;
+    }
+    return k == bg::kParamsTotal ? BG_OK : BG_ERR_INVALID;
+}
+
+namespace {
+// Snapshot layout: header, then master / m / v (BG_GUIDE_PARAMS floats each), then the replay
+// ring's cursor (4 words) and contents (48 + 16 bytes per sample of its capacity).
+struct SnapshotHeader {
+    char magic[8];
+    uint64_t version;
+    uint64_t step;
+    float lo[3], hi[3];
+    uint64_t replay_cap;
+    uint64_t bytes;
+};
+constexpr char kSnapMagic[8] = {'B', 'G', 'G', 'U', 'I', 'D', 'E', '1'};
+uint64_t snapshot_bytes(uint64_t replay_cap) {
+    return sizeof(SnapshotHeader) + 3ull * bg::kParamsTotal * 4 + (replay_cap ? 16 + 64ull * replay_cap : 0);
+}
+}  // namespace
+
+int bg_guide_snapshot(bg_guide* g, void* buf, uint64_t* size) {
+    if (!g || !size) return BG_ERR_INVALID;
+    bg_ctx* ctx = g->ctx;
+    const uint64_t cap = g->replay.state ? g->replay.cap : 0;
+    const uint64_t need = snapshot_bytes(cap);
+    if (!buf) {
+        *size = need;
+        return BG_OK;
+    }
+    if (*size < need) return fail(ctx, BG_ERR_INVALID, "bg_guide_snapshot: buffer too small");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    char* p = static_cast<char*>(buf);
+    SnapshotHeader h{};
+    std::memcpy(h.magic, kSnapMagic, 8);
+    h.version = 1;
+    h.step = (uint64_t)g->step;
+    for (int k = 0; k < 3; ++k) {
+        h.lo[k] = g->box.lo[k];
+        h.hi[k] = g->box.hi[k];
+    }
+    h.replay_cap = cap;
+    h.bytes = need;
+    std::memcpy(p, &h, sizeof(h));
+    char* q = p + sizeof(h);
+    const size_t pb = bg::kParamsTotal * 4;
+    HIP_TRY(ctx, hipMemcpyAsync(q, g->dev.master, pb, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipMemcpyAsync(q + pb, g->dev.m, pb, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipMemcpyAsync(q + 2 * pb, g->dev.v, pb, hipMemcpyDeviceToHost, s));
+    q += 3 * pb;
+    if (cap) {
+        HIP_TRY(ctx, hipMemcpyAsync(q, g->replay.state, 16, hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipMemcpyAsync(q + 16, g->replay.in, 48 * cap, hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipMemcpyAsync(q + 16 + 48 * cap, g->replay.tgt, 16 * cap, hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    *size = need;
+    return BG_OK;
+}
+
+int bg_guide_restore(bg_guide* g, const void* buf, uint64_t size) {
+    if (!g || !buf) return BG_ERR_INVALID;
+    bg_ctx* ctx = g->ctx;
+    SnapshotHeader h;
+    if (size < sizeof(h)) return fail(ctx, BG_ERR_INVALID, "bg_guide_restore: not a guide snapshot");
+    std::memcpy(&h, buf, sizeof(h));
+    if (std::memcmp(h.magic, kSnapMagic, 8) != 0 || h.version != 1 || h.replay_cap > 0xFFFFFFFFull ||
+        h.bytes != snapshot_bytes(h.replay_cap) || size < h.bytes)
+        return fail(ctx, BG_ERR_INVALID, "bg_guide_restore: not a guide snapshot of this version");
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const char* q = static_cast<const char*>(buf) + sizeof(h);
+    const size_t pb = bg::kParamsTotal * 4;
+    HIP_TRY(ctx, hipMemcpyAsync(g->dev.master, q, pb, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipMemcpyAsync(g->dev.m, q + pb, pb, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipMemcpyAsync(g->dev.v, q + 2 * pb, pb, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, bg::guide_refresh(g->dev, s));
+    q += 3 * pb;
+    if (h.replay_cap) {
+        const int rc = bg::guide_ensure_replay(g, (uint32_t)h.replay_cap);
+        if (rc) return rc;
+        HIP_TRY(ctx, hipMemcpyAsync(g->replay.state, q, 16, hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(g->replay.in, q + 16, 48 * h.replay_cap, hipMemcpyHostToDevice, s));
+        HIP_TRY(ctx, hipMemcpyAsync(g->replay.tgt, q + 16 + 48 * h.replay_cap, 16 * h.replay_cap,
+                                    hipMemcpyHostToDevice, s));
+    } else if (g->replay.state) {
+        HIP_TRY(ctx, hipMemsetAsync(g->replay.state, 0, 16, s));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    g->step = (int)h.step;
+    for (int k = 0; k < 3; ++k) {
+        g->box.lo[k] = h.lo[k];
+        g->box.hi[k] = h.hi[k];
     }
     return BG_OK;
 }

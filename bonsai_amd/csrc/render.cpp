@@ -187,6 +187,9 @@ hipError_t trace_pass(const bg_scene* s, bg_ctx* ctx, const float4* rays, uint4*
 
 }  // namespace
 
+// The guide's replay ring of `cap` samples (kept when the capacity is unchanged).
+int bg::guide_ensure_replay(bg_guide* g, uint32_t cap) { return ensure_replay(g, cap); }
+
 extern "C" {
 
 int bg_film_create(bg_ctx* ctx, uint32_t width, uint32_t height, uint32_t flags, bg_film** out) {

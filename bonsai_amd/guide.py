@@ -28,8 +28,18 @@ def pack_samples(p, wo, n) -> np.ndarray:
     return out
 
 
+def init_params(seed: int) -> np.ndarray:
+    """Seeded initial parameters (bg_guide_init_params): He-uniform weights, zero biases."""
+    out = np.empty(N_PARAMS, np.float32)
+    N.check(N.lib().bg_guide_init_params(int(seed), out.ctypes.data))
+    return out
+
+
 class GuideMLP:
-    def __init__(self, ctx: Context, params: np.ndarray, max_batch: int):
+    def __init__(self, ctx: Context, params: np.ndarray | int, max_batch: int):
+        """params: BG_GUIDE_PARAMS floats, or an int seed for init_params."""
+        if isinstance(params, (int, np.integer)):
+            params = init_params(int(params))
         params = np.ascontiguousarray(params, np.float32)
         if params.shape != (N_PARAMS,):
             raise ValueError(f"params must have shape ({N_PARAMS},)")
@@ -59,6 +69,19 @@ class GuideMLP:
         p, m, v = (np.empty(N_PARAMS, np.float32) for _ in range(3))
         N.check(N.lib().bg_guide_get_params(self.h, p.ctypes.data, m.ctypes.data, v.ctypes.data), self.ctx.h)
         return p, m, v
+
+    def snapshot(self) -> bytes:
+        """Parameters, Adam state, position box and replay ring (bg_guide_snapshot)."""
+        size = C.c_uint64(0)
+        N.check(N.lib().bg_guide_snapshot(self.h, None, C.byref(size)), self.ctx.h)
+        buf = C.create_string_buffer(size.value)
+        N.check(N.lib().bg_guide_snapshot(self.h, buf, C.byref(size)), self.ctx.h)
+        return buf.raw[:size.value]
+
+    def restore(self, snap: bytes) -> None:
+        """Continue from a snapshot (bg_guide_restore): later calls give bit-identical results."""
+        buf = C.create_string_buffer(snap, len(snap))
+        N.check(N.lib().bg_guide_restore(self.h, buf, len(snap)), self.ctx.h)
 
     def set_bounds(self, lo, hi) -> None:
         lo = np.ascontiguousarray(lo, np.float32)

@@ -423,6 +423,22 @@ int bg_guide_set_params(bg_guide* guide, const float* params);
 int bg_guide_get_params(bg_guide* guide, float* params, float* adam_m, float* adam_v);
 /* Box the positions are normalised to (each axis maps to [0, 1]). */
 int bg_guide_set_bounds(bg_guide* guide, const float lo[3], const float hi[3]);
+/* Initial parameters for bg_guide_create / bg_guide_set_params (host only, no
+ * device needed): He-uniform weights, each U(-sqrt(6 / fan_in), sqrt(6 /
+ * fan_in)) from the counter-based uniform bgx_uniform(seed, parameter index)
+ * (24 random bits), and zero biases. params: BG_GUIDE_PARAMS floats. */
+int bg_guide_init_params(uint64_t seed, float* params);
+/* Snapshot of everything a guided render's training carries from one call to
+ * the next: the FP32 parameters, Adam's moments and step count, the position
+ * box and the replay ring of training samples (its capacity, contents and
+ * cursor). bg_guide_restore of a snapshot into any guide of the same context
+ * (its max_batch at least the snapshot's minibatch use) makes the following
+ * calls give bit-identical results to those the snapshotted guide would have
+ * given. With buf NULL *size receives the byte count; otherwise *size is
+ * buf's capacity on entry and the byte count on return (BG_ERR_INVALID if the
+ * capacity is short). No counterpart in the reference (it has no guiding). */
+int bg_guide_snapshot(bg_guide* guide, void* buf, uint64_t* size);
+int bg_guide_restore(bg_guide* guide, const void* buf, uint64_t size);
 /* samples: n x 12 floats (p.xyz, -, wo.xyz, -, n.xyz, -); out: n x 40. */
 int bg_guide_eval(bg_guide* guide, const float* samples, uint64_t n, float* out);
 /* One Adam step on the batch mean of -weight * log p(direction).

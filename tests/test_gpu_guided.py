@@ -174,3 +174,43 @@ def test_growing_path_buffers_keeps_the_guiding_buffers(ctx):
     gi.render_into(bvh, film, first_sample=4)
     after = free_bytes()
     assert after == before, (before, after)
+
+
+def test_snapshot_and_restore_resume_bit_for_bit(ctx):
+    """A guided render trained for two calls, snapshotted, then continued for two more, equals a
+    fresh guide restored from the snapshot and given the same two calls: film sums, parameters
+    and Adam moments bit for bit."""
+    from bonsai_amd.guide import GuideMLP as GM
+    s = cornell_box(width=64, height=64)
+    bvh = GpuBvh(ctx, s)
+
+    def integ(g):
+        return GuidedPathIntegrator(ctx, s.camera, spp=2, max_depth=5, guide=g, train=True, train_paths=8192,
+                                    steps=4, batch=4096)
+
+    g1 = GM(ctx, 7, 65536)
+    warm = Film(ctx, 64, 64)
+    for k in range(2):
+        integ(g1).render_into(bvh, warm, spp=2, first_sample=2 * k)
+    snap = g1.snapshot()
+
+    def continue_from(g):
+        film = Film(ctx, 64, 64)
+        for k in range(2, 4):
+            integ(g).render_into(bvh, film, spp=2, first_sample=2 * k)
+        return film.sums(), g.params()
+
+    (a_rgb, a_w), a_par = continue_from(g1)
+    g2 = GM(ctx, 99, 65536)
+    g2.restore(snap)
+    (b_rgb, b_w), b_par = continue_from(g2)
+    np.testing.assert_array_equal(b_rgb, a_rgb)
+    np.testing.assert_array_equal(b_w, a_w)
+    for x, y in zip(a_par, b_par):
+        np.testing.assert_array_equal(x, y)
+    # without the snapshot the same calls give another film
+    g3 = GM(ctx, 99, 65536)
+    (c_rgb, _), _ = continue_from(g3)
+    assert not np.array_equal(c_rgb, a_rgb)
+    with pytest.raises(Exception):
+        g3.restore(b"not a snapshot" * 10)
