@@ -110,17 +110,19 @@ def test_refill_traversal_with_per_ray_roots(ctx):
         ctx.set_launch(*saved)
 
 
-def test_compacted_batch_equals_per_scene(ctx):
+@pytest.mark.parametrize("variant", [0, 2])
+def test_compacted_batch_equals_per_scene(ctx, variant):
     """A batch whose passes cover over 1M path slots and hold a scene with spheres (so the launches do
     not refill) compacts its traversal launches, per-ray root records included; each item's film
-    equals its single-scene render (whose smaller launches do not compact)."""
+    equals its single-scene render (whose smaller launches do not compact). Variant 2 takes the
+    compacted rays from the atomic work queue, reading their count from the device."""
     scenes = [path_tracer_scene(512, 256, small=True), cornell_box(width=512, height=256, with_plastic=True),
               batch_scene(1, scale=0.02)]
     bvhs = [GpuBvh(ctx, s) for s in scenes]
     items = [(b, s.camera, 300 + k) for k, (b, s) in enumerate(zip(bvhs, scenes))]
     ref = per_item(ctx, items, 4, 5, MODE_CORRECTED)
     saved, saved_c = ctx.launch(), ctx.compaction()
-    ctx.set_launch(256, 8, 32, 0, compact=True)
+    ctx.set_launch(256, 8, 32, variant, compact=True)
     try:
         br = BatchRenderer(ctx, items)
         total = sum(c.width * c.height for _, c, _ in items)

@@ -1,0 +1,172 @@
+"""The configurations at their own sizes on the GPU, against the oracle where it
+can keep up:
+
+- Cornell box (256 x 256, 16 spp, depth 5, both semantics, one- and
+  double-sided light): the whole film against the oracle on the same streams;
+- path-tracer scene kind, reference semantics: the pixels that carry almost
+  all of the image's energy (paths through near-normal glossy bounces, whose
+  pdf goes to 0 with G1 = Lambda, Q6) at the same places with the same values,
+  and the same image mean;
+- scene batch (64 seeded scenes of 1k-500k triangles at 128^2-1024^2, 16 spp,
+  depth 6): every item's film from one batched loop equal to its own render bit
+  for bit, and the two smallest items against the oracle;
+- door-gap interior (1,513,386 triangles, 1920 x 1080, depth 10): a window
+  of the plain corrected render against the oracle on the same streams, and the
+  guided render's image mean equal to the plain one's within its statistical
+  error over the whole film.
+"""
+import numpy as np
+import pytest
+
+import cpu_oracle as O
+from bonsai_amd import GpuBvh
+from bonsai_amd.guide import GuideMLP
+from bonsai_amd.integrator import (MODE_CORRECTED, MODE_REF, BatchRenderer, Film, GpuPathIntegrator,
+                                   GuidedPathIntegrator, rel_mse, resolve)
+from bonsai_amd.scenes import batch_scene, cornell_box, door_gap_room, path_tracer_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_for(scene, bvh):
+    o = O.OracleScene(scene)
+    nodes, slots = bvh.bvh()
+    o.use_bvh(nodes, slots)
+    o.set_shading(scene)
+    return o
+
+
+def gpu_sums(ctx, integ, bvh, cam, **kw):
+    film = Film(ctx, cam.width, cam.height)
+    integ.render_into(bvh, film, **kw)
+    rgb, w = film.sums()
+    film.close()
+    return rgb.astype(np.float64), w.astype(np.float64)
+
+
+@pytest.mark.parametrize("mode", [MODE_REF, MODE_CORRECTED])
+@pytest.mark.parametrize("double_sided", [False, True])
+def test_cornell_configuration_matches_oracle(ctx, mode, double_sided):
+    s = cornell_box(double_sided_light=double_sided)  # 256 x 256
+    assert (s.camera.width, s.camera.height) == (256, 256)
+    bvh = GpuBvh(ctx, s)
+    gi = GpuPathIntegrator(ctx, s.camera, spp=16, max_depth=5, mode=mode, seed=11)
+    rgb, w = gpu_sums(ctx, gi, bvh, s.camera)
+    orgb, ow = oracle_for(s, bvh).render(s, 16, 5, mode=mode, seed=11)
+    assert np.array_equal(w, ow)
+    img, ref = resolve(rgb, w), resolve(orgb, ow)
+    err = rel_mse(img, ref)
+    print(f"Cornell 256^2 16 spp mode {mode} double-sided {double_sided}: relMSE {err:.2e}, "
+          f"mean {img.mean():.5f} / {ref.mean():.5f}")
+    assert err < 1e-4
+    assert abs(img.mean() / ref.mean() - 1.0) < 1e-3
+
+
+def test_reference_semantics_outliers_match_the_oracle(ctx):
+    """With G1 = Lambda (Q6) the reference's microfacet pdf vanishes toward normal incidence, and with
+    the camera ray re-intersected instead of continued (Q1) a glossy vertex can repeat its f cos / pdf
+    factor, so a few paths carry values many orders above the rest and dominate the image mean (the
+    guards of integrator.rs:92-117 zero only NaN, negative and infinite samples). Same streams: the
+    GPU finds the same dominant pixels with the same values, so the means agree too."""
+    s = path_tracer_scene(160, 90, small=True)
+    bvh = GpuBvh(ctx, s)
+    gi = GpuPathIntegrator(ctx, s.camera, spp=4, max_depth=8, mode=MODE_REF, seed=9)
+    rgb, w = gpu_sums(ctx, gi, bvh, s.camera)
+    orgb, ow = oracle_for(s, bvh).render(s, 4, 8, mode=MODE_REF, seed=9)
+    img, ref = resolve(rgb, w), resolve(orgb, ow)
+    lum = img.sum(-1).ravel()
+    rlum = ref.sum(-1).ravel()
+    k = 20
+    top_g, top_o = np.argsort(lum)[-k:], np.argsort(rlum)[-k:]
+    share = rlum[top_o].sum() / rlum.sum()
+    print(f"reference semantics: mean GPU {img.mean():.6g} oracle {ref.mean():.6g}; top {k} pixels hold "
+          f"{share:.3f} of the oracle's energy, max {rlum.max():.6g} / {lum.max():.6g}; "
+          f"common top pixels {len(set(top_g) & set(top_o))}")
+    assert len(set(top_g) & set(top_o)) >= k - 2
+    np.testing.assert_allclose(lum[top_o], rlum[top_o], rtol=1e-2)
+    assert abs(img.mean() / ref.mean() - 1.0) < 1e-2
+
+
+@pytest.fixture(scope="module")
+def batch64(ctx):
+    scenes = [batch_scene(i) for i in range(64)]
+    bvhs = [GpuBvh(ctx, s) for s in scenes]
+    return scenes, bvhs
+
+
+@pytest.mark.parametrize("mode", [MODE_REF, MODE_CORRECTED])
+def test_full_scene_batch_equals_per_scene_renders(ctx, batch64, mode):
+    scenes, bvhs = batch64
+    items = [(b, s.camera, 1000 + k) for k, (b, s) in enumerate(zip(bvhs, scenes))]
+    tris = [s.n_tris for s in scenes]
+    print(f"batch: {sum(tris)} triangles ({min(tris)}-{max(tris)}), "
+          f"{sum(s.camera.width * s.camera.height for s in scenes)} pixels", flush=True)
+    br = BatchRenderer(ctx, items)
+    br.render(16, 6, mode=mode, autotune=True)
+    for k, (bvh, cam, seed) in enumerate(items):
+        rgb, w = gpu_sums(ctx, GpuPathIntegrator(ctx, cam, spp=16, max_depth=6, mode=mode, seed=seed), bvh, cam)
+        brgb, bw = br.sums(k)
+        np.testing.assert_array_equal(bw, w.astype(np.float32))
+        np.testing.assert_array_equal(brgb, rgb.astype(np.float32))
+    # the two smallest scenes against the oracle
+    for k in np.argsort(tris)[:2]:
+        s, bvh = scenes[k], bvhs[k]
+        orgb, ow = oracle_for(s, bvh).render(s, 16, 6, mode=mode, seed=1000 + int(k))
+        brgb, bw = br.sums(int(k))
+        assert np.array_equal(bw.astype(np.float64), ow)
+        err = rel_mse(resolve(brgb.astype(np.float64), bw.astype(np.float64)), resolve(orgb, ow))
+        print(f"batch item {k} ({tris[k]} triangles, {s.camera.width}^2): relMSE vs oracle {err:.2e}")
+        assert err < 1e-4
+    br.close()
+
+
+@pytest.fixture(scope="module")
+def door_gap(ctx):
+    s = door_gap_room()
+    return s, GpuBvh(ctx, s)
+
+
+def test_door_gap_window_matches_oracle(ctx, door_gap):
+    s, bvh = door_gap
+    assert s.n_tris == 1513386 and (s.camera.width, s.camera.height) == (1920, 1080)
+    win = (800, 480, 960, 608)
+    gi = GpuPathIntegrator(ctx, s.camera, spp=4, max_depth=10, mode=MODE_CORRECTED, seed=21)
+    rgb, w = gpu_sums(ctx, gi, bvh, s.camera, window=win)
+    x0, y0, x1, y1 = win
+    rgb, w = rgb[y0:y1, x0:x1], w[y0:y1, x0:x1]
+    orgb, ow = oracle_for(s, bvh).render(s, 4, 10, mode=MODE_CORRECTED, seed=21, window=win)
+    # the oracle's window sums include the footprint of samples from the pixels next to it, as the film
+    # does inside a window; compare pixels strictly inside
+    inner = (slice(1, -1), slice(1, -1))
+    assert np.array_equal(w[inner], ow[inner])
+    img, ref = resolve(rgb[inner], w[inner]), resolve(orgb[inner], ow[inner])
+    err = rel_mse(img, ref)
+    print(f"door gap window {win}: relMSE {err:.2e}, mean {img.mean():.5f} / {ref.mean():.5f}")
+    assert err < 1e-4
+
+
+def test_door_gap_guided_mean_equals_plain(ctx, door_gap):
+    """Over the whole 1920 x 1080 film the guided estimator (network trained during a warm-up
+    render, then frozen) and the plain path give the same mean: per pixel both are unbiased for the
+    same value and their noise is independent, so the mean of the difference image is zero within
+    four of its standard errors."""
+    s, bvh = door_gap
+    W, H = s.camera.width, s.camera.height
+    plain = GpuPathIntegrator(ctx, s.camera, spp=8, max_depth=10, mode=MODE_CORRECTED, seed=41, rr_threshold=0.0)
+    prgb, pw = gpu_sums(ctx, plain, bvh, s.camera)
+    guide = GuideMLP(ctx, 3, 65536)
+    warm = GuidedPathIntegrator(ctx, s.camera, spp=2, max_depth=10, guide=guide, seed=42, train=True, steps=48,
+                                bounces=4, rr_threshold=0.0)
+    film = Film(ctx, W, H)
+    for k in range(8):
+        warm.render_into(bvh, film, spp=2, first_sample=(1 << 20) + 2 * k)
+    film.close()
+    guided = GuidedPathIntegrator(ctx, s.camera, spp=8, max_depth=10, guide=guide, seed=43, train=False,
+                                  bounces=4, rr_threshold=0.0)
+    grgb, gw = gpu_sums(ctx, guided, bvh, s.camera)
+    d = (resolve(grgb, gw) - resolve(prgb, pw)).ravel()
+    se = d.std() / np.sqrt(d.size)
+    pm = resolve(prgb, pw).mean()
+    print(f"door gap 1080p: plain mean {pm:.6f}, guided - plain {d.mean():.3e} (standard error {se:.3e})")
+    assert abs(d.mean()) < 4 * se
+    assert se < 0.05 * pm

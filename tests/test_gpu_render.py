@@ -189,3 +189,26 @@ def test_compaction_does_not_change_the_film(ctx, mode, size):
         np.testing.assert_array_equal(rgb1, rgb0)
         np.testing.assert_array_equal(w1, w0)
         assert r1 == r0
+
+
+@pytest.mark.parametrize("value", [float("nan"), -2.0, float("inf")])
+@pytest.mark.parametrize("both", [False, True])
+def test_sample_guards_match_the_oracle(ctx, value, both):
+    """integrator.rs:92-117 on the GPU: a light emitting NaN, negative or infinite radiance. Samples of
+    NaN radiance, luminance below -1e-5 or infinite luminance go to the film as black with their
+    filter weight; the film's sums equal the oracle's and hold no NaN or infinity."""
+    from test_oracle_integrator import poisoned_cornell
+    s = poisoned_cornell(value, both=both, res=48)
+    bvh = GpuBvh(ctx, s)
+    for mode in (MODE_REF, MODE_CORRECTED):
+        film = Film(ctx, 48, 48)
+        GpuPathIntegrator(ctx, s.camera, spp=8, max_depth=5, mode=mode, seed=3).render_into(bvh, film)
+        rgb, w = film.sums()
+        film.close()
+        ref, wref = oracle_image(s, 8, 5, mode, seed=3, bvh=bvh)
+        assert np.all(np.isfinite(rgb)) and np.all(w == 8.0)
+        assert np.array_equal(w, wref.astype(np.float32))
+        img = resolve(rgb.astype(np.float64), w.astype(np.float64))
+        assert rel_mse(img, ref) < 1e-4, (value, both, mode)
+        if both and value != -2.0:
+            assert np.all(rgb == 0.0)

@@ -92,3 +92,43 @@ def test_every_sample_lands_in_its_pixel():
     o.set_shading(s)
     _, w = o.render(s, 3, 1, mode=0)
     assert w.sum() == 20 * 20 * 3
+
+
+def poisoned_cornell(value, both=False, res=24):
+    """The Cornell box with light triangle 11 (and with `both` also 10) emitting `value` (NaN, a
+    negative or an infinite radiance): integrator.rs:92-117 sets every camera sample whose radiance
+    is NaN, of luminance below -1e-5 or infinite to black, and still counts its filter weight."""
+    s = cornell_box(double_sided_light=True, width=res, height=res)
+    s.lights[1].radiance[:] = [value] * 3
+    if both:
+        s.lights[0].radiance[:] = [value] * 3
+    return s
+
+
+def test_sample_guards_zero_nan_negative_and_infinite_radiance():
+    for value in (float("nan"), -2.0, float("inf")):
+        s = poisoned_cornell(value, both=True)
+        o = O.OracleScene(s)
+        o.build()
+        o.set_shading(s)
+        for mode in (0, 1):
+            rgb, w = o.render(s, 8, 5, mode=mode, seed=3)
+            # every sample that saw the light directly or by next-event estimation was zeroed (the
+            # others carry nothing), and every weight still counts
+            assert np.all(w == 8.0)
+            if value < 0:
+                # a negative sample survives only with luminance >= -1e-5
+                lum = rgb @ np.array([0.212671, 0.715160, 0.072169])
+                assert np.all(rgb <= 0.0) and np.all(lum >= -8e-5), (mode, lum.min())
+                assert np.mean(rgb == 0.0) > 0.5
+            else:
+                assert np.all(rgb == 0.0), (value, mode)
+        # one poisoned triangle: the samples that picked the other light keep their value
+        s = poisoned_cornell(value)
+        o = O.OracleScene(s)
+        o.build()
+        o.set_shading(s)
+        rgb, w = o.render(s, 8, 5, mode=1, seed=3)
+        assert np.all(np.isfinite(rgb)) and rgb.max() > 0.0 and np.all(w == 8.0)
+        lum = rgb @ np.array([0.212671, 0.715160, 0.072169])
+        assert np.all(lum >= -8e-5) if value < 0 else np.all(rgb >= 0.0)
