@@ -15,12 +15,19 @@ over its HIP-event-timed average duration. The CPU baseline is the oracle
 (C++ restatement of the reference traversal) on a bounded random sample of
 the same four query sets, using every host core available to this process.
 
-Two further objects ride on the same line: `path_tracer` (the plain GPU path
-tracer at 1920 x 1080, 64 spp, depth 8, samples/s and Mrays/s) and `guiding`
-(the door-gap interior at 1920 x 1080, depth 10: per-sample variance and
-seconds per sample of the plain, network-free and neural-guided estimators,
-and the time to relMSE 0.02 they predict), and `batch` (64 seeded scenes
-rendered by one hipGraph-replayed wavefront loop against one call per scene).
+Further objects ride on the same line: `time_to_relmse` (the door-gap interior
+at 1920 x 1080, depth 10: the measured wall time of progressive renders until
+relMSE against a reference first reaches 0.02, neural-guided against the plain
+path with the reference's Russian roulette and at its best threshold);
+`cornell` (the Cornell box configuration on the GPU and the oracle's CPU
+render of it on every host core); `path_tracer` (the plain GPU path tracer at
+1920 x 1080, 64 spp, depth 8: samples/s, Mrays/s, the traversal launches'
+algorithmic bytes over their HIP-event time against the HBM roofline, and the
+oracle's CPU samples/s); `guiding` (per-sample variance and seconds per sample
+of the estimators, the time to relMSE 0.02 they predict, and the measured
+renders behind `time_to_relmse`); `batch` (64 seeded scenes rendered by one
+hipGraph-replayed wavefront loop against one call per scene); `mlp` and
+`integrators`.
 """
 from __future__ import annotations
 
@@ -49,6 +56,12 @@ def log(msg: str) -> None:
 
 def alg_bytes(n_rays: int, st) -> float:
     return 32.0 * n_rays + 16.0 * n_rays + 32.0 * st.nodes_visited + 36.0 * st.tris_tested + 64.0 * st.spheres_tested
+
+
+def path_alg_bytes(st) -> float:
+    """Algorithmic bytes of a render's traversal launches: 32 B ray + 16 B hit per query, 32 B per box
+    test, 36 B per triangle test, 64 B per sphere test (counted by the launches themselves)."""
+    return alg_bytes(int(st.rays), st)
 
 
 def traffic_from(prof_dir: str, any_hit: bool, set_index: int):
@@ -117,7 +130,7 @@ def cpu_baseline(scene, bvh, sets, target_s: float):
     }, samples, outs
 
 
-def path_tracer_leg(ctx, spp: int) -> dict:
+def path_tracer_leg(ctx, spp: int, cpu: bool) -> dict:
     """The plain GPU path tracer at 1920 x 1080: 2,048,000 triangles + 256 spheres, 4 quad lights,
     `spp` samples per pixel (64 in the configuration), max depth 8, corrected (pbrt-v3) semantics."""
     from bonsai_amd import GpuBvh
@@ -132,16 +145,118 @@ def path_tracer_leg(ctx, spp: int) -> dict:
         gi = GpuPathIntegrator(ctx, scene.camera, spp=spp, max_depth=8, mode=mode, seed=5)
         film = Film(ctx, scene.camera.width, scene.camera.height)
         gi.render_into(bvh, film, spp=1)  # warm-up (allocations, code objects)
+        # untimed: the box / primitive tests of every traversal launch (the algorithmic bytes)
+        ctx.set_profiling(count=True)
         film.clear()
+        cst = gi.render_into(bvh, film)
+        # timed, with a HIP event pair around each traversal kernel
+        ctx.set_profiling(traversal_time=True)
+        film.clear()
+        torch_sync()
         t0 = time.perf_counter()
         st = gi.render_into(bvh, film)
         wall = time.perf_counter() - t0
+        ctx.set_profiling()
+        trav = path_alg_bytes(cst)
+        whole = trav + 16.0 * st.samples
         out[name] = {"seconds": round(wall, 3), "samples_per_s": round(st.samples / wall, 1),
                      "Msamples_s": round(st.samples / wall / 1e6, 2), "rays": int(st.rays),
                      "Mrays_s": round(st.rays / wall / 1e6, 1), "shadow_rays": int(st.shadow_rays),
-                     "kernel_ms": round(st.kernel_ms, 1), "image_mean": round(float(film.image().mean()), 5)}
-        log(f"path tracer ({name}): {spp} spp in {wall:.2f} s, {st.rays / wall / 1e6:.0f} Mrays/s")
+                     "kernel_ms": round(st.kernel_ms, 1), "image_mean": float(f"{film.image().mean():.6g}"),
+                     "roofline": {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "traversal_ms": round(st.traverse_ms, 2),
+                                  "traversal_bytes": trav,
+                                  "traversal_achieved": round(trav / st.traverse_ms / 1e6, 1),
+                                  "traversal_frac": round(trav / st.traverse_ms / 1e6 / HBM_PEAK_GBS, 4),
+                                  "render_bytes": whole,
+                                  "render_achieved": round(whole / st.kernel_ms / 1e6, 1),
+                                  "render_frac": round(whole / st.kernel_ms / 1e6 / HBM_PEAK_GBS, 4),
+                                  "box_tests": int(cst.nodes_visited), "triangle_tests": int(cst.tris_tested),
+                                  "sphere_tests": int(cst.spheres_tested), "queries": int(cst.rays)}}
+        assert cst.rays == st.rays
+        log(f"path tracer ({name}): {spp} spp in {wall:.2f} s, {st.rays / wall / 1e6:.0f} Mrays/s, traversal "
+            f"{st.traverse_ms:.1f} ms = {trav / st.traverse_ms / 1e6:.0f} GB/s algorithmic")
         film.close()
+    if cpu:
+        c = cpu_path_rate(scene, bvh, 1, 8, MODE_CORRECTED)
+        c["gpu_vs_cpu"] = round(out["corrected"]["samples_per_s"] / c["samples_per_s"], 1)
+        out["cpu_baseline"] = c
+        log(f"path tracer, CPU oracle: {c['Msamples_s']} Msamples/s on {c['cores']} cores")
+    bvh.close()
+    return out
+
+
+def cpu_path_rate(scene, bvh, spp: int, depth: int, mode: int, rr: float = 1.0, target_s: float = 6.0) -> dict:
+    """The oracle's path integrator (the reference's rayon path restated in C++, every host core) on a
+    horizontal band of the film through its middle, sized to about target_s of CPU time."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import cpu_oracle as O
+
+    cores = len(os.sched_getaffinity(0))
+    o = O.OracleScene(scene)
+    nodes, slots = bvh.bvh()
+    o.use_bvh(nodes, slots)
+    o.set_shading(scene)
+    W, H = scene.camera.width, scene.camera.height
+
+    def band(rows):
+        y0 = max(0, H // 2 - rows // 2)
+        t0 = time.perf_counter()
+        o.render(scene, spp, depth, mode=mode, seed=5, rr=rr, window=(0, y0, W, y0 + rows), threads=cores)
+        return time.perf_counter() - t0
+
+    probe = 16
+    dt = band(probe)
+    rows = int(min(H, max(probe, probe * target_s / max(dt, 1e-6))))
+    dt = band(rows)
+    n = W * rows * spp
+    return {"samples_per_s": round(n / dt, 1), "Msamples_s": round(n / dt / 1e6, 4), "cores": cores,
+            "kind": "port", "sample": f"{W}x{rows} band through the film's middle at {spp} spp ({dt:.1f} s)"}
+
+
+def cornell_leg(ctx, cpu: bool) -> dict:
+    """The plain CPU path tracer's configuration: Cornell box 256 x 256, 16 spp, max depth 5, the quad
+    light one-sided, in both semantics: the GPU path (median of 5 renders) and, beside it, the oracle's
+    full render of the same configuration on every host core."""
+    from bonsai_amd import GpuBvh
+    from bonsai_amd.integrator import MODE_CORRECTED, MODE_REF, Film, GpuPathIntegrator
+    from bonsai_amd.scenes import cornell_box
+
+    s = cornell_box()
+    bvh = GpuBvh(ctx, s)
+    out = {"workload": "Cornell box 256x256, 16 spp, max depth 5: 10 wall + 2 light triangles, 2 spheres"}
+    for mode, name in ((MODE_REF, "reference_semantics"), (MODE_CORRECTED, "corrected")):
+        gi = GpuPathIntegrator(ctx, s.camera, spp=16, max_depth=5, mode=mode, seed=1)
+        film = Film(ctx, 256, 256)
+        gi.render_into(bvh, film)
+        times = []
+        for _ in range(5):
+            film.clear()
+            torch_sync()
+            t0 = time.perf_counter()
+            st = gi.render_into(bvh, film)
+            times.append(time.perf_counter() - t0)
+        film.close()
+        wall = float(np.median(times))
+        entry = {"gpu_seconds": round(wall, 5), "gpu_Msamples_s": round(st.samples / wall / 1e6, 1),
+                 "gpu_Mrays_s": round(st.rays / wall / 1e6, 1), "rays_per_sample": round(st.rays / st.samples, 3)}
+        if cpu:
+            sys.path.insert(0, str(ROOT / "tests"))
+            import cpu_oracle as O
+            cores = len(os.sched_getaffinity(0))
+            o = O.OracleScene(s)
+            nodes, slots = bvh.bvh()
+            o.use_bvh(nodes, slots)
+            o.set_shading(s)
+            t0 = time.perf_counter()
+            o.render(s, 16, 5, mode=mode, seed=1, threads=cores)
+            dt = time.perf_counter() - t0
+            entry["cpu_baseline"] = {"seconds": round(dt, 3), "Msamples_s": round(st.samples / dt / 1e6, 3),
+                                     "cores": cores, "kind": "port", "sample": "the full configuration"}
+            entry["gpu_vs_cpu"] = round(dt / wall, 1)
+        out[name] = entry
+        log(f"Cornell ({name}): GPU {wall * 1e3:.2f} ms" + (f", CPU oracle {entry['cpu_baseline']['seconds']:.2f} s"
+                                                            if cpu else ""))
     bvh.close()
     return out
 
@@ -344,7 +459,119 @@ GUIDE_WARMUP_FACTOR = 3
 PLAIN_BEST_RR = 0.05
 
 
-def guiding_leg(ctx, spp: int) -> dict:
+RELMSE_TARGET = 0.02
+
+
+def rel_mse_terms(img: np.ndarray, ref: np.ndarray) -> float:
+    """relMSE = mean over pixels and RGB of (I - R)^2 / (R^2 + 0.01), FP64."""
+    return float(np.mean((img - ref) ** 2 / (ref * ref + 0.01)))
+
+
+def measured_time_to_relmse(ctx, scene, bvh, ref_spp: int, max_seconds: float) -> dict:
+    """Progressive 1920 x 1080 renders of the door-gap interior, each checked against a reference
+    image after every step (the check is not timed): the wall time at which relMSE = mean (I - R)^2 /
+    (R^2 + 0.01) first reaches RELMSE_TARGET. The reference R is the frozen-network guided estimator
+    (unbiased, with the lowest variance per second) at ref_spp samples per pixel in two independent
+    halves R1, R2 with FP64 film sums; its own error, mean ((R1 - R2) / 2)^2 / (R^2 + 0.01), is
+    subtracted from every measurement (I and R are independent, so E (I - R)^2 = Var I + Var R).
+    Plain estimators step 256 spp at a time, the guided one 64 spp after its timed warm-up."""
+    from bonsai_amd.guide import GuideMLP, init_params
+    from bonsai_amd.integrator import MODE_CORRECTED, Film, GpuPathIntegrator, GuidedPathIntegrator, resolve
+
+    W, H = scene.camera.width, scene.camera.height
+
+    def guided(g, seed, train, n):
+        return GuidedPathIntegrator(ctx, scene.camera, spp=n, max_depth=10, guide=g, seed=seed, select_prob=0.5,
+                                    steps=GUIDE_STEPS, train=train, bounces=GUIDE_BOUNCES, rr_threshold=GUIDE_RR)
+
+    def warm_up(g, seed):
+        film = Film(ctx, W, H)
+        torch_sync()
+        t0 = time.perf_counter()
+        n = GUIDE_WARMUP_FACTOR * 32
+        for k in range(0, n, 2):
+            guided(g, seed, True, 2).render_into(bvh, film, spp=2, first_sample=(1 << 20) + k)
+        torch_sync()
+        film.close()
+        return time.perf_counter() - t0
+
+    def image(film):
+        rgb, w = film.sums(f64=True)
+        return resolve(rgb, w)
+
+    # the reference
+    gref = GuideMLP(ctx, init_params(11), 65536)
+    warm_up(gref, 800)
+    halves = []
+    for seed in (801, 802):
+        film = Film(ctx, W, H, f64=True)
+        integ = guided(gref, seed, False, ref_spp // 2)
+        for k in range(0, ref_spp // 2, 256):
+            integ.render_into(bvh, film, spp=min(256, ref_spp // 2 - k), first_sample=k)
+        halves.append(image(film))
+        film.close()
+    R = 0.5 * (halves[0] + halves[1])
+    ref_err = float(np.mean(0.25 * (halves[0] - halves[1]) ** 2 / (R * R + 0.01)))
+    out = {"reference": {"estimator": "guided, frozen network", "spp": ref_spp, "relmse_of_reference": round(ref_err, 5)},
+           "target": RELMSE_TARGET, "check_excluded_from_time": True}
+    log(f"measured relMSE: reference {ref_spp} spp, its own relMSE {ref_err:.4f}")
+
+    def progressive(name, make, step, warm_s):
+        film = Film(ctx, W, H, f64=True)
+        t, n, hist = warm_s, 0, []
+        reached = None
+        while t < max_seconds:
+            integ = make(step)
+            torch_sync()
+            t0 = time.perf_counter()
+            integ.render_into(bvh, film, spp=step, first_sample=n)
+            torch_sync()
+            t += time.perf_counter() - t0
+            n += step
+            e = rel_mse_terms(image(film), R) - ref_err
+            hist.append((n, round(t, 3), round(e, 5)))
+            if e <= RELMSE_TARGET:
+                reached = (n, t, e)
+                break
+        film.close()
+        rec = {"step_spp": step, "warmup_seconds": round(warm_s, 3), "checks": hist[-4:]}
+        if reached:
+            n1, t1, e1 = reached
+            # between checks relMSE falls as 1/spp: the sample count and time where it crosses the target
+            n0, t0_, e0 = hist[-2] if len(hist) > 1 else (0, warm_s, float("inf"))
+            n_star = n1 * e1 / RELMSE_TARGET
+            t_star = t0_ + (t1 - t0_) * (n_star - n0) / (n1 - n0) if n1 > n0 else t1
+            rec.update({"seconds_at_first_check_below": round(t1, 2), "spp_at_first_check_below": n1,
+                        "seconds": round(min(t1, max(t_star, t0_)), 2), "spp": int(round(n_star))})
+        else:
+            n1, t1, e1 = hist[-1]
+            rec.update({"seconds": None, "not_reached_within_s": max_seconds, "relmse_at_end": e1,
+                        "seconds_extrapolated": round(warm_s + (t1 - warm_s) * e1 / RELMSE_TARGET, 1)})
+        log(f"measured relMSE {name}: " + (f"{rec['seconds']} s ({rec['spp']} spp)" if reached else
+                                          f"not reached in {max_seconds} s, relMSE {e1:.4f}"))
+        return rec
+
+    def plain(rr, seed):
+        return lambda n: GpuPathIntegrator(ctx, scene.camera, spp=n, max_depth=10, mode=MODE_CORRECTED, seed=seed,
+                                           rr_threshold=rr)
+
+    out["plain"] = progressive("plain", plain(1.0, 811), 256, 0.0)
+    out["plain"]["rr_threshold"] = 1.0
+    out["plain_best_rr"] = progressive("plain_best_rr", plain(PLAIN_BEST_RR, 812), 256, 0.0)
+    out["plain_best_rr"]["rr_threshold"] = PLAIN_BEST_RR
+    g = GuideMLP(ctx, init_params(3), 65536)
+    warm_s = warm_up(g, 813)
+    out["guided"] = progressive("guided", lambda n: guided(g, 814, False, n), 64, warm_s)
+    gd = out["guided"]["seconds"]
+    for key, ref in (("speedup_guided_vs_plain", "plain"), ("speedup_guided_vs_plain_best_rr", "plain_best_rr")):
+        p = out[ref]["seconds"]
+        out[key] = round(p / gd, 3) if (p and gd) else None
+    gref.close()
+    g.close()
+    return out
+
+
+def guiding_leg(ctx, spp: int, measure: bool = True, ref_spp: int = 2048, max_seconds: float = 150.0) -> dict:
     """Door-gap interior at 1920 x 1080, depth 10: per-sample relative variance v (from two
     independent renders of `spp` samples each) and seconds per sample of each estimator; the
     predicted time to relMSE 0.02 is v * seconds_per_spp / 0.02 (Monte Carlo error falls as
@@ -415,6 +642,8 @@ def guiding_leg(ctx, spp: int) -> dict:
     for key, ref in (("speedup_guided_vs_plain", "plain"), ("speedup_guided_vs_plain_best_rr", "plain_best_rr")):
         p = out[ref]["predicted_seconds_to_relmse_0.02"]
         out[key] = round(p / gd, 3) if gd > 0 else None
+    if measure:
+        out["measured"] = measured_time_to_relmse(ctx, scene, bvh, ref_spp, max_seconds)
     bvh.close()
     return out
 
@@ -435,6 +664,12 @@ def main() -> int:
     ap.add_argument("--guide-spp", type=int, default=32, help="guiding leg samples per render (0: skip)")
     ap.add_argument("--batch-spp", type=int, default=16, help="scene-batch leg samples per pixel (0: skip)")
     ap.add_argument("--no-mlp", action="store_true", help="skip the guiding-network leg")
+    ap.add_argument("--no-relmse", action="store_true",
+                    help="skip the measured time-to-relMSE renders of the guiding leg (the prediction stays)")
+    ap.add_argument("--relmse-ref-spp", type=int, default=2048, help="reference samples per pixel (two halves)")
+    ap.add_argument("--relmse-max-seconds", type=float, default=150.0,
+                    help="longest progressive render per estimator before giving up")
+    ap.add_argument("--no-cornell", action="store_true", help="skip the Cornell-box (CPU path tracer) leg")
     ap.add_argument("--integrators-spp", type=int, default=16,
                     help="direct / Whitted / plastic leg samples per pixel (0: skip)")
     ap.add_argument("--skip-intersection", action="store_true",
@@ -554,8 +789,18 @@ def main() -> int:
     b, s_, c, v_ = ctx.launch()
     hits = coh = inc = None
     torch.cuda.empty_cache()
-    pt = path_tracer_leg(ctx, args.render_spp) if args.render_spp > 0 else None
-    gl = guiding_leg(ctx, args.guide_spp) if args.guide_spp > 0 else None
+    cl = cornell_leg(ctx, not args.no_cpu) if not args.no_cornell else None
+    pt = path_tracer_leg(ctx, args.render_spp, not args.no_cpu) if args.render_spp > 0 else None
+    gl = (guiding_leg(ctx, args.guide_spp, not args.no_relmse, args.relmse_ref_spp, args.relmse_max_seconds)
+          if args.guide_spp > 0 else None)
+    ttr = None
+    if gl and "measured" in gl:
+        m = gl["measured"]
+        ttr = {"workload": "door-gap interior 1920x1080, depth 10", "target_relmse": RELMSE_TARGET,
+               "guided_s": m["guided"]["seconds"], "plain_s": m["plain"]["seconds"],
+               "plain_best_rr_s": m["plain_best_rr"]["seconds"],
+               "speedup_vs_plain": m["speedup_guided_vs_plain"],
+               "speedup_vs_plain_best_rr": m["speedup_guided_vs_plain_best_rr"], "measured": True}
     bl = batch_leg(ctx, args.batch_spp) if args.batch_spp > 0 else None
     ml = mlp_leg(ctx) if not args.no_mlp else None
     il = integrators_leg(ctx, args.integrators_spp) if args.integrators_spp > 0 else None
@@ -580,6 +825,8 @@ def main() -> int:
         "breakdown": breakdown,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "time_to_relmse": ttr,
+        "cornell": cl,
         "path_tracer": pt,
         "guiding": gl,
         "batch": bl,

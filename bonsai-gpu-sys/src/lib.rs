@@ -11,7 +11,7 @@ pub const BG_ERR_INVALID: c_int = -1;
 pub const BG_ERR_HIP: c_int = -2;
 pub const BG_ERR_NOMEM: c_int = -3;
 pub const BG_ERR_UNSUPPORTED: c_int = -4;
-pub const BG_ABI_VERSION: c_int = 1;
+pub const BG_ABI_VERSION: c_int = 2;
 
 pub const BG_CLOSEST: u32 = 0;
 pub const BG_ANY: u32 = 1;
@@ -25,6 +25,8 @@ pub const BG_MAT_PLASTIC: u32 = 2;
 pub const BG_OBJ_NORMALS: u32 = 1;
 pub const BG_OBJ_UVS: u32 = 2;
 pub const BG_NOALLOC_SCRATCH_BYTES: u32 = 128 << 20;
+pub const BG_PROFILE_COUNT: u32 = 1;
+pub const BG_PROFILE_TRAVERSAL_TIME: u32 = 2;
 
 pub const BG_MODE_REF: u32 = 0;
 pub const BG_MODE_CORRECTED: u32 = 1;
@@ -149,6 +151,7 @@ pub struct bg_stats {
     pub samples: u64,
     pub shadow_rays: u64,
     pub train_steps: u64,
+    pub traverse_ms: f64,
 }
 
 #[repr(C)]
@@ -279,6 +282,7 @@ extern "C" {
                                    hits: *const bg_hit, n: usize, out: *mut bg_surface) -> c_int;
     pub fn bg_set_launch_config(ctx: *mut bg_ctx, cfg: *const bg_launch_config) -> c_int;
     pub fn bg_get_launch_config(ctx: *const bg_ctx, cfg: *mut bg_launch_config) -> c_int;
+    pub fn bg_set_profiling(ctx: *mut bg_ctx, flags: u32) -> c_int;
 
     // camera (perspective.rs:49-175, transform.rs:152-189)
     pub fn bg_look_at(pos: *const f32, look: *const f32, up: *const f32, m: *mut f32, m_inv: *mut f32) -> c_int;

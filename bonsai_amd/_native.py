@@ -18,6 +18,8 @@ BG_ERR_UNSUPPORTED = -4
 BG_CLOSEST = 0
 BG_ANY = 1
 BG_COUNT = 2
+BG_PROFILE_COUNT = 1
+BG_PROFILE_TRAVERSAL_TIME = 2
 BG_NOALLOC_SCRATCH_BYTES = 128 << 20
 BG_MISS = 0xFFFFFFFF
 BG_MAT_MATTE = 0
@@ -78,7 +80,8 @@ class bg_scene_info(C.Structure):
 class bg_stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("rays", C.c_uint64), ("launches", C.c_uint64),
                 ("nodes_visited", C.c_uint64), ("tris_tested", C.c_uint64), ("spheres_tested", C.c_uint64),
-                ("samples", C.c_uint64), ("shadow_rays", C.c_uint64), ("train_steps", C.c_uint64)]
+                ("samples", C.c_uint64), ("shadow_rays", C.c_uint64), ("train_steps", C.c_uint64),
+                ("traverse_ms", C.c_double)]
 
 
 class bg_launch_config(C.Structure):
@@ -171,6 +174,7 @@ SIGNATURES = {
     "bg_intersect_device_noalloc": (C.c_int, [P, P, P, C.c_size_t, P, C.c_uint32, C.POINTER(bg_stats)]),
     "bg_set_launch_config": (C.c_int, [P, C.POINTER(bg_launch_config)]),
     "bg_get_launch_config": (C.c_int, [P, C.POINTER(bg_launch_config)]),
+    "bg_set_profiling": (C.c_int, [P, C.c_uint32]),
     "bg_camera_rays_device": (C.c_int, [P, C.POINTER(bg_camera_desc), C.c_float, C.c_float, P]),
     "bg_camera_rays": (C.c_int, [P, C.POINTER(bg_camera_desc), C.c_float, C.c_float, P]),
     "bg_look_at": (C.c_int, [P, P, P, P, P]),

@@ -78,7 +78,7 @@ struct bg_batch {
     size_t root_cap = 0;
     bg::LaunchShape shape;
     bool tuned = false;
-    uint32_t multi_lobe = 0;
+    uint32_t multi_lobe = 0, sphere_lights = 0;
     double tune_ms = 0.0;
 };
 
@@ -201,6 +201,7 @@ int bg_batch_create(bg_ctx* ctx, const bg_batch_item* items, uint32_t n, bg_batc
         bt->n_spheres += s->n_spheres;
         n_mats += s->materials.size();
         bt->multi_lobe |= bg::has_multi_lobe(s->materials);
+        bt->sphere_lights |= bg::has_sphere_lights(s->lights, s->n_tris);
         bt->n_lights += (uint32_t)s->lights.size();
         bt->max_depth = std::max(bt->max_depth, s->bvh.max_depth);
         bt->scene_root.push_back((uint32_t)pair_base[u]);  // the scene's virtual root record
@@ -399,6 +400,7 @@ int bg_render_batch(bg_ctx* ctx, bg_batch* bt, const bg_batch_desc* d, bg_stats*
     sc.n_tris = (uint32_t)bt->n_tris;
     sc.n_lights = bt->n_lights;
     sc.multi_lobe = bt->multi_lobe;
+    sc.sphere_lights = bt->sphere_lights;
 
     std::vector<bg::LaunchShape> cands;
     if (d->autotune && !bt->tuned)

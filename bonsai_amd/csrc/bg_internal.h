@@ -148,6 +148,7 @@ struct ShadeSceneDev {
     const SphereShadeDev* spheres;
     uint32_t n_tris, n_lights;
     uint32_t multi_lobe = 0;  // a material with two lobes (plastic) is present
+    uint32_t sphere_lights = 0;  // an area light is on a sphere (selects the kernels that sample spheres)
 };
 
 // Wavefront path state (structure of arrays) for up to `cap` paths.
@@ -401,6 +402,11 @@ struct bg_scene {
 
 namespace bg {
 // a material whose BSDF holds two lobes (plastic) selects the two-lobe shading kernel
+inline uint32_t has_sphere_lights(const std::vector<bg_area_light>& lights, uint64_t n_tris) {
+    for (const bg_area_light& l : lights)
+        if (l.prim_id >= n_tris) return 1u;
+    return 0u;
+}
 inline uint32_t has_multi_lobe(const std::vector<bg_material>& mats) {
     for (const bg_material& m : mats)
         if (m.type == BG_MAT_PLASTIC) return 1u;
@@ -423,4 +429,8 @@ struct bg_ctx {
     size_t io_bytes = 0;
     bg::PathBuffers paths;
     bg::DirectSlots direct;
+    // render-call measurement (bg_set_profiling): flags, and the event pairs timing traversal launches
+    uint32_t profile = 0;
+    std::vector<hipEvent_t> prof_events;
+    size_t prof_used = 0;
 };

@@ -257,6 +257,7 @@ int bg_create(int device, bg_ctx** out) {
 void bg_destroy(bg_ctx* ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->device);
+    for (hipEvent_t e : ctx->prof_events) hipEventDestroy(e);
     if (ctx->spill) hipFree(ctx->spill);
     if (ctx->counters) hipFree(ctx->counters);
     if (ctx->io) hipFree(ctx->io);
@@ -283,6 +284,13 @@ int bg_set_launch_config(bg_ctx* ctx, const bg_launch_config* cfg) {
     s.compact = cfg->compact != 0;
     if (!bg::traverse_shape_supported(s)) return fail(ctx, BG_ERR_INVALID, "unsupported launch config");
     ctx->shape = s;
+    return BG_OK;
+}
+
+int bg_set_profiling(bg_ctx* ctx, uint32_t flags) {
+    if (!ctx) return BG_ERR_INVALID;
+    if (flags & ~(BG_PROFILE_COUNT | BG_PROFILE_TRAVERSAL_TIME)) return fail(ctx, BG_ERR_INVALID, "unknown profiling flags");
+    ctx->profile = flags;
     return BG_OK;
 }
 

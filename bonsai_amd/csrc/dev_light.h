@@ -208,12 +208,15 @@ __device__ __noinline__ void sphere_sample_cone(const SphereShadeDev& sp, V c, V
 // DiffuseAreaLight::sample_point (diffuse.rs:134-167) over Shape::sample_point
 // (shape.rs:17-34) and Triangle::sample (triangle.rs:522-559) or Sphere::sample
 // (sphere.rs:328-354; corrected semantics sample a sphere seen from outside by its cone).
+// SPL: the scene has lights on spheres (compiled out otherwise: that code costs the shading
+// kernels registers and call stack even where it never runs).
+template <bool SPL>
 __device__ __forceinline__ LightPoint sample_light_point(const ShadeSceneDev& sc, const LightDev& light, const Surf& s,
                                                          float ul0, float ul1, bool ref) {
     LightPoint o;
     float lpdf;
     bool solid_angle = false;
-    if (light.prim >= sc.n_tris) {
+    if (SPL && light.prim >= sc.n_tris) {
         const SphereShadeDev& sp = sc.spheres[light.prim - sc.n_tris];
         const V c = sphere_center(sp);
         if (!ref && !sphere_contains_origin(sp, s, c)) {
@@ -267,8 +270,9 @@ __device__ __forceinline__ void write_shadow_ray(float4* rays, uint64_t i, const
 
 // Shape::pdf (shape.rs:40-57) on a light's triangle, or sphere.rs:356-373 on a sphere (the cone
 // pdf from outside, whatever wi): solid-angle pdf of direction wi.
+template <bool SPL>
 __device__ __noinline__ float light_point_pdf(const ShadeSceneDev& sc, const LightDev& l, const Surf& s, V wi, bool ref) {
-    if (l.prim >= sc.n_tris) {
+    if (SPL && l.prim >= sc.n_tris) {
         const SphereShadeDev& sp = sc.spheres[l.prim - sc.n_tris];
         const V c = sphere_center(sp);
         if (!sphere_contains_origin(sp, s, c)) {
@@ -285,7 +289,7 @@ __device__ __noinline__ float light_point_pdf(const ShadeSceneDev& sc, const Lig
     ray_setup(rs);
     V hp, hn;
     float area;
-    if (l.prim >= sc.n_tris) {
+    if (SPL && l.prim >= sc.n_tris) {
         const SphereShadeDev& sp = sc.spheres[l.prim - sc.n_tris];
         Surf h;
         float t;
@@ -311,9 +315,10 @@ __device__ __noinline__ float light_point_pdf(const ShadeSceneDev& sc, const Lig
 
 // Emission of light li seen along the ray (r0, r1) that hit primitive `prim` (the hit's
 // geometric normal: the triangle's, flipped with its orientation, or the sphere's at the hit).
+template <bool SPL>
 __device__ __forceinline__ C3 emission_at(const ShadeSceneDev& sc, int32_t li, uint32_t prim, float4 r0, float4 r1) {
     V n;
-    if (prim >= sc.n_tris) {
+    if (SPL && prim >= sc.n_tris) {
         RayState rs;
         rs.ox = r0.x; rs.oy = r0.y; rs.oz = r0.z; rs.t_max = r0.w;
         rs.dx = r1.x; rs.dy = r1.y; rs.dz = r1.z;
@@ -335,9 +340,10 @@ __device__ __forceinline__ C3 emission_at(const ShadeSceneDev& sc, int32_t li, u
 // the point the ray hit at distance t, |cos| cosl there (the guided estimator's MIS weight of
 // emission found by its continuation): area sampling converted by t^2 / (cosl area), or the cone
 // density of a sphere seen from outside.
+template <bool SPL>
 __device__ __forceinline__ float light_pdf_of_hit(const ShadeSceneDev& sc, const LightDev& l, float4 r0, float t,
                                                   float cosl) {
-    if (l.prim >= sc.n_tris) {
+    if (SPL && l.prim >= sc.n_tris) {
         const SphereShadeDev& sp = sc.spheres[l.prim - sc.n_tris];
         const V c = sphere_center(sp);
         const float d2 = len2(mkv(r0.x, r0.y, r0.z) - c);

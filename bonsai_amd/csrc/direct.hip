@@ -33,14 +33,14 @@ constexpr uint32_t kSlotLight = 1u, kSlotBsdf = 2u;
 
 // One estimate_direct (integrator.rs:330-437) for light idx: writes the slot's
 // shadow ray and light-branch term, and its MIS ray, weight and pdf.
-template <int NL>
+template <int NL, bool SPL>
 __device__ __forceinline__ void direct_slot(const ShadeSceneDev& sc, const DirectSlots& d, uint64_t slot,
                                             const Surf& s, const BsdfN<NL>& bs, uint32_t idx, float us0, float us1,
                                             float ul0, float ul1, bool ref) {
     const uint32_t nonspec = kAll & ~kSpecular;
     const LightDev& light = sc.lights[idx];
     uint32_t flags = 0;
-    const LightPoint lpt = sample_light_point(sc, light, s, ul0, ul1, ref);
+    const LightPoint lpt = sample_light_point<SPL>(sc, light, s, ul0, ul1, ref);
     float scattering_pdf = 0.0f;
     C3 term = mkc(0.0f);
     if (lpt.pdf > 0.0f && !black(lpt.rad)) {
@@ -59,7 +59,7 @@ __device__ __forceinline__ void direct_slot(const ShadeSceneDev& sc, const Direc
     const float spdf = ref ? scattering_pdf : bsm.pdf;
     const V mdir = ref ? lpt.wi : bsm.wi;
     if (!black(fb) && spdf > 0.0f) {
-        const float lpp = light_point_pdf(sc, light, s, mdir, ref);
+        const float lpp = light_point_pdf<SPL>(sc, light, s, mdir, ref);
         if (lpp != 0.0f) {
             V o;
             float tm;
@@ -74,7 +74,7 @@ __device__ __forceinline__ void direct_slot(const ShadeSceneDev& sc, const Direc
 }
 
 // KIND: 0 direct lighting, all lights; 1 direct lighting, one light; 2 Whitted.
-template <int KIND, bool MULTI>
+template <int KIND, bool MULTI, bool SPL>
 __global__ __launch_bounds__(128) void k_direct_shade(RenderParamsDev p, ShadeSceneDev sc, PathBuffers b,
                                                       DirectSlots d) {
     const uint64_t i = (uint64_t)blockIdx.x * 128 + threadIdx.x;
@@ -107,7 +107,7 @@ __global__ __launch_bounds__(128) void k_direct_shade(RenderParamsDev p, ShadeSc
         for (uint32_t j = 0; j < nl; ++j) {
             const float ul0 = draw(key, dim), ul1 = draw(key, dim);
             const float us0 = draw(key, dim), us1 = draw(key, dim);
-            direct_slot(sc, d, (uint64_t)j * p.n_paths + i, s, bs, j, us0, us1, ul0, ul1, ref);
+            direct_slot<MULTI ? 2 : 1, SPL>(sc, d, (uint64_t)j * p.n_paths + i, s, bs, j, us0, us1, ul0, ul1, ref);
         }
     } else if (KIND == 1) {  // integrator.rs:309-328
         if (nl > 0) {
@@ -116,13 +116,13 @@ __global__ __launch_bounds__(128) void k_direct_shade(RenderParamsDev p, ShadeSc
             const uint32_t idx = (uint32_t)rmin(u * fn, fn - 1.0f);
             const float ul0 = draw(key, dim), ul1 = draw(key, dim);
             const float us0 = draw(key, dim), us1 = draw(key, dim);
-            direct_slot(sc, d, i, s, bs, idx, us0, us1, ul0, ul1, ref);
+            direct_slot<MULTI ? 2 : 1, SPL>(sc, d, i, s, bs, idx, us0, us1, ul0, ul1, ref);
         }
     } else {  // whitted.rs:67-84
         for (uint32_t j = 0; j < nl; ++j) {
             const uint64_t slot = (uint64_t)j * p.n_paths + i;
             const float u0 = draw(key, dim), u1 = draw(key, dim);
-            const LightPoint lpt = sample_light_point(sc, sc.lights[j], s, u0, u1, ref);
+            const LightPoint lpt = sample_light_point<SPL>(sc, sc.lights[j], s, u0, u1, ref);
             if (black(lpt.rad) || lpt.pdf == 0.0f) continue;
             const C3 f = bs.f(s.wo, lpt.wi, kAll);
             if (!black(f)) {
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(128) void k_direct_shade(RenderParamsDev p, ShadeSc
 
 // Adds each path's light slots to its radiance in light order and records the
 // rays it issued (beta.w: low 16 bits all queries, high 16 bits any-hit ones).
-template <int KIND>
+template <int KIND, bool SPL>
 __global__ __launch_bounds__(256) void k_direct_resolve(RenderParamsDev p, ShadeSceneDev sc, PathBuffers b,
                                                         DirectSlots d) {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) void k_direct_resolve(RenderParamsDev p, Shade
                 // integrator.rs:412-419: the reference accepts any area light, pbrt only the sampled one
                 const bool same = p.mode == 0 ? (li >= 0) : (li == (int32_t)__float_as_uint(c.y));
                 if (same) {
-                    const C3 Le = emission_at(sc, li, hp, d.mray[2 * slot], d.mray[2 * slot + 1]);
+                    const C3 Le = emission_at<SPL>(sc, li, hp, d.mray[2 * slot], d.mray[2 * slot + 1]);
                     if (!black(Le)) {
                         const float4 fw = d.b[slot];
                         ed = ed + C3{fw.x, fw.y, fw.z} * Le * fw.w / c.x;
@@ -190,20 +190,20 @@ hipError_t launch_direct_shade(const RenderParamsDev& p, const ShadeSceneDev& sc
                                const DirectSlots& d, int kind, hipStream_t s) {
     if (p.n_paths == 0) return hipSuccess;
     const dim3 grid((uint32_t)((p.n_paths + 127) / 128)), block(128);
-    const bool m = sc.multi_lobe != 0;
+    const bool m = sc.multi_lobe != 0, spl = sc.sphere_lights != 0;
+#define BG_DIRECT(K)                                                                                   \
+    do {                                                                                               \
+        if (m && spl) hipLaunchKernelGGL((k_direct_shade<K, true, true>), grid, block, 0, s, p, sc, b, d);     \
+        else if (m) hipLaunchKernelGGL((k_direct_shade<K, true, false>), grid, block, 0, s, p, sc, b, d);      \
+        else if (spl) hipLaunchKernelGGL((k_direct_shade<K, false, true>), grid, block, 0, s, p, sc, b, d);    \
+        else hipLaunchKernelGGL((k_direct_shade<K, false, false>), grid, block, 0, s, p, sc, b, d);            \
+    } while (0)
     switch (kind) {
-        case 0:
-            if (m) hipLaunchKernelGGL((k_direct_shade<0, true>), grid, block, 0, s, p, sc, b, d);
-            else hipLaunchKernelGGL((k_direct_shade<0, false>), grid, block, 0, s, p, sc, b, d);
-            break;
-        case 1:
-            if (m) hipLaunchKernelGGL((k_direct_shade<1, true>), grid, block, 0, s, p, sc, b, d);
-            else hipLaunchKernelGGL((k_direct_shade<1, false>), grid, block, 0, s, p, sc, b, d);
-            break;
-        default:
-            if (m) hipLaunchKernelGGL((k_direct_shade<2, true>), grid, block, 0, s, p, sc, b, d);
-            else hipLaunchKernelGGL((k_direct_shade<2, false>), grid, block, 0, s, p, sc, b, d);
+        case 0: BG_DIRECT(0); break;
+        case 1: BG_DIRECT(1); break;
+        default: BG_DIRECT(2);
     }
+#undef BG_DIRECT
     return hipGetLastError();
 }
 
@@ -211,9 +211,13 @@ hipError_t launch_direct_resolve(const RenderParamsDev& p, const ShadeSceneDev& 
                                  const DirectSlots& d, int kind, hipStream_t s) {
     if (p.n_paths == 0) return hipSuccess;
     const dim3 grid((uint32_t)((p.n_paths + 255) / 256)), block(256);
-    if (kind == 0) hipLaunchKernelGGL(k_direct_resolve<0>, grid, block, 0, s, p, sc, b, d);
-    else if (kind == 1) hipLaunchKernelGGL(k_direct_resolve<1>, grid, block, 0, s, p, sc, b, d);
-    else hipLaunchKernelGGL(k_direct_resolve<2>, grid, block, 0, s, p, sc, b, d);
+    const bool spl = sc.sphere_lights != 0;
+    if (kind == 0 && spl) hipLaunchKernelGGL((k_direct_resolve<0, true>), grid, block, 0, s, p, sc, b, d);
+    else if (kind == 0) hipLaunchKernelGGL((k_direct_resolve<0, false>), grid, block, 0, s, p, sc, b, d);
+    else if (kind == 1 && spl) hipLaunchKernelGGL((k_direct_resolve<1, true>), grid, block, 0, s, p, sc, b, d);
+    else if (kind == 1) hipLaunchKernelGGL((k_direct_resolve<1, false>), grid, block, 0, s, p, sc, b, d);
+    else if (spl) hipLaunchKernelGGL((k_direct_resolve<2, true>), grid, block, 0, s, p, sc, b, d);
+    else hipLaunchKernelGGL((k_direct_resolve<2, false>), grid, block, 0, s, p, sc, b, d);
     return hipGetLastError();
 }
 

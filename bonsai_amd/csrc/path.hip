@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void k_init_paths(RenderParamsDev p, PathBuffe
 }
 
 // MULTI: the scene has a two-lobe material (plastic)
-template <bool GUIDED, bool MULTI>
+template <bool GUIDED, bool MULTI, bool SPL>
 __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev sc, PathBuffers b, uint32_t bounce) {
     const uint64_t i = (uint64_t)blockIdx.x * 128 + threadIdx.x;
     if (i >= p.n_paths) return;
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
             const float cosl = __builtin_fabsf(dot(s.n, rd));
             const float pc = l4.w;  // continuation pdf of the previous vertex
             if (cosl > 0.0f && pc > 0.0f) {
-                const float pl = light_pdf_of_hit(sc, lt, r0, t, cosl) / (float)sc.n_lights;
+                const float pl = light_pdf_of_hit<SPL>(sc, lt, r0, t, cosl) / (float)sc.n_lights;
                 L = L + beta * emission(lt, s.n, -rd) * power_heuristic(pc, pl);
             }
         }
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
             const LightDev& light = sc.lights[idx];
             uint32_t nee = kHasNee;
             // diffuse.rs:134-167: a point on the light's triangle (triangle.rs:522-559)
-            const LightPoint lpt = sample_light_point(sc, light, s, ul0, ul1, ref);
+            const LightPoint lpt = sample_light_point<SPL>(sc, light, s, ul0, ul1, ref);
             const float lpdf = lpt.pdf;
             const V lwi = lpt.wi;
             const C3 lrad = lpt.rad;
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
             const float spdf = ref ? scattering_pdf : bsm.pdf;
             const V mdir = ref ? lwi : bsm.wi;  // the reference traces the light-sample direction
             if (!black(fb) && spdf > 0.0f) {
-                const float lpp = light_point_pdf(sc, light, s, mdir, ref);
+                const float lpp = light_point_pdf<SPL>(sc, light, s, mdir, ref);
                 if (lpp != 0.0f) {
                     const float w = power_heuristic(spdf, lpp);
                     V o;
@@ -319,6 +319,7 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
     b.state[i] = make_uint2(dim, alive ? (flags | kAlive) : 0u);
 }
 
+template <bool SPL>
 __global__ __launch_bounds__(256) void k_nee_resolve(RenderParamsDev p, ShadeSceneDev sc, PathBuffers b,
                                                      uint32_t bounce) {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
@@ -335,7 +336,7 @@ __global__ __launch_bounds__(256) void k_nee_resolve(RenderParamsDev p, ShadeSce
             const float4 dd = b.nee_d[i];
             const bool same = p.mode == 0 ? (li >= 0) : (li == (int32_t)__float_as_uint(dd.y));
             if (same) {
-                const C3 Le = emission_at(sc, li, hp, b.mray[2 * i], b.mray[2 * i + 1]);
+                const C3 Le = emission_at<SPL>(sc, li, hp, b.mray[2 * i], b.mray[2 * i + 1]);
                 if (!black(Le)) {
                     const float4 fw = b.nee_b[i];
                     out = out + C3{fw.x, fw.y, fw.z} * Le * fw.w / dd.x;
@@ -629,19 +630,27 @@ hipError_t launch_shade(const RenderParamsDev& p, const ShadeSceneDev& sc, const
                         hipStream_t s) {
     if (p.n_paths == 0) return hipSuccess;
     const dim3 grid((uint32_t)((p.n_paths + 127) / 128));
+#define BG_SHADE(G, M)                                                                                  \
+    do {                                                                                                \
+        if (sc.sphere_lights) hipLaunchKernelGGL((k_shade<G, M, true>), grid, dim3(128), 0, s, p, sc, b, bounce); \
+        else hipLaunchKernelGGL((k_shade<G, M, false>), grid, dim3(128), 0, s, p, sc, b, bounce);                 \
+    } while (0)
     if (p.guided) {
-        if (sc.multi_lobe) hipLaunchKernelGGL((k_shade<true, true>), grid, dim3(128), 0, s, p, sc, b, bounce);
-        else hipLaunchKernelGGL((k_shade<true, false>), grid, dim3(128), 0, s, p, sc, b, bounce);
+        if (sc.multi_lobe) BG_SHADE(true, true);
+        else BG_SHADE(true, false);
     } else {
-        if (sc.multi_lobe) hipLaunchKernelGGL((k_shade<false, true>), grid, dim3(128), 0, s, p, sc, b, bounce);
-        else hipLaunchKernelGGL((k_shade<false, false>), grid, dim3(128), 0, s, p, sc, b, bounce);
+        if (sc.multi_lobe) BG_SHADE(false, true);
+        else BG_SHADE(false, false);
     }
+#undef BG_SHADE
     return hipGetLastError();
 }
 hipError_t launch_nee_resolve(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b,
                               uint32_t bounce, hipStream_t s) {
     if (p.n_paths == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_nee_resolve, dim3((uint32_t)((p.n_paths + 255) / 256)), dim3(256), 0, s, p, sc, b, bounce);
+    const dim3 grid((uint32_t)((p.n_paths + 255) / 256));
+    if (sc.sphere_lights) hipLaunchKernelGGL(k_nee_resolve<true>, grid, dim3(256), 0, s, p, sc, b, bounce);
+    else hipLaunchKernelGGL(k_nee_resolve<false>, grid, dim3(256), 0, s, p, sc, b, bounce);
     return hipGetLastError();
 }
 hipError_t launch_count_rays(const RenderParamsDev& p, const PathBuffers& b, unsigned long long* counters,

@@ -172,6 +172,27 @@ bg::TraverseArgs traverse_args(const bg_scene* s, bg_ctx* ctx, const float4* ray
 // Launches that refill finished lanes skip it: there a lane at an ended path takes the next ray at
 // once (measured: compacting them cost 10% on a triangle scene, where it saved 11-14% on the
 // sphere-and-triangle path-tracer scene); so do launches of fewer than 1M slots.
+// A render's traversal launch with the context's measurement (bg_set_profiling): test counting
+// into ctx->counters, and a HIP event pair around the kernel.
+hipError_t profiled_traverse(bg_ctx* ctx, bg::TraverseArgs a, bool any, int grid, hipStream_t st) {
+    const bool count = (ctx->profile & BG_PROFILE_COUNT) != 0;
+    if (count) a.counters = ctx->counters;
+    const bool timed = (ctx->profile & BG_PROFILE_TRAVERSAL_TIME) != 0;
+    if (timed) {
+        while (ctx->prof_events.size() < 2 * (ctx->prof_used + 1)) {
+            hipEvent_t e;
+            const hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) return r;
+            ctx->prof_events.push_back(e);
+        }
+        const hipError_t r = hipEventRecord(ctx->prof_events[2 * ctx->prof_used], st);
+        if (r != hipSuccess) return r;
+    }
+    const hipError_t e = bg::launch_traverse(a, any, count, ctx->shape, grid, st);
+    if (e != hipSuccess || !timed) return e;
+    return hipEventRecord(ctx->prof_events[2 * ctx->prof_used++ + 1], st);
+}
+
 hipError_t trace_pass(const bg_scene* s, bg_ctx* ctx, const float4* rays, uint4* hits, uint64_t n, bool any, int grid,
                       hipStream_t st) {
     bg::TraverseArgs a = traverse_args(s, ctx, rays, hits, n);
@@ -182,7 +203,38 @@ hipError_t trace_pass(const bg_scene* s, bg_ctx* ctx, const float4* rays, uint4*
         a.idx = b.ridx;
         a.n_idx = b.rcount;
     }
-    return bg::launch_traverse(a, any, false, ctx->shape, grid, st);
+    return profiled_traverse(ctx, a, any, grid, st);
+}
+
+// Start and end of a render call's measurement.
+int profile_begin(bg_ctx* ctx) {
+    ctx->prof_used = 0;
+    if (ctx->profile & BG_PROFILE_COUNT) {
+        if (!ctx->counters) HIP_TRY(ctx, hipMalloc((void**)&ctx->counters, 3 * sizeof(unsigned long long)));
+        HIP_TRY(ctx, hipMemsetAsync(ctx->counters, 0, 3 * sizeof(unsigned long long), ctx->stream));
+    }
+    return BG_OK;
+}
+
+int profile_end(bg_ctx* ctx, bg_stats* stats) {
+    if (!stats) return BG_OK;
+    if (ctx->profile & BG_PROFILE_COUNT) {
+        unsigned long long c[3] = {0, 0, 0};
+        HIP_TRY(ctx, hipMemcpy(c, ctx->counters, sizeof(c), hipMemcpyDeviceToHost));
+        stats->nodes_visited = c[0];
+        stats->tris_tested = c[1];
+        stats->spheres_tested = c[2];
+    }
+    if (ctx->profile & BG_PROFILE_TRAVERSAL_TIME) {
+        double sum = 0.0;
+        for (size_t k = 0; k < ctx->prof_used; ++k) {
+            float ms = 0.0f;
+            HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->prof_events[2 * k], ctx->prof_events[2 * k + 1]));
+            sum += ms;
+        }
+        stats->traverse_ms = sum;
+    }
+    return BG_OK;
 }
 
 }  // namespace
@@ -325,6 +377,7 @@ bg::ShadeSceneDev shade_scene(const bg_scene* s) {
     sc.n_tris = (uint32_t)s->n_tris;
     sc.n_lights = (uint32_t)s->lights.size();
     sc.multi_lobe = bg::has_multi_lobe(s->materials);
+    sc.sphere_lights = bg::has_sphere_lights(s->lights, s->n_tris);
     return sc;
 }
 
@@ -397,6 +450,8 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
     hipStream_t st = ctx->stream;
 
     HIP_TRY(ctx, hipMemsetAsync(b.counters, 0, 2 * sizeof(unsigned long long), st));
+    rc = profile_begin(ctx);
+    if (rc) return rc;
     auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
     uint64_t launches = 0;
@@ -478,6 +533,8 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
         stats->samples = npix * d->spp;
         stats->train_steps = train_steps;
     }
+    rc = profile_end(ctx, stats);
+    if (rc) return rc;
     return BG_OK;
 }
 
@@ -559,6 +616,8 @@ int direct_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, const b
     const bg::DirectSlots& ds = ctx->direct;
     hipStream_t st = ctx->stream;
     HIP_TRY(ctx, hipMemsetAsync(b.counters, 0, 2 * sizeof(unsigned long long), st));
+    rc = profile_begin(ctx);
+    if (rc) return rc;
     auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
     uint64_t launches = 0;
@@ -571,14 +630,14 @@ int direct_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, const b
         const int g = (int)std::min<uint64_t>(grid, (p.n_paths + block - 1) / block);
         const int gq = (int)std::max<uint64_t>(1, std::min<uint64_t>(grid, (nq + block - 1) / block));
         HIP_TRY(ctx, bg::launch_init_paths(p, b, st));
-        HIP_TRY(ctx, bg::launch_traverse(traverse_args(s, ctx, b.ray, b.hit, p.n_paths), false, false, ctx->shape, g, st));
+        HIP_TRY(ctx, profiled_traverse(ctx, traverse_args(s, ctx, b.ray, b.hit, p.n_paths), false, g, st));
         HIP_TRY(ctx, bg::launch_direct_shade(p, sc, b, ds, kind, st));
         launches += 3;
         if (nq) {
-            HIP_TRY(ctx, bg::launch_traverse(traverse_args(s, ctx, ds.sray, ds.shit, nq), true, false, ctx->shape, gq, st));
+            HIP_TRY(ctx, profiled_traverse(ctx, traverse_args(s, ctx, ds.sray, ds.shit, nq), true, gq, st));
             ++launches;
             if (kind != 2) {
-                HIP_TRY(ctx, bg::launch_traverse(traverse_args(s, ctx, ds.mray, ds.mhit, nq), false, false, ctx->shape, gq, st));
+                HIP_TRY(ctx, profiled_traverse(ctx, traverse_args(s, ctx, ds.mray, ds.mhit, nq), false, gq, st));
                 ++launches;
             }
         }
@@ -602,7 +661,7 @@ int direct_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, const b
         stats->launches = launches;
         stats->samples = npix * d->spp;
     }
-    return BG_OK;
+    return profile_end(ctx, stats);
 }
 
 }  // namespace

@@ -56,6 +56,12 @@ class Context:
         cfg = N.bg_launch_config(block, stack, blocks_per_cu, variant, 1 if compact else 0)
         N.check(N.lib().bg_set_launch_config(self.h, C.byref(cfg)), self.h)
 
+    def set_profiling(self, count: bool = False, traversal_time: bool = False) -> None:
+        """Render calls count their traversal launches' box / primitive tests (stats.nodes_visited,
+        tris_tested, spheres_tested) and / or time them (stats.traverse_ms)."""
+        flags = (N.BG_PROFILE_COUNT if count else 0) | (N.BG_PROFILE_TRAVERSAL_TIME if traversal_time else 0)
+        N.check(N.lib().bg_set_profiling(self.h, flags), self.h)
+
     def compaction(self) -> bool:
         cfg = N.bg_launch_config()
         N.check(N.lib().bg_get_launch_config(self.h, C.byref(cfg)), self.h)

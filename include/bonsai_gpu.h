@@ -35,7 +35,7 @@ extern "C" {
 #define BG_ERR_NOMEM (-3)
 #define BG_ERR_UNSUPPORTED (-4)
 
-#define BG_ABI_VERSION 1
+#define BG_ABI_VERSION 2
 
 /* Query flags for bg_intersect / bg_intersect_device. */
 #define BG_CLOSEST 0u
@@ -147,6 +147,10 @@ typedef struct bg_stats {
     uint64_t samples;     /* camera samples completed */
     uint64_t shadow_rays; /* any-hit queries among `rays` */
     uint64_t train_steps; /* guiding-network Adam steps taken by the call */
+    /* Filled by the path and direct-lighting renders when the context times
+     * traversal launches (BG_PROFILE_TRAVERSAL_TIME): summed HIP-event duration
+     * of their traversal kernels alone. */
+    double traverse_ms;
 } bg_stats;
 
 /* Library */
@@ -252,6 +256,17 @@ typedef struct bg_launch_config {
                              instead of every path slot; the hits are the same */
 } bg_launch_config;
 int bg_set_launch_config(bg_ctx* ctx, const bg_launch_config* cfg);
+/* Measurement of the render calls' traversal launches (bg_render_film,
+ * bg_render_film_guided, bg_render_film_direct, bg_render; not scene
+ * batches): BG_PROFILE_COUNT counts their box and primitive tests into
+ * bg_stats as BG_COUNT does for bg_intersect (slower: for the algorithmic
+ * bytes of a render, in an untimed call); BG_PROFILE_TRAVERSAL_TIME records a
+ * HIP event pair around each traversal kernel and sums the durations into
+ * bg_stats.traverse_ms. 0 turns both off (the default). The images do not
+ * change. */
+#define BG_PROFILE_COUNT 1u
+#define BG_PROFILE_TRAVERSAL_TIME 2u
+int bg_set_profiling(bg_ctx* ctx, uint32_t flags);
 int bg_get_launch_config(const bg_ctx* ctx, bg_launch_config* cfg);
 
 /* Camera (src/cameras/perspective.rs:49-114 with src/base/transform.rs). */

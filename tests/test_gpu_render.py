@@ -212,3 +212,30 @@ def test_sample_guards_match_the_oracle(ctx, value, both):
         assert rel_mse(img, ref) < 1e-4, (value, both, mode)
         if both and value != -2.0:
             assert np.all(rgb == 0.0)
+
+
+def test_profiling_counts_and_times_traversal_without_changing_the_image(ctx):
+    """bg_set_profiling: the render's traversal launches count their box and primitive tests and are
+    timed by HIP event pairs; the film is the same bit for bit."""
+    s = cornell_box(width=64, height=64)
+    bvh = GpuBvh(ctx, s)
+    gi = GpuPathIntegrator(ctx, s.camera, spp=8, max_depth=5, mode=MODE_CORRECTED, seed=4)
+    films = []
+    stats = []
+    for flags in ((False, False), (True, True), (False, True)):
+        ctx.set_profiling(*flags)
+        try:
+            film = Film(ctx, 64, 64)
+            stats.append(gi.render_into(bvh, film))
+            films.append(film.sums())
+            film.close()
+        finally:
+            ctx.set_profiling()
+    for rgb, w in films[1:]:
+        np.testing.assert_array_equal(rgb, films[0][0])
+        np.testing.assert_array_equal(w, films[0][1])
+    plain, both, timed = stats
+    assert plain.nodes_visited == 0 and plain.traverse_ms == 0.0
+    assert both.nodes_visited > both.rays > 0 and both.tris_tested > 0 and both.spheres_tested > 0
+    assert both.rays == plain.rays
+    assert 0.0 < timed.traverse_ms <= timed.kernel_ms and timed.nodes_visited == 0
