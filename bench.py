@@ -530,6 +530,7 @@ def measured_time_to_relmse(ctx, scene, bvh, ref_spp: int, max_seconds: float) -
             n += step
             e = rel_mse_terms(image(film), R) - ref_err
             hist.append((n, round(t, 3), round(e, 5)))
+            log(f"  {name}: {n} spp, {t:.1f} s, relMSE {e:.4f}")
             if e <= RELMSE_TARGET:
                 reached = (n, t, e)
                 break
