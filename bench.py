@@ -208,6 +208,8 @@ def cpu_path_rate(scene, bvh, spp: int, depth: int, mode: int, rr: float = 1.0, 
     probe = 16
     dt = band(probe)
     rows = int(min(H, max(probe, probe * target_s / max(dt, 1e-6))))
+    if rows == H:  # the whole film is quicker than the target: more samples per pixel
+        spp = max(spp, int(spp * target_s * probe / (H * max(dt, 1e-6))))
     dt = band(rows)
     n = W * rows * spp
     return {"samples_per_s": round(n / dt, 1), "Msamples_s": round(n / dt / 1e6, 4), "cores": cores,
@@ -572,7 +574,8 @@ def measured_time_to_relmse(ctx, scene, bvh, ref_spp: int, max_seconds: float) -
     return out
 
 
-def guiding_leg(ctx, spp: int, measure: bool = True, ref_spp: int = 2048, max_seconds: float = 150.0) -> dict:
+def guiding_leg(ctx, spp: int, measure: bool = True, ref_spp: int = 2048, max_seconds: float = 150.0,
+                cpu: bool = True) -> dict:
     """Door-gap interior at 1920 x 1080, depth 10: per-sample relative variance v (from two
     independent renders of `spp` samples each) and seconds per sample of each estimator; the
     predicted time to relMSE 0.02 is v * seconds_per_spp / 0.02 (Monte Carlo error falls as
@@ -643,6 +646,11 @@ def guiding_leg(ctx, spp: int, measure: bool = True, ref_spp: int = 2048, max_se
     for key, ref in (("speedup_guided_vs_plain", "plain"), ("speedup_guided_vs_plain_best_rr", "plain_best_rr")):
         p = out[ref]["predicted_seconds_to_relmse_0.02"]
         out[key] = round(p / gd, 3) if gd > 0 else None
+    if cpu:
+        c = cpu_path_rate(scene, bvh, 1, 10, MODE_CORRECTED)
+        c["gpu_vs_cpu_plain"] = round(1.0 / out["plain"]["seconds_per_spp"] * W * H / c["samples_per_s"], 1)
+        out["cpu_baseline"] = c
+        log(f"door gap, CPU oracle: {c['Msamples_s']} Msamples/s on {c['cores']} cores")
     if measure:
         out["measured"] = measured_time_to_relmse(ctx, scene, bvh, ref_spp, max_seconds)
     bvh.close()
@@ -792,8 +800,8 @@ def main() -> int:
     torch.cuda.empty_cache()
     cl = cornell_leg(ctx, not args.no_cpu) if not args.no_cornell else None
     pt = path_tracer_leg(ctx, args.render_spp, not args.no_cpu) if args.render_spp > 0 else None
-    gl = (guiding_leg(ctx, args.guide_spp, not args.no_relmse, args.relmse_ref_spp, args.relmse_max_seconds)
-          if args.guide_spp > 0 else None)
+    gl = (guiding_leg(ctx, args.guide_spp, not args.no_relmse, args.relmse_ref_spp, args.relmse_max_seconds,
+                      not args.no_cpu) if args.guide_spp > 0 else None)
     ttr = None
     if gl and "measured" in gl:
         m = gl["measured"]

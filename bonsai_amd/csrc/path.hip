@@ -490,54 +490,11 @@ __global__ __launch_bounds__(256) void k_live_count(uint64_t n, PathBuffers b) {
     if (threadIdx.x == 0) b.gblk[blockIdx.x] = total;
 }
 
-// Exclusive prefix of nblk counts in place, and their sum, in one block: each thread sums a
-// run of consecutive counts (loads issued eight at a time), a shuffle scan within each
-// wavefront and one over the 16 wavefront totals give each run's offset, and each thread then
-// rewrites its run.
+// Exclusive prefix of nblk counts in place, and their sum, in one block (dev::scan_counts).
 __global__ __launch_bounds__(1024) void k_live_scan(uint32_t nblk, uint32_t* __restrict__ blk, uint32_t* __restrict__ total) {
     __shared__ uint32_t wsum[16];
-    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const uint32_t per = (nblk + 1023) / 1024;
-    const uint32_t b0 = min(t * per, nblk), b1 = min(b0 + per, nblk);
-    uint32_t sum = 0, k = b0;
-    for (; k + 8 <= b1; k += 8) {
-        uint32_t v[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = blk[k + j];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) sum += v[j];
-    }
-    for (; k < b1; ++k) sum += blk[k];
-    uint32_t x = sum;  // inclusive scan over the wavefront
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t y = __shfl_up(x, d, 64);
-        if (lane >= (uint32_t)d) x += y;
-    }
-    if (lane == 63) wsum[wave] = x;
-    __syncthreads();
-    uint32_t run = x - sum, all = 0;
-    for (uint32_t w = 0; w < 16; ++w) {
-        const uint32_t v = wsum[w];
-        if (w < wave) run += v;
-        all += v;
-    }
-    if (t == 0) *total = all;
-    for (k = b0; k + 8 <= b1; k += 8) {
-        uint32_t v[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = blk[k + j];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            blk[k + j] = run;
-            run += v[j];
-        }
-    }
-    for (; k < b1; ++k) {
-        const uint32_t v = blk[k];
-        blk[k] = run;
-        run += v;
-    }
+    const uint32_t all = dev::scan_counts(nblk, blk, wsum);
+    if (threadIdx.x == 0) *total = all;
 }
 
 // film.rs:232-284 with a radius-0.5 box filter, gathered per pixel: a sample at

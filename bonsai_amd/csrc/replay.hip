@@ -24,36 +24,17 @@ __global__ __launch_bounds__(kBlk) void k_replay_count(uint64_t n, const float4*
     if (threadIdx.x == 0) blk[blockIdx.x] = total;
 }
 
-// One block: exclusive scan of the block counts in place, then the ring state.
+// One block: exclusive scan of the block counts in place (dev::scan_counts), then the ring state.
 __global__ __launch_bounds__(1024) void k_replay_scan(uint32_t nblk, uint32_t* __restrict__ blk, uint32_t* __restrict__ state,
                                                       uint32_t cap) {
-    __shared__ uint32_t part[1024];
-    const uint32_t t = threadIdx.x;
-    const uint32_t per = (nblk + 1023) / 1024;
-    const uint32_t b0 = t * per, b1 = min(b0 + per, nblk);
-    uint32_t sum = 0;
-    for (uint32_t b = b0; b < b1; ++b) sum += blk[b];
-    part[t] = sum;
-    __syncthreads();
-    if (t == 0) {  // 1024 partial sums, scanned serially
-        uint32_t run = 0;
-        for (int k = 0; k < 1024; ++k) {
-            const uint32_t v = part[k];
-            part[k] = run;
-            run += v;
-        }
+    __shared__ uint32_t wsum[16];
+    const uint32_t run = dev::scan_counts(nblk, blk, wsum);
+    if (threadIdx.x == 0) {
         const uint32_t head = state[0];
         state[2] = head;  // base of this append
         state[3] = run;   // size of this append
         state[0] = (uint32_t)(((uint64_t)head + run) % cap);
         state[1] = (uint32_t)min<uint64_t>((uint64_t)state[1] + run, cap);
-    }
-    __syncthreads();
-    uint32_t run = part[t];
-    for (uint32_t b = b0; b < b1; ++b) {
-        const uint32_t v = blk[b];
-        blk[b] = run;
-        run += v;
     }
 }
 
