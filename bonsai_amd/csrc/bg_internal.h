@@ -313,12 +313,11 @@ struct GuideBox {
     float lo[3], hi[3];
 };
 struct GuideScratch {
-    uint16_t* xs = nullptr;    // [5][64][cap] layer inputs, BF16
-    uint16_t* dys = nullptr;   // [5][64][cap] layer output gradients, BF16
     float* loss = nullptr;     // [cap]
-    float* partial = nullptr;  // [chunks][5][64][64]
-    float* bpartial = nullptr; // [chunks][5][64]
-    uint64_t cap = 0, chunk = 2048;
+    float* partial = nullptr;  // [blocks][5][64][64] per-block weight gradients
+    float* bpartial = nullptr; // [blocks][5][64] per-block bias gradients
+    uint64_t cap = 0;
+    uint32_t blocks = 0;       // most blocks a training step launches
 };
 hipError_t guide_refresh(const GuideDevice& g, hipStream_t s);
 // n_dev (optional) holds the sample count on the device; n then only bounds the grid.
@@ -326,6 +325,8 @@ hipError_t guide_refresh(const GuideDevice& g, hipStream_t s);
 // Blocks per CU of the inference kernel: it is held to 3 waves per SIMD, so 3
 // blocks of 4 waves fill every SIMD.
 constexpr int kInferBlocksPerCu = 3;
+// Blocks per CU of the fused training kernel (its LDS and registers allow one).
+constexpr int kTrainBlocksPerCu = 1;
 hipError_t guide_infer(const GuideDevice& g, const GuideBox& box, const float4* in, uint64_t n, float* out, int grid,
                        hipStream_t s, const uint32_t* n_dev = nullptr, bool mixture = false);
 hipError_t guide_train_step(const GuideDevice& g, const GuideBox& box, const float4* in, const float4* target,

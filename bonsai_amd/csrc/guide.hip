@@ -15,8 +15,11 @@
 // stay in registers from layer to layer; only the bias, ReLU and the BF16
 // rounding touch them in between. The backward pass reuses the same scheme
 // with the transposed weights. Weight gradients (dW = dY X^T summed over the
-// batch) are split over sample chunks and reduced in the fused Adam kernel.
+// batch) are accumulated per block in the training kernel and reduced, in a
+// fixed order, by the Adam kernel.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "bg_internal.h"
 #include "dev_guide.h"
@@ -156,13 +159,30 @@ __device__ __forceinline__ void normalise_pos(const Sample& s, const GuideBox& b
 
 // Copies the 5 BF16 weight matrices (dense rows of 64) into padded LDS rows,
 // 8 bytes per thread and step; with permute_out the last layer's accumulator
-// row p takes output row out_row(p).
-__device__ void load_weights_lds(const uint16_t* __restrict__ wg, uint16_t* __restrict__ lds, bool permute_out = false) {
+// row p takes output row out_row(p), with permute_cols (the transposed copy)
+// its column p takes output column out_row(p). out_row keeps groups of 4
+// together, so whole quads move. Every load of a thread is issued before the
+// first LDS write, so the block waits for one memory latency, not twenty.
+constexpr int kBlock = 256;  // threads of the MLP kernels
+constexpr int kQuadsPerThread = kLayers * 64 * 16 / kBlock;
+static_assert(kLayers * 64 * 16 % kBlock == 0, "weight quads split evenly over the block");
+__device__ void load_weights_lds(const uint16_t* __restrict__ wg, uint16_t* __restrict__ lds, bool permute_out = false,
+                                 bool permute_cols = false) {
     const uint2* src = reinterpret_cast<const uint2*>(wg);
-    for (int i = threadIdx.x; i < kLayers * 64 * 16; i += blockDim.x) {
+    uint2 v[kQuadsPerThread];
+#pragma unroll
+    for (int k = 0; k < kQuadsPerThread; ++k) {
+        const int i = threadIdx.x + k * kBlock;
         const int row = i / 16, q = i % 16;  // row over all layers, 4-element quad in the row
-        const int from = permute_out && row >= (kLayers - 1) * 64 ? (kLayers - 1) * 64 + out_row(row % 64) : row;
-        *reinterpret_cast<uint2*>(lds + row * kLd + 4 * q) = src[from * 16 + q];
+        const bool last = row >= (kLayers - 1) * 64;
+        const int from = permute_out && last ? (kLayers - 1) * 64 + out_row(row % 64) : row;
+        const int fq = permute_cols && last ? out_row(4 * q) / 4 : q;
+        v[k] = src[from * 16 + fq];
+    }
+#pragma unroll
+    for (int k = 0; k < kQuadsPerThread; ++k) {
+        const int i = threadIdx.x + k * kBlock;
+        *reinterpret_cast<uint2*>(lds + (i / 16) * kLd + 4 * (i % 16)) = v[k];
     }
     __syncthreads();
 }
@@ -174,12 +194,14 @@ __device__ void load_weights_lds(const uint16_t* __restrict__ wg, uint16_t* __re
 // Held to 3 waves per SIMD (156 VGPRs, the accumulators in VGPRs): left alone
 // the compiler settles at 2.
 template <bool MIX>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_guide_infer(GuideDevice g, GuideBox box, const float4* __restrict__ in,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_guide_infer(GuideDevice g, GuideBox box, const float4* __restrict__ in,
                                                      uint64_t n, const uint32_t* __restrict__ n_dev,
                                                      float* __restrict__ out) {
     __shared__ uint16_t wl[kLayers * kLayerLds];
+    __shared__ float bl[kLayers * 64];
     if (n_dev) n = *n_dev;
     if ((uint64_t)blockIdx.x * 128 >= n) return;  // the whole block is idle (uniform)
+    for (int k = threadIdx.x; k < kLayers * 64; k += kBlock) bl[k] = g.bias[k];
     load_weights_lds(g.wbf, wl, true);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, h = lane >> 5;
     const uint64_t groups = (n + 31) / 32;
@@ -194,7 +216,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
         f16v acc[2];
         for (int l = 0; l < kLayers; ++l) {
             layer_mm(wl + l * kLayerLds, xb, lane, acc);
-            epilogue(acc, g.bias + l * 64, l < kLayers - 1, lane, xb, l == kLayers - 1);
+            epilogue(acc, bl + l * 64, l < kLayers - 1, lane, xb, l == kLayers - 1);
         }
         // this lane holds outputs 20h .. 20h + 19 of sample l % 32 (lobes 4h .. 4h + 3): accumulators
         // 0-15 of block 0, then 0-3 of block 1 (out_row)
@@ -214,198 +236,271 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 }
 
 // ------------------------------------------------------------- mixture math
-struct Lobe8 {
-    float pi[8], mu[8][3], kappa[8], nm[8], s[8];  // s: tanh((s_raw - C) / R)
-};
-__device__ __forceinline__ void mixture_of(const float* o, Lobe8& L) {
-    float amax = -3.0e38f;
-    for (int j = 0; j < 8; ++j) amax = fmaxf(amax, o[5 * j]);
-    float sum = 0.0f;
-    for (int j = 0; j < 8; ++j) { L.pi[j] = expf(o[5 * j] - amax); sum += L.pi[j]; }
-    for (int j = 0; j < 8; ++j) {
-        L.pi[j] /= sum;
-        const float mx = o[5 * j + 1], my = o[5 * j + 2], mz = o[5 * j + 3];
-        const float nm = sqrtf(mx * mx + my * my + mz * mz + 1e-12f);
-        L.nm[j] = nm;
-        L.mu[j][0] = mx / nm; L.mu[j][1] = my / nm; L.mu[j][2] = mz / nm;
-        L.s[j] = tanhf((o[5 * j + 4] - kLogKappaC) / kLogKappaR);  // kept for the chain rule
-        L.kappa[j] = expf(kLogKappaC + kLogKappaR * L.s[j]);
-    }
-}
 __device__ __forceinline__ float vmf(const float* mu, float kappa, const float* w) {
     const float c = kappa / (2.0f * kPiF * (-expm1f(-2.0f * kappa)));
     return c * expf(kappa * (mu[0] * w[0] + mu[1] * w[1] + mu[2] * w[2] - 1.0f));
 }
 
-// d(-weight * log p(w)) / d out, scaled by inv_batch (oracle loss_grad_out).
-__device__ void out_grad(const float* o, const float* w, float weight, float inv_batch, float* g, float* loss) {
-    if (weight == 0.0f) {  // empty record: no loss, no gradient (even where the mixture underflows)
-        for (int k = 0; k < kOut; ++k) g[k] = 0.0f;
-        *loss = 0.0f;
-        return;
+// d(-weight * log p(w)) / d out, scaled by inv_batch (oracle loss_grad_out), for
+// the 4 lobes this lane holds (lobes 4h .. 4h + 3 of its sample, o: their 20
+// outputs); the two halves of the wavefront exchange the logit maximum,
+// the exp sum and the mixture density, each adding its own part first (the sums
+// are the same on both halves). g gets d loss / d o for the same 20 outputs and
+// *loss the sample's loss. All 64 lanes must call it (it shuffles).
+__device__ __forceinline__ void out_grad_half(const float* o, const float* w, float weight, float inv_batch, float* g,
+                                              float* loss) {
+    float amax = -3.0e38f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) amax = fmaxf(amax, o[5 * j]);
+    amax = fmaxf(amax, __shfl_xor(amax, 32));
+    float pi[4], sum = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        pi[j] = expf(o[5 * j] - amax);
+        sum += pi[j];
     }
-    Lobe8 L;
-    mixture_of(o, L);
-    float f[8], p = 0.0f;
-    for (int j = 0; j < 8; ++j) { f[j] = vmf(L.mu[j], L.kappa[j], w); p += L.pi[j] * f[j]; }
+    sum += __shfl_xor(sum, 32);
+    float mu[4][3], nm[4], sv[4], kap[4], f[4], p = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        pi[j] /= sum;
+        const float mx = o[5 * j + 1], my = o[5 * j + 2], mz = o[5 * j + 3];
+        nm[j] = sqrtf(mx * mx + my * my + mz * mz + 1e-12f);
+        mu[j][0] = mx / nm[j];
+        mu[j][1] = my / nm[j];
+        mu[j][2] = mz / nm[j];
+        sv[j] = tanhf((o[5 * j + 4] - kLogKappaC) / kLogKappaR);
+        kap[j] = expf(kLogKappaC + kLogKappaR * sv[j]);
+        f[j] = vmf(mu[j], kap[j], w);
+        p += pi[j] * f[j];
+    }
+    p += __shfl_xor(p, 32);
     p = fmaxf(p, 1e-30f);  // a direction where every lobe underflows in FP32
-    *loss = -weight * logf(p);
+    const bool live = weight != 0.0f;  // an empty record has no loss and no gradient
+    *loss = live ? -weight * logf(p) : 0.0f;
     const float sc = -weight * inv_batch;
-    for (int j = 0; j < 8; ++j) {
-        const float r = L.pi[j] * f[j] / p;
-        g[5 * j] = sc * (r - L.pi[j]);
-        const float k = L.kappa[j];
-        const float muw = L.mu[j][0] * w[0] + L.mu[j][1] * w[1] + L.mu[j][2] * w[2];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float r = pi[j] * f[j] / p;
+        const float k = kap[j];
+        const float muw = mu[j][0] * w[0] + mu[j][1] * w[1] + mu[j][2] * w[2];
         const float dlogk = 1.0f / k - 2.0f * expf(-2.0f * k) / (-expm1f(-2.0f * k)) + (muw - 1.0f);
-        g[5 * j + 4] = sc * r * dlogk * k * (1.0f - L.s[j] * L.s[j]);
+        g[5 * j] = live ? sc * (r - pi[j]) : 0.0f;
+        g[5 * j + 4] = live ? sc * r * dlogk * k * (1.0f - sv[j] * sv[j]) : 0.0f;
+#pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const float proj = k * w[c] - k * muw * L.mu[j][c];
-            g[5 * j + 1 + c] = sc * r * proj / L.nm[j];
+            const float proj = k * w[c] - k * muw * mu[j][c];
+            g[5 * j + 1 + c] = live ? sc * r * proj / nm[j] : 0.0f;
         }
     }
 }
 
 // ------------------------------------------------------------- training
-// Forward + loss gradient + backward for a batch. Stores every layer's input
-// activations X_l and output gradients dY_l as BF16, feature-major [64][S],
-// for the weight-gradient kernel. Per-sample losses go to loss_out.
-__global__ __launch_bounds__(256) void k_guide_train(GuideDevice g, GuideBox box, const float4* __restrict__ in,
-                                                     const float4* __restrict__ target, uint64_t n, float inv_batch,
-                                                     uint16_t* __restrict__ xs, uint16_t* __restrict__ dys,
-                                                     float* __restrict__ loss_out) {
+// One fused pass per block over its share of the batch: forward, loss gradient,
+// backward and the weight gradients, with nothing but the per-sample loss and
+// one partial gradient per block leaving the chip.
+//
+// Each wave takes 32-sample groups; the 4 waves of a block run in step, one
+// group each per round. Every layer input X_l stays in registers (B-operand
+// layout) from the forward to the backward pass, where it also gives the ReLU
+// masks. The last layer's rows are permuted as in inference (out_row), so each
+// half-wavefront holds four whole lobes of its sample and forms their loss
+// gradient in registers; the transposed copy of that layer has its columns
+// permuted the same way, and its weight and bias gradient rows are put back
+// in output order when they are written.
+//
+// For the weight gradient dW_l = sum over samples of dY_l X_l^T the block's 128
+// samples of X_l and dY_l are staged in LDS feature-major ([64 features][128
+// samples], BF16), the layout in which both MFMA operands read samples along k;
+// wave w accumulates block (w / 2, w % 2) of dW_l (32 x 32, 16 k-steps) in
+// registers across rounds, and thread t the bias gradient of row t / 4 over
+// samples 32 (t % 4) .. + 31. LDS holds the weights, their transposes (the
+// backward products' A operands) and the biases, 88 KB, and the staging, 33 KB.
+// Registers (the 5 layer inputs and 5 gradient blocks stay live through the
+// backward pass) allow one wave per SIMD: one block per CU. Summation order is
+// fixed, so results repeat bit for bit.
+constexpr int kStage = 132;  // staging row stride in BF16 (66 dwords: rows two banks apart)
+constexpr int kStageTile = 64 * kStage;
+
+// Writes a wave's 64 x 32 B-operand tile (feature 8s + 4h + j, sample lane % 32) into the
+// block's staging tile at sample column 32 * wave.
+__device__ __forceinline__ void stage(uint16_t* __restrict__ t, const s4 v[8], int lane, int wave) {
+    const int col = 32 * wave + (lane & 31), h = lane >> 5;
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) t[(8 * s + 4 * h + j) * kStage + col] = (uint16_t)v[s][j];
+}
+
+__global__ __launch_bounds__(kBlock, 1) void k_guide_train(GuideDevice g, GuideBox box, const float4* __restrict__ in,
+                                                        const float4* __restrict__ target, uint64_t n, float inv_batch,
+                                                        uint32_t rounds, float* __restrict__ partial,
+                                                        float* __restrict__ bpartial, float* __restrict__ loss_out) {
     __shared__ uint16_t wl[kLayers * kLayerLds];
     __shared__ uint16_t wt[kLayers * kLayerLds];
-    __shared__ float tile[4][32][65];
-    load_weights_lds(g.wbf, wl);
-    load_weights_lds(g.wbft, wt);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint64_t groups = (n + 31) / 32;
-    // every wave of the block runs the same number of rounds (the rounds hold block barriers)
-    for (uint64_t base = (uint64_t)blockIdx.x * 4; base < groups; base += (uint64_t)gridDim.x * 4) {
-        const uint64_t i = (base + wave) * 32 + (lane & 31);
+    __shared__ uint16_t stg[2 * kStageTile];  // X tile, dY tile
+    __shared__ float bl[kLayers * 64];
+    uint16_t* xt = stg;
+    uint16_t* dt = stg + kStageTile;
+    for (int k = threadIdx.x; k < kLayers * 64; k += kBlock) bl[k] = g.bias[k];
+    load_weights_lds(g.wbf, wl, true);
+    load_weights_lds(g.wbft, wt, false, true);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, h = lane >> 5;
+    const int mbw = wave >> 1, nbw = wave & 1;  // this wave's block of every dW_l
+    const int brow = threadIdx.x >> 2, bq = threadIdx.x & 3;  // this thread's bias row and sample quarter
+    f16v dw[kLayers];
+    float bsum[kLayers];
+#pragma unroll
+    for (int l = 0; l < kLayers; ++l) {
+        dw[l] = (f16v){0};
+        bsum[l] = 0.0f;
+    }
+    // sample index of this lane in round r; a round's inputs are loaded one round ahead
+    auto index = [&](uint32_t r) { return (((uint64_t)blockIdx.x * rounds + r) * 4 + wave) * 32 + (lane & 31); };
+    auto fetch = [&](uint64_t i, Sample& smp, float4& tg) {
+        smp = i < n ? load_sample(in, i) : Sample{{0, 0, 0}, {0, 0, 1}, {0, 0, 1}};
+        tg = i < n ? target[i] : make_float4(0, 0, 1, 0);
+    };
+    Sample next_smp;
+    float4 next_tg;
+    fetch(index(0), next_smp, next_tg);
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint64_t i = index(r);
         const bool valid = i < n;
-        Sample smp = valid ? load_sample(in, i) : Sample{{0, 0, 0}, {0, 0, 1}, {0, 0, 1}};
+        const Sample smp = next_smp;
+        const float4 tg = next_tg;
+        if (r + 1 < rounds) fetch(index(r + 1), next_smp, next_tg);
         float q[3];
         normalise_pos(smp, box, q);
-        s4 xb[8];
-        encode_b(q, smp.wo, smp.n, lane, xb);
+        s4 xs[kLayers][8];
+        encode_b(q, smp.wo, smp.n, lane, xs[0]);
         f16v acc[2];
+        s4 nx[8];
+#pragma unroll
         for (int l = 0; l < kLayers; ++l) {
-            // store X_l (B operand, feature-major) for dW
-            if (valid)
-                for (int s = 0; s < 8; ++s) {
-                    const int f = 8 * s + 4 * (lane >> 5);
-                    for (int j = 0; j < 4; ++j) xs[((uint64_t)l * 64 + f + j) * n + i] = (uint16_t)xb[s][j];
-                }
-            layer_mm(wl + l * kLayerLds, xb, lane, acc);
-            epilogue(acc, g.bias + l * 64, l < kLayers - 1, lane, xb);
+            layer_mm(wl + l * kLayerLds, xs[l], lane, acc);
+            epilogue(acc, bl + l * 64, l < kLayers - 1, lane, nx, l == kLayers - 1);
+            if (l + 1 < kLayers)
+#pragma unroll
+                for (int s = 0; s < 8; ++s) xs[l + 1][s] = nx[s];
         }
-        // gather the 40 outputs of each sample, form d loss / d out
-        for (int mb = 0; mb < 2; ++mb)
-            for (int r = 0; r < 16; ++r) tile[wave][lane & 31][acc_row(mb, r, lane)] = acc[mb][r];
-        __syncthreads();
-        if (lane < 32) {
-            float o[kOut], gr[kOut], ls = 0.0f;
-            for (int k = 0; k < kOut; ++k) o[k] = tile[wave][lane][k];
-            const float4 tg = valid ? target[i] : make_float4(0, 0, 1, 0);
-            const float w[3] = {tg.x, tg.y, tg.z};
-            out_grad(o, w, valid ? tg.w : 0.0f, inv_batch, gr, &ls);
-            for (int k = 0; k < 64; ++k) tile[wave][lane][k] = k < kOut ? gr[k] : 0.0f;
-            if (valid) loss_out[i] = ls;
-        }
-        __syncthreads();
-        // backward: d of layer-l input = W_l^T dY_l, masked by the ReLU of that input
+        // outputs 20h .. 20h + 19 of this lane's sample: accumulators 0-15 of block 0, 0-3 of block 1
+        float o[20], gr[20], ls;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) o[k] = acc[0][k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) o[16 + k] = acc[1][k];
+        const float w[3] = {tg.x, tg.y, tg.z};
+        out_grad_half(o, w, tg.w, inv_batch, gr, &ls);
+        if (valid && h == 0) loss_out[i] = ls;
+        // dY of the last layer in the same (permuted) accumulator layout, as B operands
         s4 db[8];
-        for (int s = 0; s < 8; ++s) {
-            const int f = 8 * s + 4 * (lane >> 5);
-            db[s] = pack4(tile[wave][lane & 31][f], tile[wave][lane & 31][f + 1], tile[wave][lane & 31][f + 2],
-                          tile[wave][lane & 31][f + 3]);
-        }
-        __syncthreads();
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) db[qq] = pack4(gr[4 * qq], gr[4 * qq + 1], gr[4 * qq + 2], gr[4 * qq + 3]);
+        db[4] = pack4(gr[16], gr[17], gr[18], gr[19]);
+        db[5] = db[6] = db[7] = (s4){0, 0, 0, 0};
+#pragma unroll
         for (int l = kLayers - 1; l >= 0; --l) {
-            if (valid)
-                for (int s = 0; s < 8; ++s) {
-                    const int f = 8 * s + 4 * (lane >> 5);
-                    for (int j = 0; j < 4; ++j) dys[((uint64_t)l * 64 + f + j) * n + i] = (uint16_t)db[s][j];
+            __syncthreads();  // the previous reads of the staging tiles are done
+            stage(xt, xs[l], lane, wave);
+            stage(dt, db, lane, wave);
+            __syncthreads();
+            // dW_l over the block's 128 samples of this round
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                const s4 a = *reinterpret_cast<const s4*>(dt + (32 * mbw + (lane & 31)) * kStage + 8 * t + 4 * h);
+                const s4 b = *reinterpret_cast<const s4*>(xt + (32 * nbw + (lane & 31)) * kStage + 8 * t + 4 * h);
+                dw[l] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, dw[l], 0, 0, 0);
+            }
+            {  // bias: row brow, samples 32 bq .. 32 bq + 31, in order
+                const uint2* rp = reinterpret_cast<const uint2*>(dt + brow * kStage + 32 * bq);
+                float sum = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const uint2 v = rp[k];
+                    sum += __uint_as_float(v.x << 16);
+                    sum += __uint_as_float(v.x & 0xFFFF0000u);
+                    sum += __uint_as_float(v.y << 16);
+                    sum += __uint_as_float(v.y & 0xFFFF0000u);
                 }
+                bsum[l] += sum;
+            }
             if (l == 0) break;
+            // d of layer-l input = W_l^T dY_l, masked by the ReLU that made X_l: X_l is +0 or
+            // positive in BF16, so the mask is X_l != 0, applied to the rounded values
             layer_mm(wt + l * kLayerLds, db, lane, acc);
-            // mask by ReLU'(pre-activation of layer l-1) = (X_l > 0)
+#pragma unroll
             for (int mb = 0; mb < 2; ++mb)
-                for (int r = 0; r < 16; ++r) {
-                    const int row = acc_row(mb, r, lane);
-                    const uint16_t xv = valid ? xs[((uint64_t)l * 64 + row) * n + i] : 0;
-                    if (!(bf2f(xv) > 0.0f)) acc[mb][r] = 0.0f;
+#pragma unroll
+                for (int qq = 0; qq < 4; ++qq) {
+                    const s4 d = pack4(acc[mb][4 * qq], acc[mb][4 * qq + 1], acc[mb][4 * qq + 2], acc[mb][4 * qq + 3]);
+                    db[4 * mb + qq] = d & (xs[l][4 * mb + qq] != (s4){0, 0, 0, 0});
                 }
-            for (int mb = 0; mb < 2; ++mb)
-                for (int qq = 0; qq < 4; ++qq)
-                    db[4 * mb + qq] = pack4(acc[mb][4 * qq], acc[mb][4 * qq + 1], acc[mb][4 * qq + 2], acc[mb][4 * qq + 3]);
         }
+    }
+    // this block's partial gradients (summed over blocks in the Adam kernel), last layer rows in
+    // output order
+    float* dst = partial + (uint64_t)blockIdx.x * kLayers * 4096;
+#pragma unroll
+    for (int l = 0; l < kLayers; ++l)
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int row = acc_row(mbw, k, lane);
+            dst[l * 4096 + (l == kLayers - 1 ? out_row(row) : row) * 64 + 32 * nbw + (lane & 31)] = dw[l][k];
+        }
+#pragma unroll
+    for (int l = 0; l < kLayers; ++l) {
+        float v = bsum[l];
+        v += __shfl_xor(v, 1);  // (q + (q ^ 1)) + ((q ^ 2) + (q ^ 3))
+        v += __shfl_xor(v, 2);
+        if (bq == 0) bpartial[((uint64_t)blockIdx.x * kLayers + l) * 64 + (l == kLayers - 1 ? out_row(brow) : brow)] = v;
     }
 }
 
-// Weight gradients: for layer l, output block mb, input block nb and sample
-// chunk c, partial[c][l][32mb + row][32nb + col] = sum over the chunk of
-// dY_l[row][s] * X_l[col][s]; bias partials = sum of dY_l over the chunk.
-__global__ __launch_bounds__(64) void k_guide_dw(const uint16_t* __restrict__ xs, const uint16_t* __restrict__ dys,
-                                                 uint64_t n, uint64_t chunk, float* __restrict__ partial,
-                                                 float* __restrict__ bpartial) {
-    const int lane = threadIdx.x;
-    const int l = blockIdx.y / 4, mb = (blockIdx.y / 2) & 1, nb = blockIdx.y & 1;
-    const uint64_t c = blockIdx.x;
-    const uint64_t s0 = c * chunk, s1 = s0 + chunk < n ? s0 + chunk : n;
-    f16v acc = {0};
-    float bsum = 0.0f;
-    const uint64_t rowA = (uint64_t)l * 64 + 32 * mb + (lane & 31);
-    const uint64_t rowB = (uint64_t)l * 64 + 32 * nb + (lane & 31);
-    for (uint64_t s = s0; s < s1; s += 8) {
-        s4 a, b;
-        for (int j = 0; j < 4; ++j) {
-            const uint64_t k = s + 4 * (lane >> 5) + j;
-            a[j] = (short)(k < s1 ? dys[rowA * n + k] : 0);
-            b[j] = (short)(k < s1 ? xs[rowB * n + k] : 0);
-            if (nb == 0) bsum += bf2f((uint16_t)a[j]);
-        }
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
-    }
-    float* dst = partial + ((c * kLayers + l) * 64) * 64;
-    for (int r = 0; r < 16; ++r) dst[(uint64_t)(32 * mb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)) * 64 + 32 * nb + (lane & 31)] = acc[r];
-    if (nb == 0) {
-        // lanes l and l+32 hold halves of the same row's sum
-        const float other = __shfl_down(bsum, 32);
-        if (lane < 32) bpartial[(c * kLayers + l) * 64 + 32 * mb + lane] = bsum + other;
-    }
-}
-
-// Sums the partials into the gradient of every parameter (oracle flat order),
-// takes one Adam step on the FP32 master weights and refreshes the BF16 copies.
-__global__ __launch_bounds__(256) void k_guide_adam(GuideDevice g, const float* __restrict__ partial,
-                                                    const float* __restrict__ bpartial, uint64_t chunks, float lr,
-                                                    float b1, float b2, float eps, int t, float* __restrict__ grad_out) {
-    const int idx = blockIdx.x * 256 + threadIdx.x;
-    if (idx >= kParamsTotal) return;
+// Sums the per-block partials into the gradient of every parameter (oracle flat
+// order), takes one Adam step on the FP32 master weights and refreshes the BF16
+// copies. A block takes 64 consecutive parameters with 16 slices of threads:
+// slice k sums partials k, k + 16, ..., then the 16 slice sums are added in
+// slice order, so the result does not depend on timing.
+constexpr int kAdamParams = 64, kAdamSlices = 16;
+__global__ __launch_bounds__(kAdamParams* kAdamSlices) void k_guide_adam(
+    GuideDevice g, const float* __restrict__ partial, const float* __restrict__ bpartial, uint64_t chunks, float lr,
+    float b1, float b2, float eps, int t, float* __restrict__ grad_out) {
+    __shared__ float red[kAdamSlices][kAdamParams];
+    const int p = threadIdx.x % kAdamParams, slice = threadIdx.x / kAdamParams;
+    const int idx = blockIdx.x * kAdamParams + p;
     // locate (layer, row, col) of flat index idx: per layer W (rows x cols) then b (rows)
     int l = 0, base = 0;
-    int rows = 64, cols = 64;
-    for (;;) {
+    int rows = 64;
+    const int cols = 64;
+    for (; l < kLayers; ++l) {
         rows = l == kLayers - 1 ? kOut : 64;
-        cols = 64;
         const int sz = rows * cols + rows;
         if (idx < base + sz) break;
         base += sz;
-        ++l;
     }
     const int off = idx - base;
-    float gsum = 0.0f;
     int row, col = -1;
+    const float* src;
+    uint64_t stride;
     if (off < rows * cols) {
         row = off / cols;
         col = off % cols;
-        for (uint64_t c = 0; c < chunks; ++c) gsum += partial[((c * kLayers + l) * 64 + row) * 64 + col];
+        src = partial + ((uint64_t)l * 64 + row) * 64 + col;
+        stride = (uint64_t)kLayers * 4096;
     } else {
         row = off - rows * cols;
-        for (uint64_t c = 0; c < chunks; ++c) gsum += bpartial[(c * kLayers + l) * 64 + row];
+        src = bpartial + (uint64_t)l * 64 + row;
+        stride = (uint64_t)kLayers * 64;
     }
+    float part = 0.0f;
+    if (idx < kParamsTotal)
+        for (uint64_t c = slice; c < chunks; c += kAdamSlices) part += src[c * stride];
+    red[slice][p] = part;
+    __syncthreads();
+    if (slice != 0 || idx >= kParamsTotal) return;
+    float gsum = 0.0f;
+    for (int k = 0; k < kAdamSlices; ++k) gsum += red[k][p];
     if (grad_out) grad_out[idx] = gsum;
     // Adam (Kingma & Ba 2015) with bias correction
     const float m = b1 * g.m[idx] + (1.0f - b1) * gsum;
@@ -450,8 +545,8 @@ hipError_t guide_infer(const GuideDevice& g, const GuideBox& box, const float4* 
     if (n == 0) return hipSuccess;
     const uint64_t need = (n + 127) / 128;
     const int gr = (int)(need < (uint64_t)grid ? need : (uint64_t)grid);
-    if (mixture) hipLaunchKernelGGL(k_guide_infer<true>, dim3(gr), dim3(256), 0, s, g, box, in, n, n_dev, out);
-    else hipLaunchKernelGGL(k_guide_infer<false>, dim3(gr), dim3(256), 0, s, g, box, in, n, n_dev, out);
+    if (mixture) hipLaunchKernelGGL(k_guide_infer<true>, dim3(gr), dim3(kBlock), 0, s, g, box, in, n, n_dev, out);
+    else hipLaunchKernelGGL(k_guide_infer<false>, dim3(gr), dim3(kBlock), 0, s, g, box, in, n, n_dev, out);
     return hipGetLastError();
 }
 
@@ -459,15 +554,16 @@ hipError_t guide_train_step(const GuideDevice& g, const GuideBox& box, const flo
                             uint64_t n, const GuideScratch& sc, float lr, int t, int grid, float* grad_out,
                             hipStream_t s) {
     if (n == 0) return hipSuccess;
-    const uint64_t need = (n + 127) / 128;
-    const int gr = (int)(need < (uint64_t)grid ? need : (uint64_t)grid);
-    hipLaunchKernelGGL(k_guide_train, dim3(gr), dim3(256), 0, s, g, box, in, target, n, 1.0f / (float)n, sc.xs,
-                       sc.dys, sc.loss);
-    const uint64_t chunks = (n + sc.chunk - 1) / sc.chunk;
-    hipLaunchKernelGGL(k_guide_dw, dim3((uint32_t)chunks, kLayers * 4), dim3(64), 0, s, sc.xs, sc.dys, n, sc.chunk,
-                       sc.partial, sc.bpartial);
-    hipLaunchKernelGGL(k_guide_adam, dim3((kParamsTotal + 255) / 256), dim3(256), 0, s, g, sc.partial, sc.bpartial,
-                       chunks, lr, 0.9f, 0.999f, 1e-8f, t, grad_out);
+    // blocks of 4 waves x `rounds` groups of 32 samples, at most sc.blocks of them (their partials)
+    const uint64_t groups = (n + 31) / 32;
+    uint64_t blocks = std::min<uint64_t>((uint64_t)std::min(grid, (int)sc.blocks), (groups + 3) / 4);
+    const uint32_t rounds = (uint32_t)((groups + 4 * blocks - 1) / (4 * blocks));
+    blocks = (groups + 4ull * rounds - 1) / (4ull * rounds);
+    hipLaunchKernelGGL(k_guide_train, dim3((uint32_t)blocks), dim3(kBlock), 0, s, g, box, in, target, n,
+                       1.0f / (float)n, rounds, sc.partial, sc.bpartial, sc.loss);
+    hipLaunchKernelGGL(k_guide_adam, dim3((kParamsTotal + kAdamParams - 1) / kAdamParams),
+                       dim3(kAdamParams * kAdamSlices), 0, s, g, sc.partial, sc.bpartial, blocks, lr, 0.9f, 0.999f,
+                       1e-8f, t, grad_out);
     return hipGetLastError();
 }
 

@@ -34,7 +34,7 @@ struct GuideHostBufs {
 // Device views of the guide's single allocation.
 GuideHostBufs carve(bg_guide* g, size_t* total_out) {
     const uint64_t cap = g->scratch.cap;
-    const uint64_t chunks = (cap + g->scratch.chunk - 1) / g->scratch.chunk;
+    const uint64_t blocks = g->scratch.blocks;
     char* p = (char*)g->mem;
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -53,16 +53,14 @@ GuideHostBufs carve(bg_guide* g, size_t* total_out) {
     b.in = (float4*)take(48 * cap);
     b.target = (float4*)take(16 * cap);
     b.out = (float*)take(4 * BG_GUIDE_OUTPUTS * cap);
-    g->scratch.xs = (uint16_t*)take(5 * 64 * 2 * cap);
-    g->scratch.dys = (uint16_t*)take(5 * 64 * 2 * cap);
     g->scratch.loss = (float*)take(4 * cap);
-    g->scratch.partial = (float*)take(chunks * 5 * 64 * 64 * 4);
-    g->scratch.bpartial = (float*)take(chunks * 5 * 64 * 4);
+    g->scratch.partial = (float*)take(blocks * 5 * 64 * 64 * 4);
+    g->scratch.bpartial = (float*)take(blocks * 5 * 64 * 4);
     if (total_out) *total_out = off;
     return b;
 }
 
-int grid_of(const bg_guide* g) { return 2 * (g->ctx->num_cus > 0 ? g->ctx->num_cus : 1); }
+int grid_of(const bg_guide* g) { return bg::kTrainBlocksPerCu * (g->ctx->num_cus > 0 ? g->ctx->num_cus : 1); }
 int infer_grid_of(const bg_guide* g) { return bg::kInferBlocksPerCu * (g->ctx->num_cus > 0 ? g->ctx->num_cus : 1); }
 
 }  // namespace
@@ -78,6 +76,7 @@ int bg_guide_create(bg_ctx* ctx, const float* params, uint64_t max_batch, bg_gui
     if (!g) return fail(ctx, BG_ERR_NOMEM, "bg_guide_create: out of host memory");
     g->ctx = ctx;
     g->scratch.cap = max_batch;
+    g->scratch.blocks = (uint32_t)grid_of(g);
     size_t total = 0;
     carve(g, &total);
     hipError_t e = hipMalloc(&g->mem, total);

@@ -442,7 +442,7 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
             guide->box.hi[k] = root.bmax[k] + 1e-3f * ext;
         }
     }
-    const int guide_grid = 2 * std::max(ctx->num_cus, 1);
+    const int guide_grid = bg::kTrainBlocksPerCu * std::max(ctx->num_cus, 1);
     const int infer_grid = bg::kInferBlocksPerCu * std::max(ctx->num_cus, 1);
     uint64_t train_steps = 0;
     const bg::ShadeSceneDev sc = shade_scene(s);
@@ -512,7 +512,7 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
                 HIP_TRY(ctx, bg::guide_train_step(guide->dev, guide->box, guide->batch_in, guide->batch_tgt, nb,
                                                   guide->scratch, lr, guide->step, guide_grid, nullptr, st));
                 ++train_steps;
-                launches += 4;
+                launches += 3;  // gather, train, Adam
             }
             launches += 4;
         }

@@ -90,12 +90,24 @@ def test_ragged_batches_are_slices_of_one_batch(ctx):
     np.testing.assert_array_equal(g.eval(s[300:333]), full[300:333])
 
 
-def test_gradient_matches_fp64(ctx):
+@pytest.mark.parametrize("n", [5000, 40000])
+def test_gradient_matches_fp64(ctx, n):
+    """n = 5000: 157 groups of 32 samples on 40 blocks, one round each, the last group ragged;
+    n = 40000: 1250 groups, several rounds per block (each block accumulates its partial
+    gradient across rounds).
+
+    The backward pass is checked on the GPU's own forward outputs: the BF16-emulating oracle's
+    backward from them, with the same rounding points. The loss gradient of a few samples (a
+    narrow lobe whose responsibility turns on the last bits of the outputs) moves by a large
+    fraction under the 1e-3-relative output differences between the GPU's and the emulated
+    forward pass, and at 40000 samples one such sample (6529) alone moves layer 0's weight
+    gradient by 3%: that is the forward pass's rounding, bounded on the outputs here, not an
+    error of the backward pass. End to end the gradients are also held to the FP64 oracle."""
     flat = params(5)
-    n = 5000  # three sample chunks of the weight-gradient reduction, the last one ragged
     p, wo, nrm, w, weight = batch(6, n)
     g = make(ctx, flat, n)
     targets = np.concatenate([w, weight[:, None]], 1).astype(np.float32)
+    gout = g.eval(pack_samples(p, wo, nrm)).astype(np.float64)
     loss, grad = g.train(pack_samples(p, wo, nrm), targets, lr=1e-3, want_grad=True)
     X = oracle_inputs(p, wo, nrm)
     # oracle on the GPU's own target directions (rounded to f32, as the GPU sees them)
@@ -105,7 +117,8 @@ def test_gradient_matches_fp64(ctx):
     wt = targets[:, 3].astype(np.float64)
     ref = G.backward(flat.astype(np.float64), cache, G.loss_grad_out(out, w32, wt))
     eout, ecache = G.forward_bf16(flat.astype(np.float64), X)
-    emu = G.backward_bf16(flat.astype(np.float64), ecache, G.loss_grad_out(eout, w32, wt))
+    assert rel_l2(gout, eout) < 2e-3
+    emu = G.backward_bf16(flat.astype(np.float64), ecache, G.loss_grad_out(gout, w32, wt))
     assert abs(loss - ref_loss) < 1e-2 * abs(ref_loss)
 
     def per_part(r):
@@ -118,7 +131,8 @@ def test_gradient_matches_fp64(ctx):
                 o += size
         return errs
 
-    for label, r, bound in (("BF16-emulating oracle", emu, (1.5e-2, 0.9999)), ("FP64 oracle", ref, (0.15, 0.99))):
+    for label, r, bound in (("BF16-emulating backward on the GPU outputs", emu, (1e-2, 0.9999)),
+                            ("FP64 oracle", ref, (0.15, 0.99))):
         errs = per_part(r)
         print(f"gradients vs {label} (layer+part rel-L2/cosine):",
               ", ".join(f"{li}{nm} {e:.1e}/{c:.5f}" for li, nm, e, c in errs))
