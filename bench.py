@@ -64,6 +64,28 @@ def path_alg_bytes(st) -> float:
     return alg_bytes(int(st.rays), st)
 
 
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "k_trav_pairs_intersection_1.31Mtri_2p24rays.md")
+
+
+def traffic_from_profile(path: str, query: str):
+    """HBM bytes per launch of one query's traversal kernel as the committed rocprofv3 summary of this
+    workload gives them: its FETCH_SIZE and WRITE_SIZE table (KiB per dispatch) scaled by the copy
+    calibration factors stated beside it. None if the summary is absent or lacks the row."""
+    import re
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    ff = re.search(r"FETCH_SIZE \(factor ([\d.]+)\)", text)
+    wf = re.search(r"WRITE_SIZE \(factor ([\d.]+)\)", text)
+    label = query.replace("_", " ").replace(" hit", "")  # "incoherent_closest" -> "incoherent closest"
+    row = re.search(r"^\| " + re.escape(label) + r" \| ([\d,]+) \| [\d.]+ \| ([\d,]+) ->", text, flags=re.M)
+    if not (ff and wf and row):
+        return None
+    f, w = (float(row.group(k).replace(",", "")) for k in (1, 2))
+    return round((f * float(ff.group(1)) + w * float(wf.group(1))) * 1024.0, 0)
+
+
 def traffic_from(prof_dir: str, any_hit: bool, set_index: int):
     """HBM bytes per launch of the timed traversal kernel for one query, from rocprofv3 --pmc runs of
     `bench.py --no-cpu --steps 1 --warmup 0` (FETCH_SIZE and WRITE_SIZE, KiB per dispatch) scaled by the
@@ -776,7 +798,9 @@ def main() -> int:
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": (traffic_from(args.traffic_from, dom.endswith("_any"), 1 if dom.startswith("incoherent") else 0)
-                                if args.traffic_from else None),
+                                if args.traffic_from else traffic_from_profile(TRAFFIC_PROFILE, dom)),
+                    "traffic_source": (f"rocprofv3 --pmc runs in {args.traffic_from}" if args.traffic_from else
+                                       "profiles/" + os.path.basename(TRAFFIC_PROFILE)),
                     "kernel": f"k_traverse ({dom})",
                     "bytes_per_launch": counts[dom][0]}
 
