@@ -120,8 +120,9 @@ int grid_for(const bg_ctx* ctx, const bg::LaunchShape& sh, uint64_t n) {
     return (int)std::max<uint64_t>(1, std::min<uint64_t>(std::min(tiles, cap), 1u << 20));
 }
 
+// mask: the ray masks the kernel that wrote the rays stored (PathBuffers::wmask).
 hipError_t traverse(const bg_batch* bt, bg_ctx* ctx, const bg::LaunchShape& sh, const float4* rays, uint4* hits,
-                    uint64_t n, const uint32_t* roots, bool any, hipStream_t st) {
+                    uint64_t n, const uint32_t* roots, bool any, const uint64_t* mask, hipStream_t st) {
     const bool queue = sh.variant == 2;
     if (queue) {
         hipError_t e = bg::launch_zero_u64(bt->d_queue, st);
@@ -134,7 +135,7 @@ hipError_t traverse(const bg_batch* bt, bg_ctx* ctx, const bg::LaunchShape& sh, 
     if (bg::traverse_compacts(a, shape, grid)) {
         // only the rays that can hit, through a device index list (the others' hits become misses)
         const bg::PathBuffers& b = ctx->paths;
-        const hipError_t e = bg::launch_ray_compact(rays, n, b.rblk, b.rmask, b.ridx, b.rcount, hits, st);
+        const hipError_t e = bg::launch_ray_compact(rays, n, b.rblk, b.rmask, b.ridx, b.rcount, hits, st, mask);
         if (e != hipSuccess) return e;
         a.idx = b.ridx;
         a.n_idx = b.rcount;
@@ -421,10 +422,10 @@ int bg_render_batch(bg_ctx* ctx, bg_batch* bt, const bg_batch_desc* d, bg_stats*
         HIP_TRY(ctx, hipMemsetAsync(bt->d_pass, 0, sizeof(uint32_t), st));
         HIP_TRY(ctx, bg::launch_init_paths(p, b, st));
         std::vector<float> score(cands.size(), 0.0f);
-        auto time_all = [&](const float4* rays, uint4* hits, bool any) -> int {
+        auto time_all = [&](const float4* rays, uint4* hits, bool any, const uint64_t* mask) -> int {
             for (size_t c = 0; c < cands.size(); ++c) {
                 HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
-                HIP_TRY(ctx, traverse(bt, ctx, cands[c], rays, hits, cap, b.root, any, st));
+                HIP_TRY(ctx, traverse(bt, ctx, cands[c], rays, hits, cap, b.root, any, mask, st));
                 HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
                 HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
                 float ms = 0.0f;
@@ -433,19 +434,19 @@ int bg_render_batch(bg_ctx* ctx, bg_batch* bt, const bg_batch_desc* d, bg_stats*
             }
             return BG_OK;
         };
-        rc = time_all(b.ray, b.hit, false);
+        rc = time_all(b.ray, b.hit, false, b.wmask[0]);
         if (rc) return rc;
         HIP_TRY(ctx, bg::launch_shade(p, sc, b, 0, st));
-        rc = time_all(b.ray, b.hit, false);
-        if (!rc && sc.n_lights) rc = time_all(b.sray, b.shit, true);
+        rc = time_all(b.ray, b.hit, false, b.wmask[0]);
+        if (!rc && sc.n_lights) rc = time_all(b.sray, b.shit, true, b.wmask[1]);
         if (rc) return rc;
         // and a deep bounce, where ended paths leave most slots empty (what compaction is for)
         const uint32_t deep = std::min<uint32_t>(4, d->max_depth);
         for (uint32_t bounce = 1; bounce < deep; ++bounce) {
-            HIP_TRY(ctx, traverse(bt, ctx, bt->shape, b.ray, b.hit, cap, b.root, false, st));
+            HIP_TRY(ctx, traverse(bt, ctx, bt->shape, b.ray, b.hit, cap, b.root, false, b.wmask[0], st));
             HIP_TRY(ctx, bg::launch_shade(p, sc, b, bounce, st));
         }
-        if (deep > 1) rc = time_all(b.ray, b.hit, false);
+        if (deep > 1) rc = time_all(b.ray, b.hit, false, b.wmask[0]);
         if (rc) return rc;
         size_t best = 0;
         for (size_t c = 1; c < cands.size(); ++c)
@@ -464,12 +465,12 @@ int bg_render_batch(bg_ctx* ctx, bg_batch* bt, const bg_batch_desc* d, bg_stats*
         if ((e = bg::launch_init_paths(p, b, s)) != hipSuccess) return e;
         ++launches;
         for (uint32_t bounce = 0; bounce <= d->max_depth; ++bounce) {
-            if ((e = traverse(bt, ctx, sh, b.ray, b.hit, cap, b.root, false, s)) != hipSuccess) return e;
+            if ((e = traverse(bt, ctx, sh, b.ray, b.hit, cap, b.root, false, b.wmask[0], s)) != hipSuccess) return e;
             if ((e = bg::launch_shade(p, sc, b, bounce, s)) != hipSuccess) return e;
             launches += 2;
             if (sc.n_lights > 0 && bounce < d->max_depth) {
-                if ((e = traverse(bt, ctx, sh, b.sray, b.shit, cap, b.root, true, s)) != hipSuccess) return e;
-                if ((e = traverse(bt, ctx, sh, b.mray, b.mhit, cap, b.root, false, s)) != hipSuccess) return e;
+                if ((e = traverse(bt, ctx, sh, b.sray, b.shit, cap, b.root, true, b.wmask[1], s)) != hipSuccess) return e;
+                if ((e = traverse(bt, ctx, sh, b.mray, b.mhit, cap, b.root, false, b.wmask[2], s)) != hipSuccess) return e;
                 if ((e = bg::launch_nee_resolve(p, sc, b, bounce, s)) != hipSuccess) return e;
                 launches += 3;
             }

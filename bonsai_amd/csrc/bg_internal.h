@@ -179,6 +179,9 @@ struct PathBuffers {
     uint32_t* ridx = nullptr;   // compacted launches: indices of the rays that can hit
     uint32_t* rblk = nullptr;   // per 256 rays: count, then its exclusive prefix
     uint64_t* rmask = nullptr;  // per 64 rays: which of them can hit (bit l for ray 64 k + l)
+    // the same masks for the extension, shadow and MIS-branch rays (ray, sray, mray), stored by the
+    // kernel that wrote the rays (k_init_paths, k_shade) so a compacted launch need not read them
+    uint64_t* wmask[3] = {nullptr, nullptr, nullptr};
     uint32_t* rcount = nullptr; // their number (device)
     uint32_t* gslot = nullptr; // per path: its slot in gin/gout this bounce, ~0 when not live
     uint32_t* gcount = nullptr; // live-path count of this bounce (device)
@@ -256,8 +259,6 @@ hipError_t launch_nee_resolve(const RenderParamsDev& p, const ShadeSceneDev& sc,
                               uint32_t bounce, hipStream_t s);
 hipError_t launch_count_rays(const RenderParamsDev& p, const PathBuffers& b, unsigned long long* counters,
                              hipStream_t s);
-// Index list of the rays among n with t_max > 0 (ascending), their count in *count,
-// and a miss written to hits[i] of every other ray.
 // Whether launch_traverse with this shape and grid runs the refill kernel (variant 4 where it
 // refills: four or more rays per lane, no spheres, no index list).
 bool traverse_refills(const TraverseArgs& a, const LaunchShape& shape, int grid);
@@ -265,9 +266,11 @@ bool traverse_refills(const TraverseArgs& a, const LaunchShape& shape, int grid)
 // not refill, and it covers at least 1M ray slots.
 bool traverse_compacts(const TraverseArgs& a, const LaunchShape& shape, int grid);
 // Writes idx (ascending indices of the rays with t_max > 0), their number to *count and misses
-// for the other rays; blk holds ceil(n / 256) words and mask 4 * ceil(n / 256).
+// for the other rays; blk holds ceil(n / 256) words and mask 4 * ceil(n / 256). With `produced`
+// (the masks the kernel that wrote the rays stored, one word per 64 rays, the first ceil(n / 64)
+// of them read) the rays are not read at all; otherwise their t_max values make the masks in mask.
 hipError_t launch_ray_compact(const float4* rays, uint64_t n, uint32_t* blk, uint64_t* mask, uint32_t* idx,
-                              uint32_t* count, uint4* hits, hipStream_t s);
+                              uint32_t* count, uint4* hits, hipStream_t s, const uint64_t* produced = nullptr);
 hipError_t launch_guide_inputs(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b, hipStream_t s);
 hipError_t launch_guide_targets(const RenderParamsDev& p, const PathBuffers& b, hipStream_t s);
 // Appends the pass's nonzero-weight records to the ring (order-preserving,

@@ -261,11 +261,14 @@ __device__ __forceinline__ LightPoint sample_light_point(const ShadeSceneDev& sc
     return o;
 }
 
-// VisibilityTester's ray (spawn_ray_to_it, interaction.rs:132-143), t_max 1 - 1e-4.
-__device__ __forceinline__ void write_shadow_ray(float4* rays, uint64_t i, const Surf& s, const LightPoint& l) {
+// VisibilityTester's ray (spawn_ray_to_it, interaction.rs:132-143), t_max 1 - 1e-4; returns
+// the t_max it wrote.
+__device__ __forceinline__ float write_shadow_ray(float4* rays, uint64_t i, const Surf& s, const LightPoint& l) {
     const V o = offset_ray_origin(s.p, s.perr, s.n, l.p - s.p);
     const V tgt = offset_ray_origin(l.p, l.err, l.n, o - l.p);
-    write_ray(rays, i, o, 1.0f - 0.0001f, tgt - o, s.time);
+    const float t_max = 1.0f - 0.0001f;
+    write_ray(rays, i, o, t_max, tgt - o, s.time);
+    return t_max;
 }
 
 // Shape::pdf (shape.rs:40-57) on a light's triangle, or sphere.rs:356-373 on a sphere (the cone

@@ -66,9 +66,20 @@ __device__ __forceinline__ uint32_t item_of_pixel(const RenderParamsDev& p, uint
     return lo;
 }
 
-__global__ __launch_bounds__(256) void k_init_paths(RenderParamsDev p, PathBuffers b) {
-    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= p.n_paths) return;
+// Stores, from lane 0 of each wavefront, one word per ray set k < NSETS: bit l for slot
+// 64 w + l set where bit k of that lane's `live` is (the ray written there can hit). Every lane
+// of the wavefront must call it; lanes past the slot count pass 0.
+template <int NSETS>
+__device__ __forceinline__ void store_ray_masks(const PathBuffers& b, uint64_t i, uint32_t live) {
+#pragma unroll
+    for (int k = 0; k < NSETS; ++k) {
+        const uint64_t m = __ballot((live >> k) & 1u);
+        if ((threadIdx.x & 63) == 0) b.wmask[k][i >> 6] = m;
+    }
+}
+
+// One path's camera ray and state; returns the ray's t_max.
+__device__ __forceinline__ float init_path(const RenderParamsDev& p, const PathBuffers& b, uint64_t i) {
     const CameraDev* cam = &p.cam;
     uint64_t seed = p.seed, li = i;
     uint32_t x0 = p.x0, y0 = p.y0, x1 = p.x1, y1 = p.y1, item = 0;
@@ -103,13 +114,20 @@ __global__ __launch_bounds__(256) void k_init_paths(RenderParamsDev p, PathBuffe
     b.key[i] = make_uint2((uint32_t)key, (uint32_t)(key >> 32));
     b.state[i] = make_uint2(dim, kAlive | (item << kItemShift));  // the batch item rides in the flags
     b.film[i] = make_float2(fx, fy);
+    return r.t_max;
 }
 
-// MULTI: the scene has a two-lobe material (plastic)
+__global__ __launch_bounds__(256) void k_init_paths(RenderParamsDev p, PathBuffers b) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const uint32_t live = i < p.n_paths && init_path(p, b, i) > 0.0f ? 1u : 0u;
+    store_ray_masks<1>(b, i, live);
+}
+
+// One path's vertex; returns which of the rays it wrote can hit: bit 0 the extension ray, bit 1 the
+// shadow ray, bit 2 the MIS-branch ray (a killed ray has t_max = -inf).
 template <bool GUIDED, bool MULTI, bool SPL>
-__global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev sc, PathBuffers b, uint32_t bounce) {
-    const uint64_t i = (uint64_t)blockIdx.x * 128 + threadIdx.x;
-    if (i >= p.n_paths) return;
+__device__ __forceinline__ uint32_t shade_path(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b,
+                                               uint32_t bounce, uint64_t i) {
     uint2 st = b.state[i];
     uint32_t flags = st.y;
     b.nee_a[i].w = 0.0f;
@@ -118,14 +136,15 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
     uint64_t rec = 0;
     const bool recording = rec_slot(p, i, bounce, &rec);
     if (recording) b.rec_tgt[rec] = make_float4(0.0f, 0.0f, 1.0f, 0.0f);  // .w = 0: no sample at this slot
-    if (!(flags & kAlive)) return;
+    if (!(flags & kAlive)) return 0u;
     const bool ref = p.mode == 0;
+    float ext_t = -__builtin_inff(), shadow_t = -__builtin_inff(), mis_t = -__builtin_inff();
     const uint4 h = b.hit[i];
     const float4 r0 = b.ray[2 * i], r1 = b.ray[2 * i + 1];
     if (h.x == 0xFFFFFFFFu) {
         b.state[i] = make_uint2(st.x, 0u);
         kill_ray(b.ray, i);
-        return;
+        return 0u;
     }
     const uint32_t prim = h.x;
     Surf s;
@@ -133,7 +152,7 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
     if (!hit_surface(sc, r0, r1, prim, &s, &t)) {  // unreachable for a primitive the traversal accepted on this ray
         b.state[i] = make_uint2(st.x, 0u);
         kill_ray(b.ray, i);
-        return;
+        return 0u;
     }
     const uint64_t key = key_of(b.key[i]);
     uint32_t dim = st.x;
@@ -210,7 +229,7 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
                 const C3 f = bs.f(s.wo, lwi, nonspec) * __builtin_fabsf(dot(lwi, bs.ns));
                 scattering_pdf = bs.pdf(s.wo, lwi, nonspec);
                 if (!black(f)) {
-                    write_shadow_ray(b.sray, i, s, lpt);
+                    shadow_t = write_shadow_ray(b.sray, i, s, lpt);
                     float w;
                     if (GUIDED) {
                         const float pc = (1.0f - alpha) * scattering_pdf + (alpha > 0.0f ? alpha * guide_pdf(gm, lwi) : 0.0f);
@@ -236,6 +255,7 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
                     float tm;
                     spawn(s, mdir, ref, &o, &tm);
                     write_ray(b.mray, i, o, tm, mdir, s.time);
+                    mis_t = tm;
                     b.nee_b[i] = make_float4(fb.r, fb.g, fb.b, w);
                     b.nee_d[i] = make_float4(spdf, __uint_as_float(idx), 0.0f, 0.0f);
                     nee |= kHasBsdf;
@@ -303,11 +323,13 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
             if (alive) {
                 if (ref) {
                     b.ray[2 * i].w = t;  // Q1: the camera ray is re-intersected with t_max = t_hit
+                    ext_t = t;
                 } else {
                     V o;
                     float tm;
                     spawn(s, wi, false, &o, &tm);
                     write_ray(b.ray, i, o, tm, wi, s.time);
+                    ext_t = tm;
                 }
             }
         }
@@ -317,6 +339,15 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
     b.beta[i] = make_float4(beta.r, beta.g, beta.b, __uint_as_float(nrays));
     b.L[i] = make_float4(L.r, L.g, L.b, cont_pdf);  // .w: continuation pdf, for the next vertex's MIS
     b.state[i] = make_uint2(dim, alive ? (flags | kAlive) : 0u);
+    return (alive && ext_t > 0.0f ? 1u : 0u) | (shadow_t > 0.0f ? 2u : 0u) | (mis_t > 0.0f ? 4u : 0u);
+}
+
+// MULTI: the scene has a two-lobe material (plastic)
+template <bool GUIDED, bool MULTI, bool SPL>
+__global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev sc, PathBuffers b, uint32_t bounce) {
+    const uint64_t i = (uint64_t)blockIdx.x * 128 + threadIdx.x;
+    const uint32_t live = i < p.n_paths ? shade_path<GUIDED, MULTI, SPL>(p, sc, b, bounce, i) : 0u;
+    store_ray_masks<3>(b, i, live);
 }
 
 template <bool SPL>
@@ -465,15 +496,30 @@ __global__ __launch_bounds__(256) void k_ray_count(const float4* __restrict__ ra
     if (threadIdx.x == 0) blk[blockIdx.x] = total;
 }
 
+// Per 256 rays, how many can hit, from the wavefront masks the kernel that wrote the rays stored
+// (store_ray_masks): one thread per 256 rays, reading the block's words below ceil(n / 64).
+__global__ __launch_bounds__(256) void k_mask_count(const uint64_t* __restrict__ mask, uint64_t n,
+                                                    uint32_t* __restrict__ blk) {
+    const uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= (n + 255) / 256) return;
+    const uint64_t nw = (n + 63) / 64;
+    uint32_t c = 0;
+#pragma unroll
+    for (uint64_t w = 4 * j; w < 4 * j + 4; ++w)
+        if (w < nw) c += (uint32_t)__popcll(mask[w]);
+    blk[j] = c;
+}
+
 // After k_live_scan over blk: each ray that can hit at its block's offset plus its rank in the
 // block (ascending ray order); every other ray gets the miss the traversal would report. The
-// ranks come from the wavefront masks k_ray_count stored (bits past n are zero), so the rays
-// are not read again.
+// ranks come from wavefront masks (k_ray_count's, or the ray writer's: bits past n are zero,
+// words from ceil(n / 64) on are not read), so the rays are not read again.
 __global__ __launch_bounds__(256) void k_ray_compact(uint64_t n, const uint32_t* __restrict__ blk,
                                                      const uint64_t* __restrict__ mask, uint32_t* __restrict__ idx,
                                                      uint4* __restrict__ hits) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if ((i & ~63ull) >= n) return;  // a wavefront wholly past the rays (uniform)
     const uint64_t* bm = mask + (uint64_t)blockIdx.x * 4;
     uint32_t off = blk[blockIdx.x];
     for (uint32_t w = 0; w < wave; ++w) off += (uint32_t)__popcll(bm[w]);
@@ -618,12 +664,13 @@ hipError_t launch_count_rays(const RenderParamsDev& p, const PathBuffers& b, uns
     return hipGetLastError();
 }
 hipError_t launch_ray_compact(const float4* rays, uint64_t n, uint32_t* blk, uint64_t* mask, uint32_t* idx,
-                              uint32_t* count, uint4* hits, hipStream_t s) {
+                              uint32_t* count, uint4* hits, hipStream_t s, const uint64_t* produced) {
     if (n == 0) return hipSuccess;
     const uint32_t nblk = (uint32_t)((n + 255) / 256);
-    hipLaunchKernelGGL(k_ray_count, dim3(nblk), dim3(256), 0, s, rays, n, blk, mask);
+    if (produced) hipLaunchKernelGGL(k_mask_count, dim3((nblk + 255) / 256), dim3(256), 0, s, produced, n, blk);
+    else hipLaunchKernelGGL(k_ray_count, dim3(nblk), dim3(256), 0, s, rays, n, blk, mask);
     hipLaunchKernelGGL(k_live_scan, dim3(1), dim3(1024), 0, s, nblk, blk, count);
-    hipLaunchKernelGGL(k_ray_compact, dim3(nblk), dim3(256), 0, s, n, blk, mask, idx, hits);
+    hipLaunchKernelGGL(k_ray_compact, dim3(nblk), dim3(256), 0, s, n, blk, produced ? produced : mask, idx, hits);
     return hipGetLastError();
 }
 hipError_t launch_guide_inputs(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b, hipStream_t s) {

@@ -48,7 +48,7 @@ int bg::ensure_paths(bg_ctx* ctx, size_t cap) {
     b.last_records = kept.last_records;
     const size_t per_path[] = {32, 16, 16, 16, 8, 8, 8, 32, 16, 32, 16, 16, 16, 16, 16, 4};
     size_t total = 256 + ((4 * (cap / 256 + 1) + 255) / 256) * 256 + 256;  // + rblk, rcount
-    total += ((32 * (cap / 256 + 1) + 255) / 256) * 256;                    // + rmask
+    total += 4 * (((32 * (cap / 256 + 1) + 255) / 256) * 256);              // + rmask, wmask[3]
     for (size_t s : per_path) total += ((s * cap + 255) / 256) * 256;
     void* mem = nullptr;
     HIP_TRY(ctx, hipMalloc(&mem, total));
@@ -77,6 +77,7 @@ int bg::ensure_paths(bg_ctx* ctx, size_t cap) {
     b.ridx = (uint32_t*)take(4 * cap);
     b.rblk = (uint32_t*)take(4 * (cap / 256 + 1));
     b.rmask = (uint64_t*)take(32 * (cap / 256 + 1));
+    for (int k = 0; k < 3; ++k) b.wmask[k] = (uint64_t*)take(32 * (cap / 256 + 1));
     b.rcount = (uint32_t*)take(256);
     b.counters = (unsigned long long*)take(256);  // the 256 spare bytes of `total`
     b.cap = cap;
@@ -193,12 +194,13 @@ hipError_t profiled_traverse(bg_ctx* ctx, bg::TraverseArgs a, bool any, int grid
     return hipEventRecord(ctx->prof_events[2 * ctx->prof_used++ + 1], st);
 }
 
+// mask: the ray masks the kernel that wrote the rays stored (PathBuffers::wmask).
 hipError_t trace_pass(const bg_scene* s, bg_ctx* ctx, const float4* rays, uint4* hits, uint64_t n, bool any, int grid,
-                      hipStream_t st) {
+                      const uint64_t* mask, hipStream_t st) {
     bg::TraverseArgs a = traverse_args(s, ctx, rays, hits, n);
     if (bg::traverse_compacts(a, ctx->shape, grid)) {
         const bg::PathBuffers& b = ctx->paths;
-        const hipError_t e = bg::launch_ray_compact(rays, n, b.rblk, b.rmask, b.ridx, b.rcount, hits, st);
+        const hipError_t e = bg::launch_ray_compact(rays, n, b.rblk, b.rmask, b.ridx, b.rcount, hits, st, mask);
         if (e != hipSuccess) return e;
         a.idx = b.ridx;
         a.n_idx = b.rcount;
@@ -468,7 +470,7 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
         }
         HIP_TRY(ctx, bg::launch_init_paths(p, b, st));
         for (uint32_t bounce = 0; bounce <= d->max_depth; ++bounce) {
-            HIP_TRY(ctx, trace_pass(s, ctx, b.ray, b.hit, p.n_paths, false, g, st));
+            HIP_TRY(ctx, trace_pass(s, ctx, b.ray, b.hit, p.n_paths, false, g, b.wmask[0], st));
             if (guided && bounce < d->max_depth && bounce < p.guide_bounces && (p.guide_alpha > 0.0f || train_paths)) {
                 HIP_TRY(ctx, bg::launch_guide_inputs(p, sc, b, st));  // the records copy these inputs
                 launches += 3;
@@ -480,10 +482,10 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
             }
             HIP_TRY(ctx, bg::launch_shade(p, sc, b, bounce, st));
             if (sc.n_lights > 0 && bounce < d->max_depth) {
-                HIP_TRY(ctx, trace_pass(s, ctx, b.sray, b.shit, p.n_paths, true, g, st));
+                HIP_TRY(ctx, trace_pass(s, ctx, b.sray, b.shit, p.n_paths, true, g, b.wmask[1], st));
                 // the guided estimator has no separate MIS ray: its continuation plays that part
                 if (!guided) {
-                    HIP_TRY(ctx, trace_pass(s, ctx, b.mray, b.mhit, p.n_paths, false, g, st));
+                    HIP_TRY(ctx, trace_pass(s, ctx, b.mray, b.mhit, p.n_paths, false, g, b.wmask[2], st));
                     ++launches;
                 }
                 HIP_TRY(ctx, bg::launch_nee_resolve(p, sc, b, bounce, st));
