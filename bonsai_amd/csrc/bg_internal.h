@@ -268,7 +268,8 @@ bool traverse_compacts(const TraverseArgs& a, const LaunchShape& shape, int grid
 // Writes idx (ascending indices of the rays with t_max > 0), their number to *count and misses
 // for the other rays; blk holds ceil(n / 256) words and mask 4 * ceil(n / 256). With `produced`
 // (the masks the kernel that wrote the rays stored, one word per 64 rays, the first ceil(n / 64)
-// of them read) the rays are not read at all; otherwise their t_max values make the masks in mask.
+// of them read) the rays are not read at all and no misses are written (the path kernels read
+// only hits of rays that can hit); otherwise their t_max values make the masks in mask.
 hipError_t launch_ray_compact(const float4* rays, uint64_t n, uint32_t* blk, uint64_t* mask, uint32_t* idx,
                               uint32_t* count, uint4* hits, hipStream_t s, const uint64_t* produced = nullptr);
 hipError_t launch_guide_inputs(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b, hipStream_t s);

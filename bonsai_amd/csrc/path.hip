@@ -27,7 +27,12 @@ namespace {
 using namespace dev;
 using dev::block_exclusive;
 
-constexpr uint32_t kAlive = 1u, kSpecFlag = 2u, kItemShift = 16u;
+// state.y flags; kQuiet: dead since an earlier bounce, with its rays killed and no NEE pending, so
+// shading has nothing to write for it. Every ray a live path issues has t_max > 0 (a camera ray
+// +inf less its error offset, an extension ray +inf or the re-intersection distance, a shadow
+// ray 1 - 1e-4, an MIS ray 1 - 1e-4 or +inf), so the kernels that read a hit read only hits of
+// rays the ray masks mark, and a compacted launch need not write misses for the others.
+constexpr uint32_t kAlive = 1u, kSpecFlag = 2u, kQuiet = 4u, kItemShift = 16u;
 // nee_a.w flags
 constexpr uint32_t kHasNee = 1u, kHasLight = 2u, kHasBsdf = 4u;
 
@@ -112,7 +117,9 @@ __device__ __forceinline__ float init_path(const RenderParamsDev& p, const PathB
     b.beta[i] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(1u));  // .w: rays issued (camera ray)
     b.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     b.key[i] = make_uint2((uint32_t)key, (uint32_t)(key >> 32));
-    b.state[i] = make_uint2(dim, kAlive | (item << kItemShift));  // the batch item rides in the flags
+    // the batch item rides in the flags; a camera ray that can hit nothing (t_max NaN or <= 0, from a
+    // degenerate camera transform) ends the path here, as its miss would at the first shading
+    b.state[i] = make_uint2(dim, (r.t_max > 0.0f ? kAlive : 0u) | (item << kItemShift));
     b.film[i] = make_float2(fx, fy);
     return r.t_max;
 }
@@ -130,13 +137,17 @@ __device__ __forceinline__ uint32_t shade_path(const RenderParamsDev& p, const S
                                                uint32_t bounce, uint64_t i) {
     uint2 st = b.state[i];
     uint32_t flags = st.y;
-    b.nee_a[i].w = 0.0f;
-    kill_ray(b.sray, i);
-    kill_ray(b.mray, i);
     uint64_t rec = 0;
     const bool recording = rec_slot(p, i, bounce, &rec);
     if (recording) b.rec_tgt[rec] = make_float4(0.0f, 0.0f, 1.0f, 0.0f);  // .w = 0: no sample at this slot
-    if (!(flags & kAlive)) return 0u;
+    if (flags & kQuiet) return 0u;
+    b.nee_a[i].w = 0.0f;
+    kill_ray(b.sray, i);
+    kill_ray(b.mray, i);
+    if (!(flags & kAlive)) {  // its extension ray was killed when it ended
+        b.state[i] = make_uint2(st.x, kQuiet);
+        return 0u;
+    }
     const bool ref = p.mode == 0;
     float ext_t = -__builtin_inff(), shadow_t = -__builtin_inff(), mis_t = -__builtin_inff();
     const uint4 h = b.hit[i];
@@ -514,6 +525,9 @@ __global__ __launch_bounds__(256) void k_mask_count(const uint64_t* __restrict__
 // block (ascending ray order); every other ray gets the miss the traversal would report. The
 // ranks come from wavefront masks (k_ray_count's, or the ray writer's: bits past n are zero,
 // words from ceil(n / 64) on are not read), so the rays are not read again.
+// MISSES: write the misses (not needed where the masks came from the ray writer: every reader of a
+// hit of these rays reads only hits of rays that can hit, see kQuiet).
+template <bool MISSES>
 __global__ __launch_bounds__(256) void k_ray_compact(uint64_t n, const uint32_t* __restrict__ blk,
                                                      const uint64_t* __restrict__ mask, uint32_t* __restrict__ idx,
                                                      uint4* __restrict__ hits) {
@@ -525,7 +539,7 @@ __global__ __launch_bounds__(256) void k_ray_compact(uint64_t n, const uint32_t*
     for (uint32_t w = 0; w < wave; ++w) off += (uint32_t)__popcll(bm[w]);
     const uint64_t m = bm[wave];
     if ((m >> lane) & 1ull) idx[off + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)i;
-    else if (i < n) hits[i] = make_uint4(0xFFFFFFFFu, 0u, 0u, 0u);
+    else if (MISSES && i < n) hits[i] = make_uint4(0xFFFFFFFFu, 0u, 0u, 0u);
 }
 
 __global__ __launch_bounds__(256) void k_live_count(uint64_t n, PathBuffers b) {
@@ -670,7 +684,8 @@ hipError_t launch_ray_compact(const float4* rays, uint64_t n, uint32_t* blk, uin
     if (produced) hipLaunchKernelGGL(k_mask_count, dim3((nblk + 255) / 256), dim3(256), 0, s, produced, n, blk);
     else hipLaunchKernelGGL(k_ray_count, dim3(nblk), dim3(256), 0, s, rays, n, blk, mask);
     hipLaunchKernelGGL(k_live_scan, dim3(1), dim3(1024), 0, s, nblk, blk, count);
-    hipLaunchKernelGGL(k_ray_compact, dim3(nblk), dim3(256), 0, s, n, blk, produced ? produced : mask, idx, hits);
+    if (produced) hipLaunchKernelGGL(k_ray_compact<false>, dim3(nblk), dim3(256), 0, s, n, blk, produced, idx, hits);
+    else hipLaunchKernelGGL(k_ray_compact<true>, dim3(nblk), dim3(256), 0, s, n, blk, mask, idx, hits);
     return hipGetLastError();
 }
 hipError_t launch_guide_inputs(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b, hipStream_t s) {
