@@ -95,7 +95,7 @@ void free_batch(bg_batch* b) {
 bg::TraverseArgs batch_args(const bg_batch* bt, const bg_ctx* ctx, const float4* rays, uint4* hits, uint64_t n,
                             const uint32_t* roots, bool queue) {
     bg::TraverseArgs a;
-    a.nodes = nullptr;  // batches traverse the pair layout only (variants 0, 2 and 4)
+    a.nodes = nullptr;  // batches traverse the pair layout only (variants 0, 2, 4 and 5)
     a.pairs = bt->pairs;
     a.leaves = bt->leaves;
     a.tri_verts = bt->tri_verts;
@@ -123,13 +123,9 @@ int grid_for(const bg_ctx* ctx, const bg::LaunchShape& sh, uint64_t n) {
 // mask: the ray masks the kernel that wrote the rays stored (PathBuffers::wmask).
 hipError_t traverse(const bg_batch* bt, bg_ctx* ctx, const bg::LaunchShape& sh, const float4* rays, uint4* hits,
                     uint64_t n, const uint32_t* roots, bool any, const uint64_t* mask, hipStream_t st) {
-    const bool queue = sh.variant == 2;
-    if (queue) {
-        hipError_t e = bg::launch_zero_u64(bt->d_queue, st);
-        if (e != hipSuccess) return e;
-    }
+    const bool queue = sh.variant == 2 || sh.variant == 5;  // (the launch zeroes the counter)
     bg::LaunchShape shape = sh;
-    if (shape.variant != 2 && shape.variant != 4) shape.variant = 0;  // the node-record variants need node roots
+    if (shape.variant != 2 && shape.variant != 4 && shape.variant != 5) shape.variant = 0;  // the node-record variants need node roots
     bg::TraverseArgs a = batch_args(bt, ctx, rays, hits, n, roots, queue);
     const int grid = grid_for(ctx, shape, n);
     if (bg::traverse_compacts(a, shape, grid)) {
@@ -403,13 +399,15 @@ int bg_render_batch(bg_ctx* ctx, bg_batch* bt, const bg_batch_desc* d, bg_stats*
     sc.multi_lobe = bt->multi_lobe;
     sc.sphere_lights = bt->sphere_lights;
 
+    // The autotuner times two stages of candidates: the traversal variant and compaction at two
+    // shapes (16), then the shape (block x stack x blocks per CU, 16) with each of the best two
+    // (variant, compaction) pairs (32): 48 candidates instead of the 128 of the full product, each
+    // timed on all the traversal launches of one pass.
     std::vector<bg::LaunchShape> cands;
+    const int kBlocks[] = {128, 256}, kStacks[] = {8, 16}, kBpc[] = {8, 16, 32, 64};
     if (d->autotune && !bt->tuned)
-        for (int block : {128, 256})
-            for (int stack : {8, 16})
-                for (int bpc : {8, 16, 32, 64})
-                    for (int variant : {0, 2, 4})
-                        for (int compact : {0, 1}) cands.push_back(bg::LaunchShape{block, stack, bpc, variant, compact});
+        for (int block : kBlocks)
+            for (int bpc : kBpc) cands.push_back(bg::LaunchShape{block, 16, bpc, 0, 1});
     // spill rows for the largest grid any launch shape of this call uses
     uint64_t max_threads = (uint64_t)grid_for(ctx, bt->shape, cap) * bt->shape.block;
     for (const bg::LaunchShape& c : cands) max_threads = std::max<uint64_t>(max_threads, (uint64_t)grid_for(ctx, c, cap) * c.block);
@@ -417,41 +415,66 @@ int bg_render_batch(bg_ctx* ctx, bg_batch* bt, const bg_batch_desc* d, bg_stats*
     if (rc) return rc;
 
     if (d->autotune && !bt->tuned) {
-        // time every candidate on the pass's camera rays, then on bounce-1 and shadow rays
+        // time every candidate of a stage on every traversal launch of one pass (the extension,
+        // shadow and MIS-branch rays of each bounce, on the rays the shading kernels wrote: every
+        // candidate gives the same hits); the candidate of least total time wins
         auto t0 = std::chrono::steady_clock::now();
-        HIP_TRY(ctx, hipMemsetAsync(bt->d_pass, 0, sizeof(uint32_t), st));
-        HIP_TRY(ctx, bg::launch_init_paths(p, b, st));
-        std::vector<float> score(cands.size(), 0.0f);
-        auto time_all = [&](const float4* rays, uint4* hits, bool any, const uint64_t* mask) -> int {
-            for (size_t c = 0; c < cands.size(); ++c) {
-                HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
-                HIP_TRY(ctx, traverse(bt, ctx, cands[c], rays, hits, cap, b.root, any, mask, st));
-                HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
-                HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
-                float ms = 0.0f;
-                HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-                score[c] += ms;
+        // the candidates of `cs` ordered by their total time, fastest first
+        auto tune = [&](const std::vector<bg::LaunchShape>& cs, std::vector<bg::LaunchShape>* ranked) -> int {
+            HIP_TRY(ctx, hipMemsetAsync(bt->d_pass, 0, sizeof(uint32_t), st));
+            HIP_TRY(ctx, bg::launch_init_paths(p, b, st));
+            std::vector<float> score(cs.size(), 0.0f);
+            auto time_all = [&](const float4* rays, uint4* hits, bool any, const uint64_t* mask) -> int {
+                for (size_t c = 0; c < cs.size(); ++c) {
+                    HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
+                    HIP_TRY(ctx, traverse(bt, ctx, cs[c], rays, hits, cap, b.root, any, mask, st));
+                    HIP_TRY(ctx, hipEventRecord(ctx->ev1, st));
+                    HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+                    float ms = 0.0f;
+                    HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+                    score[c] += ms;
+                }
+                return BG_OK;
+            };
+            for (uint32_t bounce = 0; bounce <= d->max_depth; ++bounce) {
+                int r = time_all(b.ray, b.hit, false, b.wmask[0]);
+                if (r) return r;
+                HIP_TRY(ctx, bg::launch_shade(p, sc, b, bounce, st));
+                if (sc.n_lights > 0 && bounce < d->max_depth) {
+                    if ((r = time_all(b.sray, b.shit, true, b.wmask[1])) != BG_OK) return r;
+                    if ((r = time_all(b.mray, b.mhit, false, b.wmask[2])) != BG_OK) return r;
+                }
             }
+            std::vector<size_t> order(cs.size());
+            for (size_t c = 0; c < cs.size(); ++c) order[c] = c;
+            std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return score[x] < score[y]; });
+            ranked->clear();
+            for (size_t c : order) ranked->push_back(cs[c]);
             return BG_OK;
         };
-        rc = time_all(b.ray, b.hit, false, b.wmask[0]);
+        std::vector<bg::LaunchShape> stage, ranked;
+        for (int variant : {0, 2, 4, 5})
+            for (int compact : {0, 1}) {
+                stage.push_back(bg::LaunchShape{256, 8, 32, variant, compact});
+                stage.push_back(bg::LaunchShape{128, 8, 64, variant, compact});
+            }
+        rc = tune(stage, &ranked);
         if (rc) return rc;
-        HIP_TRY(ctx, bg::launch_shade(p, sc, b, 0, st));
-        rc = time_all(b.ray, b.hit, false, b.wmask[0]);
-        if (!rc && sc.n_lights) rc = time_all(b.sray, b.shit, true, b.wmask[1]);
-        if (rc) return rc;
-        // and a deep bounce, where ended paths leave most slots empty (what compaction is for)
-        const uint32_t deep = std::min<uint32_t>(4, d->max_depth);
-        for (uint32_t bounce = 1; bounce < deep; ++bounce) {
-            HIP_TRY(ctx, traverse(bt, ctx, bt->shape, b.ray, b.hit, cap, b.root, false, b.wmask[0], st));
-            HIP_TRY(ctx, bg::launch_shade(p, sc, b, bounce, st));
+        std::vector<bg::LaunchShape> pairs;  // the (variant, compaction) pairs by their best time
+        for (const bg::LaunchShape& c : ranked) {
+            bool seen = false;
+            for (const bg::LaunchShape& q : pairs) seen = seen || (q.variant == c.variant && q.compact == c.compact);
+            if (!seen) pairs.push_back(c);
         }
-        if (deep > 1) rc = time_all(b.ray, b.hit, false, b.wmask[0]);
+        stage.clear();
+        for (size_t k = 0; k < 2 && k < pairs.size(); ++k)
+            for (int block : kBlocks)
+                for (int stack : kStacks)
+                    for (int bpc : kBpc)
+                        stage.push_back(bg::LaunchShape{block, stack, bpc, pairs[k].variant, pairs[k].compact});
+        rc = tune(stage, &ranked);
         if (rc) return rc;
-        size_t best = 0;
-        for (size_t c = 1; c < cands.size(); ++c)
-            if (score[c] < score[best]) best = c;
-        bt->shape = cands[best];
+        bt->shape = ranked[0];
         bt->tuned = true;
         bt->tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }

@@ -75,7 +75,7 @@ struct TraverseArgs {
     const float4* pairs = nullptr;     // PairRecord array (4 float4 each)
     const uint2* leaves = nullptr;     // leaf table of the pair layout
     const uint32_t* roots = nullptr;   // per ray: root record (scene batches); record/node 0 when null
-    unsigned long long* queue = nullptr;  // work-queue counter (variant 2), zero at launch
+    unsigned long long* queue = nullptr;  // work-queue counter (variants 2 and 5; zeroed by the launch)
     uint32_t chunk_shift = 9;             // variant 4: log2 of the rays per chunk (set at launch)
     uint32_t refill_min = 16;             // variant 4: finished lanes that trigger a refill
     // Compacted launches: logical ray j is ray idx[j] (read from rays, written to hits), for
@@ -93,7 +93,8 @@ struct LaunchShape {
     int blocks_per_cu = 32;
     // 0: pair records, one ray per loop round; 1: node records, per-lane refill; 2: as 0 with rays
     // taken from an atomic work queue; 3: node records, one ray per loop round; 4: pair records,
-    // batched refill of finished lanes from per-wavefront ray sequences
+    // batched refill of finished lanes from per-wavefront ray sequences; 5: pair records, batched
+    // refill from runs of rays taken from an atomic work queue (persistent; index lists and spheres)
     int variant = 4;
     // render passes: trace only the rays that can hit (t_max > 0), compacted into an index list
     // (the others' hits are written as misses), instead of launching over every path slot
@@ -429,6 +430,7 @@ struct bg_ctx {
     // reusable device buffers
     uint32_t* spill = nullptr;
     unsigned long long* counters = nullptr;
+    unsigned long long* queue = nullptr;  // work-queue counter of the traversal launches (variant 5)
     size_t spill_bytes = 0;
     void* io = nullptr;
     size_t io_bytes = 0;

@@ -73,12 +73,13 @@ int traverse_grid(const bg_ctx* ctx, size_t n) {
 }
 
 // Device memory a traversal call works in beyond its rays and hits: the spill
-// rows of the LDS stacks and the three BG_COUNT counters. Null members: the
-// context's buffers, grown on demand.
+// rows of the LDS stacks, the three BG_COUNT counters and the work-queue
+// counter. Null members: the context's buffers, grown on demand.
 struct TraverseScratch {
     uint32_t* spill = nullptr;
     size_t spill_bytes = 0;
     unsigned long long* counters = nullptr;
+    unsigned long long* queue = nullptr;
 };
 
 int run_traverse(bg_ctx* ctx, const bg_scene* s, const void* d_rays, size_t n, void* d_hits, uint32_t flags,
@@ -121,6 +122,7 @@ int run_traverse(bg_ctx* ctx, const bg_scene* s, const void* d_rays, size_t n, v
     a.spill = fixed ? fixed->spill : ctx->spill;
     a.spill_depth = depth;
     a.counters = nullptr;
+    a.queue = fixed ? fixed->queue : ctx->queue;
     const bool any = (flags & BG_ANY) != 0;
     const bool count = (flags & BG_COUNT) != 0;
     const unsigned long long zeros[3] = {0, 0, 0};
@@ -243,7 +245,8 @@ int bg_create(int device, bg_ctx** out) {
     }
     ctx->num_cus = prop.multiProcessorCount;
     // the static scratch of bg_intersect_device_noalloc: its address on this device, taken once
-    if (bg::noalloc_scratch_init(device) != hipSuccess) {
+    if (bg::noalloc_scratch_init(device) != hipSuccess || hipMalloc((void**)&ctx->queue, 256) != hipSuccess) {
+        if (ctx->queue) hipFree(ctx->queue);
         hipEventDestroy(ctx->ev0);
         hipEventDestroy(ctx->ev1);
         hipStreamDestroy(ctx->stream);
@@ -260,6 +263,7 @@ void bg_destroy(bg_ctx* ctx) {
     for (hipEvent_t e : ctx->prof_events) hipEventDestroy(e);
     if (ctx->spill) hipFree(ctx->spill);
     if (ctx->counters) hipFree(ctx->counters);
+    if (ctx->queue) hipFree(ctx->queue);
     if (ctx->io) hipFree(ctx->io);
     if (ctx->paths.mem) hipFree(ctx->paths.mem);
     if (ctx->paths.guide_mem) hipFree(ctx->paths.guide_mem);
@@ -556,9 +560,10 @@ int bg_intersect_device_noalloc(bg_ctx* ctx, const bg_scene* s, const void* d_ra
         if (stats) *stats = bg_stats{0.0, 0.0, n, 0};
         return BG_OK;
     }
-    // layout: counters in the first 256 bytes, spill rows after them
+    // layout: counters and the queue counter in the first 256 bytes, spill rows after them
     TraverseScratch sc;
     sc.counters = reinterpret_cast<unsigned long long*>(base);
+    sc.queue = reinterpret_cast<unsigned long long*>(base + 64);
     sc.spill = reinterpret_cast<uint32_t*>(base + 256);
     sc.spill_bytes = (size_t)BG_NOALLOC_SCRATCH_BYTES - 256;
     return run_traverse(ctx, s, d_rays, n, d_hits, flags, stats, &sc);

@@ -164,6 +164,7 @@ bg::TraverseArgs traverse_args(const bg_scene* s, bg_ctx* ctx, const float4* ray
     a.spill = ctx->spill;
     a.spill_depth = s->bvh.max_depth + 1;
     a.counters = nullptr;
+    a.queue = ctx->queue;
     return a;
 }
 

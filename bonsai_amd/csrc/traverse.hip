@@ -551,6 +551,77 @@ void k_trav_pairs_refill(TraverseArgs a) {
     flush_counts<COUNT>(a, cnt);
 }
 
+// Persistent traversal with per-lane refill from an atomic work queue (variant 5): the grid stays
+// resident; each wavefront takes runs of kQueueRun consecutive logical rays from a.queue (one
+// atomic per run, zero at launch) and, as in k_trav_pairs_refill, once refill_min of its lanes
+// (or all) have finished, those lanes store their hits and take the next rays of the wavefront's
+// current run (fetching the next run when it cannot serve them all). Unlike the static sequences
+// of k_trav_pairs_refill, a wavefront whose rays end early takes more, so scene batches with
+// uneven rays and compacted launches (an index list, the ray count on the device) balance to the
+// end. Each lane traces each ray exactly as above: the hits are the same.
+constexpr uint64_t kQueueRun = 256;  // rays per atomic fetch: 4 per lane
+template <int BLOCK, int STACK, bool ANY, bool SPH>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(!SPH ? 8 : 1, 8)))
+void k_trav_pairs_queue(TraverseArgs a) {
+    __shared__ uint2 lds[STACK * BLOCK];
+    const int tid = threadIdx.x;
+    const uint64_t gtid = (uint64_t)blockIdx.x * BLOCK + tid;
+    uint2* __restrict__ spill = reinterpret_cast<uint2*>(a.spill) + gtid * a.spill_depth;
+    uint2* __restrict__ ring = lds + tid;
+    PairCounts cnt;
+
+    const int lane = tid & 63;
+    const uint64_t n = a.n_idx ? (uint64_t)*a.n_idx : a.n_rays;
+    // the wavefront's unassigned logical rays [cur, end) of its current run, and whether the queue
+    // has run dry (all wavefront-uniform)
+    uint64_t cur = 0, end = 0;
+    bool dry = false;
+    uint64_t ray = ~0ull;  // this lane's ray (physical id), ~0 for none
+    PairLane L;
+    L.ref = kRefNone;
+    for (;;) {
+        const uint64_t m_leaf = __builtin_amdgcn_ballot_w64((int32_t)L.ref < (int32_t)kRefNone);
+        const uint64_t m_int = __builtin_amdgcn_ballot_w64((int32_t)L.ref >= 0);
+        const uint64_t m_free = ~(m_leaf | m_int);
+        const uint32_t n_free = popc64(m_free);
+        if (n_free != 0 && (n_free >= a.refill_min || n_free == 64) && !(dry && cur >= end)) {
+            const uint64_t rem = end - cur;
+            uint64_t run = 0;
+            bool fetched = false;
+            if (rem < n_free && !dry) {
+                unsigned long long b0 = 0;
+                if (lane == 0) b0 = atomicAdd(a.queue, (unsigned long long)kQueueRun);
+                run = __builtin_amdgcn_readfirstlane((uint32_t)b0) | ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(b0 >> 32)) << 32);
+                if (run >= n) dry = true;
+                else fetched = true;
+            }
+            if (L.ref == kRefNone) {
+                if (ray != ~0ull) pair_store(a, ray, L);
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m_free >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)m_free, 0u));
+                const uint64_t r = rank < rem ? cur + rank : (fetched ? run + (rank - rem) : ~0ull);
+                ray = ~0ull;
+                if (r < n) {
+                    ray = a.idx ? (uint64_t)a.idx[r] : r;
+                    pair_start(a, ray, L);
+                }
+            }
+            if (rem >= n_free) {
+                cur += n_free;
+            } else if (fetched) {
+                cur = run + (n_free - rem);
+                end = run + kQueueRun;
+            } else {
+                cur = end;
+            }
+            continue;
+        }
+        if ((m_leaf | m_int) == 0) break;
+        pair_step<BLOCK, STACK, ANY, false, SPH>(a, L, ring, spill, m_leaf, m_int, cnt);
+    }
+    if (ray != ~0ull) pair_store(a, ray, L);
+}
+
 // Chunk of the refill kernel's ray sequences: rays / wavefronts rounded down to a
 // power of two, within [64, 512], so that every wavefront has rays and a
 // wavefront's consecutive rays come from one stretch of the ray array.
@@ -572,6 +643,15 @@ void launch4(const TraverseArgs& a_in, int variant, int grid, hipStream_t s) {
     // runs take rays in grid-stride order); 4: pair records with batched refill; 1: node records
     // with per-lane refill; 3: node records
     TraverseArgs a = a_in;
+    // variant 5 needs a queue counter and is not a counting kernel: otherwise it runs as 4
+    if (variant == 5 && (COUNT || a.queue == nullptr || a.pairs == nullptr)) variant = 4;
+    if (variant == 5) {
+        launch_zero_u64(a.queue, s);
+        if (a.spheres != nullptr) hipLaunchKernelGGL((k_trav_pairs_queue<BLOCK, STACK, ANY, true>), dim3(grid), dim3(BLOCK), 0, s, a);
+        else hipLaunchKernelGGL((k_trav_pairs_queue<BLOCK, STACK, ANY, false>), dim3(grid), dim3(BLOCK), 0, s, a);
+        return;
+    }
+    if (variant == 2 && a.queue != nullptr && !COUNT) launch_zero_u64(a.queue, s);
     // compacted launches (an index list) run the grid-stride or queue pair kernel
     if (a.idx != nullptr && a.pairs != nullptr && variant != 2) variant = 0;
     const bool q = variant == 2 && a.queue != nullptr && !COUNT;
@@ -634,7 +714,7 @@ bool traverse_compacts(const TraverseArgs& a, const LaunchShape& shape, int grid
 bool traverse_shape_supported(const LaunchShape& s) {
     const bool b = s.block == 64 || s.block == 128 || s.block == 256;
     const bool st = s.stack == 8 || s.stack == 16 || s.stack == 32;
-    return b && st && s.blocks_per_cu >= 0 && s.variant >= 0 && s.variant <= 4;
+    return b && st && s.blocks_per_cu >= 0 && s.variant >= 0 && s.variant <= 5;
 }
 
 hipError_t launch_traverse(const TraverseArgs& a, bool any_hit, bool count, const LaunchShape& shape, int grid,

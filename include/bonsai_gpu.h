@@ -250,7 +250,10 @@ typedef struct bg_launch_config {
                              4 (default): once 16 lanes of a wavefront have finished their
                                 rays, they take the next rays of the wavefront's sequence
                                 (runs of consecutive rays) together; launches of fewer than
-                                four rays per lane, and scenes with spheres, run it as 0 */
+                                four rays per lane, and scenes with spheres, run it as 0;
+                             5: as 4, the wavefronts taking runs of 256 rays from an atomic
+                                work queue as they need them (persistent; also with spheres
+                                and on compacted path-integrator launches) */
     uint32_t compact;     /* path-integrator passes: nonzero (default) traces only the rays that
                              can hit (t_max > 0), compacted on the device into an index list,
                              instead of every path slot; the hits are the same */

@@ -51,11 +51,11 @@ def test_batch_equals_per_scene(ctx, items, mode, graph):
         np.testing.assert_array_equal(w, ref[k][1])
         np.testing.assert_array_equal(rgb, ref[k][0])
     block, stack, bpc, variant = br.launch()
-    assert block in (128, 256) and stack in (8, 16) and bpc in (8, 16, 32, 64) and variant in (0, 2, 4)
+    assert block in (128, 256) and stack in (8, 16) and bpc in (8, 16, 32, 64) and variant in (0, 2, 4, 5)
     br.close()
 
 
-@pytest.mark.parametrize("variant", [0, 2, 4])
+@pytest.mark.parametrize("variant", [0, 2, 4, 5])
 def test_fixed_shapes_and_work_queue(ctx, items, variant):
     b0, s0, c0, v0 = ctx.launch()
     ctx.set_launch(128, 8, 4, variant)
@@ -110,12 +110,13 @@ def test_refill_traversal_with_per_ray_roots(ctx):
         ctx.set_launch(*saved)
 
 
-@pytest.mark.parametrize("variant", [0, 2])
+@pytest.mark.parametrize("variant", [0, 2, 5])
 def test_compacted_batch_equals_per_scene(ctx, variant):
     """A batch whose passes cover over 1M path slots and hold a scene with spheres (so the launches do
     not refill) compacts its traversal launches, per-ray root records included; each item's film
-    equals its single-scene render (whose smaller launches do not compact). Variant 2 takes the
-    compacted rays from the atomic work queue, reading their count from the device."""
+    equals its single-scene render (whose smaller launches do not compact). Variants 2 and 5 take the
+    compacted rays from the atomic work queue, reading their count from the device (5 refilling
+    finished lanes from the runs it takes)."""
     scenes = [path_tracer_scene(512, 256, small=True), cornell_box(width=512, height=256, with_plastic=True),
               batch_scene(1, scale=0.02)]
     bvhs = [GpuBvh(ctx, s) for s in scenes]

@@ -161,7 +161,8 @@ def deep_rays(n, seed):
 
 @pytest.mark.parametrize("block,stack,variant", [(64, 8, 0), (128, 16, 0), (256, 32, 0), (128, 8, 0), (64, 32, 0),
                                                   (64, 8, 1), (128, 16, 1), (256, 32, 1), (64, 8, 3), (256, 16, 3),
-                                                  (128, 16, 2), (64, 8, 4), (256, 8, 4), (128, 32, 4)])
+                                                  (128, 16, 2), (64, 8, 4), (256, 8, 4), (128, 32, 4),
+                                                  (256, 8, 5), (64, 16, 5)])
 def test_launch_shapes_do_not_change_results(ctx, block, stack, variant):
     saved = ctx.launch()
     try:
@@ -208,6 +209,35 @@ def test_refill_variant_with_many_rays_per_lane(ctx, block, stack):
         # a scene with spheres runs variant 4 as 0: the same results
         add_spheres(s2, rng.uniform(-1, 1, (50, 3)), rng.uniform(0.01, 0.2, 50))
         run_both(ctx, s2, incoherent_rays(100_000, lo, hi, 4))
+    finally:
+        ctx.set_launch(*saved)
+
+
+@pytest.mark.parametrize("block,stack", [(64, 8), (256, 8), (128, 16)])
+def test_queue_refill_variant(ctx, block, stack):
+    """Variant 5: persistent wavefronts take runs of 256 rays from an atomic counter and refill
+    finished lanes from them. With one block per CU every wavefront takes many runs, the ray count
+    is not a multiple of a run, and the scene has spheres in one case (the refill kernel with the
+    sphere test): hits equal the oracle's, and the counter is zeroed by each launch (a second
+    query of the same rays gives the same hits)."""
+    saved = ctx.launch()
+    ctx.set_launch(block, stack, 1, 5)
+    try:
+        s = deep_scene()
+        rays = np.concatenate([deep_rays(150_001, 7), box_rays(150_002, -0.5, 1.5, 8, 0, 1)])
+        rays = rays[np.random.default_rng(9).permutation(len(rays))]
+        bvh = GpuBvh(ctx, s)
+        o = oracle_for(bvh, s)
+        for any_hit in (False, True):
+            ref = o.intersect(rays, any_hit=any_hit)
+            for _ in range(2):
+                g = bvh.intersect_test(rays) if any_hit else bvh.intersect(rays)
+                assert_hits_equal(g, ref, s.n_tris, any_hit)
+        s2 = SceneData(tri_positions=icosphere(4, (0.0, 0.0, 0.0), 1.0, 0.15, 4.0, 3))
+        lo, hi = s2.bounds()
+        rng = np.random.default_rng(11)
+        add_spheres(s2, rng.uniform(-1, 1, (50, 3)), rng.uniform(0.01, 0.2, 50))
+        run_both(ctx, s2, incoherent_rays(300_003, lo, hi, 5))
     finally:
         ctx.set_launch(*saved)
 

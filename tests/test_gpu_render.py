@@ -155,7 +155,9 @@ def test_compaction_does_not_change_the_film(ctx, mode, size):
     path-tracer scene kind (spheres and triangles, 8 light
     triangles, roulette ending paths) and, in the corrected semantics, for the guided estimator
     with a frozen network. 1024 x 600 x 8 spp = 4,915,200 slots gives 19,200 block counts, so
-    the one-block scan runs both its eight-wide and its single-count loops."""
+    the one-block scan runs both its eight-wide and its single-count loops. Variant 5 (refill from
+    an atomic work queue, persistent, on the compacted index list or on every slot) gives the same
+    sums too."""
     import sys
     from pathlib import Path
     from bonsai_amd.guide import GuideMLP
@@ -169,9 +171,10 @@ def test_compaction_does_not_change_the_film(ctx, mode, size):
     guide = GuideMLP(ctx, G.init_params(5).astype(np.float32), 65536)
     saved, saved_c = ctx.launch(), ctx.compaction()
     out = {}
+    configs = [(False, saved[3]), (True, saved[3]), (True, 5), (False, 5)]
     try:
-        for compact in (False, True):
-            ctx.set_launch(*saved, compact=compact)
+        for compact, variant in configs:
+            ctx.set_launch(saved[0], saved[1], saved[2], variant, compact=compact)
             integs = [GpuPathIntegrator(ctx, s.camera, spp=8, max_depth=8, mode=mode, seed=19)]
             if mode == MODE_CORRECTED:
                 integs.append(GuidedPathIntegrator(ctx, s.camera, spp=8, max_depth=8, guide=guide, seed=23,
@@ -182,13 +185,14 @@ def test_compaction_does_not_change_the_film(ctx, mode, size):
                 st = gi.render_into(bvh, film)
                 sums.append(film.sums() + (st.rays,))
                 film.close()
-            out[compact] = sums
+            out[(compact, variant)] = sums
     finally:
         ctx.set_launch(*saved, compact=saved_c)
-    for (rgb0, w0, r0), (rgb1, w1, r1) in zip(out[False], out[True]):
-        np.testing.assert_array_equal(rgb1, rgb0)
-        np.testing.assert_array_equal(w1, w0)
-        assert r1 == r0
+    for key in configs[1:]:
+        for (rgb0, w0, r0), (rgb1, w1, r1) in zip(out[configs[0]], out[key]):
+            np.testing.assert_array_equal(rgb1, rgb0)
+            np.testing.assert_array_equal(w1, w0)
+            assert r1 == r0
 
 
 @pytest.mark.parametrize("value", [float("nan"), -2.0, float("inf")])
