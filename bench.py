@@ -221,18 +221,25 @@ def cpu_path_rate(scene, bvh, spp: int, depth: int, mode: int, rr: float = 1.0, 
     o.set_shading(scene)
     W, H = scene.camera.width, scene.camera.height
 
-    def band(rows):
+    def band(rows, n_spp):
         y0 = max(0, H // 2 - rows // 2)
         t0 = time.perf_counter()
-        o.render(scene, spp, depth, mode=mode, seed=5, rr=rr, window=(0, y0, W, y0 + rows), threads=cores)
+        o.render(scene, n_spp, depth, mode=mode, seed=5, rr=rr, window=(0, y0, W, y0 + rows), threads=cores)
         return time.perf_counter() - t0
 
-    probe = 16
-    dt = band(probe)
-    rows = int(min(H, max(probe, probe * target_s / max(dt, 1e-6))))
-    if rows == H:  # the whole film is quicker than the target: more samples per pixel
-        spp = max(spp, int(spp * target_s * probe / (H * max(dt, 1e-6))))
-    dt = band(rows)
+    # grow the band (then, with the whole film, the samples per pixel) until a render takes at least
+    # half the target: a small probe's time is mostly fixed cost, so one extrapolation falls short
+    rows = 16
+    dt = band(rows, spp)
+    for _ in range(6):
+        if dt >= 0.5 * target_s:
+            break
+        f = min(64.0, target_s / max(dt, 1e-3))
+        if rows < H:
+            rows = int(min(H, max(rows + 1, rows * f)))
+        else:
+            spp = max(spp + 1, int(spp * f))
+        dt = band(rows, spp)
     n = W * rows * spp
     return {"samples_per_s": round(n / dt, 1), "Msamples_s": round(n / dt / 1e6, 4), "cores": cores,
             "kind": "port", "sample": f"{W}x{rows} band through the film's middle at {spp} spp ({dt:.1f} s)"}
