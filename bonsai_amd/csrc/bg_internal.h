@@ -260,11 +260,8 @@ hipError_t launch_nee_resolve(const RenderParamsDev& p, const ShadeSceneDev& sc,
                               uint32_t bounce, hipStream_t s);
 hipError_t launch_count_rays(const RenderParamsDev& p, const PathBuffers& b, unsigned long long* counters,
                              hipStream_t s);
-// Whether launch_traverse with this shape and grid runs the refill kernel (variant 4 where it
-// refills: four or more rays per lane, no spheres, no index list).
-bool traverse_refills(const TraverseArgs& a, const LaunchShape& shape, int grid);
-// Whether a render pass compacts this launch's rays first: the shape asks for it, the launch does
-// not refill, and it covers at least 1M ray slots.
+// Whether a render pass compacts this launch's rays first: the shape asks for it and the launch
+// covers at least 1M ray slots.
 bool traverse_compacts(const TraverseArgs& a, const LaunchShape& shape, int grid);
 // Writes idx (ascending indices of the rays with t_max > 0), their number to *count and misses
 // for the other rays; blk holds ceil(n / 256) words and mask 4 * ceil(n / 256). With `produced`

@@ -281,6 +281,16 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_refill(TraverseArgs a) {
     }
 }
 
+// Chunk of the refill kernels' ray sequences: rays / wavefronts rounded down to a
+// power of two, within [64, 512], so that every wavefront has rays and a
+// wavefront's consecutive rays come from one stretch of the ray array.
+__host__ __device__ inline uint32_t refill_shift(uint64_t n_rays, uint64_t n_waves) {
+    const uint64_t per = n_rays / (n_waves ? n_waves : 1);
+    uint32_t sh = 6;
+    while (sh < 9 && (2ull << sh) <= per) ++sh;
+    return sh;
+}
+
 // Per-lane state of the pair-record traversal.
 struct PairLane {
     RayState rs;
@@ -514,11 +524,16 @@ void k_trav_pairs_refill(TraverseArgs a) {
     uint2* __restrict__ ring = lds + tid;
     PairCounts cnt;
 
-    const uint32_t shift = a.chunk_shift;
-    const uint64_t chunk = 1ull << shift;
     const uint64_t n_waves = (uint64_t)gridDim.x * (BLOCK / 64);
+    // With an index list the live-ray count is on the device: the chunk and the refill threshold
+    // follow from it here (below four rays per lane a wavefront refills only when all its lanes
+    // are free, taking its rays in rounds as the grid-stride kernel does).
+    const uint64_t n_rays = a.n_idx ? (uint64_t)*a.n_idx : a.n_rays;
+    const uint32_t shift = a.n_idx ? refill_shift(n_rays, n_waves) : a.chunk_shift;
+    const uint32_t refill_min = a.n_idx && n_rays < 256 * n_waves ? 64u : a.refill_min;
+    const uint64_t chunk = 1ull << shift;
     const uint64_t wave = (uint64_t)blockIdx.x * (BLOCK / 64) + (uint32_t)__builtin_amdgcn_readfirstlane(tid >> 6);
-    // position s of this wavefront's sequence -> ray id (increasing in s)
+    // position s of this wavefront's sequence -> logical ray (increasing in s)
     auto seq_ray = [&](uint64_t s) { return (((s >> shift) * n_waves + wave) << shift) + (s & (chunk - 1)); };
     uint64_t pos = 0;               // next sequence position (the same in every lane)
     uint64_t ray = ~0ull;           // this lane's ray, ~0 for none
@@ -529,15 +544,15 @@ void k_trav_pairs_refill(TraverseArgs a) {
         const uint64_t m_int = __builtin_amdgcn_ballot_w64((int32_t)L.ref >= 0);
         const uint64_t m_free = ~(m_leaf | m_int);
         const uint32_t n_free = popc64(m_free);
-        if (n_free != 0 && (n_free >= a.refill_min || n_free == 64) && seq_ray(pos) < a.n_rays) {
+        if (n_free != 0 && (n_free >= refill_min || n_free == 64) && seq_ray(pos) < n_rays) {
             if (L.ref == kRefNone) {
                 if (ray != ~0ull) pair_store(a, ray, L);
                 const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m_free >> 32),
                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)m_free, 0u));
                 const uint64_t r = seq_ray(pos + rank);
                 ray = ~0ull;
-                if (r < a.n_rays) {
-                    ray = r;
+                if (r < n_rays) {
+                    ray = a.idx ? (uint64_t)a.idx[r] : r;
                     pair_start(a, ray, L);
                 }
             }
@@ -622,18 +637,12 @@ void k_trav_pairs_queue(TraverseArgs a) {
     if (ray != ~0ull) pair_store(a, ray, L);
 }
 
-// Chunk of the refill kernel's ray sequences: rays / wavefronts rounded down to a
-// power of two, within [64, 512], so that every wavefront has rays and a
-// wavefront's consecutive rays come from one stretch of the ray array.
-uint32_t refill_chunk_shift(uint64_t n_rays, uint64_t n_waves) {
-    const uint64_t per = n_rays / (n_waves ? n_waves : 1);
-    uint32_t sh = 6;
-    while (sh < 9 && (2ull << sh) <= per) ++sh;
-    return sh;
-}
+uint32_t refill_chunk_shift(uint64_t n_rays, uint64_t n_waves) { return refill_shift(n_rays, n_waves); }
 
+// Variant 4 refills on triangle scenes: over an index list (the kernel takes the live count from
+// the device), or over every ray where there are at least four rays per lane.
 bool refill_applies(const TraverseArgs& a, uint64_t n_waves) {
-    return a.idx == nullptr && a.spheres == nullptr && a.n_rays >= 256 * n_waves;
+    return a.spheres == nullptr && (a.idx != nullptr || a.n_rays >= 256 * n_waves);
 }
 
 template <int BLOCK, int STACK, bool ANY, bool COUNT>
@@ -652,8 +661,9 @@ void launch4(const TraverseArgs& a_in, int variant, int grid, hipStream_t s) {
         return;
     }
     if (variant == 2 && a.queue != nullptr && !COUNT) launch_zero_u64(a.queue, s);
-    // compacted launches (an index list) run the grid-stride or queue pair kernel
-    if (a.idx != nullptr && a.pairs != nullptr && variant != 2) variant = 0;
+    // compacted launches (an index list) run the grid-stride, queue or (triangle scenes) refill
+    // pair kernel
+    if (a.idx != nullptr && a.pairs != nullptr && variant != 2 && !(variant == 4 && a.spheres == nullptr)) variant = 0;
     const bool q = variant == 2 && a.queue != nullptr && !COUNT;
     const bool pairs = (variant == 0 || variant == 2 || variant == 4) && a.pairs != nullptr;
     // Variant 4 refills only where lanes have rays to refill from: at least four rays per lane
@@ -702,13 +712,9 @@ hipError_t launch1(const TraverseArgs& a, bool any, bool count, int stack, int v
 
 }  // namespace
 
-bool traverse_refills(const TraverseArgs& a, const LaunchShape& shape, int grid) {
-    return shape.variant == 4 && a.pairs != nullptr && refill_applies(a, (uint64_t)grid * (shape.block / 64));
-}
-
 bool traverse_compacts(const TraverseArgs& a, const LaunchShape& shape, int grid) {
     // (below about 1M slots the three compaction launches cost more than the empty slots)
-    return shape.compact && a.pairs != nullptr && a.n_rays >= (1ull << 20) && !traverse_refills(a, shape, grid);
+    return shape.compact && a.pairs != nullptr && a.n_rays >= (1ull << 20);
 }
 
 bool traverse_shape_supported(const LaunchShape& s) {

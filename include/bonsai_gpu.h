@@ -249,8 +249,10 @@ typedef struct bg_launch_config {
                              3: as 0 on node records (scene batches run it as 0);
                              4 (default): once 16 lanes of a wavefront have finished their
                                 rays, they take the next rays of the wavefront's sequence
-                                (runs of consecutive rays) together; launches of fewer than
-                                four rays per lane, and scenes with spheres, run it as 0;
+                                (runs of consecutive rays) together, also over the compacted
+                                rays of path-integrator launches; uncompacted launches of
+                                fewer than four rays per lane, and scenes with spheres, run
+                                it as 0;
                              5: as 4, the wavefronts taking runs of 256 rays from an atomic
                                 work queue as they need them (persistent; also with spheres
                                 and on compacted path-integrator launches) */
