@@ -64,6 +64,23 @@ def path_alg_bytes(st) -> float:
     return alg_bytes(int(st.rays), st)
 
 
+def traversal_roofline(ctx, render) -> dict:
+    """The traversal roofline of a render: `render()` runs it once and returns its bg_stats; it runs
+    twice more, once counting every traversal launch's box and primitive tests (the algorithmic
+    bytes, path_alg_bytes) and once with a HIP event pair around each traversal launch."""
+    ctx.set_profiling(count=True)
+    cst = render()
+    ctx.set_profiling(traversal_time=True)
+    st = render()
+    ctx.set_profiling()
+    trav = path_alg_bytes(cst)
+    return {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "traversal_ms": round(st.traverse_ms, 2),
+            "traversal_bytes": trav, "traversal_achieved": round(trav / st.traverse_ms / 1e6, 1),
+            "traversal_frac": round(trav / st.traverse_ms / 1e6 / HBM_PEAK_GBS, 4),
+            "box_tests": int(cst.nodes_visited), "triangle_tests": int(cst.tris_tested),
+            "sphere_tests": int(cst.spheres_tested), "queries": int(cst.rays)}
+
+
 TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                                "k_trav_pairs_intersection_1.31Mtri_2p24rays.md")
 
@@ -409,6 +426,9 @@ def batch_leg(ctx, spp: int) -> dict:
     out["batched"] = {"seconds": round(wall, 3), "Msamples_s": round(st.samples / wall / 1e6, 2),
                       "Mrays_s": round(st.rays / wall / 1e6, 1), "launch": {"block": b, "stack": s_,
                       "blocks_per_cu": c, "variant": v}, "first_call_seconds_incl_autotune": round(tune_s, 2)}
+    # (measured renders run their passes without the graph)
+    out["batched"]["roofline"] = traversal_roofline(ctx, lambda: br.render(spp, 6, mode=MODE_CORRECTED,
+                                                                            autotune=False))
     br.close()
     films = [Film(ctx, s.camera.width, s.camera.height) for s in scenes]
     for (bvh, cam, seed), f in zip(items, films):  # warm-up
@@ -423,7 +443,9 @@ def batch_leg(ctx, spp: int) -> dict:
     out["per_scene"] = {"seconds": round(wall1, 3), "Msamples_s": round(pixels * spp / wall1 / 1e6, 2),
                         "Mrays_s": round(rays / wall1 / 1e6, 1)}
     out["speedup_batched_vs_per_scene"] = round(wall1 / wall, 3)
-    log(f"batch leg: batched {wall:.2f} s, per-scene {wall1:.2f} s")
+    rf = out["batched"]["roofline"]
+    log(f"batch leg: batched {wall:.2f} s, per-scene {wall1:.2f} s; batched traversal {rf['traversal_ms']:.1f} ms = "
+        f"{rf['traversal_achieved']:.0f} GB/s algorithmic")
     for f in films:
         f.close()
     for b_ in bvhs:
@@ -669,6 +691,20 @@ def guiding_leg(ctx, spp: int, measure: bool = True, ref_spp: int = 2048, max_se
                      "predicted_seconds_to_relmse_0.02": round(warm_s + v * t_spp / 0.02, 1)}
         if warm:
             out[name]["warmup_training_seconds"] = round(warm_s, 3)
+        if name in ("plain", "guided"):  # (a training render would change the network it measures)
+            def one(g=g, sel=sel, rr=rr):
+                if g is None:
+                    gi = GpuPathIntegrator(ctx, scene.camera, spp=8, max_depth=10, mode=MODE_CORRECTED, seed=33,
+                                           rr_threshold=rr)
+                else:
+                    gi = GuidedPathIntegrator(ctx, scene.camera, spp=8, max_depth=10, guide=g, seed=33,
+                                              select_prob=sel, steps=GUIDE_STEPS, train=False,
+                                              bounces=GUIDE_BOUNCES, rr_threshold=rr)
+                film = Film(ctx, W, H)
+                st = gi.render_into(bvh, film)
+                film.close()
+                return st
+            out[name]["roofline_8spp"] = traversal_roofline(ctx, one)
         log(f"guiding leg {name}: v {v:.2f}, {t_spp * 1e3:.1f} ms/spp, {rays / secs / 1e6:.0f} Mrays/s"
             + (f", warm-up {warm_s:.2f} s" if warm else ""))
     gd = out["guided"]["predicted_seconds_to_relmse_0.02"]

@@ -120,9 +120,11 @@ int grid_for(const bg_ctx* ctx, const bg::LaunchShape& sh, uint64_t n) {
     return (int)std::max<uint64_t>(1, std::min<uint64_t>(std::min(tiles, cap), 1u << 20));
 }
 
-// mask: the ray masks the kernel that wrote the rays stored (PathBuffers::wmask).
+// mask: the ray masks the kernel that wrote the rays stored (PathBuffers::wmask); measured: with the
+// context's measurement (bg_set_profiling), as a render's launches are (the autotuner's are not).
 hipError_t traverse(const bg_batch* bt, bg_ctx* ctx, const bg::LaunchShape& sh, const float4* rays, uint4* hits,
-                    uint64_t n, const uint32_t* roots, bool any, const uint64_t* mask, hipStream_t st) {
+                    uint64_t n, const uint32_t* roots, bool any, const uint64_t* mask, hipStream_t st,
+                    bool measured = false) {
     const bool queue = sh.variant == 2 || sh.variant == 5;  // (the launch zeroes the counter)
     bg::LaunchShape shape = sh;
     if (shape.variant != 2 && shape.variant != 4 && shape.variant != 5) shape.variant = 0;  // the node-record variants need node roots
@@ -136,6 +138,7 @@ hipError_t traverse(const bg_batch* bt, bg_ctx* ctx, const bg::LaunchShape& sh, 
         a.idx = b.ridx;
         a.n_idx = b.rcount;
     }
+    if (measured) return bg::profiled_traverse(ctx, a, any, shape, grid, st);
     return bg::launch_traverse(a, any, false, shape, grid, st);
 }
 
@@ -488,12 +491,12 @@ int bg_render_batch(bg_ctx* ctx, bg_batch* bt, const bg_batch_desc* d, bg_stats*
         if ((e = bg::launch_init_paths(p, b, s)) != hipSuccess) return e;
         ++launches;
         for (uint32_t bounce = 0; bounce <= d->max_depth; ++bounce) {
-            if ((e = traverse(bt, ctx, sh, b.ray, b.hit, cap, b.root, false, b.wmask[0], s)) != hipSuccess) return e;
+            if ((e = traverse(bt, ctx, sh, b.ray, b.hit, cap, b.root, false, b.wmask[0], s, true)) != hipSuccess) return e;
             if ((e = bg::launch_shade(p, sc, b, bounce, s)) != hipSuccess) return e;
             launches += 2;
             if (sc.n_lights > 0 && bounce < d->max_depth) {
-                if ((e = traverse(bt, ctx, sh, b.sray, b.shit, cap, b.root, true, b.wmask[1], s)) != hipSuccess) return e;
-                if ((e = traverse(bt, ctx, sh, b.mray, b.mhit, cap, b.root, false, b.wmask[2], s)) != hipSuccess) return e;
+                if ((e = traverse(bt, ctx, sh, b.sray, b.shit, cap, b.root, true, b.wmask[1], s, true)) != hipSuccess) return e;
+                if ((e = traverse(bt, ctx, sh, b.mray, b.mhit, cap, b.root, false, b.wmask[2], s, true)) != hipSuccess) return e;
                 if ((e = bg::launch_nee_resolve(p, sc, b, bounce, s)) != hipSuccess) return e;
                 launches += 3;
             }
@@ -507,9 +510,13 @@ int bg_render_batch(bg_ctx* ctx, bg_batch* bt, const bg_batch_desc* d, bg_stats*
     const uint32_t passes = d->spp / spp_pass;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // a measured render (bg_set_profiling) runs its passes as direct launches, the event pairs
+    // around its traversal launches outside any graph
+    rc = bg::profile_begin(ctx);
+    if (rc) return rc;
     auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
-    if (d->use_graph) {
+    if (d->use_graph && !ctx->profile) {
         HIP_TRY(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
         const hipError_t e = enqueue_pass(st);
         hipGraph_t g = nullptr;
@@ -553,7 +560,7 @@ int bg_render_batch(bg_ctx* ctx, bg_batch* bt, const bg_batch_desc* d, bg_stats*
         stats->rays = nr[0];
         stats->shadow_rays = nr[1];
     }
-    return BG_OK;
+    return bg::profile_end(ctx, stats);
 }
 
 }  // extern "C"

@@ -260,6 +260,13 @@ hipError_t launch_nee_resolve(const RenderParamsDev& p, const ShadeSceneDev& sc,
                               uint32_t bounce, hipStream_t s);
 hipError_t launch_count_rays(const RenderParamsDev& p, const PathBuffers& b, unsigned long long* counters,
                              hipStream_t s);
+// A render's traversal launch with the context's measurement (bg_set_profiling): test counting
+// into ctx->counters, and a HIP event pair around the kernel; profile_begin / profile_end open a
+// render call's measurement and write it into its stats.
+hipError_t profiled_traverse(bg_ctx* ctx, TraverseArgs a, bool any, const LaunchShape& shape, int grid,
+                             hipStream_t st);
+int profile_begin(bg_ctx* ctx);
+int profile_end(bg_ctx* ctx, bg_stats* stats);
 // Whether a render pass compacts this launch's rays first: the shape asks for it and the launch
 // covers at least 1M ray slots.
 bool traverse_compacts(const TraverseArgs& a, const LaunchShape& shape, int grid);

@@ -174,9 +174,25 @@ bg::TraverseArgs traverse_args(const bg_scene* s, bg_ctx* ctx, const float4* ray
 // Launches that refill finished lanes skip it: there a lane at an ended path takes the next ray at
 // once (measured: compacting them cost 10% on a triangle scene, where it saved 11-14% on the
 // sphere-and-triangle path-tracer scene); so do launches of fewer than 1M slots.
+hipError_t trace_pass(const bg_scene* s, bg_ctx* ctx, const float4* rays, uint4* hits, uint64_t n, bool any, int grid,
+                      const uint64_t* mask, hipStream_t st) {
+    bg::TraverseArgs a = traverse_args(s, ctx, rays, hits, n);
+    if (bg::traverse_compacts(a, ctx->shape, grid)) {
+        const bg::PathBuffers& b = ctx->paths;
+        const hipError_t e = bg::launch_ray_compact(rays, n, b.rblk, b.rmask, b.ridx, b.rcount, hits, st, mask);
+        if (e != hipSuccess) return e;
+        a.idx = b.ridx;
+        a.n_idx = b.rcount;
+    }
+    return bg::profiled_traverse(ctx, a, any, ctx->shape, grid, st);
+}
+
+}  // namespace
+
 // A render's traversal launch with the context's measurement (bg_set_profiling): test counting
 // into ctx->counters, and a HIP event pair around the kernel.
-hipError_t profiled_traverse(bg_ctx* ctx, bg::TraverseArgs a, bool any, int grid, hipStream_t st) {
+hipError_t bg::profiled_traverse(bg_ctx* ctx, bg::TraverseArgs a, bool any, const bg::LaunchShape& shape, int grid,
+                                  hipStream_t st) {
     const bool count = (ctx->profile & BG_PROFILE_COUNT) != 0;
     if (count) a.counters = ctx->counters;
     const bool timed = (ctx->profile & BG_PROFILE_TRAVERSAL_TIME) != 0;
@@ -190,27 +206,14 @@ hipError_t profiled_traverse(bg_ctx* ctx, bg::TraverseArgs a, bool any, int grid
         const hipError_t r = hipEventRecord(ctx->prof_events[2 * ctx->prof_used], st);
         if (r != hipSuccess) return r;
     }
-    const hipError_t e = bg::launch_traverse(a, any, count, ctx->shape, grid, st);
+    const hipError_t e = bg::launch_traverse(a, any, count, shape, grid, st);
     if (e != hipSuccess || !timed) return e;
     return hipEventRecord(ctx->prof_events[2 * ctx->prof_used++ + 1], st);
 }
 
 // mask: the ray masks the kernel that wrote the rays stored (PathBuffers::wmask).
-hipError_t trace_pass(const bg_scene* s, bg_ctx* ctx, const float4* rays, uint4* hits, uint64_t n, bool any, int grid,
-                      const uint64_t* mask, hipStream_t st) {
-    bg::TraverseArgs a = traverse_args(s, ctx, rays, hits, n);
-    if (bg::traverse_compacts(a, ctx->shape, grid)) {
-        const bg::PathBuffers& b = ctx->paths;
-        const hipError_t e = bg::launch_ray_compact(rays, n, b.rblk, b.rmask, b.ridx, b.rcount, hits, st, mask);
-        if (e != hipSuccess) return e;
-        a.idx = b.ridx;
-        a.n_idx = b.rcount;
-    }
-    return profiled_traverse(ctx, a, any, grid, st);
-}
-
 // Start and end of a render call's measurement.
-int profile_begin(bg_ctx* ctx) {
+int bg::profile_begin(bg_ctx* ctx) {
     ctx->prof_used = 0;
     if (ctx->profile & BG_PROFILE_COUNT) {
         if (!ctx->counters) HIP_TRY(ctx, hipMalloc((void**)&ctx->counters, 3 * sizeof(unsigned long long)));
@@ -219,7 +222,7 @@ int profile_begin(bg_ctx* ctx) {
     return BG_OK;
 }
 
-int profile_end(bg_ctx* ctx, bg_stats* stats) {
+int bg::profile_end(bg_ctx* ctx, bg_stats* stats) {
     if (!stats) return BG_OK;
     if (ctx->profile & BG_PROFILE_COUNT) {
         unsigned long long c[3] = {0, 0, 0};
@@ -239,8 +242,6 @@ int profile_end(bg_ctx* ctx, bg_stats* stats) {
     }
     return BG_OK;
 }
-
-}  // namespace
 
 // The guide's replay ring of `cap` samples (kept when the capacity is unchanged).
 int bg::guide_ensure_replay(bg_guide* g, uint32_t cap) { return ensure_replay(g, cap); }
@@ -453,7 +454,7 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
     hipStream_t st = ctx->stream;
 
     HIP_TRY(ctx, hipMemsetAsync(b.counters, 0, 2 * sizeof(unsigned long long), st));
-    rc = profile_begin(ctx);
+    rc = bg::profile_begin(ctx);
     if (rc) return rc;
     auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
@@ -536,7 +537,7 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
         stats->samples = npix * d->spp;
         stats->train_steps = train_steps;
     }
-    rc = profile_end(ctx, stats);
+    rc = bg::profile_end(ctx, stats);
     if (rc) return rc;
     return BG_OK;
 }
@@ -619,7 +620,7 @@ int direct_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, const b
     const bg::DirectSlots& ds = ctx->direct;
     hipStream_t st = ctx->stream;
     HIP_TRY(ctx, hipMemsetAsync(b.counters, 0, 2 * sizeof(unsigned long long), st));
-    rc = profile_begin(ctx);
+    rc = bg::profile_begin(ctx);
     if (rc) return rc;
     auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(ctx, hipEventRecord(ctx->ev0, st));
@@ -633,14 +634,14 @@ int direct_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, const b
         const int g = (int)std::min<uint64_t>(grid, (p.n_paths + block - 1) / block);
         const int gq = (int)std::max<uint64_t>(1, std::min<uint64_t>(grid, (nq + block - 1) / block));
         HIP_TRY(ctx, bg::launch_init_paths(p, b, st));
-        HIP_TRY(ctx, profiled_traverse(ctx, traverse_args(s, ctx, b.ray, b.hit, p.n_paths), false, g, st));
+        HIP_TRY(ctx, bg::profiled_traverse(ctx, traverse_args(s, ctx, b.ray, b.hit, p.n_paths), false, ctx->shape, g, st));
         HIP_TRY(ctx, bg::launch_direct_shade(p, sc, b, ds, kind, st));
         launches += 3;
         if (nq) {
-            HIP_TRY(ctx, profiled_traverse(ctx, traverse_args(s, ctx, ds.sray, ds.shit, nq), true, gq, st));
+            HIP_TRY(ctx, bg::profiled_traverse(ctx, traverse_args(s, ctx, ds.sray, ds.shit, nq), true, ctx->shape, gq, st));
             ++launches;
             if (kind != 2) {
-                HIP_TRY(ctx, profiled_traverse(ctx, traverse_args(s, ctx, ds.mray, ds.mhit, nq), false, gq, st));
+                HIP_TRY(ctx, bg::profiled_traverse(ctx, traverse_args(s, ctx, ds.mray, ds.mhit, nq), false, ctx->shape, gq, st));
                 ++launches;
             }
         }
@@ -664,7 +665,7 @@ int direct_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, const b
         stats->launches = launches;
         stats->samples = npix * d->spp;
     }
-    return profile_end(ctx, stats);
+    return bg::profile_end(ctx, stats);
 }
 
 }  // namespace

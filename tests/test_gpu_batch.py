@@ -73,6 +73,30 @@ def test_fixed_shapes_and_work_queue(ctx, items, variant):
         ctx.set_launch(b0, s0, c0, v0)
 
 
+def test_measured_batch_render(ctx, items):
+    """With the context's measurement on (bg_set_profiling), a batch render counts its traversal
+    launches' tests and times them (passes run as direct launches instead of graph replays); the
+    films are the same as without it."""
+    br = BatchRenderer(ctx, items[:3])
+    st0 = br.render(2, 4, autotune=False)
+    ref = [br.sums(k) for k in range(3)]
+    assert st0.nodes_visited == 0 and st0.traverse_ms == 0.0
+    try:
+        ctx.set_profiling(count=True)
+        st1 = br.render(2, 4, autotune=False)
+        ctx.set_profiling(traversal_time=True)
+        st2 = br.render(2, 4, autotune=False)
+    finally:
+        ctx.set_profiling()
+    assert st1.nodes_visited > st1.rays > 0 and st1.tris_tested > 0 and st1.rays == st0.rays
+    assert st2.traverse_ms > 0.0 and st2.traverse_ms <= st2.kernel_ms and st2.rays == st0.rays
+    for k in range(3):
+        rgb, w = br.sums(k)
+        np.testing.assert_array_equal(rgb, ref[k][0])
+        np.testing.assert_array_equal(w, ref[k][1])
+    br.close()
+
+
 def test_batch_rerender_clears_the_film(ctx, items):
     br = BatchRenderer(ctx, items[:3])
     br.render(2, 3, autotune=False)
