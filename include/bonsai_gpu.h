@@ -262,8 +262,9 @@ typedef struct bg_launch_config {
 } bg_launch_config;
 int bg_set_launch_config(bg_ctx* ctx, const bg_launch_config* cfg);
 /* Measurement of the render calls' traversal launches (bg_render_film,
- * bg_render_film_guided, bg_render_film_direct, bg_render; not scene
- * batches): BG_PROFILE_COUNT counts their box and primitive tests into
+ * bg_render_film_guided, bg_render_film_direct, bg_render, bg_render_batch;
+ * a measured batch render runs its passes as direct launches instead of a
+ * graph replay): BG_PROFILE_COUNT counts their box and primitive tests into
  * bg_stats as BG_COUNT does for bg_intersect (slower: for the algorithmic
  * bytes of a render, in an untimed call); BG_PROFILE_TRAVERSAL_TIME records a
  * HIP event pair around each traversal kernel and sums the durations into
