@@ -11,7 +11,7 @@ pub const BG_ERR_INVALID: c_int = -1;
 pub const BG_ERR_HIP: c_int = -2;
 pub const BG_ERR_NOMEM: c_int = -3;
 pub const BG_ERR_UNSUPPORTED: c_int = -4;
-pub const BG_ABI_VERSION: c_int = 2;
+pub const BG_ABI_VERSION: c_int = 3;
 
 pub const BG_CLOSEST: u32 = 0;
 pub const BG_ANY: u32 = 1;
@@ -249,6 +249,7 @@ pub struct bg_guiding_desc {
     pub replay: u32,
     pub lobes: u32,
     pub bounces: u32,
+    pub rr_start: u32,
 }
 
 extern "C" {

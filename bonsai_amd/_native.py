@@ -125,7 +125,7 @@ class bg_render_desc(C.Structure):
 class bg_guiding_desc(C.Structure):
     _fields_ = [("select_prob", C.c_float), ("train", C.c_uint32), ("train_paths", C.c_uint32),
                 ("batch", C.c_uint32), ("steps", C.c_uint32), ("lr", C.c_float), ("replay", C.c_uint32),
-                ("lobes", C.c_uint32), ("bounces", C.c_uint32)]
+                ("lobes", C.c_uint32), ("bounces", C.c_uint32), ("rr_start", C.c_uint32)]
 
 
 class bg_direct_desc(C.Structure):

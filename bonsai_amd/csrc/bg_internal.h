@@ -244,6 +244,7 @@ struct RenderParamsDev {
     float guide_alpha = 0.0f;
     uint32_t guide_lobes = 3;  // bit 0: guide at diffuse vertices, bit 1: at glossy vertices
     uint32_t guide_bounces = 0xFFFFFFFFu;  // vertices (bounces) below this are guided
+    uint32_t rr_start = 4;                 // Russian roulette from this bounce on (path.rs:133: bounces > 3)
     uint32_t rec_stride = 1, rec_paths = 0;
     // scene batches: items sorted by path_base (and pix_base); the first sample of the pass
     // is read from pass_first (device) so a captured pass can be replayed

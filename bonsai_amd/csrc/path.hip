@@ -323,7 +323,7 @@ __device__ __forceinline__ uint32_t shade_path(const RenderParamsDev& p, const S
         } else {
             beta = beta * (fs * __builtin_fabsf(dot(wi, bs.ns)) / pdf);
             flags = (stype & kSpecular) ? (flags | kSpecFlag) : (flags & ~kSpecFlag);
-            if (cmax(beta) < p.rr && bounce > 3) {
+            if (cmax(beta) < p.rr && bounce >= p.rr_start) {
                 const float q = rmax(1.0f - cmax(beta), 0.5f);
                 if (draw(key, dim) < q) alive = false;
                 else beta = beta / (1.0f - q);

@@ -428,6 +428,7 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
         p.guide_alpha = gd->select_prob;
         p.guide_lobes = gd->lobes ? gd->lobes : 3u;
         p.guide_bounces = gd->bounces ? gd->bounces : 0xFFFFFFFFu;
+        if (gd->rr_start) p.rr_start = gd->rr_start;
         if (gd->train) {
             train_paths = gd->train_paths ? gd->train_paths : (1u << 20);
             batch = std::min<uint64_t>(gd->batch ? gd->batch : 16384, guide->scratch.cap);

@@ -192,11 +192,11 @@ class GuidedPathIntegrator(GpuPathIntegrator):
     def __init__(self, ctx: Context, camera: Camera, spp: int, max_depth: int, guide, rr_threshold: float = 1.0,
                  seed: int = 1, select_prob: float = 0.5, train: bool = True, train_paths: int = 1 << 20,
                  batch: int = 16384, steps: int = 8, lr: float = 1e-2, replay: int = 1 << 20, lobes: int = 0,
-                 bounces: int = 0):
+                 bounces: int = 0, rr_start: int = 0):
         super().__init__(ctx, camera, spp, max_depth, rr_threshold, MODE_CORRECTED, seed)
         self.guide = guide
         self.guiding = N.bg_guiding_desc(select_prob, 1 if train else 0, train_paths, batch, steps, lr, replay,
-                                         lobes, bounces)
+                                         lobes, bounces, rr_start)
 
     def render_into(self, bvh: GpuBvh, film: Film, spp=None, first_sample=0, window=None,
                     paths_per_pass=0) -> N.bg_stats:

@@ -35,7 +35,7 @@ extern "C" {
 #define BG_ERR_NOMEM (-3)
 #define BG_ERR_UNSUPPORTED (-4)
 
-#define BG_ABI_VERSION 2
+#define BG_ABI_VERSION 3
 
 /* Query flags for bg_intersect / bg_intersect_device. */
 #define BG_CLOSEST 0u
@@ -496,6 +496,8 @@ typedef struct bg_guiding_desc {
     uint32_t bounces;      /* guide (and evaluate the network) at the first `bounces` vertices of a
                               path only; later vertices use the same estimator with select_prob 0
                               and are not recorded for training (0: every vertex) */
+    uint32_t rr_start;     /* Russian roulette (rr_threshold) from this bounce on (0: bounce 4, as
+                              the plain path, path.rs:133) */
 } bg_guiding_desc;
 #define BG_GUIDE_DIFFUSE 1u
 #define BG_GUIDE_GLOSSY 2u

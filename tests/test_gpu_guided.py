@@ -58,6 +58,33 @@ def test_select_prob_zero_ignores_the_network(ctx):
     np.testing.assert_array_equal(films[0][1], films[1][1])
 
 
+def test_roulette_start(ctx):
+    """rr_start sets the first bounce Russian roulette may end a guided path at: 0 is the plain
+    path's bounce 4 (the same film as 4), and a start past the last bounce never draws, which is the
+    film of no roulette; a start of 2 ends paths earlier (fewer rays, another film)."""
+    s = cornell_box(width=64, height=64)
+    bvh = GpuBvh(ctx, s)
+    g = new_guide(ctx)
+
+    def film(rr, start):
+        gi = GuidedPathIntegrator(ctx, s.camera, spp=8, max_depth=6, guide=g, seed=4, train=False,
+                                  rr_threshold=rr, rr_start=start)
+        f = Film(ctx, 64, 64)
+        st = gi.render_into(bvh, f)
+        out = f.sums() + (st.rays,)
+        f.close()
+        return out
+
+    a, b = film(1.0, 0), film(1.0, 4)
+    np.testing.assert_array_equal(a[0], b[0])
+    assert a[2] == b[2]
+    c, d = film(1.0, 7), film(0.0, 0)
+    np.testing.assert_array_equal(c[0], d[0])
+    assert c[2] == d[2]
+    e = film(1.0, 2)
+    assert e[2] < a[2] < d[2]
+
+
 @pytest.mark.parametrize("with_plastic", [False, True])
 def test_guided_agrees_with_plain_on_cornell(ctx, with_plastic):
     s = cornell_box(width=48, height=48, with_plastic=with_plastic)
