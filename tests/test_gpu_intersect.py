@@ -345,8 +345,10 @@ def test_deep_tree_really_is_deep(ctx):
 
 
 def test_full_size_intersection_benchmark_subset(ctx):
-    """The 2^24-ray, 1.31M-triangle workload: every ray on the GPU, a 40k-ray random subset against the
-    oracle, and a size-independent property on all rays (any-hit reports a hit exactly when closest-hit does)."""
+    """The 2^24-ray, 1.31M-triangle workload (coherent and incoherent sets): every ray on the GPU, random
+    subsets of closest and any hits (the any-hit id is the first hit in the reference's traversal
+    order) against the oracle, and a size-independent property on all rays (any-hit reports a hit
+    exactly when closest-hit does)."""
     s = intersection_scene(8)
     assert s.n_tris == 1_310_720
     bvh = GpuBvh(ctx, s)
@@ -360,9 +362,16 @@ def test_full_size_intersection_benchmark_subset(ctx):
     idx = rng.choice(rays.shape[0], 40_000, replace=False)
     ref = o.intersect(rays[idx])
     assert np.array_equal(closest[idx, 0], ref[:, 0]) and same_bits(closest[idx, 1:], ref[:, 1:])
+    ref = o.intersect(rays[idx], any_hit=True)
+    assert np.array_equal(anyh[idx, 0], ref[:, 0]) and same_bits(anyh[idx, 1:], ref[:, 1:])
+    del rays, closest, anyh
     lo, hi = s.bounds()
-    inc = incoherent_rays(1 << 20, lo, hi, 11)
+    inc = incoherent_rays(1 << 24, lo, hi, 11)
     g = O.hits_as_words(bvh.intersect(inc))
+    ga = O.hits_as_words(bvh.intersect_test(inc))
+    assert np.array_equal(g[:, 0] != MISS, ga[:, 0] != MISS)
     idx = rng.choice(inc.shape[0], 20_000, replace=False)
     ref = o.intersect(inc[idx])
     assert np.array_equal(g[idx, 0], ref[:, 0]) and same_bits(g[idx, 1:], ref[:, 1:])
+    ref = o.intersect(inc[idx], any_hit=True)
+    assert np.array_equal(ga[idx, 0], ref[:, 0]) and same_bits(ga[idx, 1:], ref[:, 1:])
