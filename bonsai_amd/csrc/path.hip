@@ -507,27 +507,51 @@ __global__ __launch_bounds__(256) void k_ray_count(const float4* __restrict__ ra
     if (threadIdx.x == 0) blk[blockIdx.x] = total;
 }
 
-// Per 256 rays, how many can hit, from the wavefront masks the kernel that wrote the rays stored
-// (store_ray_masks): one thread per 256 rays, reading the block's words below ceil(n / 64).
+// Per group of 4096 rays (64 mask words), how many can hit, from the wavefront masks the kernel that
+// wrote the rays stored (store_ray_masks): one wavefront per group, lane l reading word 64 g + l
+// (words from ceil(n / 64) on count as empty).
 __global__ __launch_bounds__(256) void k_mask_count(const uint64_t* __restrict__ mask, uint64_t n,
-                                                    uint32_t* __restrict__ blk) {
-    const uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (j >= (n + 255) / 256) return;
-    const uint64_t nw = (n + 63) / 64;
-    uint32_t c = 0;
+                                                    uint32_t* __restrict__ gcnt) {
+    const uint64_t g = ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t nw = (n + 63) / 64, ng = (nw + 63) / 64;
+    if (g >= ng) return;  // (whole wavefronts)
+    const uint64_t w = 64 * g + lane;
+    uint32_t c = w < nw ? (uint32_t)__popcll(mask[w]) : 0u;
 #pragma unroll
-    for (uint64_t w = 4 * j; w < 4 * j + 4; ++w)
-        if (w < nw) c += (uint32_t)__popcll(mask[w]);
-    blk[j] = c;
+    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+    if (lane == 0) gcnt[g] = c;
+}
+
+// After k_live_scan over the group counts: each ray that can hit at its group's offset, plus the
+// live rays of the group's earlier words (a wavefront-wide scan over the group's 64 words), plus
+// its rank in its word; ascending ray order. No misses are written (see kQuiet).
+__global__ __launch_bounds__(256) void k_mask_compact(uint64_t n, const uint32_t* __restrict__ gcnt,
+                                                      const uint64_t* __restrict__ mask, uint32_t* __restrict__ idx) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t word = ((uint64_t)blockIdx.x * 256 + threadIdx.x) >> 6;  // this wavefront's word
+    const uint64_t nw = (n + 63) / 64;
+    if (word >= nw) return;  // (whole wavefronts)
+    const uint64_t g = word >> 6;
+    const uint64_t w = 64 * g + lane;
+    const uint32_t c = w < nw ? (uint32_t)__popcll(mask[w]) : 0u;
+    uint32_t x = c;  // inclusive scan over the group's words
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    const uint32_t k = (uint32_t)(word & 63);
+    const uint32_t before = __shfl(x - c, k, 64);  // live rays in the group's words before this one
+    const uint64_t m = mask[word];
+    if ((m >> lane) & 1ull)
+        idx[gcnt[g] + before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)(64 * word + lane);
 }
 
 // After k_live_scan over blk: each ray that can hit at its block's offset plus its rank in the
 // block (ascending ray order); every other ray gets the miss the traversal would report. The
-// ranks come from wavefront masks (k_ray_count's, or the ray writer's: bits past n are zero,
-// words from ceil(n / 64) on are not read), so the rays are not read again.
-// MISSES: write the misses (not needed where the masks came from the ray writer: every reader of a
-// hit of these rays reads only hits of rays that can hit, see kQuiet).
-template <bool MISSES>
+// ranks come from k_ray_count's wavefront masks (bits past n are zero), so the rays are not read
+// again.
 __global__ __launch_bounds__(256) void k_ray_compact(uint64_t n, const uint32_t* __restrict__ blk,
                                                      const uint64_t* __restrict__ mask, uint32_t* __restrict__ idx,
                                                      uint4* __restrict__ hits) {
@@ -539,7 +563,7 @@ __global__ __launch_bounds__(256) void k_ray_compact(uint64_t n, const uint32_t*
     for (uint32_t w = 0; w < wave; ++w) off += (uint32_t)__popcll(bm[w]);
     const uint64_t m = bm[wave];
     if ((m >> lane) & 1ull) idx[off + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)i;
-    else if (MISSES && i < n) hits[i] = make_uint4(0xFFFFFFFFu, 0u, 0u, 0u);
+    else if (i < n) hits[i] = make_uint4(0xFFFFFFFFu, 0u, 0u, 0u);
 }
 
 __global__ __launch_bounds__(256) void k_live_count(uint64_t n, PathBuffers b) {
@@ -681,11 +705,17 @@ hipError_t launch_ray_compact(const float4* rays, uint64_t n, uint32_t* blk, uin
                               uint32_t* count, uint4* hits, hipStream_t s, const uint64_t* produced) {
     if (n == 0) return hipSuccess;
     const uint32_t nblk = (uint32_t)((n + 255) / 256);
-    if (produced) hipLaunchKernelGGL(k_mask_count, dim3((nblk + 255) / 256), dim3(256), 0, s, produced, n, blk);
-    else hipLaunchKernelGGL(k_ray_count, dim3(nblk), dim3(256), 0, s, rays, n, blk, mask);
+    if (produced) {
+        // counts per 4096 rays, so the one-block scan covers 16 times fewer counts
+        const uint32_t ngrp = (uint32_t)((n + 4095) / 4096);
+        hipLaunchKernelGGL(k_mask_count, dim3((ngrp + 3) / 4), dim3(256), 0, s, produced, n, blk);
+        hipLaunchKernelGGL(k_live_scan, dim3(1), dim3(1024), 0, s, ngrp, blk, count);
+        hipLaunchKernelGGL(k_mask_compact, dim3(nblk), dim3(256), 0, s, n, blk, produced, idx);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_ray_count, dim3(nblk), dim3(256), 0, s, rays, n, blk, mask);
     hipLaunchKernelGGL(k_live_scan, dim3(1), dim3(1024), 0, s, nblk, blk, count);
-    if (produced) hipLaunchKernelGGL(k_ray_compact<false>, dim3(nblk), dim3(256), 0, s, n, blk, produced, idx, hits);
-    else hipLaunchKernelGGL(k_ray_compact<true>, dim3(nblk), dim3(256), 0, s, n, blk, mask, idx, hits);
+    hipLaunchKernelGGL(k_ray_compact, dim3(nblk), dim3(256), 0, s, n, blk, mask, idx, hits);
     return hipGetLastError();
 }
 hipError_t launch_guide_inputs(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b, hipStream_t s) {
