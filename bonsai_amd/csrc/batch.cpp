@@ -133,10 +133,10 @@ hipError_t traverse(const bg_batch* bt, bg_ctx* ctx, const bg::LaunchShape& sh, 
     if (bg::traverse_compacts(a, shape, grid)) {
         // only the rays that can hit, through a device index list (the others' hits become misses)
         const bg::PathBuffers& b = ctx->paths;
-        const hipError_t e = bg::launch_ray_compact(rays, n, b.rblk, b.rmask, b.ridx, b.rcount, hits, st, mask);
+        const hipError_t e = bg::launch_ray_compact(rays, n, b.rblk[0], b.rmask[0], b.ridx[0], b.rcount[0], hits, st, mask);
         if (e != hipSuccess) return e;
-        a.idx = b.ridx;
-        a.n_idx = b.rcount;
+        a.idx = b.ridx[0];
+        a.n_idx = b.rcount[0];
     }
     if (measured) return bg::profiled_traverse(ctx, a, any, shape, grid, st);
     return bg::launch_traverse(a, any, false, shape, grid, st);

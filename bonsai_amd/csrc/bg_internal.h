@@ -177,13 +177,16 @@ struct PathBuffers {
     void* guide_mem = nullptr;
     float4* gin = nullptr;     // 3 per live path (compacted): network input (p, wo, n) at the current hit
     float* gout = nullptr;     // 40 per live path (compacted): network output
-    uint32_t* ridx = nullptr;   // compacted launches: indices of the rays that can hit
-    uint32_t* rblk = nullptr;   // per 256 rays: count, then its exclusive prefix
-    uint64_t* rmask = nullptr;  // per 64 rays: which of them can hit (bit l for ray 64 k + l)
+    // compacted launches, one set per ray set (extension, shadow, MIS branch) so that the three
+    // can run at once: indices of the rays that can hit, per 256 rays (or 4096) their count then
+    // its exclusive prefix, per 64 rays which can hit (k_ray_count), their number (device)
+    uint32_t* ridx[3] = {nullptr, nullptr, nullptr};
+    uint32_t* rblk[3] = {nullptr, nullptr, nullptr};
+    uint64_t* rmask[3] = {nullptr, nullptr, nullptr};
+    uint32_t* rcount[3] = {nullptr, nullptr, nullptr};
     // the same masks for the extension, shadow and MIS-branch rays (ray, sray, mray), stored by the
     // kernel that wrote the rays (k_init_paths, k_shade) so a compacted launch need not read them
     uint64_t* wmask[3] = {nullptr, nullptr, nullptr};
-    uint32_t* rcount = nullptr; // their number (device)
     uint32_t* gslot = nullptr; // per path: its slot in gin/gout this bounce, ~0 when not live
     uint32_t* gcount = nullptr; // live-path count of this bounce (device)
     uint32_t* gblk = nullptr;   // per 256 paths: live count, then its exclusive prefix
@@ -435,8 +438,15 @@ struct bg_ctx {
     // reusable device buffers
     uint32_t* spill = nullptr;
     unsigned long long* counters = nullptr;
-    unsigned long long* queue = nullptr;  // work-queue counter of the traversal launches (variant 5)
+    // work-queue counters of the traversal launches (variant 5): one per ray set, 64 bytes apart
+    unsigned long long* queue = nullptr;
     size_t spill_bytes = 0;
+    // path renders trace a bounce's shadow and MIS-branch rays on two more streams (default
+    // priority), beside the next bounce's extension rays; each ray set has its own spill rows
+    // (spill_set_bytes apart) and compaction buffers
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t fork_ev = nullptr, join_ev[2] = {nullptr, nullptr};
+    size_t spill_set_bytes = 0;
     void* io = nullptr;
     size_t io_bytes = 0;
     bg::PathBuffers paths;

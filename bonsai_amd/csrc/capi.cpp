@@ -270,6 +270,11 @@ void bg_destroy(bg_ctx* ctx) {
     if (ctx->direct.mem) hipFree(ctx->direct.mem);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
+    for (int k = 0; k < 2; ++k) {
+        if (ctx->side[k]) hipStreamDestroy(ctx->side[k]);
+        if (ctx->join_ev[k]) hipEventDestroy(ctx->join_ev[k]);
+    }
+    if (ctx->fork_ev) hipEventDestroy(ctx->fork_ev);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -46,9 +46,11 @@ int bg::ensure_paths(bg_ctx* ctx, size_t cap) {
     b.rec_aux = kept.rec_aux;
     b.rec_blk = kept.rec_blk;
     b.last_records = kept.last_records;
-    const size_t per_path[] = {32, 16, 16, 16, 8, 8, 8, 32, 16, 32, 16, 16, 16, 16, 16, 4};
-    size_t total = 256 + ((4 * (cap / 256 + 1) + 255) / 256) * 256 + 256;  // + rblk, rcount
-    total += 4 * (((32 * (cap / 256 + 1) + 255) / 256) * 256);              // + rmask, wmask[3]
+    const size_t per_path[] = {32, 16, 16, 16, 8, 8, 8, 32, 16, 32, 16, 16, 16, 16, 16};
+    size_t total = 256;                                                            // counters
+    total += 3 * (((4 * (cap / 256 + 1) + 255) / 256) * 256 + 256);              // rblk[3], rcount[3]
+    total += 6 * (((32 * (cap / 256 + 1) + 255) / 256) * 256);                   // rmask[3], wmask[3]
+    total += 3 * (((4 * cap + 255) / 256) * 256);                                // ridx[3]
     for (size_t s : per_path) total += ((s * cap + 255) / 256) * 256;
     void* mem = nullptr;
     HIP_TRY(ctx, hipMalloc(&mem, total));
@@ -74,12 +76,14 @@ int bg::ensure_paths(bg_ctx* ctx, size_t cap) {
     b.nee_b = (float4*)take(16 * cap);
     b.nee_c = (float4*)take(16 * cap);
     b.nee_d = (float4*)take(16 * cap);
-    b.ridx = (uint32_t*)take(4 * cap);
-    b.rblk = (uint32_t*)take(4 * (cap / 256 + 1));
-    b.rmask = (uint64_t*)take(32 * (cap / 256 + 1));
-    for (int k = 0; k < 3; ++k) b.wmask[k] = (uint64_t*)take(32 * (cap / 256 + 1));
-    b.rcount = (uint32_t*)take(256);
-    b.counters = (unsigned long long*)take(256);  // the 256 spare bytes of `total`
+    for (int k = 0; k < 3; ++k) {
+        b.ridx[k] = (uint32_t*)take(4 * cap);
+        b.rblk[k] = (uint32_t*)take(4 * (cap / 256 + 1));
+        b.rmask[k] = (uint64_t*)take(32 * (cap / 256 + 1));
+        b.wmask[k] = (uint64_t*)take(32 * (cap / 256 + 1));
+        b.rcount[k] = (uint32_t*)take(256);
+    }
+    b.counters = (unsigned long long*)take(256);
     b.cap = cap;
     return BG_OK;
 }
@@ -148,7 +152,21 @@ int ensure_replay(bg_guide* g, uint32_t cap) {
     return BG_OK;
 }
 
-bg::TraverseArgs traverse_args(const bg_scene* s, bg_ctx* ctx, const float4* rays, uint4* hits, uint64_t n) {
+// The two side streams (default priority) and the events that fork and join them, made once.
+int ensure_side_streams(bg_ctx* ctx) {
+    if (ctx->side[0]) return BG_OK;
+    for (int k = 0; k < 2; ++k) {
+        HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->side[k], hipStreamNonBlocking));
+        HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->join_ev[k], hipEventDisableTiming));
+    }
+    HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+    return BG_OK;
+}
+
+// set: the ray set (0 extension, 1 shadow, 2 MIS branch) whose spill rows and queue counter the
+// launch uses, so that launches of different sets can run at once.
+bg::TraverseArgs traverse_args(const bg_scene* s, bg_ctx* ctx, const float4* rays, uint4* hits, uint64_t n,
+                               int set = 0) {
     bg::TraverseArgs a;
     a.nodes = s->d_nodes;
     a.pairs = s->d_pairs;
@@ -161,28 +179,27 @@ bg::TraverseArgs traverse_args(const bg_scene* s, bg_ctx* ctx, const float4* ray
     a.rays = rays;
     a.hits = hits;
     a.n_rays = n;
-    a.spill = ctx->spill;
+    a.spill = ctx->spill + (set ? set * (ctx->spill_set_bytes / sizeof(uint32_t)) : 0);
     a.spill_depth = s->bvh.max_depth + 1;
     a.counters = nullptr;
-    a.queue = ctx->queue;
+    a.queue = ctx->queue + 8 * set;
     return a;
 }
 
 // One traversal launch of a path-integrator pass over n path slots (the path buffers hold at least
-// n). With the launch shape's compaction only the rays that can hit are traced, through an index
-// list built on the device, and the others' hits are written as misses: the hits are the same.
-// Launches that refill finished lanes skip it: there a lane at an ended path takes the next ray at
-// once (measured: compacting them cost 10% on a triangle scene, where it saved 11-14% on the
-// sphere-and-triangle path-tracer scene); so do launches of fewer than 1M slots.
+// n) of ray set `set`. With the launch shape's compaction (launches of 1M slots or more) only the
+// rays that can hit are traced, through an index list built on the device from the masks the ray
+// writer stored; the kernels that read hits read only those of rays that can hit.
 hipError_t trace_pass(const bg_scene* s, bg_ctx* ctx, const float4* rays, uint4* hits, uint64_t n, bool any, int grid,
-                      const uint64_t* mask, hipStream_t st) {
-    bg::TraverseArgs a = traverse_args(s, ctx, rays, hits, n);
+                      int set, const uint64_t* mask, hipStream_t st) {
+    bg::TraverseArgs a = traverse_args(s, ctx, rays, hits, n, set);
     if (bg::traverse_compacts(a, ctx->shape, grid)) {
         const bg::PathBuffers& b = ctx->paths;
-        const hipError_t e = bg::launch_ray_compact(rays, n, b.rblk, b.rmask, b.ridx, b.rcount, hits, st, mask);
+        const hipError_t e =
+            bg::launch_ray_compact(rays, n, b.rblk[set], b.rmask[set], b.ridx[set], b.rcount[set], hits, st, mask);
         if (e != hipSuccess) return e;
-        a.idx = b.ridx;
-        a.n_idx = b.rcount;
+        a.idx = b.ridx[set];
+        a.n_idx = b.rcount[set];
     }
     return bg::profiled_traverse(ctx, a, any, ctx->shape, grid, st);
 }
@@ -232,12 +249,28 @@ int bg::profile_end(bg_ctx* ctx, bg_stats* stats) {
         stats->spheres_tested = c[2];
     }
     if (ctx->profile & BG_PROFILE_TRAVERSAL_TIME) {
-        double sum = 0.0;
+        // the time during which at least one traversal kernel ran: launches of different ray sets
+        // overlap on the side streams, so the union of the [start, end] intervals, each taken from
+        // the first launch's start event
+        std::vector<std::pair<double, double>> iv(ctx->prof_used);
         for (size_t k = 0; k < ctx->prof_used; ++k) {
-            float ms = 0.0f;
-            HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->prof_events[2 * k], ctx->prof_events[2 * k + 1]));
-            sum += ms;
+            float a = 0.0f, e = 0.0f;
+            HIP_TRY(ctx, hipEventElapsedTime(&a, ctx->prof_events[0], ctx->prof_events[2 * k]));
+            HIP_TRY(ctx, hipEventElapsedTime(&e, ctx->prof_events[0], ctx->prof_events[2 * k + 1]));
+            iv[k] = {a, e};
         }
+        std::sort(iv.begin(), iv.end());
+        double sum = 0.0, lo = 0.0, hi = -1.0;
+        for (const auto& x : iv) {
+            if (x.first > hi) {
+                if (hi > lo) sum += hi - lo;
+                lo = x.first;
+                hi = x.second;
+            } else if (x.second > hi) {
+                hi = x.second;
+            }
+        }
+        if (hi > lo) sum += hi - lo;
         stats->traverse_ms = sum;
     }
     return BG_OK;
@@ -406,7 +439,12 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
     const int block = ctx->shape.block;
     const uint64_t grid_cap = ctx->shape.blocks_per_cu > 0 ? (uint64_t)ctx->num_cus * ctx->shape.blocks_per_cu : (1u << 20);
     const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((cap + block - 1) / block, std::min<uint64_t>(grid_cap, 1u << 20)));
-    rc = bg::ensure_spill(ctx, (size_t)grid * block * (s->bvh.max_depth + 1) * bg::kSpillWords * sizeof(uint32_t));
+    // spill rows for each of the three ray sets, which trace at once
+    const size_t per_set = (((size_t)grid * block * (s->bvh.max_depth + 1) * bg::kSpillWords * sizeof(uint32_t) + 255) / 256) * 256;
+    rc = bg::ensure_spill(ctx, 3 * per_set);
+    if (rc) return rc;
+    ctx->spill_set_bytes = per_set;
+    rc = ensure_side_streams(ctx);
     if (rc) return rc;
 
     bg::RenderParamsDev p;
@@ -472,8 +510,8 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
             p.rec_paths = (uint32_t)std::min<uint64_t>(train_paths, (p.n_paths + p.rec_stride - 1) / p.rec_stride);
         }
         HIP_TRY(ctx, bg::launch_init_paths(p, b, st));
+        HIP_TRY(ctx, trace_pass(s, ctx, b.ray, b.hit, p.n_paths, false, g, 0, b.wmask[0], st));
         for (uint32_t bounce = 0; bounce <= d->max_depth; ++bounce) {
-            HIP_TRY(ctx, trace_pass(s, ctx, b.ray, b.hit, p.n_paths, false, g, b.wmask[0], st));
             if (guided && bounce < d->max_depth && bounce < p.guide_bounces && (p.guide_alpha > 0.0f || train_paths)) {
                 HIP_TRY(ctx, bg::launch_guide_inputs(p, sc, b, st));  // the records copy these inputs
                 launches += 3;
@@ -484,18 +522,36 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
                 }
             }
             HIP_TRY(ctx, bg::launch_shade(p, sc, b, bounce, st));
-            if (sc.n_lights > 0 && bounce < d->max_depth) {
-                HIP_TRY(ctx, trace_pass(s, ctx, b.sray, b.shit, p.n_paths, true, g, b.wmask[1], st));
+            // The shadow and MIS-branch rays of this bounce trace on the side streams while the next
+            // bounce's extension rays trace here: the three read only what shading wrote and write
+            // their own hits, compaction buffers and spill rows. NEE resolves after all of them.
+            const bool nee = sc.n_lights > 0 && bounce < d->max_depth;
+            if (nee) {
+                HIP_TRY(ctx, hipEventRecord(ctx->fork_ev, st));
+                HIP_TRY(ctx, hipStreamWaitEvent(ctx->side[0], ctx->fork_ev, 0));
+                HIP_TRY(ctx, trace_pass(s, ctx, b.sray, b.shit, p.n_paths, true, g, 1, b.wmask[1], ctx->side[0]));
+                HIP_TRY(ctx, hipEventRecord(ctx->join_ev[0], ctx->side[0]));
                 // the guided estimator has no separate MIS ray: its continuation plays that part
                 if (!guided) {
-                    HIP_TRY(ctx, trace_pass(s, ctx, b.mray, b.mhit, p.n_paths, false, g, b.wmask[2], st));
+                    HIP_TRY(ctx, hipStreamWaitEvent(ctx->side[1], ctx->fork_ev, 0));
+                    HIP_TRY(ctx, trace_pass(s, ctx, b.mray, b.mhit, p.n_paths, false, g, 2, b.wmask[2], ctx->side[1]));
+                    HIP_TRY(ctx, hipEventRecord(ctx->join_ev[1], ctx->side[1]));
                     ++launches;
                 }
+            }
+            if (bounce < d->max_depth) {
+                HIP_TRY(ctx, trace_pass(s, ctx, b.ray, b.hit, p.n_paths, false, g, 0, b.wmask[0], st));
+                ++launches;
+            }
+            if (nee) {
+                HIP_TRY(ctx, hipStreamWaitEvent(st, ctx->join_ev[0], 0));
+                if (!guided) HIP_TRY(ctx, hipStreamWaitEvent(st, ctx->join_ev[1], 0));
                 HIP_TRY(ctx, bg::launch_nee_resolve(p, sc, b, bounce, st));
                 launches += 2;
             }
-            launches += 2;
+            ++launches;
         }
+        ++launches;  // the first extension launch
         HIP_TRY(ctx, bg::launch_film(p, b, film->width, film->height, film->sums, film->sums64, st));
         HIP_TRY(ctx, bg::launch_count_rays(p, b, b.counters, st));
         launches += 3;

@@ -267,9 +267,10 @@ int bg_set_launch_config(bg_ctx* ctx, const bg_launch_config* cfg);
  * graph replay): BG_PROFILE_COUNT counts their box and primitive tests into
  * bg_stats as BG_COUNT does for bg_intersect (slower: for the algorithmic
  * bytes of a render, in an untimed call); BG_PROFILE_TRAVERSAL_TIME records a
- * HIP event pair around each traversal kernel and sums the durations into
- * bg_stats.traverse_ms. 0 turns both off (the default). The images do not
- * change. */
+ * HIP event pair around each traversal kernel and writes the time during
+ * which at least one of them ran (path renders trace the shadow and
+ * MIS-branch rays beside the next extension rays) into bg_stats.traverse_ms.
+ * 0 turns both off (the default). The images do not change. */
 #define BG_PROFILE_COUNT 1u
 #define BG_PROFILE_TRAVERSAL_TIME 2u
 int bg_set_profiling(bg_ctx* ctx, uint32_t flags);
