@@ -92,8 +92,10 @@ void free_batch(bg_batch* b) {
         if (p) hipFree(p);
 }
 
+// set: the ray set (0 extension, 1 shadow, 2 MIS branch): its own spill rows and queue counter, so
+// that launches of different sets can run at once.
 bg::TraverseArgs batch_args(const bg_batch* bt, const bg_ctx* ctx, const float4* rays, uint4* hits, uint64_t n,
-                            const uint32_t* roots, bool queue) {
+                            const uint32_t* roots, bool queue, int set) {
     bg::TraverseArgs a;
     a.nodes = nullptr;  // batches traverse the pair layout only (variants 0, 2, 4 and 5)
     a.pairs = bt->pairs;
@@ -106,11 +108,11 @@ bg::TraverseArgs batch_args(const bg_batch* bt, const bg_ctx* ctx, const float4*
     a.rays = rays;
     a.hits = hits;
     a.n_rays = n;
-    a.spill = ctx->spill;
+    a.spill = ctx->spill + (set ? set * (ctx->spill_set_bytes / sizeof(uint32_t)) : 0);
     a.spill_depth = bt->max_depth + 1;
     a.counters = nullptr;
     a.roots = roots;
-    a.queue = queue ? bt->d_queue : nullptr;
+    a.queue = queue ? bt->d_queue + 8 * set : nullptr;
     return a;
 }
 
@@ -120,23 +122,25 @@ int grid_for(const bg_ctx* ctx, const bg::LaunchShape& sh, uint64_t n) {
     return (int)std::max<uint64_t>(1, std::min<uint64_t>(std::min(tiles, cap), 1u << 20));
 }
 
-// mask: the ray masks the kernel that wrote the rays stored (PathBuffers::wmask); measured: with the
-// context's measurement (bg_set_profiling), as a render's launches are (the autotuner's are not).
+// mask: the ray masks the kernel that wrote the rays stored (PathBuffers::wmask); set: the ray set
+// (its spill rows, queue counter and compaction buffers); measured: with the context's measurement
+// (bg_set_profiling), as a render's launches are (the autotuner's are not).
 hipError_t traverse(const bg_batch* bt, bg_ctx* ctx, const bg::LaunchShape& sh, const float4* rays, uint4* hits,
                     uint64_t n, const uint32_t* roots, bool any, const uint64_t* mask, hipStream_t st,
-                    bool measured = false) {
+                    bool measured = false, int set = 0) {
     const bool queue = sh.variant == 2 || sh.variant == 5;  // (the launch zeroes the counter)
     bg::LaunchShape shape = sh;
     if (shape.variant != 2 && shape.variant != 4 && shape.variant != 5) shape.variant = 0;  // the node-record variants need node roots
-    bg::TraverseArgs a = batch_args(bt, ctx, rays, hits, n, roots, queue);
+    bg::TraverseArgs a = batch_args(bt, ctx, rays, hits, n, roots, queue, set);
     const int grid = grid_for(ctx, shape, n);
     if (bg::traverse_compacts(a, shape, grid)) {
         // only the rays that can hit, through a device index list (the others' hits become misses)
         const bg::PathBuffers& b = ctx->paths;
-        const hipError_t e = bg::launch_ray_compact(rays, n, b.rblk[0], b.rmask[0], b.ridx[0], b.rcount[0], hits, st, mask);
+        const hipError_t e =
+            bg::launch_ray_compact(rays, n, b.rblk[set], b.rmask[set], b.ridx[set], b.rcount[set], hits, st, mask);
         if (e != hipSuccess) return e;
-        a.idx = b.ridx[0];
-        a.n_idx = b.rcount[0];
+        a.idx = b.ridx[set];
+        a.n_idx = b.rcount[set];
     }
     if (measured) return bg::profiled_traverse(ctx, a, any, shape, grid, st);
     return bg::launch_traverse(a, any, false, shape, grid, st);
@@ -300,7 +304,7 @@ int bg_batch_create(bg_ctx* ctx, const bg_batch_item* items, uint32_t n, bg_batc
     if (!rc) rc = dalloc(ctx, &bt->d_items, n);
     if (!rc) rc = dalloc(ctx, &bt->film, bt->total_pix);
     if (!rc) rc = dalloc(ctx, &bt->d_pass, 1);
-    if (!rc) rc = dalloc(ctx, &bt->d_queue, 1);
+    if (!rc) rc = dalloc(ctx, &bt->d_queue, 24);  // one counter per ray set, 64 bytes apart
     if (rc) {
         free_batch(bt);
         delete bt;
@@ -414,7 +418,12 @@ int bg_render_batch(bg_ctx* ctx, bg_batch* bt, const bg_batch_desc* d, bg_stats*
     // spill rows for the largest grid any launch shape of this call uses
     uint64_t max_threads = (uint64_t)grid_for(ctx, bt->shape, cap) * bt->shape.block;
     for (const bg::LaunchShape& c : cands) max_threads = std::max<uint64_t>(max_threads, (uint64_t)grid_for(ctx, c, cap) * c.block);
-    rc = bg::ensure_spill(ctx, (size_t)max_threads * (bt->max_depth + 1) * bg::kSpillWords * sizeof(uint32_t));
+    // (three sets of spill rows: a pass traces its ray sets at once)
+    const size_t per_set = (((size_t)max_threads * (bt->max_depth + 1) * bg::kSpillWords * sizeof(uint32_t) + 255) / 256) * 256;
+    rc = bg::ensure_spill(ctx, 3 * per_set);
+    if (rc) return rc;
+    ctx->spill_set_bytes = per_set;
+    rc = bg::ensure_side_streams(ctx);
     if (rc) return rc;
 
     if (d->autotune && !bt->tuned) {
@@ -490,13 +499,32 @@ int bg_render_batch(bg_ctx* ctx, bg_batch* bt, const bg_batch_desc* d, bg_stats*
         hipError_t e;
         if ((e = bg::launch_init_paths(p, b, s)) != hipSuccess) return e;
         ++launches;
+        if ((e = traverse(bt, ctx, sh, b.ray, b.hit, cap, b.root, false, b.wmask[0], s, true, 0)) != hipSuccess) return e;
+        ++launches;
         for (uint32_t bounce = 0; bounce <= d->max_depth; ++bounce) {
-            if ((e = traverse(bt, ctx, sh, b.ray, b.hit, cap, b.root, false, b.wmask[0], s, true)) != hipSuccess) return e;
             if ((e = bg::launch_shade(p, sc, b, bounce, s)) != hipSuccess) return e;
-            launches += 2;
-            if (sc.n_lights > 0 && bounce < d->max_depth) {
-                if ((e = traverse(bt, ctx, sh, b.sray, b.shit, cap, b.root, true, b.wmask[1], s, true)) != hipSuccess) return e;
-                if ((e = traverse(bt, ctx, sh, b.mray, b.mhit, cap, b.root, false, b.wmask[2], s, true)) != hipSuccess) return e;
+            ++launches;
+            // the shadow and MIS-branch rays on the side streams, beside the next extension rays
+            // (as in path renders; captured as a fork and a join of the graph)
+            const bool nee = sc.n_lights > 0 && bounce < d->max_depth;
+            if (nee) {
+                if ((e = hipEventRecord(ctx->fork_ev, s)) != hipSuccess) return e;
+                for (int k = 0; k < 2; ++k) {
+                    hipStream_t side = ctx->side[k];
+                    if ((e = hipStreamWaitEvent(side, ctx->fork_ev, 0)) != hipSuccess) return e;
+                    if ((e = traverse(bt, ctx, sh, k ? b.mray : b.sray, k ? b.mhit : b.shit, cap, b.root, k == 0,
+                                      b.wmask[1 + k], side, true, 1 + k)) != hipSuccess)
+                        return e;
+                    if ((e = hipEventRecord(ctx->join_ev[k], side)) != hipSuccess) return e;
+                }
+            }
+            if (bounce < d->max_depth) {
+                if ((e = traverse(bt, ctx, sh, b.ray, b.hit, cap, b.root, false, b.wmask[0], s, true, 0)) != hipSuccess) return e;
+                ++launches;
+            }
+            if (nee) {
+                for (int k = 0; k < 2; ++k)
+                    if ((e = hipStreamWaitEvent(s, ctx->join_ev[k], 0)) != hipSuccess) return e;
                 if ((e = bg::launch_nee_resolve(p, sc, b, bounce, s)) != hipSuccess) return e;
                 launches += 3;
             }

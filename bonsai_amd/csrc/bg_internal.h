@@ -308,6 +308,7 @@ hipError_t noalloc_fill_miss(void* hits, uint64_t n, hipStream_t s);
 // render.cpp: (re)allocate the context's path state and traversal spill buffer
 int ensure_paths(bg_ctx* ctx, size_t cap);
 int ensure_spill(bg_ctx* ctx, size_t bytes);
+int ensure_side_streams(bg_ctx* ctx);
 // render.cpp: the guide's replay ring of `cap` samples (emptied when the capacity changes)
 int guide_ensure_replay(bg_guide* g, uint32_t cap);
 

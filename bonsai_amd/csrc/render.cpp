@@ -88,6 +88,17 @@ int bg::ensure_paths(bg_ctx* ctx, size_t cap) {
     return BG_OK;
 }
 
+// The two side streams (default priority) and the events that fork and join them, made once.
+int bg::ensure_side_streams(bg_ctx* ctx) {
+    if (ctx->side[0]) return BG_OK;
+    for (int k = 0; k < 2; ++k) {
+        HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->side[k], hipStreamNonBlocking));
+        HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->join_ev[k], hipEventDisableTiming));
+    }
+    HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+    return BG_OK;
+}
+
 int bg::ensure_spill(bg_ctx* ctx, size_t need) {
     if (ctx->spill_bytes >= need) return BG_OK;
     if (ctx->spill) hipFree(ctx->spill);
@@ -149,17 +160,6 @@ int ensure_replay(bg_guide* g, uint32_t cap) {
     g->replay.tgt = (float4*)(q + 256 + ((48ull * cap + 255) / 256) * 256);
     g->replay.cap = cap;
     HIP_TRY(ctx, hipMemsetAsync(g->replay.state, 0, 16, ctx->stream));
-    return BG_OK;
-}
-
-// The two side streams (default priority) and the events that fork and join them, made once.
-int ensure_side_streams(bg_ctx* ctx) {
-    if (ctx->side[0]) return BG_OK;
-    for (int k = 0; k < 2; ++k) {
-        HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->side[k], hipStreamNonBlocking));
-        HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->join_ev[k], hipEventDisableTiming));
-    }
-    HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
     return BG_OK;
 }
 
@@ -444,7 +444,7 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
     rc = bg::ensure_spill(ctx, 3 * per_set);
     if (rc) return rc;
     ctx->spill_set_bytes = per_set;
-    rc = ensure_side_streams(ctx);
+    rc = bg::ensure_side_streams(ctx);
     if (rc) return rc;
 
     bg::RenderParamsDev p;
