@@ -662,7 +662,12 @@ int direct_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, const b
     const uint64_t grid_cap = ctx->shape.blocks_per_cu > 0 ? (uint64_t)ctx->num_cus * ctx->shape.blocks_per_cu : (1u << 20);
     const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((cap * n_slots + block - 1) / block,
                                                                     std::min<uint64_t>(grid_cap, 1u << 20)));
-    rc = bg::ensure_spill(ctx, (size_t)grid * block * (s->bvh.max_depth + 1) * bg::kSpillWords * sizeof(uint32_t));
+    // spill rows for the camera rays and the two light-slot ray sets, which trace at once
+    const size_t per_set = (((size_t)grid * block * (s->bvh.max_depth + 1) * bg::kSpillWords * sizeof(uint32_t) + 255) / 256) * 256;
+    rc = bg::ensure_spill(ctx, 3 * per_set);
+    if (rc) return rc;
+    ctx->spill_set_bytes = per_set;
+    rc = bg::ensure_side_streams(ctx);
     if (rc) return rc;
 
     bg::RenderParamsDev p;
@@ -695,12 +700,22 @@ int direct_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, const b
         HIP_TRY(ctx, bg::launch_direct_shade(p, sc, b, ds, kind, st));
         launches += 3;
         if (nq) {
-            HIP_TRY(ctx, bg::profiled_traverse(ctx, traverse_args(s, ctx, ds.sray, ds.shit, nq), true, ctx->shape, gq, st));
+            // the shadow and BSDF-sampling slots trace at once on the side streams
+            HIP_TRY(ctx, hipEventRecord(ctx->fork_ev, st));
+            HIP_TRY(ctx, hipStreamWaitEvent(ctx->side[0], ctx->fork_ev, 0));
+            HIP_TRY(ctx, bg::profiled_traverse(ctx, traverse_args(s, ctx, ds.sray, ds.shit, nq, 1), true, ctx->shape, gq,
+                                               ctx->side[0]));
+            HIP_TRY(ctx, hipEventRecord(ctx->join_ev[0], ctx->side[0]));
             ++launches;
             if (kind != 2) {
-                HIP_TRY(ctx, bg::profiled_traverse(ctx, traverse_args(s, ctx, ds.mray, ds.mhit, nq), false, ctx->shape, gq, st));
+                HIP_TRY(ctx, hipStreamWaitEvent(ctx->side[1], ctx->fork_ev, 0));
+                HIP_TRY(ctx, bg::profiled_traverse(ctx, traverse_args(s, ctx, ds.mray, ds.mhit, nq, 2), false, ctx->shape,
+                                                   gq, ctx->side[1]));
+                HIP_TRY(ctx, hipEventRecord(ctx->join_ev[1], ctx->side[1]));
                 ++launches;
             }
+            HIP_TRY(ctx, hipStreamWaitEvent(st, ctx->join_ev[0], 0));
+            if (kind != 2) HIP_TRY(ctx, hipStreamWaitEvent(st, ctx->join_ev[1], 0));
         }
         HIP_TRY(ctx, bg::launch_direct_resolve(p, sc, b, ds, kind, st));
         HIP_TRY(ctx, bg::launch_film(p, b, film->width, film->height, film->sums, film->sums64, st));
