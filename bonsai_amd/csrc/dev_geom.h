@@ -49,39 +49,49 @@ __device__ __forceinline__ float max3(float x, float y, float z) { return rmax(x
 
 __device__ __forceinline__ float sel3(float x, float y, float z, int k) { return k == 0 ? x : (k == 1 ? y : z); }
 
-// Per-ray state of a query.
+// Per-ray state of a query. The traversal loops hold one per lane for the whole
+// query, so it carries no value another field already determines: the
+// direction signs of bvh.rs:281-290 are the signs of ix, iy, iz, and the
+// permutation (kx, ky) of triangle.rs:148-156 follows from kz.
 struct RayState {
     float ox, oy, oz;       // origin
     float dx, dy, dz;       // direction
     float t_max;
     float ix, iy, iz;       // 1 / d
-    int neg0, neg1, neg2;   // inv < 0
-    uint32_t negbits;       // neg0 | neg1 << 1 | neg2 << 2 (bit 3, a record's "no axis", is 0)
-    int kx, ky, kz;         // triangle.rs:148-156 permutation
+    uint32_t negbits;       // (ix < 0) | (iy < 0) << 1 | (iz < 0) << 2 (bit 3, a record's "no axis", is 0)
+    int kz;                 // triangle.rs:148-156: kz = max_dimension(|d|), kx = kz + 1, ky = kx + 1 (mod 3)
     float okx, oky, okz;    // origin components in permuted order
     float sx, sy, sz;       // triangle.rs:163-165 shear
+    __host__ __device__ bool neg0() const { return ix < 0.0f; }
+    __host__ __device__ bool neg1() const { return iy < 0.0f; }
+    __host__ __device__ bool neg2() const { return iz < 0.0f; }
 };
+
+// Components of (x, y, z) in the ray's permuted order, the x, y and z of triangle.rs:157-160.
+__device__ __forceinline__ float sel_kx(const RayState& r, float x, float y, float z) {
+    return r.kz == 2 ? x : (r.kz == 0 ? y : z);
+}
+__device__ __forceinline__ float sel_ky(const RayState& r, float x, float y, float z) {
+    return r.kz == 1 ? x : (r.kz == 2 ? y : z);
+}
+__device__ __forceinline__ float sel_kz(const RayState& r, float x, float y, float z) {
+    return r.kz == 0 ? x : (r.kz == 1 ? y : z);
+}
 
 __device__ __forceinline__ void ray_setup(RayState& r) {
     r.ix = 1.0f / r.dx;
     r.iy = 1.0f / r.dy;
     r.iz = 1.0f / r.dz;
-    r.neg0 = r.ix < 0.0f;
-    r.neg1 = r.iy < 0.0f;
-    r.neg2 = r.iz < 0.0f;
-    r.negbits = (uint32_t)r.neg0 | (uint32_t)r.neg1 << 1 | (uint32_t)r.neg2 << 2;
+    r.negbits = (uint32_t)r.neg0() | (uint32_t)r.neg1() << 1 | (uint32_t)r.neg2() << 2;
     const float ax = __builtin_fabsf(r.dx), ay = __builtin_fabsf(r.dy), az = __builtin_fabsf(r.dz);
     // vec3.rs:159-171 max_dimension
-    int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
-    int kx = kz + 1; if (kx == 3) kx = 0;
-    int ky = kx + 1; if (ky == 3) ky = 0;
-    r.kx = kx; r.ky = ky; r.kz = kz;
-    const float dpx = sel3(r.dx, r.dy, r.dz, kx);
-    const float dpy = sel3(r.dx, r.dy, r.dz, ky);
-    const float dpz = sel3(r.dx, r.dy, r.dz, kz);
-    r.okx = sel3(r.ox, r.oy, r.oz, kx);
-    r.oky = sel3(r.ox, r.oy, r.oz, ky);
-    r.okz = sel3(r.ox, r.oy, r.oz, kz);
+    r.kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    const float dpx = sel_kx(r, r.dx, r.dy, r.dz);
+    const float dpy = sel_ky(r, r.dx, r.dy, r.dz);
+    const float dpz = sel_kz(r, r.dx, r.dy, r.dz);
+    r.okx = sel_kx(r, r.ox, r.oy, r.oz);
+    r.oky = sel_ky(r, r.ox, r.oy, r.oz);
+    r.okz = sel_kz(r, r.ox, r.oy, r.oz);
     r.sx = -dpx / dpz;
     r.sy = -dpy / dpz;
     r.sz = 1.0f / dpz;
@@ -89,17 +99,17 @@ __device__ __forceinline__ void ray_setup(RayState& r) {
 
 // bounds3.rs:246-288 with node bounds (lo, hi).
 __device__ __forceinline__ bool slab(const RayState& r, float lx, float ly, float lz, float hx, float hy, float hz) {
-    float t_min = ((r.neg0 ? hx : lx) - r.ox) * r.ix;
-    float t_max = ((r.neg0 ? lx : hx) - r.ox) * r.ix;
-    const float ty_min = ((r.neg1 ? hy : ly) - r.oy) * r.iy;
-    float ty_max = ((r.neg1 ? ly : hy) - r.oy) * r.iy;
+    float t_min = ((r.neg0() ? hx : lx) - r.ox) * r.ix;
+    float t_max = ((r.neg0() ? lx : hx) - r.ox) * r.ix;
+    const float ty_min = ((r.neg1() ? hy : ly) - r.oy) * r.iy;
+    float ty_max = ((r.neg1() ? ly : hy) - r.oy) * r.iy;
     t_max *= kSlabPad;
     ty_max *= kSlabPad;
     if (t_min > ty_max || ty_min > t_max) return false;
     if (ty_min > t_min) t_min = ty_min;
     if (ty_max < t_max) t_max = ty_max;
-    const float tz_min = ((r.neg2 ? hz : lz) - r.oz) * r.iz;
-    float tz_max = ((r.neg2 ? lz : hz) - r.oz) * r.iz;
+    const float tz_min = ((r.neg2() ? hz : lz) - r.oz) * r.iz;
+    float tz_max = ((r.neg2() ? lz : hz) - r.oz) * r.iz;
     tz_max *= kSlabPad;
     if (t_min > tz_max || tz_min > t_max) return false;
     if (tz_min > t_min) t_min = tz_min;
@@ -128,21 +138,32 @@ __device__ __forceinline__ bool slab(const RayState& r, float lx, float ly, floa
 // So entered <=> max(entries) <= min(exits) && min(exits) > 0, with the entry
 // distance max(entries) (equal to the reference's up to the sign of a zero,
 // which no comparison sees).
+//
+// The three exits are padded once, after the minimum: x -> round(pad * x) is
+// non-decreasing for the positive pad, so it commutes with the minimum, and a
+// NaN exit stays NaN, which minNum ignores in both orders; the minimum can
+// differ only in the sign of a zero, which neither comparison sees.
+//
+// slab_enter returns "entered" and stores the entry distance in *t_in (also
+// for a box not entered, where the caller must not use it).
+__host__ __device__ __forceinline__ bool slab_enter(const RayState& r, float lx, float ly, float lz, float hx,
+                                                    float hy, float hz, float* t_in) {
+    const float tx_min = ((r.neg0() ? hx : lx) - r.ox) * r.ix;
+    const float tx_max = ((r.neg0() ? lx : hx) - r.ox) * r.ix;
+    const float ty_min = ((r.neg1() ? hy : ly) - r.oy) * r.iy;
+    const float ty_max = ((r.neg1() ? ly : hy) - r.oy) * r.iy;
+    const float tz_min = ((r.neg2() ? hz : lz) - r.oz) * r.iz;
+    const float tz_max = ((r.neg2() ? lz : hz) - r.oz) * r.iz;
+    *t_in = __builtin_fmaxf(__builtin_fmaxf(tx_min, ty_min), tz_min);
+    const float t_out = kSlabPad * __builtin_fminf(__builtin_fminf(tx_max, ty_max), tz_max);
+    return !__builtin_isunordered(tx_min, tx_max) && *t_in <= t_out && t_out > 0.0f;
+}
+
+// The entry distance of a box the ray enters, NaN for one it does not.
 __host__ __device__ __forceinline__ float slab_entry(const RayState& r, float lx, float ly, float lz, float hx,
                                                      float hy, float hz) {
-    const float tx_min = ((r.neg0 ? hx : lx) - r.ox) * r.ix;
-    float tx_max = ((r.neg0 ? lx : hx) - r.ox) * r.ix;
-    const float ty_min = ((r.neg1 ? hy : ly) - r.oy) * r.iy;
-    float ty_max = ((r.neg1 ? ly : hy) - r.oy) * r.iy;
-    const float tz_min = ((r.neg2 ? hz : lz) - r.oz) * r.iz;
-    float tz_max = ((r.neg2 ? lz : hz) - r.oz) * r.iz;
-    tx_max *= kSlabPad;
-    ty_max *= kSlabPad;
-    tz_max *= kSlabPad;
-    const float t_in = __builtin_fmaxf(__builtin_fmaxf(tx_min, ty_min), tz_min);
-    const float t_out = __builtin_fminf(__builtin_fminf(tx_max, ty_max), tz_max);
-    const bool entered = !__builtin_isunordered(tx_min, tx_max) && t_in <= t_out && t_out > 0.0f;
-    return entered ? t_in : __builtin_nanf("");
+    float t_in;
+    return slab_enter(r, lx, ly, lz, hx, hy, hz, &t_in) ? t_in : __builtin_nanf("");
 }
 
 using bg::cross64_len2;
@@ -173,12 +194,12 @@ __device__ __forceinline__ bool tri_hit_full(const RayState& r, const float* __r
     const float p1x = v[3], p1y = v[4], p1z = v[5];
     const float p2x = v[6], p2y = v[7], p2z = v[8];
     // translate, permute (triangle.rs:142-160)
-    float a_x = sel3(p0x, p0y, p0z, r.kx) - r.okx, a_y = sel3(p0x, p0y, p0z, r.ky) - r.oky,
-          a_z = sel3(p0x, p0y, p0z, r.kz) - r.okz;
-    float b_x = sel3(p1x, p1y, p1z, r.kx) - r.okx, b_y = sel3(p1x, p1y, p1z, r.ky) - r.oky,
-          b_z = sel3(p1x, p1y, p1z, r.kz) - r.okz;
-    float c_x = sel3(p2x, p2y, p2z, r.kx) - r.okx, c_y = sel3(p2x, p2y, p2z, r.ky) - r.oky,
-          c_z = sel3(p2x, p2y, p2z, r.kz) - r.okz;
+    float a_x = sel_kx(r, p0x, p0y, p0z) - r.okx, a_y = sel_ky(r, p0x, p0y, p0z) - r.oky,
+          a_z = sel_kz(r, p0x, p0y, p0z) - r.okz;
+    float b_x = sel_kx(r, p1x, p1y, p1z) - r.okx, b_y = sel_ky(r, p1x, p1y, p1z) - r.oky,
+          b_z = sel_kz(r, p1x, p1y, p1z) - r.okz;
+    float c_x = sel_kx(r, p2x, p2y, p2z) - r.okx, c_y = sel_ky(r, p2x, p2y, p2z) - r.oky,
+          c_z = sel_kz(r, p2x, p2y, p2z) - r.okz;
     // shear (triangle.rs:166-171)
     a_x += r.sx * a_z; a_y += r.sy * a_z;
     b_x += r.sx * b_z; b_y += r.sy * b_z;

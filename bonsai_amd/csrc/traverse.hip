@@ -117,7 +117,7 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(TraverseArgs a) {
                 } else {
                     // interior: push far child, go to near child (bvh.rs:331-339)
                     const uint32_t axis = meta & 3u;
-                    const int neg = axis == 0 ? rs.neg0 : (axis == 1 ? rs.neg1 : rs.neg2);
+                    const int neg = axis == 0 ? rs.neg0() : (axis == 1 ? rs.neg1() : rs.neg2());
                     const uint32_t far = neg ? cur + 1 : off;
                     cur = neg ? off : cur + 1;
                     if (sp - base == STACK) {
@@ -236,7 +236,7 @@ __global__ __launch_bounds__(BLOCK) void k_traverse_refill(TraverseArgs a) {
                     }
                 } else {
                     const uint32_t axis = meta & 3u;
-                    const int neg = axis == 0 ? rs.neg0 : (axis == 1 ? rs.neg1 : rs.neg2);
+                    const int neg = axis == 0 ? rs.neg0() : (axis == 1 ? rs.neg1() : rs.neg2());
                     const uint32_t far = neg ? cur + 1 : off;
                     cur = neg ? off : cur + 1;
                     if (sp - base == STACK) {
@@ -394,21 +394,27 @@ __device__ __forceinline__ void pair_step(const TraverseArgs& a, PairLane& L, ui
         const float4 q0 = P[0], q1 = P[1], q2 = P[2];
         const uint4 q3 = *reinterpret_cast<const uint4*>(P + 3);
         const bool swap = (L.rs.negbits >> q3.z) & 1u;  // near child by the split axis (3: none)
-        // entry distances, NaN for a box the ray does not enter (NaN < t_max is false)
-        const float t0 = slab_entry(L.rs, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y);
-        const float t1 = slab_entry(L.rs, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w);
         const uint32_t rn = swap ? q3.y : q3.x, rf = swap ? q3.x : q3.y;
-        const float tn = swap ? t1 : t0, tf = swap ? t0 : t1;
+        float t0, t1;
+        const bool e0 = slab_enter(L.rs, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, &t0);
+        const bool e1 = slab_enter(L.rs, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, &t1);
+        // each child's box admitted at the current t_max (entered, entry below t_max)
+        const bool h0 = e0 && t0 < L.rs.t_max, h1 = e1 && t1 < L.rs.t_max;
+        const bool hn = swap ? h1 : h0, hf = swap ? h0 : h1;
         if (COUNT) ++c.nodes;  // the near child's box test
-        if (rf != kRefNone && (COUNT || tf < L.rs.t_max)) {
+        // counting pushes every far child, with a NaN entry for a box not entered (NaN < t_max
+        // is false when it is popped); otherwise only admitted far children are pushed
+        if (rf != kRefNone && (COUNT || hf)) {
             if (L.sp - L.base == STACK) {
                 spill[L.base] = ring[(L.base & kMask) * BLOCK];
                 ++L.base;
             }
-            ring[(L.sp & kMask) * BLOCK] = make_uint2(rf, __float_as_uint(tf));
+            const float tf = swap ? t0 : t1;
+            const bool ef = swap ? e0 : e1;
+            ring[(L.sp & kMask) * BLOCK] = make_uint2(rf, __float_as_uint(COUNT && !ef ? __builtin_nanf("") : tf));
             ++L.sp;
         }
-        if (tn < L.rs.t_max && rn != kRefNone) L.ref = rn;
+        if (hn && rn != kRefNone) L.ref = rn;
         else next = true;
     }
     if (next) {
@@ -539,9 +545,15 @@ void k_trav_pairs_refill(TraverseArgs a) {
     uint64_t ray = ~0ull;           // this lane's ray, ~0 for none
     PairLane L;
     L.ref = kRefNone;
+    // (one path back to the loop head: a refill that re-entered the loop head through its own
+    // edge made the compiler copy the whole lane state between two register sets on every step.
+    // One refill serves every free lane the sequence still has rays for, and pair_start puts a
+    // lane at a record, never at kRefNone, so after a refill a lane is free only once the
+    // sequence has no rays left: the refill condition cannot hold again right after it, and a
+    // wavefront with no lane at a record has finished.)
     for (;;) {
-        const uint64_t m_leaf = __builtin_amdgcn_ballot_w64((int32_t)L.ref < (int32_t)kRefNone);
-        const uint64_t m_int = __builtin_amdgcn_ballot_w64((int32_t)L.ref >= 0);
+        uint64_t m_leaf = __builtin_amdgcn_ballot_w64((int32_t)L.ref < (int32_t)kRefNone);
+        uint64_t m_int = __builtin_amdgcn_ballot_w64((int32_t)L.ref >= 0);
         const uint64_t m_free = ~(m_leaf | m_int);
         const uint32_t n_free = popc64(m_free);
         if (n_free != 0 && (n_free >= refill_min || n_free == 64) && seq_ray(pos) < n_rays) {
@@ -557,7 +569,8 @@ void k_trav_pairs_refill(TraverseArgs a) {
                 }
             }
             pos += n_free;
-            continue;
+            m_leaf = __builtin_amdgcn_ballot_w64((int32_t)L.ref < (int32_t)kRefNone);
+            m_int = __builtin_amdgcn_ballot_w64((int32_t)L.ref >= 0);
         }
         if ((m_leaf | m_int) == 0) break;
         pair_step<BLOCK, STACK, ANY, COUNT, SPH>(a, L, ring, spill, m_leaf, m_int, cnt);
@@ -594,9 +607,12 @@ void k_trav_pairs_queue(TraverseArgs a) {
     uint64_t ray = ~0ull;  // this lane's ray (physical id), ~0 for none
     PairLane L;
     L.ref = kRefNone;
+    // (one path back to the loop head, as in k_trav_pairs_refill; lanes a refill leaves free are
+    // served at a later step, and a wavefront with no lane at a record after a refill has no rays
+    // left: a lane that got none drew a position at or past the end of the queue)
     for (;;) {
-        const uint64_t m_leaf = __builtin_amdgcn_ballot_w64((int32_t)L.ref < (int32_t)kRefNone);
-        const uint64_t m_int = __builtin_amdgcn_ballot_w64((int32_t)L.ref >= 0);
+        uint64_t m_leaf = __builtin_amdgcn_ballot_w64((int32_t)L.ref < (int32_t)kRefNone);
+        uint64_t m_int = __builtin_amdgcn_ballot_w64((int32_t)L.ref >= 0);
         const uint64_t m_free = ~(m_leaf | m_int);
         const uint32_t n_free = popc64(m_free);
         if (n_free != 0 && (n_free >= a.refill_min || n_free == 64) && !(dry && cur >= end)) {
@@ -629,7 +645,8 @@ void k_trav_pairs_queue(TraverseArgs a) {
             } else {
                 cur = end;
             }
-            continue;
+            m_leaf = __builtin_amdgcn_ballot_w64((int32_t)L.ref < (int32_t)kRefNone);
+            m_int = __builtin_amdgcn_ballot_w64((int32_t)L.ref >= 0);
         }
         if ((m_leaf | m_int) == 0) break;
         pair_step<BLOCK, STACK, ANY, false, SPH>(a, L, ring, spill, m_leaf, m_int, cnt);

@@ -67,7 +67,6 @@ int main(int argc, char** argv) {
         rs.ox = o[0]; rs.oy = o[1]; rs.oz = o[2];
         rs.dx = d[0]; rs.dy = d[1]; rs.dz = d[2];
         rs.ix = 1.0f / d[0]; rs.iy = 1.0f / d[1]; rs.iz = 1.0f / d[2];
-        rs.neg0 = rs.ix < 0.0f; rs.neg1 = rs.iy < 0.0f; rs.neg2 = rs.iz < 0.0f;
         const float entry = bg::dev::slab_entry(rs, lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]);
         // the oracle: the reference's ray and box, inv_dir and dir_is_neg (bvh.rs:281-290)
         orc::Bounds3 b;
