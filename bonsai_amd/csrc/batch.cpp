@@ -348,8 +348,11 @@ int bg_render_batch(bg_ctx* ctx, bg_batch* bt, const bg_batch_desc* d, bg_stats*
     const uint32_t n = (uint32_t)bt->items.size();
     HIP_TRY(ctx, hipMemsetAsync(bt->film, 0, bt->total_pix * sizeof(float4), ctx->stream));
     if (d->spp == 0) return BG_OK;
-    // samples per pass: a divisor of spp so that every replay of the captured pass is identical
-    const uint64_t target = d->paths_per_pass ? d->paths_per_pass : (uint64_t)32 << 20;
+    // samples per pass: a divisor of spp so that every replay of the captured pass is identical.
+    // Default about 256M paths (264 bytes each: up to 68 GB of the 288 GB); the 64-scene,
+    // 28.3M-pixel, 16-spp batch at depth 6 measured 0.366 s in passes of 1 spp, 0.348 s of 2,
+    // 0.337 s of 4 and 0.315 s of 8: larger passes keep the deep bounces' launches full.
+    const uint64_t target = d->paths_per_pass ? d->paths_per_pass : (uint64_t)256 << 20;
     uint32_t spp_pass = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(d->spp, target / bt->total_pix));
     while (d->spp % spp_pass) --spp_pass;
     const uint64_t cap = bt->total_pix * spp_pass;

@@ -316,8 +316,10 @@ typedef struct bg_render_desc {
     uint32_t mode;          /* BG_MODE_REF / BG_MODE_CORRECTED */
     uint64_t seed;
     uint32_t x0, y0, x1, y1; /* pixel window; x1 == 0 and y1 == 0 means the full film */
-    uint32_t paths_per_pass; /* 0: about 16M for the path integrators (whole samples per pixel,
-                                at least one), about 4M for direct lighting and Whitted */
+    uint32_t paths_per_pass; /* 0: about 64M for the path integrators (16M while a guide trains:
+                                one training round follows each pass; whole samples per pixel, at
+                                least one), about 64M slots (paths times light slots) for
+                                direct lighting and Whitted */
 } bg_render_desc;
 
 /* Film: per-pixel RGB sums and filter-weight sums (src/base/film.rs). */
@@ -410,7 +412,7 @@ typedef struct bg_batch_desc {
     uint32_t max_depth;
     float rr_threshold;
     uint32_t mode;           /* BG_MODE_REF / BG_MODE_CORRECTED */
-    uint32_t paths_per_pass; /* 0: about 32M */
+    uint32_t paths_per_pass; /* 0: about 256M (whole samples per pixel dividing spp) */
     uint32_t use_graph;      /* nonzero: capture one pass in a hipGraph and replay it */
     uint32_t autotune;       /* nonzero: time the traversal launch shapes on the first call */
 } bg_batch_desc;
