@@ -30,6 +30,11 @@ struct PairRecord {
     uint32_t ref0, ref1, axis, pad;
 };
 constexpr uint32_t kRefLeaf = 0x80000000u, kRefNone = 0xFFFFFFFFu;
+// build_pairs numbers the first kTopRecords records of a scene breadth-first from the virtual root
+// (record 0, the BVH root's record 1, its children's records 2 and 3, ...; the rest follow in
+// depth-first order), so the traversal kernels can hold a prefix of the record array, the top
+// levels every ray passes through, in LDS.
+constexpr uint32_t kTopRecords = 64;
 // Words of one traversal stack entry (reference, entry distance) in the spill buffer.
 constexpr size_t kSpillWords = 2;
 
@@ -82,6 +87,8 @@ struct TraverseArgs {
     // j < *n_idx (a count on the device); n_rays is then only the upper bound for the grid.
     const uint32_t* idx = nullptr;
     const uint32_t* n_idx = nullptr;
+    // records 0 .. n_top - 1 are also read from a copy in LDS (single-scene launches; 0 for none)
+    uint32_t n_top = 0;
 };
 
 // Default = the best of a sweep over block {64,128,256} x stack {8,16} x
@@ -391,6 +398,7 @@ struct bg_scene {
     // device
     float4* d_nodes = nullptr;
     float4* d_pairs = nullptr;
+    uint64_t n_pairs = 0;
     uint2* d_leaves = nullptr;
     float* d_tri_verts = nullptr;
     uint32_t* d_slot_prim = nullptr;

@@ -287,6 +287,31 @@ void build_pairs(const HostBVH& bvh, std::vector<PairRecord>* pairs, std::vector
         r.axis = nd.meta & 3u;
         r.pad = 0;
     }
+    // Renumber: the first kTopRecords records breadth-first from the virtual root (child 0 before
+    // child 1), the others in their depth-first order after them. Only the numbering changes; every
+    // record keeps its boxes, children and split axis.
+    const size_t n_rec = pairs->size();
+    std::vector<uint32_t> id(n_rec, kRefNone), bfs;
+    bfs.reserve(std::min<size_t>(n_rec, 2 * (size_t)kTopRecords + 2));
+    bfs.push_back(0);
+    uint32_t next = 0;
+    for (size_t head = 0; head < bfs.size() && next < kTopRecords; ++head) {
+        const uint32_t r = bfs[head];
+        id[r] = next++;
+        for (uint32_t c : {(*pairs)[r].ref0, (*pairs)[r].ref1})
+            if (c != kRefNone && !(c & kRefLeaf)) bfs.push_back(c);
+    }
+    for (size_t r = 0; r < n_rec; ++r)
+        if (id[r] == kRefNone) id[r] = next++;
+    auto remap = [&](uint32_t c) { return (c == kRefNone || (c & kRefLeaf)) ? c : id[c]; };
+    std::vector<PairRecord> out(n_rec);
+    for (size_t r = 0; r < n_rec; ++r) {
+        PairRecord q = (*pairs)[r];
+        q.ref0 = remap(q.ref0);
+        q.ref1 = remap(q.ref1);
+        out[id[r]] = q;
+    }
+    pairs->swap(out);
 }
 
 }  // namespace bg

@@ -110,6 +110,7 @@ int run_traverse(bg_ctx* ctx, const bg_scene* s, const void* d_rays, size_t n, v
     bg::TraverseArgs a;
     a.nodes = s->d_nodes;
     a.pairs = s->d_pairs;
+    a.n_top = (uint32_t)std::min<uint64_t>(s->n_pairs, bg::kTopRecords);
     a.leaves = s->d_leaves;
     a.tri_verts = s->d_tri_verts;
     a.slot_prim = s->d_slot_prim;
@@ -401,6 +402,7 @@ int bg_scene_create(bg_ctx* ctx, const bg_scene_desc* d, bg_scene** out) {
         std::vector<uint32_t> leaves;
         bg::build_pairs(s->bvh, &pairs, &leaves);
         if (!rc) rc = upload(ctx, (bg::PairRecord**)&s->d_pairs, pairs.data(), pairs.size(), &bytes);
+        s->n_pairs = pairs.size();
         if (!rc) rc = upload(ctx, (uint32_t**)&s->d_leaves, leaves.data(), leaves.size(), &bytes);
     }
     if (!rc) rc = upload(ctx, &s->d_tri_verts, verts.data(), verts.size(), &bytes);

@@ -170,6 +170,7 @@ bg::TraverseArgs traverse_args(const bg_scene* s, bg_ctx* ctx, const float4* ray
     bg::TraverseArgs a;
     a.nodes = s->d_nodes;
     a.pairs = s->d_pairs;
+    a.n_top = (uint32_t)std::min<uint64_t>(s->n_pairs, bg::kTopRecords);
     a.leaves = s->d_leaves;
     a.tri_verts = s->d_tri_verts;
     a.slot_prim = s->d_slot_prim;

@@ -302,6 +302,9 @@ struct PairLane {
     float bt, bb1, bb2;
 };
 
+// A float in LDS, by address space.
+typedef __attribute__((address_space(3))) float LdsF;
+
 struct PairCounts {
     uint64_t nodes = 0, tris = 0, sph = 0;
 };
@@ -340,8 +343,8 @@ __device__ __forceinline__ uint32_t popc64(uint64_t m) {
 // visit counts are unchanged.
 template <int BLOCK, int STACK, bool ANY, bool COUNT, bool SPH>
 __device__ __forceinline__ void pair_step(const TraverseArgs& a, PairLane& L, uint2* __restrict__ ring,
-                                          uint2* __restrict__ spill, uint64_t m_leaf, uint64_t m_int,
-                                          PairCounts& c) {
+                                          uint2* __restrict__ spill, const LdsF* __restrict__ top,
+                                          uint32_t n_top, uint64_t m_leaf, uint64_t m_int, PairCounts& c) {
     constexpr uint32_t kMask = STACK - 1;
     const bool at_leaf = (int32_t)L.ref < (int32_t)kRefNone;
     const bool at_int = (int32_t)L.ref >= 0;
@@ -390,9 +393,22 @@ __device__ __forceinline__ void pair_step(const TraverseArgs& a, PairLane& L, ui
         }
     }
     if (at_int) {
-        const float4* P = a.pairs + 4 * (uint64_t)L.ref;
-        const float4 q0 = P[0], q1 = P[1], q2 = P[2];
-        const uint4 q3 = *reinterpret_cast<const uint4*>(P + 3);
+        // the record, from the LDS copy of the top levels where it is one of them
+        float4 q0, q1, q2;
+        uint4 q3;
+        if (L.ref < n_top) {
+            // (an LDS-typed pointer: the two branches then stay apart, LDS reads here and global
+            // loads below, instead of one generic load that would take both paths for every lane)
+            const LdsF* T = top + 16 * L.ref;
+            q0 = make_float4(T[0], T[1], T[2], T[3]);
+            q1 = make_float4(T[4], T[5], T[6], T[7]);
+            q2 = make_float4(T[8], T[9], T[10], T[11]);
+            q3 = make_uint4(__float_as_uint(T[12]), __float_as_uint(T[13]), __float_as_uint(T[14]), __float_as_uint(T[15]));
+        } else {
+            const float4* P = a.pairs + 4 * (uint64_t)L.ref;
+            q0 = P[0]; q1 = P[1]; q2 = P[2];
+            q3 = *reinterpret_cast<const uint4*>(P + 3);
+        }
         const bool swap = (L.rs.negbits >> q3.z) & 1u;  // near child by the split axis (3: none)
         const uint32_t rn = swap ? q3.y : q3.x, rf = swap ? q3.x : q3.y;
         float t0, t1;
@@ -438,6 +454,32 @@ __device__ __forceinline__ void pair_step(const TraverseArgs& a, PairLane& L, ui
     }
 }
 
+// Records of the top levels a block holds in LDS: BLOCK / 16 (16 for 256 threads: the virtual root
+// and the BVH's top four levels, 1 KB beside the 16 KB stack ring of an 8-entry stack). Measured on
+// the 2^24-ray sets (256 threads, 8-entry stack, 8 blocks per CU): 16 records 5,117 Mrays/s
+// against 5,005 with none; 64 records (4 KB) served more loads from LDS (incoherent closest-hit
+// 4.03 -> 3.99 ms) but, at 20 KB per block, cost the coherent sets more (2.89 -> 3.00 ms):
+// 5,085 Mrays/s.
+// The sphere-capable kernels hold none: the LDS branch of the record fetch took the closest-hit
+// one from 96 to 98 VGPRs, past the 5-waves-per-SIMD limit (1080p path tracer 0.144 -> 0.148 s).
+template <int BLOCK, bool SPH>
+constexpr uint32_t kBlockTop = SPH ? 0 : BLOCK / 16;
+
+// Copies the first n_top records (the top levels of the tree, build_pairs numbering) into LDS, one
+// float4 per thread. Every thread of the block calls it once, before any traversal.
+template <int BLOCK, bool SPH>
+__device__ __forceinline__ uint32_t load_top(const TraverseArgs& a, LdsF* __restrict__ top) {
+    if (kBlockTop<BLOCK, SPH> == 0) return 0;
+    const uint32_t n_top = a.n_top < kBlockTop<BLOCK, SPH> ? a.n_top : kBlockTop<BLOCK, SPH>;
+    if (threadIdx.x < 4 * n_top) {
+        const float4 v = a.pairs[threadIdx.x];
+        LdsF* d = top + 4 * threadIdx.x;
+        d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+    }
+    __syncthreads();
+    return n_top;
+}
+
 template <bool COUNT>
 __device__ __forceinline__ void flush_counts(const TraverseArgs& a, const PairCounts& c) {
     if (COUNT) {
@@ -467,6 +509,9 @@ template <int BLOCK, int STACK, bool ANY, bool COUNT, bool SPH, bool QUEUE>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(!SPH && !COUNT && !QUEUE ? 8 : 1, 8)))
 void k_trav_pairs(TraverseArgs a) {
     __shared__ uint2 lds[STACK * BLOCK];
+    __shared__ float4 top_[SPH ? 1 : 4 * kBlockTop<BLOCK, SPH>];
+    LdsF* top = (LdsF*)top_;
+    const uint32_t n_top = load_top<BLOCK, SPH>(a, top);
     const int tid = threadIdx.x;
     const uint64_t gtid = (uint64_t)blockIdx.x * BLOCK + tid;
     const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
@@ -500,7 +545,7 @@ void k_trav_pairs(TraverseArgs a) {
                 const uint64_t m_leaf = __builtin_amdgcn_ballot_w64((int32_t)L.ref < (int32_t)kRefNone);
                 const uint64_t m_int = __builtin_amdgcn_ballot_w64((int32_t)L.ref >= 0);
                 if ((m_leaf | m_int) == 0) break;
-                pair_step<BLOCK, STACK, ANY, COUNT, SPH>(a, L, ring, spill, m_leaf, m_int, cnt);
+                pair_step<BLOCK, STACK, ANY, COUNT, SPH>(a, L, ring, spill, top, n_top, m_leaf, m_int, cnt);
             }
             pair_store(a, rid, L);
         }
@@ -524,6 +569,9 @@ template <int BLOCK, int STACK, bool ANY, bool COUNT, bool SPH>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(!SPH ? 8 : 1, 8)))
 void k_trav_pairs_refill(TraverseArgs a) {
     __shared__ uint2 lds[STACK * BLOCK];
+    __shared__ float4 top_[SPH ? 1 : 4 * kBlockTop<BLOCK, SPH>];
+    LdsF* top = (LdsF*)top_;
+    const uint32_t n_top = load_top<BLOCK, SPH>(a, top);
     const int tid = threadIdx.x;
     const uint64_t gtid = (uint64_t)blockIdx.x * BLOCK + tid;
     uint2* __restrict__ spill = reinterpret_cast<uint2*>(a.spill) + gtid * a.spill_depth;
@@ -573,7 +621,7 @@ void k_trav_pairs_refill(TraverseArgs a) {
             m_int = __builtin_amdgcn_ballot_w64((int32_t)L.ref >= 0);
         }
         if ((m_leaf | m_int) == 0) break;
-        pair_step<BLOCK, STACK, ANY, COUNT, SPH>(a, L, ring, spill, m_leaf, m_int, cnt);
+        pair_step<BLOCK, STACK, ANY, COUNT, SPH>(a, L, ring, spill, top, n_top, m_leaf, m_int, cnt);
     }
     if (ray != ~0ull) pair_store(a, ray, L);
     flush_counts<COUNT>(a, cnt);
@@ -592,6 +640,9 @@ template <int BLOCK, int STACK, bool ANY, bool SPH>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(!SPH ? 8 : 1, 8)))
 void k_trav_pairs_queue(TraverseArgs a) {
     __shared__ uint2 lds[STACK * BLOCK];
+    __shared__ float4 top_[SPH ? 1 : 4 * kBlockTop<BLOCK, SPH>];
+    LdsF* top = (LdsF*)top_;
+    const uint32_t n_top = load_top<BLOCK, SPH>(a, top);
     const int tid = threadIdx.x;
     const uint64_t gtid = (uint64_t)blockIdx.x * BLOCK + tid;
     uint2* __restrict__ spill = reinterpret_cast<uint2*>(a.spill) + gtid * a.spill_depth;
@@ -649,7 +700,7 @@ void k_trav_pairs_queue(TraverseArgs a) {
             m_int = __builtin_amdgcn_ballot_w64((int32_t)L.ref >= 0);
         }
         if ((m_leaf | m_int) == 0) break;
-        pair_step<BLOCK, STACK, ANY, false, SPH>(a, L, ring, spill, m_leaf, m_int, cnt);
+        pair_step<BLOCK, STACK, ANY, false, SPH>(a, L, ring, spill, top, n_top, m_leaf, m_int, cnt);
     }
     if (ray != ~0ull) pair_store(a, ray, L);
 }
