@@ -652,7 +652,7 @@ def guiding_leg(ctx, spp: int, measure: bool = True, ref_spp: int = 2048, max_se
 
     def render(g, sel, train, seed, rr, n=spp, first=0):
         # training renders call once per 2 samples (the network trains after every call); the
-        # others render all samples in one call, in the library's passes (about 16M paths)
+        # others render all samples in one call, in the library's passes (about 64M paths)
         if g is None:
             gi = GpuPathIntegrator(ctx, scene.camera, spp=n, max_depth=10, mode=MODE_CORRECTED, seed=seed,
                                    rr_threshold=rr)
