@@ -406,7 +406,7 @@ def batch_leg(ctx, spp: int) -> dict:
     uniform in 128-1024), 16 spp, depth 6, mixed materials: one batched wavefront loop (hipGraph
     replay of a captured pass, autotuned traversal launch shape) against one render call per scene."""
     from bonsai_amd import GpuBvh
-    from bonsai_amd.integrator import MODE_CORRECTED, BatchRenderer, Film, GpuPathIntegrator
+    from bonsai_amd.integrator import MODE_CORRECTED, SCHEDULE_PERSISTENT, BatchRenderer, Film, GpuPathIntegrator
     from bonsai_amd.scenes import batch_scene
 
     scenes = [batch_scene(i) for i in range(64)]
@@ -429,6 +429,16 @@ def batch_leg(ctx, spp: int) -> dict:
     # (measured renders run their passes without the graph)
     out["batched"]["roofline"] = traversal_roofline(ctx, lambda: br.render(spp, 6, mode=MODE_CORRECTED,
                                                                             autotune=False))
+    # the same batch with each pass as one persistent path kernel (64-path groups from an atomic
+    # counter; the same films): scheduling measured against the wavefront passes
+    br.render(1, 6, mode=MODE_CORRECTED, autotune=False, schedule=SCHEDULE_PERSISTENT)
+    torch_sync()
+    t0 = time.perf_counter()
+    stp = br.render(spp, 6, mode=MODE_CORRECTED, autotune=False, schedule=SCHEDULE_PERSISTENT)
+    wallp = time.perf_counter() - t0
+    out["persistent"] = {"seconds": round(wallp, 3), "Msamples_s": round(stp.samples / wallp / 1e6, 2),
+                         "Mrays_s": round(stp.rays / wallp / 1e6, 1), "wavefront_speedup": round(wallp / wall, 3)}
+    log(f"batch leg: persistent path kernel {wallp:.2f} s against {wall:.2f} s for the wavefront passes")
     br.close()
     films = [Film(ctx, s.camera.width, s.camera.height) for s in scenes]
     for (bvh, cam, seed), f in zip(items, films):  # warm-up

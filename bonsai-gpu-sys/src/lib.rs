@@ -11,7 +11,7 @@ pub const BG_ERR_INVALID: c_int = -1;
 pub const BG_ERR_HIP: c_int = -2;
 pub const BG_ERR_NOMEM: c_int = -3;
 pub const BG_ERR_UNSUPPORTED: c_int = -4;
-pub const BG_ABI_VERSION: c_int = 3;
+pub const BG_ABI_VERSION: c_int = 4;
 
 pub const BG_CLOSEST: u32 = 0;
 pub const BG_ANY: u32 = 1;
@@ -235,7 +235,11 @@ pub struct bg_batch_desc {
     pub paths_per_pass: u32,
     pub use_graph: u32,
     pub autotune: u32,
+    pub schedule: u32,
 }
+
+pub const BG_SCHEDULE_WAVEFRONT: u32 = 0;
+pub const BG_SCHEDULE_PERSISTENT: u32 = 1;
 
 #[repr(C)]
 #[derive(Clone, Copy, Debug, Default)]

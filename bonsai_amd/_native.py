@@ -138,11 +138,14 @@ class bg_batch_item(C.Structure):
 
 class bg_batch_desc(C.Structure):
     _fields_ = [("spp", C.c_uint32), ("max_depth", C.c_uint32), ("rr_threshold", C.c_float), ("mode", C.c_uint32),
-                ("paths_per_pass", C.c_uint32), ("use_graph", C.c_uint32), ("autotune", C.c_uint32)]
+                ("paths_per_pass", C.c_uint32), ("use_graph", C.c_uint32), ("autotune", C.c_uint32),
+                ("schedule", C.c_uint32)]
 
 
 BG_MODE_REF = 0
 BG_MODE_CORRECTED = 1
+BG_SCHEDULE_WAVEFRONT = 0
+BG_SCHEDULE_PERSISTENT = 1
 BG_FILM_F64 = 1
 BG_INTEGRATOR_DIRECT = 1
 BG_INTEGRATOR_WHITTED = 2

@@ -304,7 +304,7 @@ int bg_batch_create(bg_ctx* ctx, const bg_batch_item* items, uint32_t n, bg_batc
     if (!rc) rc = dalloc(ctx, &bt->d_items, n);
     if (!rc) rc = dalloc(ctx, &bt->film, bt->total_pix);
     if (!rc) rc = dalloc(ctx, &bt->d_pass, 1);
-    if (!rc) rc = dalloc(ctx, &bt->d_queue, 24);  // one counter per ray set, 64 bytes apart
+    if (!rc) rc = dalloc(ctx, &bt->d_queue, 32);  // one counter per ray set and the persistent kernel's, 64 bytes apart
     if (rc) {
         free_batch(bt);
         delete bt;
@@ -343,6 +343,8 @@ int bg_batch_launch_config(const bg_batch* bt, bg_launch_config* cfg) {
 int bg_render_batch(bg_ctx* ctx, bg_batch* bt, const bg_batch_desc* d, bg_stats* stats) {
     if (!ctx || !bt || !d || bt->ctx != ctx) return fail(ctx, BG_ERR_INVALID, "null or foreign argument");
     if (d->mode > BG_MODE_CORRECTED) return fail(ctx, BG_ERR_INVALID, "unknown mode");
+    if (d->schedule > BG_SCHEDULE_PERSISTENT) return fail(ctx, BG_ERR_INVALID, "unknown schedule");
+    const bool persistent = d->schedule == BG_SCHEDULE_PERSISTENT;
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     if (stats) *stats = bg_stats{};
     const uint32_t n = (uint32_t)bt->items.size();
@@ -415,12 +417,16 @@ int bg_render_batch(bg_ctx* ctx, bg_batch* bt, const bg_batch_desc* d, bg_stats*
     // timed on all the traversal launches of one pass.
     std::vector<bg::LaunchShape> cands;
     const int kBlocks[] = {128, 256}, kStacks[] = {8, 16}, kBpc[] = {8, 16, 32, 64};
-    if (d->autotune && !bt->tuned)
+    const bool tuning = d->autotune && !bt->tuned && !persistent;  // (the persistent kernel has one shape)
+    if (tuning)
         for (int block : kBlocks)
             for (int bpc : kBpc) cands.push_back(bg::LaunchShape{block, 16, bpc, 0, 1});
     // spill rows for the largest grid any launch shape of this call uses
     uint64_t max_threads = (uint64_t)grid_for(ctx, bt->shape, cap) * bt->shape.block;
     for (const bg::LaunchShape& c : cands) max_threads = std::max<uint64_t>(max_threads, (uint64_t)grid_for(ctx, c, cap) * c.block);
+    // the persistent path kernel: as many 256-thread blocks as stay resident
+    const int pgrid = persistent ? std::max(1, ctx->num_cus) * bg::path_persistent_blocks_per_cu(sc, bt->spheres != nullptr) : 0;
+    max_threads = std::max<uint64_t>(max_threads, (uint64_t)pgrid * 256);
     // (three sets of spill rows: a pass traces its ray sets at once)
     const size_t per_set = (((size_t)max_threads * (bt->max_depth + 1) * bg::kSpillWords * sizeof(uint32_t) + 255) / 256) * 256;
     rc = bg::ensure_spill(ctx, 3 * per_set);
@@ -429,7 +435,7 @@ int bg_render_batch(bg_ctx* ctx, bg_batch* bt, const bg_batch_desc* d, bg_stats*
     rc = bg::ensure_side_streams(ctx);
     if (rc) return rc;
 
-    if (d->autotune && !bt->tuned) {
+    if (tuning) {
         // time every candidate of a stage on every traversal launch of one pass (the extension,
         // shadow and MIS-branch rays of each bounce, on the rays the shading kernels wrote: every
         // candidate gives the same hits); the candidate of least total time wins
@@ -500,6 +506,21 @@ int bg_render_batch(bg_ctx* ctx, bg_batch* bt, const bg_batch_desc* d, bg_stats*
     uint64_t launches = 0;
     auto enqueue_pass = [&](hipStream_t s) -> hipError_t {
         hipError_t e;
+        if (persistent) {
+            // one launch runs the pass's paths; its three ray sets share set 0's spill rows (each
+            // thread traces them one after another)
+            const bg::TraverseArgs ext = batch_args(bt, ctx, b.ray, b.hit, cap, b.root, false, 0);
+            const bg::TraverseArgs sha = batch_args(bt, ctx, b.sray, b.shit, cap, b.root, false, 0);
+            const bg::TraverseArgs mis = batch_args(bt, ctx, b.mray, b.mhit, cap, b.root, false, 0);
+            if ((e = bg::launch_path_persistent(p, sc, b, ext, sha, mis, bt->d_queue + 24, pgrid, s)) != hipSuccess)
+                return e;
+            launches += 2;
+            if ((e = bg::launch_batch_film(p, b, bt->total_pix, bt->film, s)) != hipSuccess) return e;
+            if ((e = bg::launch_count_rays(p, b, b.counters, s)) != hipSuccess) return e;
+            if ((e = bg::launch_pass_advance(bt->d_pass, spp_pass, s)) != hipSuccess) return e;
+            launches += 3;
+            return hipSuccess;
+        }
         if ((e = bg::launch_init_paths(p, b, s)) != hipSuccess) return e;
         ++launches;
         if ((e = traverse(bt, ctx, sh, b.ray, b.hit, cap, b.root, false, b.wmask[0], s, true, 0)) != hipSuccess) return e;

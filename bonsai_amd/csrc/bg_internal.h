@@ -301,6 +301,14 @@ hipError_t launch_batch_film(const RenderParamsDev& p, const PathBuffers& b, uin
 // Adds `spp` to *pass_first (the last node of a captured pass).
 hipError_t launch_pass_advance(uint32_t* pass_first, uint32_t spp, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long* p, hipStream_t s);
+// A plain pass as one persistent launch (k_path_persistent): zeroes *queue, then `grid` blocks of
+// 256 threads run whole paths of 64-path groups taken from it. The three argument sets are the
+// extension, shadow and MIS-branch queries (their spill rows: ext's, one row per thread).
+hipError_t launch_path_persistent(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b,
+                                  const TraverseArgs& ext, const TraverseArgs& sha, const TraverseArgs& mis,
+                                  unsigned long long* queue, int grid, hipStream_t s);
+// Resident 256-thread blocks per CU of the persistent path kernel for this scene's instantiation.
+int path_persistent_blocks_per_cu(const ShadeSceneDev& sc, bool spheres);
 hipError_t launch_film(const RenderParamsDev& p, const PathBuffers& b, uint32_t width, uint32_t height, float4* film,
                        double4* film64, hipStream_t s);
 

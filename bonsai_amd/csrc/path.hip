@@ -20,6 +20,7 @@
 #include "dev_light.h"
 #include "dev_scan.h"
 #include "dev_shade.h"
+#include "dev_trav.h"
 
 namespace bg {
 namespace {
@@ -361,11 +362,11 @@ __global__ __launch_bounds__(128) void k_shade(RenderParamsDev p, ShadeSceneDev 
     store_ray_masks<3>(b, i, live);
 }
 
+// One path's next-event estimate of this vertex, from its shadow and MIS-branch hits, added to
+// its radiance.
 template <bool SPL>
-__global__ __launch_bounds__(256) void k_nee_resolve(RenderParamsDev p, ShadeSceneDev sc, PathBuffers b,
-                                                     uint32_t bounce) {
-    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= p.n_paths) return;
+__device__ __forceinline__ void nee_resolve_path(const RenderParamsDev& p, const ShadeSceneDev& sc,
+                                                 const PathBuffers& b, uint32_t bounce, uint64_t i) {
     const float4 a = b.nee_a[i];
     const uint32_t nee = __float_as_uint(a.w);
     if (!(nee & kHasNee)) return;
@@ -393,6 +394,60 @@ __global__ __launch_bounds__(256) void k_nee_resolve(RenderParamsDev p, ShadeSce
     b.L[i] = make_float4(L.r, L.g, L.b, l4.w);
     uint64_t rec;
     if (rec_slot(p, i, bounce, &rec) && b.rec_tgt[rec].w != 0.0f) b.rec_aux[2 * rec + 1] = make_float4(L.r, L.g, L.b, 0.0f);
+}
+
+template <bool SPL>
+__global__ __launch_bounds__(256) void k_nee_resolve(RenderParamsDev p, ShadeSceneDev sc, PathBuffers b,
+                                                     uint32_t bounce) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < p.n_paths) nee_resolve_path<SPL>(p, sc, b, bounce, i);
+}
+
+// The persistent path kernel of a plain pass: each wavefront takes groups of 64 consecutive path
+// slots from an atomic counter (zeroed before the launch) and runs their whole paths: the camera
+// ray, then per bounce the shading, the shadow and MIS-branch queries, the extension ray and the
+// next-event resolve. These are the per-path functions the wavefront pass launches one kernel at a
+// time (init_path, shade_path, nee_resolve_path) in the same order on the same path-state arrays,
+// and each ray is traced by the same pair traversal, so every path's radiance, film point and ray
+// count equal the wavefront render's. The three ray sets share the block's LDS stack ring and one
+// spill row per thread, being traced one after another. Path slots run item-major, then sample,
+// then pixel, so a group is a run of one scene item's pixels at one sample.
+// The kernel holds the shading's registers and the traversal's together: 201-247 VGPRs (2 waves
+// per SIMD) as the compiler picks them; held to 4 waves (128 VGPRs, ~200 spilled) it ran fastest on
+// the 64-scene batch: 2 / 3 / 4 / 5 / 6 waves 0.72 / 0.57 / 0.52 / 0.53 / 0.56 s, against 0.33 s
+// for the wavefront passes, whose traversal launches run at 8 waves per SIMD.
+template <bool MULTI, bool SPL, bool SPH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_path_persistent(RenderParamsDev p, ShadeSceneDev sc, PathBuffers b,
+                                                         TraverseArgs ext, TraverseArgs sha, TraverseArgs mis,
+                                                         unsigned long long* queue) {
+    constexpr int kBlock = 256, kStack = 8;
+    __shared__ uint2 lds[kStack * kBlock];
+    const uint64_t gtid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    uint2* __restrict__ ring = lds + threadIdx.x;
+    uint2* __restrict__ spill = reinterpret_cast<uint2*>(ext.spill) + gtid * ext.spill_depth;
+    const uint32_t lane = threadIdx.x & 63;
+    PairCounts cnt;
+    for (;;) {
+        unsigned long long b0 = 0;
+        if (lane == 0) b0 = atomicAdd(queue, 64ull);
+        const uint64_t base = (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)b0) |
+                              ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(b0 >> 32)) << 32);
+        if (base >= p.n_paths) break;
+        const uint64_t i = base + lane;
+        const bool in = i < p.n_paths;
+        const bool cam = in && init_path(p, b, i) > 0.0f;
+        trace_lane<kBlock, kStack, false, SPH>(ext, i, cam, ring, spill, cnt);
+        for (uint32_t bounce = 0; bounce <= p.max_depth; ++bounce) {
+            const uint32_t rays = in ? shade_path<false, MULTI, SPL>(p, sc, b, bounce, i) : 0u;
+            const bool nee = sc.n_lights > 0 && bounce < p.max_depth;
+            if (nee) {
+                trace_lane<kBlock, kStack, true, SPH>(sha, i, (rays & 2u) != 0, ring, spill, cnt);
+                trace_lane<kBlock, kStack, false, SPH>(mis, i, (rays & 4u) != 0, ring, spill, cnt);
+            }
+            if (bounce < p.max_depth) trace_lane<kBlock, kStack, false, SPH>(ext, i, (rays & 1u) != 0, ring, spill, cnt);
+            if (nee && in) nee_resolve_path<SPL>(p, sc, b, bounce, i);
+        }
+    }
 }
 
 // Network input at each live path's hit: position, outgoing direction and the
@@ -693,6 +748,39 @@ hipError_t launch_nee_resolve(const RenderParamsDev& p, const ShadeSceneDev& sc,
     if (sc.sphere_lights) hipLaunchKernelGGL(k_nee_resolve<true>, grid, dim3(256), 0, s, p, sc, b, bounce);
     else hipLaunchKernelGGL(k_nee_resolve<false>, grid, dim3(256), 0, s, p, sc, b, bounce);
     return hipGetLastError();
+}
+hipError_t launch_path_persistent(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b,
+                                  const TraverseArgs& ext, const TraverseArgs& sha, const TraverseArgs& mis,
+                                  unsigned long long* queue, int grid, hipStream_t s) {
+    if (p.n_paths == 0) return hipSuccess;
+    hipError_t e = launch_zero_u64(queue, s);
+    if (e != hipSuccess) return e;
+    const bool sph = ext.spheres != nullptr;
+#define BG_PERSIST(M, L, S) \
+    hipLaunchKernelGGL((k_path_persistent<M, L, S>), dim3(grid), dim3(256), 0, s, p, sc, b, ext, sha, mis, queue)
+    if (sc.multi_lobe) {
+        if (sc.sphere_lights) { if (sph) BG_PERSIST(true, true, true); else BG_PERSIST(true, true, false); }
+        else { if (sph) BG_PERSIST(true, false, true); else BG_PERSIST(true, false, false); }
+    } else {
+        if (sc.sphere_lights) { if (sph) BG_PERSIST(false, true, true); else BG_PERSIST(false, true, false); }
+        else { if (sph) BG_PERSIST(false, false, true); else BG_PERSIST(false, false, false); }
+    }
+#undef BG_PERSIST
+    return hipGetLastError();
+}
+int path_persistent_blocks_per_cu(const ShadeSceneDev& sc, bool spheres) {
+    int n = 0;
+    hipError_t e;
+#define BG_OCC(M, L, S) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_path_persistent<M, L, S>, 256, 0)
+    if (sc.multi_lobe) {
+        if (sc.sphere_lights) { if (spheres) BG_OCC(true, true, true); else BG_OCC(true, true, false); }
+        else { if (spheres) BG_OCC(true, false, true); else BG_OCC(true, false, false); }
+    } else {
+        if (sc.sphere_lights) { if (spheres) BG_OCC(false, true, true); else BG_OCC(false, true, false); }
+        else { if (spheres) BG_OCC(false, false, true); else BG_OCC(false, false, false); }
+    }
+#undef BG_OCC
+    return e == hipSuccess && n > 0 ? n : 1;
 }
 hipError_t launch_count_rays(const RenderParamsDev& p, const PathBuffers& b, unsigned long long* counters,
                              hipStream_t s) {

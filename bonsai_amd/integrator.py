@@ -15,6 +15,8 @@ from .scenes import Camera
 
 MODE_REF = N.BG_MODE_REF
 MODE_CORRECTED = N.BG_MODE_CORRECTED
+SCHEDULE_WAVEFRONT = N.BG_SCHEDULE_WAVEFRONT
+SCHEDULE_PERSISTENT = N.BG_SCHEDULE_PERSISTENT
 
 _XYZ = np.array([[0.412453, 0.357580, 0.180423], [0.212671, 0.715160, 0.072169], [0.019334, 0.119193, 0.950227]])
 _RGB = np.array([[3.240479, -1.537150, -0.498535], [-0.969256, 1.875991, 0.041556], [0.055648, -0.204043, 1.057311]])
@@ -260,9 +262,12 @@ class BatchRenderer:
             pass
 
     def render(self, spp: int, max_depth: int, mode: int = MODE_CORRECTED, rr_threshold: float = 1.0,
-               use_graph: bool = True, autotune: bool = True, paths_per_pass: int = 0) -> N.bg_stats:
+               use_graph: bool = True, autotune: bool = True, paths_per_pass: int = 0,
+               schedule: int = 0) -> N.bg_stats:
+        """schedule: SCHEDULE_WAVEFRONT (a kernel per stage of each pass) or SCHEDULE_PERSISTENT (each
+        pass as one persistent path kernel over 64-path groups taken from an atomic counter)."""
         d = N.bg_batch_desc(spp, max_depth, rr_threshold, mode, paths_per_pass, 1 if use_graph else 0,
-                            1 if autotune else 0)
+                            1 if autotune else 0, schedule)
         st = N.bg_stats()
         N.check(N.lib().bg_render_batch(self.ctx.h, self.h, C.byref(d), C.byref(st)), self.ctx.h)
         return st

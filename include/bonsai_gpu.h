@@ -35,7 +35,7 @@ extern "C" {
 #define BG_ERR_NOMEM (-3)
 #define BG_ERR_UNSUPPORTED (-4)
 
-#define BG_ABI_VERSION 3
+#define BG_ABI_VERSION 4
 
 /* Query flags for bg_intersect / bg_intersect_device. */
 #define BG_CLOSEST 0u
@@ -415,7 +415,14 @@ typedef struct bg_batch_desc {
     uint32_t paths_per_pass; /* 0: about 256M (whole samples per pixel dividing spp) */
     uint32_t use_graph;      /* nonzero: capture one pass in a hipGraph and replay it */
     uint32_t autotune;       /* nonzero: time the traversal launch shapes on the first call */
+    uint32_t schedule;       /* BG_SCHEDULE_WAVEFRONT: a kernel per stage of a pass (shade, each
+                                traversal, resolve); BG_SCHEDULE_PERSISTENT: each pass as one
+                                persistent path kernel whose wavefronts take 64-path groups (one
+                                item's pixels at one sample) from an atomic counter and run their
+                                whole paths. Both give the same films. */
 } bg_batch_desc;
+#define BG_SCHEDULE_WAVEFRONT 0u
+#define BG_SCHEDULE_PERSISTENT 1u
 /* Clears the batch film and renders spp samples of every item into it. */
 int bg_render_batch(bg_ctx* ctx, bg_batch* batch, const bg_batch_desc* desc, bg_stats* stats);
 /* Film sums of one item: width*height*4 floats (r, g, b, weight). */

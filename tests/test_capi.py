@@ -35,7 +35,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_errors_without_device():
     L = N.lib()
-    assert L.bg_abi_version() == 3
+    assert L.bg_abi_version() == 4
     cnt = C.c_int(-1)
     assert L.bg_device_count(C.byref(cnt)) == 0 and cnt.value >= 0
     assert L.bg_device_count(None) == N.BG_ERR_INVALID

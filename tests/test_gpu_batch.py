@@ -5,7 +5,8 @@ import numpy as np
 import pytest
 
 from bonsai_amd import GpuBvh
-from bonsai_amd.integrator import MODE_CORRECTED, MODE_REF, BatchRenderer, Film, GpuPathIntegrator
+from bonsai_amd.integrator import (MODE_CORRECTED, MODE_REF, SCHEDULE_PERSISTENT, SCHEDULE_WAVEFRONT, BatchRenderer,
+                                   Film, GpuPathIntegrator)
 from bonsai_amd.scenes import Camera, batch_scene, cornell_box, path_tracer_scene
 
 pytestmark = pytest.mark.gpu
@@ -52,6 +53,34 @@ def test_batch_equals_per_scene(ctx, items, mode, graph):
         np.testing.assert_array_equal(rgb, ref[k][0])
     block, stack, bpc, variant = br.launch()
     assert block in (128, 256) and stack in (8, 16) and bpc in (8, 16, 32, 64) and variant in (0, 2, 4, 5)
+    br.close()
+
+
+@pytest.mark.parametrize("mode", [MODE_REF, MODE_CORRECTED])
+@pytest.mark.parametrize("graph", [True, False])
+def test_persistent_schedule_equals_per_scene(ctx, items, mode, graph):
+    # one persistent path kernel per pass (64-path groups from an atomic counter): the same films
+    # and ray counts as the wavefront schedule and as each item rendered alone
+    spp, depth = 4, 6
+    ref = per_item(ctx, items, spp, depth, mode)
+    total = sum(c.width * c.height for _, c, _ in items)
+    br = BatchRenderer(ctx, items)
+    wf = br.render(spp, depth, mode=mode, use_graph=graph, autotune=False, paths_per_pass=2 * total,
+                   schedule=SCHEDULE_WAVEFRONT)
+    st = br.render(spp, depth, mode=mode, use_graph=graph, autotune=False, paths_per_pass=2 * total,
+                   schedule=SCHEDULE_PERSISTENT)
+    assert st.samples == total * spp and st.rays == wf.rays and st.shadow_rays == wf.shadow_rays
+    for k in range(len(items)):
+        rgb, w = br.sums(k)
+        np.testing.assert_array_equal(w, ref[k][1])
+        np.testing.assert_array_equal(rgb, ref[k][0])
+    br.close()
+
+
+def test_unknown_schedule_is_rejected(ctx, items):
+    br = BatchRenderer(ctx, items[:2])
+    with pytest.raises(RuntimeError, match="schedule"):
+        br.render(1, 2, schedule=2)
     br.close()
 
 

@@ -21,7 +21,7 @@ import pytest
 import cpu_oracle as O
 from bonsai_amd import GpuBvh
 from bonsai_amd.guide import GuideMLP
-from bonsai_amd.integrator import (MODE_CORRECTED, MODE_REF, BatchRenderer, Film, GpuPathIntegrator,
+from bonsai_amd.integrator import (MODE_CORRECTED, MODE_REF, SCHEDULE_PERSISTENT, BatchRenderer, Film, GpuPathIntegrator,
                                    GuidedPathIntegrator, rel_mse, resolve)
 from bonsai_amd.scenes import batch_scene, cornell_box, door_gap_room, path_tracer_scene
 
@@ -103,6 +103,7 @@ def test_full_scene_batch_equals_per_scene_renders(ctx, batch64, mode):
           f"{sum(s.camera.width * s.camera.height for s in scenes)} pixels", flush=True)
     br = BatchRenderer(ctx, items)
     br.render(16, 6, mode=mode, autotune=True)
+    ref_sums = [br.sums(k) for k in range(len(items))]
     for k, (bvh, cam, seed) in enumerate(items):
         rgb, w = gpu_sums(ctx, GpuPathIntegrator(ctx, cam, spp=16, max_depth=6, mode=mode, seed=seed), bvh, cam)
         brgb, bw = br.sums(k)
@@ -117,6 +118,13 @@ def test_full_scene_batch_equals_per_scene_renders(ctx, batch64, mode):
         err = rel_mse(resolve(brgb.astype(np.float64), bw.astype(np.float64)), resolve(orgb, ow))
         print(f"batch item {k} ({tris[k]} triangles, {s.camera.width}^2): relMSE vs oracle {err:.2e}")
         assert err < 1e-4
+    # the persistent schedule renders the same films
+    br.render(16, 6, mode=mode, autotune=False, schedule=SCHEDULE_PERSISTENT)
+    for k in range(len(items)):
+        prgb, pw = br.sums(k)
+        brgb, bw = ref_sums[k]
+        np.testing.assert_array_equal(pw, bw)
+        np.testing.assert_array_equal(prgb, brgb)
     br.close()
 
 

@@ -91,9 +91,9 @@ def test_batch_with_sphere_light_equals_per_scene(ctx):
               cornell_box(double_sided_light=False, width=40, height=48, sphere_light=True)]
     bvhs = [GpuBvh(ctx, s) for s in scenes]
     items = [(b, s.camera, 50 + k) for k, (b, s) in enumerate(zip(bvhs, scenes))]
-    for mode in (MODE_REF, MODE_CORRECTED):
+    for mode, schedule in ((MODE_REF, 0), (MODE_CORRECTED, 0), (MODE_REF, 1), (MODE_CORRECTED, 1)):
         br = BatchRenderer(ctx, items)
-        br.render(8, 5, mode=mode, autotune=False)
+        br.render(8, 5, mode=mode, autotune=False, schedule=schedule)
         for k, (b, cam, seed) in enumerate(items):
             film = Film(ctx, cam.width, cam.height)
             GpuPathIntegrator(ctx, cam, spp=8, max_depth=5, mode=mode, seed=seed).render_into(b, film)
