@@ -49,13 +49,27 @@ __device__ __forceinline__ void spawn(const Surf& s, V d, bool ref, V* o, float*
 // for a primitive the ray does not hit, which a traversal never reports.
 __device__ __forceinline__ bool hit_surface(const ShadeSceneDev& sc, float4 r0, float4 r1, uint32_t prim, Surf* s,
                                             float* t) {
-    RayState rs;
-    rs.ox = r0.x; rs.oy = r0.y; rs.oz = r0.z; rs.t_max = r0.w;
-    rs.dx = r1.x; rs.dy = r1.y; rs.dz = r1.z;
-    ray_setup(rs);
+    // (each branch has its own ray state and the sphere branch its own surface record: what the
+    // non-inlined sphere call takes by address then lives in scratch only on the sphere path,
+    // instead of sending every triangle hit's surface through scratch)
     bool ok;
-    if (prim < sc.n_tris) ok = tri_interaction(rs, sc.tri_orig + 9 * (uint64_t)prim, sc.tri_flip[prim] != 0, s, t);
-    else ok = sphere_interaction(rs, sc.spheres[prim - sc.n_tris], s, t);
+    if (prim < sc.n_tris) {
+        RayState rs;
+        rs.ox = r0.x; rs.oy = r0.y; rs.oz = r0.z; rs.t_max = r0.w;
+        rs.dx = r1.x; rs.dy = r1.y; rs.dz = r1.z;
+        ray_setup(rs);
+        ok = tri_interaction(rs, sc.tri_orig + 9 * (uint64_t)prim, sc.tri_flip[prim] != 0, s, t);
+    } else {
+        RayState rs;
+        rs.ox = r0.x; rs.oy = r0.y; rs.oz = r0.z; rs.t_max = r0.w;
+        rs.dx = r1.x; rs.dy = r1.y; rs.dz = r1.z;
+        ray_setup(rs);
+        Surf h;
+        float th;
+        ok = sphere_interaction(rs, sc.spheres[prim - sc.n_tris], &h, &th);
+        *s = h;
+        *t = th;
+    }
     s->time = r1.w;
     return ok;
 }
@@ -274,7 +288,7 @@ __device__ __forceinline__ float write_shadow_ray(float4* rays, uint64_t i, cons
 // Shape::pdf (shape.rs:40-57) on a light's triangle, or sphere.rs:356-373 on a sphere (the cone
 // pdf from outside, whatever wi): solid-angle pdf of direction wi.
 template <bool SPL>
-__device__ __noinline__ float light_point_pdf(const ShadeSceneDev& sc, const LightDev& l, const Surf& s, V wi, bool ref) {
+__device__ __forceinline__ float light_point_pdf(const ShadeSceneDev& sc, const LightDev& l, const Surf& s, V wi, bool ref) {
     if (SPL && l.prim >= sc.n_tris) {
         const SphereShadeDev& sp = sc.spheres[l.prim - sc.n_tris];
         const V c = sphere_center(sp);
