@@ -294,6 +294,7 @@ int bg_set_launch_config(bg_ctx* ctx, const bg_launch_config* cfg) {
     s.compact = cfg->compact != 0;
     if (!bg::traverse_shape_supported(s)) return fail(ctx, BG_ERR_INVALID, "unsupported launch config");
     ctx->shape = s;
+    ctx->render_shape = s;
     return BG_OK;
 }
 

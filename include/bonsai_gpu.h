@@ -236,7 +236,13 @@ typedef struct bg_surface {
 int bg_surface_interactions(bg_ctx* ctx, const bg_scene* scene, const bg_ray* rays, const bg_hit* hits, size_t n,
                             bg_surface* out);
 
-/* Traversal launch shape (chosen by the autotuner or the caller). */
+/* Traversal launch shape (chosen by the autotuner or the caller). A context
+ * starts with two: bg_intersect* queries and the direct-lighting and Whitted
+ * renders use 256 threads, an 8-entry stack, 32 blocks per CU and variant 4;
+ * the path and guided renders use 128 threads, an 8-entry stack, 16 blocks
+ * per CU and variant 5. bg_set_launch_config sets the one shape both then use;
+ * bg_get_launch_config reads the first. Scene batches tune their own
+ * (bg_batch_launch_config). */
 typedef struct bg_launch_config {
     uint32_t block;       /* threads per block: 64, 128, 256 */
     uint32_t stack;       /* LDS stack entries per lane: 8, 16, 32 */
