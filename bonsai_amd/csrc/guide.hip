@@ -5,15 +5,18 @@
 // (mixture logit, direction 3-vector, concentration parameter s with log kappa
 // = 1.5 + 5.5 tanh((s - 1.5) / 5.5)). Weights are kept in
 // FP32 (master copy, Adam moments) and in BF16 for the matrix cores; every
-// product is v_mfma_f32_32x32x8_bf16 with FP32 accumulation.
+// product is gfx950's v_mfma_f32_32x32x16_bf16 with FP32 accumulation.
 //
 // One wavefront evaluates 32 samples at a time as Y^T = W X^T: the weights
-// are the A operand (32 output rows x 8 inputs per instruction) and the
-// activations the B operand (8 inputs x 32 samples). The accumulator layout
-// (lane l, register r -> row 8*(r/4) + 4*(l/32) + r%4, column l%32) is
-// exactly the B-operand layout of the next layer's k-steps, so activations
-// stay in registers from layer to layer; only the bias, ReLU and the BF16
-// rounding touch them in between. The backward pass reuses the same scheme
+// are the A operand (32 output rows x 16 inputs per instruction) and the
+// activations the B operand (16 inputs x 32 samples). Lane l supplies 8 of an
+// instruction's 16 inputs, half h = l/32 of them, for both operands; which
+// feature each of those 8 slots holds is free as long as A and B agree. The
+// accumulator layout (lane l, register r -> row 8*(r/4) + 4*(l/32) + r%4,
+// column l%32) puts features 8q + 4h .. + 3 in registers 4q .. 4q + 3, so
+// k-step t takes the quads q = 2t and 2t + 1 of every lane: the activations
+// stay in registers from layer to layer, as B operands, and only the bias,
+// ReLU and the BF16 rounding touch them in between. The backward pass reuses the same scheme
 // with the transposed weights. Weight gradients (dW = dY X^T summed over the
 // batch) are accumulated per block in the training kernel and reduced, in a
 // fixed order, by the Adam kernel.
@@ -28,6 +31,8 @@ namespace bg {
 namespace {
 
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef __bf16 bf4v __attribute__((ext_vector_type(4)));
@@ -59,23 +64,21 @@ __device__ __forceinline__ s4 pack4(float a, float b, float c, float d) {
 // half-turns, so 2^k q is reduced exactly.
 // Weights in LDS: BF16 [layer][64 rows][kLd], rows padded from 64 to 68
 // elements (136 B = 34 banks) so the 32 rows one A-operand read touches spread
-// over the banks instead of all starting on bank 0. A operand of k-step s for
-// output block mb: row 32*mb + lane%32, columns 8s + 4*(lane/32) .. +3.
+// over the banks instead of all starting on bank 0. Quad q of output block mb
+// for this lane: row 32*mb + lane%32, columns 8q + 4*(lane/32) .. +3.
 constexpr int kLd = 68;
 constexpr int kLayerLds = 64 * kLd;
-__device__ __forceinline__ s4 load_a(const uint16_t* __restrict__ w, int mb, int s, int lane) {
-    const uint16_t* p = w + (32 * mb + (lane & 31)) * kLd + 8 * s + 4 * (lane >> 5);
+__device__ __forceinline__ s4 load_a(const uint16_t* __restrict__ w, int mb, int q, int lane) {
+    const uint16_t* p = w + (32 * mb + (lane & 31)) * kLd + 8 * q + 4 * (lane >> 5);
     return *reinterpret_cast<const s4*>(p);
 }
 
-// One layer: acc[mb] = W[32mb.., :] * X^T, with X^T given as B operands xb[s].
-__device__ __forceinline__ void layer_mm(const uint16_t* __restrict__ w, const s4 xb[8], int lane, f16v acc[2]) {
-    for (int mb = 0; mb < 2; ++mb) {
-        f16v c = {0};
-#pragma unroll
-        for (int s = 0; s < 8; ++s) c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(load_a(w, mb, s, lane), xb[s], c, 0, 0, 0);
-        acc[mb] = c;
-    }
+// One 32 x 32 x 16 product step, each operand given as this lane's two quads of inputs.
+__device__ __forceinline__ f16v mfma16(s4 a0, s4 a1, s4 b0, s4 b1, f16v c) {
+    const s8 a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
+    const s8 b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf8v, a), __builtin_bit_cast(bf8v, b), c, 0, 0,
+                                                   0);
 }
 
 // Row held by accumulator register r of lane `lane` in block mb.
@@ -92,25 +95,59 @@ __device__ __forceinline__ int out_row(int p) {
     return 20 * h + idx;
 }
 
-// Bias + optional ReLU, then repack the accumulators as the next layer's B operands.
-// permute_out: the accumulator rows hold the outputs out_row(row) (last layer, inference).
-__device__ __forceinline__ void epilogue(f16v acc[2], const float* __restrict__ bias, bool relu, int lane, s4 xb[8],
-                                         bool permute_out = false) {
+// One layer: acc[mb] = b + W[32mb.., :] * X^T, with X^T given as the B-operand quads xb[q]; with
+// BIAS the accumulators start from the bias b (16-byte aligned), so no add follows the products.
+// Registers 4q .. 4q + 3 hold rows r0 .. r0 + 3 (r0 = acc_row(mb, 4q, lane)), and out_row maps
+// such a quad to four consecutive outputs, so each quad of the bias is one 16-byte read.
+// permute_out: the accumulator rows hold the outputs out_row(row) (last layer).
+template <bool BIAS>
+__device__ __forceinline__ void layer_mm(const uint16_t* __restrict__ w, const s4 xb[8], int lane, f16v acc[2],
+                                         const float* __restrict__ bias = nullptr, bool permute_out = false) {
+#pragma unroll
     for (int mb = 0; mb < 2; ++mb) {
+        f16v c = {0};
+        if (BIAS)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int row = acc_row(mb, r, lane);
-            float v = acc[mb][r] + bias[permute_out ? out_row(row) : row];
-            if (relu) v = v > 0.0f ? v : 0.0f;
-            acc[mb][r] = v;
-        }
+            for (int q = 0; q < 4; ++q) {
+                const int row = acc_row(mb, 4 * q, lane);
+                const float4 b = *reinterpret_cast<const float4*>(bias + (permute_out ? out_row(row) : row));
+                c[4 * q] = b.x;
+                c[4 * q + 1] = b.y;
+                c[4 * q + 2] = b.z;
+                c[4 * q + 3] = b.w;
+            }
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-            xb[4 * mb + q] = pack4(acc[mb][4 * q], acc[mb][4 * q + 1], acc[mb][4 * q + 2], acc[mb][4 * q + 3]);
+        for (int t = 0; t < 4; ++t)
+            c = mfma16(load_a(w, mb, 2 * t, lane), load_a(w, mb, 2 * t + 1, lane), xb[2 * t], xb[2 * t + 1], c);
+        acc[mb] = c;
     }
 }
 
-// B operands of the encoding for the sample of this lane: k-step s holds
+// A hidden layer's activations as the next layer's B operands: the accumulators rounded to
+// BF16, then the ReLU on the rounded values as a signed 16-bit max with 0 (rounding keeps the
+// sign, and a negative BF16 value or -0 is a negative 16-bit integer), two values per
+// instruction. The max is written as an instruction: as a vector max on the rounded values the
+// compiler splits each two-value conversion into two one-value conversions and a permute.
+__device__ __forceinline__ uint32_t pk_max0(uint32_t x) {
+    uint32_t r;
+    asm("v_pk_max_i16 %0, %1, 0" : "=v"(r) : "v"(x));
+    return r;
+}
+__device__ __forceinline__ void relu_pack(const f16v acc[2], s4 xb[8]) {
+    typedef uint32_t u2v __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            u2v u = __builtin_bit_cast(
+                u2v, pack4(acc[mb][4 * q], acc[mb][4 * q + 1], acc[mb][4 * q + 2], acc[mb][4 * q + 3]));
+            u.x = pk_max0(u.x);
+            u.y = pk_max0(u.y);
+            xb[4 * mb + q] = __builtin_bit_cast(s4, u);
+        }
+}
+
+// B operands of the encoding for the sample of this lane: quad s holds
 // features 8s + 4h .. 8s + 4h + 3, h = lane / 32. On each axis a lane needs
 // the frequencies b, b + 1, b + 4, b + 5 (b = 2h): one sincospif at 2^b and
 // double-angle steps (sin 2x = 2 sin x cos x, cos 2x = 1 - 2 sin^2 x) for the
@@ -163,16 +200,17 @@ __device__ __forceinline__ void normalise_pos(const Sample& s, const GuideBox& b
 // its column p takes output column out_row(p). out_row keeps groups of 4
 // together, so whole quads move. Every load of a thread is issued before the
 // first LDS write, so the block waits for one memory latency, not twenty.
-constexpr int kBlock = 256;  // threads of the MLP kernels
-constexpr int kQuadsPerThread = kLayers * 64 * 16 / kBlock;
-static_assert(kLayers * 64 * 16 % kBlock == 0, "weight quads split evenly over the block");
+constexpr int kBlock = 256;  // threads of the training kernel
+template <int BLOCK = kBlock>
 __device__ void load_weights_lds(const uint16_t* __restrict__ wg, uint16_t* __restrict__ lds, bool permute_out = false,
                                  bool permute_cols = false) {
+    constexpr int kQuadsPerThread = kLayers * 64 * 16 / BLOCK;
+    static_assert(kLayers * 64 * 16 % BLOCK == 0, "weight quads split evenly over the block");
     const uint2* src = reinterpret_cast<const uint2*>(wg);
     uint2 v[kQuadsPerThread];
 #pragma unroll
     for (int k = 0; k < kQuadsPerThread; ++k) {
-        const int i = threadIdx.x + k * kBlock;
+        const int i = threadIdx.x + k * BLOCK;
         const int row = i / 16, q = i % 16;  // row over all layers, 4-element quad in the row
         const bool last = row >= (kLayers - 1) * 64;
         const int from = permute_out && last ? (kLayers - 1) * 64 + out_row(row % 64) : row;
@@ -181,7 +219,7 @@ __device__ void load_weights_lds(const uint16_t* __restrict__ wg, uint16_t* __re
     }
 #pragma unroll
     for (int k = 0; k < kQuadsPerThread; ++k) {
-        const int i = threadIdx.x + k * kBlock;
+        const int i = threadIdx.x + k * BLOCK;
         *reinterpret_cast<uint2*>(lds + (i / 16) * kLd + 4 * (i % 16)) = v[k];
     }
     __syncthreads();
@@ -191,21 +229,24 @@ __device__ void load_weights_lds(const uint16_t* __restrict__ wg, uint16_t* __re
 // in: 3 float4 per sample (p, wo, n); out: 40 floats per sample.
 // MIX: write the 10-float4 mixture record of each sample (dev_guide.h) instead
 // of its 40 raw outputs.
-// Held to 3 waves per SIMD (156 VGPRs, the accumulators in VGPRs): left alone
-// the compiler settles at 2.
+// Blocks of kInferBlock threads share one LDS copy of the weights (44 KB), so
+// LDS leaves room for kInferBlocksPerCu blocks per CU; registers are held to
+// that many waves per SIMD.
+constexpr int kInferWaves = kInferBlock / 64;
+static_assert(kInferBlocksPerCu * kInferWaves % 4 == 0, "whole waves per SIMD");
 template <bool MIX>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_guide_infer(GuideDevice g, GuideBox box, const float4* __restrict__ in,
+__global__ __launch_bounds__(kInferBlock) __attribute__((amdgpu_waves_per_eu(kInferBlocksPerCu * kInferWaves / 4, 8))) void k_guide_infer(GuideDevice g, GuideBox box, const float4* __restrict__ in,
                                                      uint64_t n, const uint32_t* __restrict__ n_dev,
                                                      float* __restrict__ out) {
     __shared__ uint16_t wl[kLayers * kLayerLds];
-    __shared__ float bl[kLayers * 64];
+    __shared__ __align__(16) float bl[kLayers * 64];
     if (n_dev) n = *n_dev;
-    if ((uint64_t)blockIdx.x * 128 >= n) return;  // the whole block is idle (uniform)
-    for (int k = threadIdx.x; k < kLayers * 64; k += kBlock) bl[k] = g.bias[k];
-    load_weights_lds(g.wbf, wl, true);
+    if ((uint64_t)blockIdx.x * 32 * kInferWaves >= n) return;  // the whole block is idle (uniform)
+    for (int k = threadIdx.x; k < kLayers * 64; k += kInferBlock) bl[k] = g.bias[k];
+    load_weights_lds<kInferBlock>(g.wbf, wl, true);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, h = lane >> 5;
     const uint64_t groups = (n + 31) / 32;
-    for (uint64_t grp = (uint64_t)blockIdx.x * 4 + wave; grp < groups; grp += (uint64_t)gridDim.x * 4) {
+    for (uint64_t grp = (uint64_t)blockIdx.x * kInferWaves + wave; grp < groups; grp += (uint64_t)gridDim.x * kInferWaves) {
         const uint64_t i = grp * 32 + (lane & 31);
         const bool valid = i < n;
         Sample smp = valid ? load_sample(in, i) : Sample{{0, 0, 0}, {0, 0, 1}, {0, 0, 1}};
@@ -214,9 +255,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3, 8))) 
         s4 xb[8];
         encode_b(q, smp.wo, smp.n, lane, xb);
         f16v acc[2];
+        // an offset the compiler cannot see through, so the weight reads stay in this loop
+        // instead of being hoisted out of it into registers that would then spill
+        int wo = 0;
+        asm volatile("" : "+s"(wo));
         for (int l = 0; l < kLayers; ++l) {
-            layer_mm(wl + l * kLayerLds, xb, lane, acc);
-            epilogue(acc, bl + l * 64, l < kLayers - 1, lane, xb, l == kLayers - 1);
+            layer_mm<true>(wl + wo + l * kLayerLds, xb, lane, acc, bl + l * 64, l == kLayers - 1);
+            if (l < kLayers - 1) relu_pack(acc, xb);
         }
         // this lane holds outputs 20h .. 20h + 19 of sample l % 32 (lobes 4h .. 4h + 3): accumulators
         // 0-15 of block 0, then 0-3 of block 1 (out_row)
@@ -312,7 +357,7 @@ __device__ __forceinline__ void out_grad_half(const float* o, const float* w, fl
 // For the weight gradient dW_l = sum over samples of dY_l X_l^T the block's 128
 // samples of X_l and dY_l are staged in LDS feature-major ([64 features][128
 // samples], BF16), the layout in which both MFMA operands read samples along k;
-// wave w accumulates block (w / 2, w % 2) of dW_l (32 x 32, 16 k-steps) in
+// wave w accumulates block (w / 2, w % 2) of dW_l (32 x 32, 8 k-steps of 16) in
 // registers across rounds, and thread t the bias gradient of row t / 4 over
 // samples 32 (t % 4) .. + 31. LDS holds the weights, their transposes (the
 // backward products' A operands) and the biases, 88 KB, and the staging, 33 KB.
@@ -339,7 +384,7 @@ __global__ __launch_bounds__(kBlock, 1) void k_guide_train(GuideDevice g, GuideB
     __shared__ uint16_t wl[kLayers * kLayerLds];
     __shared__ uint16_t wt[kLayers * kLayerLds];
     __shared__ uint16_t stg[2 * kStageTile];  // X tile, dY tile
-    __shared__ float bl[kLayers * 64];
+    __shared__ __align__(16) float bl[kLayers * 64];
     uint16_t* xt = stg;
     uint16_t* dt = stg + kStageTile;
     for (int k = threadIdx.x; k < kLayers * 64; k += kBlock) bl[k] = g.bias[k];
@@ -378,8 +423,8 @@ __global__ __launch_bounds__(kBlock, 1) void k_guide_train(GuideDevice g, GuideB
         s4 nx[8];
 #pragma unroll
         for (int l = 0; l < kLayers; ++l) {
-            layer_mm(wl + l * kLayerLds, xs[l], lane, acc);
-            epilogue(acc, bl + l * 64, l < kLayers - 1, lane, nx, l == kLayers - 1);
+            layer_mm<true>(wl + l * kLayerLds, xs[l], lane, acc, bl + l * 64, l == kLayers - 1);
+            if (l + 1 < kLayers) relu_pack(acc, nx);
             if (l + 1 < kLayers)
 #pragma unroll
                 for (int s = 0; s < 8; ++s) xs[l + 1][s] = nx[s];
@@ -407,10 +452,10 @@ __global__ __launch_bounds__(kBlock, 1) void k_guide_train(GuideDevice g, GuideB
             __syncthreads();
             // dW_l over the block's 128 samples of this round
 #pragma unroll
-            for (int t = 0; t < 16; ++t) {
-                const s4 a = *reinterpret_cast<const s4*>(dt + (32 * mbw + (lane & 31)) * kStage + 8 * t + 4 * h);
-                const s4 b = *reinterpret_cast<const s4*>(xt + (32 * nbw + (lane & 31)) * kStage + 8 * t + 4 * h);
-                dw[l] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, dw[l], 0, 0, 0);
+            for (int t = 0; t < 8; ++t) {
+                const s4* a = reinterpret_cast<const s4*>(dt + (32 * mbw + (lane & 31)) * kStage + 16 * t + 8 * h);
+                const s4* b = reinterpret_cast<const s4*>(xt + (32 * nbw + (lane & 31)) * kStage + 16 * t + 8 * h);
+                dw[l] = mfma16(a[0], a[1], b[0], b[1], dw[l]);
             }
             {  // bias: row brow, samples 32 bq .. 32 bq + 31, in order
                 const uint2* rp = reinterpret_cast<const uint2*>(dt + brow * kStage + 32 * bq);
@@ -428,7 +473,7 @@ __global__ __launch_bounds__(kBlock, 1) void k_guide_train(GuideDevice g, GuideB
             if (l == 0) break;
             // d of layer-l input = W_l^T dY_l, masked by the ReLU that made X_l: X_l is +0 or
             // positive in BF16, so the mask is X_l != 0, applied to the rounded values
-            layer_mm(wt + l * kLayerLds, db, lane, acc);
+            layer_mm<false>(wt + l * kLayerLds, db, lane, acc);
 #pragma unroll
             for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
@@ -543,10 +588,10 @@ hipError_t guide_refresh(const GuideDevice& g, hipStream_t s) {
 hipError_t guide_infer(const GuideDevice& g, const GuideBox& box, const float4* in, uint64_t n, float* out,
                        int grid, hipStream_t s, const uint32_t* n_dev, bool mixture) {
     if (n == 0) return hipSuccess;
-    const uint64_t need = (n + 127) / 128;
+    const uint64_t need = (n + 32 * kInferWaves - 1) / (32 * kInferWaves);
     const int gr = (int)(need < (uint64_t)grid ? need : (uint64_t)grid);
-    if (mixture) hipLaunchKernelGGL(k_guide_infer<true>, dim3(gr), dim3(kBlock), 0, s, g, box, in, n, n_dev, out);
-    else hipLaunchKernelGGL(k_guide_infer<false>, dim3(gr), dim3(kBlock), 0, s, g, box, in, n, n_dev, out);
+    if (mixture) hipLaunchKernelGGL(k_guide_infer<true>, dim3(gr), dim3(kInferBlock), 0, s, g, box, in, n, n_dev, out);
+    else hipLaunchKernelGGL(k_guide_infer<false>, dim3(gr), dim3(kInferBlock), 0, s, g, box, in, n, n_dev, out);
     return hipGetLastError();
 }
 
