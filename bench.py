@@ -429,8 +429,8 @@ def batch_leg(ctx, spp: int) -> dict:
     # (measured renders run their passes without the graph)
     out["batched"]["roofline"] = traversal_roofline(ctx, lambda: br.render(spp, 6, mode=MODE_CORRECTED,
                                                                             autotune=False))
-    # the same batch with each pass as one persistent path kernel (64-path groups from an atomic
-    # counter; the same films): scheduling measured against the wavefront passes
+    # the same batch with each pass as one persistent path kernel (path slots handed to free lanes
+    # from an atomic counter; the same films): scheduling measured against the wavefront passes
     br.render(1, 6, mode=MODE_CORRECTED, autotune=False, schedule=SCHEDULE_PERSISTENT)
     torch_sync()
     t0 = time.perf_counter()

@@ -403,19 +403,23 @@ __global__ __launch_bounds__(256) void k_nee_resolve(RenderParamsDev p, ShadeSce
     if (i < p.n_paths) nee_resolve_path<SPL>(p, sc, b, bounce, i);
 }
 
-// The persistent path kernel of a plain pass: each wavefront takes groups of 64 consecutive path
-// slots from an atomic counter (zeroed before the launch) and runs their whole paths: the camera
-// ray, then per bounce the shading, the shadow and MIS-branch queries, the extension ray and the
-// next-event resolve. These are the per-path functions the wavefront pass launches one kernel at a
-// time (init_path, shade_path, nee_resolve_path) in the same order on the same path-state arrays,
-// and each ray is traced by the same pair traversal, so every path's radiance, film point and ray
-// count equal the wavefront render's. The three ray sets share the block's LDS stack ring and one
-// spill row per thread, being traced one after another. Path slots run item-major, then sample,
-// then pixel, so a group is a run of one scene item's pixels at one sample.
+// The persistent path kernel of a plain pass. Each lane of a wavefront holds one path at a time,
+// at its own bounce; a lane whose path has ended takes the next slot of the run of 64 consecutive
+// slots its wavefront last claimed from an atomic counter (zeroed before the launch), claiming the
+// next run once that one is handed out, so lanes keep working while the wavefront's other paths
+// are short or long. One round of the loop traces the lanes' extension (or camera) rays, shades
+// their vertices, traces the shadow and MIS-branch rays the vertices wrote, and resolves the
+// next-event estimates. These are the per-path functions the wavefront pass launches one kernel at
+// a time (init_path, shade_path, nee_resolve_path), called for each path in the same order on the
+// same path-state arrays, and each ray is traced by the same pair traversal, so every path's
+// radiance, film point and ray count equal the wavefront render's (a path that has ended is not
+// shaded again: those shadings change none of them). The three ray sets share the block's LDS stack
+// ring and one spill row per thread, being traced one after another. Path slots run item-major,
+// then sample, then pixel, so a run is one scene item's pixels at one sample.
 // The kernel holds the shading's registers and the traversal's together: 201-247 VGPRs (2 waves
-// per SIMD) as the compiler picks them; held to 4 waves (128 VGPRs, ~200 spilled) it ran fastest on
-// the 64-scene batch: 2 / 3 / 4 / 5 / 6 waves 0.72 / 0.57 / 0.52 / 0.53 / 0.56 s, against 0.33 s
-// for the wavefront passes, whose traversal launches run at 8 waves per SIMD.
+// per SIMD) as the compiler picks them; it is held to 4 waves (128 VGPRs, the rest spilled), which
+// measured fastest on the 64-scene batch when each wavefront ran 64 whole paths in lockstep
+// (2 / 3 / 4 / 5 / 6 waves 0.72 / 0.57 / 0.52 / 0.53 / 0.56 s).
 template <bool MULTI, bool SPL, bool SPH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_path_persistent(RenderParamsDev p, ShadeSceneDev sc, PathBuffers b,
                                                          TraverseArgs ext, TraverseArgs sha, TraverseArgs mis,
@@ -427,26 +431,49 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     uint2* __restrict__ spill = reinterpret_cast<uint2*>(ext.spill) + gtid * ext.spill_depth;
     const uint32_t lane = threadIdx.x & 63;
     PairCounts cnt;
+    uint64_t i = 0;          // this lane's path slot, while `have`
+    bool have = false;
+    bool ext_live = false;   // its extension (or camera) ray is to be traced
+    uint32_t bounce = 0;
+    uint64_t run = 0, run_end = 0;  // slots claimed and not yet handed out (uniform)
+    bool drained = false;           // the counter has passed the last slot (uniform)
     for (;;) {
-        unsigned long long b0 = 0;
-        if (lane == 0) b0 = atomicAdd(queue, 64ull);
-        const uint64_t base = (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)b0) |
-                              ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(b0 >> 32)) << 32);
-        if (base >= p.n_paths) break;
-        const uint64_t i = base + lane;
-        const bool in = i < p.n_paths;
-        const bool cam = in && init_path(p, b, i) > 0.0f;
-        trace_lane<kBlock, kStack, false, SPH>(ext, i, cam, ring, spill, cnt);
-        for (uint32_t bounce = 0; bounce <= p.max_depth; ++bounce) {
-            const uint32_t rays = in ? shade_path<false, MULTI, SPL>(p, sc, b, bounce, i) : 0u;
-            const bool nee = sc.n_lights > 0 && bounce < p.max_depth;
-            if (nee) {
-                trace_lane<kBlock, kStack, true, SPH>(sha, i, (rays & 2u) != 0, ring, spill, cnt);
-                trace_lane<kBlock, kStack, false, SPH>(mis, i, (rays & 4u) != 0, ring, spill, cnt);
+        // hand out slots to the lanes without a path, in lane order
+        for (;;) {
+            const uint64_t idle = __builtin_amdgcn_ballot_w64(!have);
+            if (idle == 0 || drained) break;
+            if (run == run_end) {
+                unsigned long long b0 = 0;
+                if (lane == 0) b0 = atomicAdd(queue, 64ull);
+                run = (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)b0) |
+                      ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(b0 >> 32)) << 32);
+                if (run >= p.n_paths) {
+                    drained = true;
+                    break;
+                }
+                run_end = run + 64 < p.n_paths ? run + 64 : p.n_paths;
             }
-            if (bounce < p.max_depth) trace_lane<kBlock, kStack, false, SPH>(ext, i, (rays & 1u) != 0, ring, spill, cnt);
-            if (nee && in) nee_resolve_path<SPL>(p, sc, b, bounce, i);
+            const uint64_t avail = run_end - run;
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+            if (!have && rank < avail) {
+                i = run + rank;
+                have = true;
+                bounce = 0;
+                ext_live = init_path(p, b, i) > 0.0f;
+            }
+            const uint64_t given = (uint64_t)__builtin_popcountll(idle);
+            run += given < avail ? given : avail;
         }
+        if (__builtin_amdgcn_ballot_w64(have) == 0) break;  // no slots left and every path done
+        trace_lane<kBlock, kStack, false, SPH>(ext, i, have && ext_live, ring, spill, cnt);
+        const uint32_t rays = have ? shade_path<false, MULTI, SPL>(p, sc, b, bounce, i) : 0u;
+        trace_lane<kBlock, kStack, true, SPH>(sha, i, (rays & 2u) != 0, ring, spill, cnt);
+        trace_lane<kBlock, kStack, false, SPH>(mis, i, (rays & 4u) != 0, ring, spill, cnt);
+        if (have && sc.n_lights > 0 && bounce < p.max_depth) nee_resolve_path<SPL>(p, sc, b, bounce, i);
+        // the path goes on with its extension ray (none is written at max_depth), else the lane is free
+        ext_live = have && (rays & 1u) != 0 && bounce < p.max_depth;
+        if (ext_live) ++bounce;
+        else have = false;
     }
 }
 

@@ -265,7 +265,7 @@ class BatchRenderer:
                use_graph: bool = True, autotune: bool = True, paths_per_pass: int = 0,
                schedule: int = 0) -> N.bg_stats:
         """schedule: SCHEDULE_WAVEFRONT (a kernel per stage of each pass) or SCHEDULE_PERSISTENT (each
-        pass as one persistent path kernel over 64-path groups taken from an atomic counter)."""
+        pass as one persistent path kernel whose lanes take path slots from an atomic counter)."""
         d = N.bg_batch_desc(spp, max_depth, rr_threshold, mode, paths_per_pass, 1 if use_graph else 0,
                             1 if autotune else 0, schedule)
         st = N.bg_stats()

@@ -423,9 +423,10 @@ typedef struct bg_batch_desc {
     uint32_t autotune;       /* nonzero: time the traversal launch shapes on the first call */
     uint32_t schedule;       /* BG_SCHEDULE_WAVEFRONT: a kernel per stage of a pass (shade, each
                                 traversal, resolve); BG_SCHEDULE_PERSISTENT: each pass as one
-                                persistent path kernel whose wavefronts take 64-path groups (one
-                                item's pixels at one sample) from an atomic counter and run their
-                                whole paths. Both give the same films. */
+                                persistent path kernel: each lane runs one path at a time, and a
+                                lane whose path has ended takes the next path slot from runs of 64
+                                its wavefront claims from an atomic counter. Both give the same
+                                films. */
 } bg_batch_desc;
 #define BG_SCHEDULE_WAVEFRONT 0u
 #define BG_SCHEDULE_PERSISTENT 1u
