@@ -174,10 +174,11 @@ struct PathBuffers {
     uint4* shit = nullptr;
     float4* mray = nullptr;    // MIS-branch ray, 2 per path
     uint4* mhit = nullptr;
-    float4* nee_a = nullptr;   // light-branch term rgb, flags
-    float4* nee_b = nullptr;   // BSDF-branch f rgb, MIS weight
-    float4* nee_c = nullptr;   // beta at the vertex rgb, light pdf
-    float4* nee_d = nullptr;   // BSDF-branch pdf, light index, -, -
+    // a vertex's next-event estimate, already scaled by the throughput at the vertex and the light
+    // pick (beta / light pdf): the light branch's radiance to add when its shadow ray is unoccluded
+    // (rgb, flags), and the BSDF branch's factor of the emission its ray reaches (rgb, light index)
+    float4* nee_a = nullptr;
+    float4* nee_b = nullptr;
     unsigned long long* counters = nullptr;  // [queries, shadow queries]
     // path guiding (allocated only for guided renders)
     size_t guide_cap = 0, rec_cap = 0;

@@ -268,14 +268,14 @@ __device__ __forceinline__ uint32_t shade_path(const RenderParamsDev& p, const S
                     spawn(s, mdir, ref, &o, &tm);
                     write_ray(b.mray, i, o, tm, mdir, s.time);
                     mis_t = tm;
-                    b.nee_b[i] = make_float4(fb.r, fb.g, fb.b, w);
-                    b.nee_d[i] = make_float4(spdf, __uint_as_float(idx), 0.0f, 0.0f);
+                    const C3 kb = beta * (fb * (w / spdf) / light_pdf);
+                    b.nee_b[i] = make_float4(kb.r, kb.g, kb.b, __uint_as_float(idx));
                     nee |= kHasBsdf;
                     nrays += 1u;
                 }
             }
-            b.nee_a[i] = make_float4(term.r, term.g, term.b, __uint_as_float(nee));
-            b.nee_c[i] = make_float4(beta.r, beta.g, beta.b, light_pdf);
+            const C3 ka = beta * (term / light_pdf);
+            b.nee_a[i] = make_float4(ka.r, ka.g, ka.b, __uint_as_float(nee));
         }
         // path.rs:111-142; guided: one-sample MIS of the network's mixture and the BSDF
         V wi;
@@ -370,27 +370,24 @@ __device__ __forceinline__ void nee_resolve_path(const RenderParamsDev& p, const
     const float4 a = b.nee_a[i];
     const uint32_t nee = __float_as_uint(a.w);
     if (!(nee & kHasNee)) return;
-    C3 out = mkc(0.0f);
-    if ((nee & kHasLight) && b.shit[i].x == 0xFFFFFFFFu) out = out + C3{a.x, a.y, a.z};
+    // integrator.rs:309-434: beta (term_light + f_bsdf Le w / pdf_bsdf) / light_pdf, the scaling
+    // by beta / light_pdf applied when the terms were stored
+    C3 add = mkc(0.0f);
+    if ((nee & kHasLight) && b.shit[i].x == 0xFFFFFFFFu) add = add + C3{a.x, a.y, a.z};
     if (nee & kHasBsdf) {
         const uint32_t hp = b.mhit[i].x;
         if (hp != 0xFFFFFFFFu) {
             const int32_t li = sc.prim_light[hp];
-            const float4 dd = b.nee_d[i];
-            const bool same = p.mode == 0 ? (li >= 0) : (li == (int32_t)__float_as_uint(dd.y));
+            const float4 kb = b.nee_b[i];
+            const bool same = p.mode == 0 ? (li >= 0) : (li == (int32_t)__float_as_uint(kb.w));
             if (same) {
                 const C3 Le = emission_at<SPL>(sc, li, hp, b.mray[2 * i], b.mray[2 * i + 1]);
-                if (!black(Le)) {
-                    const float4 fw = b.nee_b[i];
-                    out = out + C3{fw.x, fw.y, fw.z} * Le * fw.w / dd.x;
-                }
+                if (!black(Le)) add = add + C3{kb.x, kb.y, kb.z} * Le;
             }
         }
     }
-    const float4 c = b.nee_c[i];
-    const C3 ld = out / c.w;
     const float4 l4 = b.L[i];
-    const C3 L = C3{l4.x, l4.y, l4.z} + C3{c.x, c.y, c.z} * ld;
+    const C3 L = C3{l4.x, l4.y, l4.z} + add;
     b.L[i] = make_float4(L.r, L.g, L.b, l4.w);
     uint64_t rec;
     if (rec_slot(p, i, bounce, &rec) && b.rec_tgt[rec].w != 0.0f) b.rec_aux[2 * rec + 1] = make_float4(L.r, L.g, L.b, 0.0f);

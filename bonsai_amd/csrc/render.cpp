@@ -46,7 +46,7 @@ int bg::ensure_paths(bg_ctx* ctx, size_t cap) {
     b.rec_aux = kept.rec_aux;
     b.rec_blk = kept.rec_blk;
     b.last_records = kept.last_records;
-    const size_t per_path[] = {32, 16, 16, 16, 8, 8, 8, 32, 16, 32, 16, 16, 16, 16, 16};
+    const size_t per_path[] = {32, 16, 16, 16, 8, 8, 8, 32, 16, 32, 16, 16, 16};
     size_t total = 256;                                                            // counters
     total += 3 * (((4 * (cap / 256 + 1) + 255) / 256) * 256 + 256);              // rblk[3], rcount[3]
     total += 6 * (((32 * (cap / 256 + 1) + 255) / 256) * 256);                   // rmask[3], wmask[3]
@@ -74,8 +74,6 @@ int bg::ensure_paths(bg_ctx* ctx, size_t cap) {
     b.mhit = (uint4*)take(16 * cap);
     b.nee_a = (float4*)take(16 * cap);
     b.nee_b = (float4*)take(16 * cap);
-    b.nee_c = (float4*)take(16 * cap);
-    b.nee_d = (float4*)take(16 * cap);
     for (int k = 0; k < 3; ++k) {
         b.ridx[k] = (uint32_t*)take(4 * cap);
         b.rblk[k] = (uint32_t*)take(4 * (cap / 256 + 1));
