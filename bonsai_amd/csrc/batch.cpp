@@ -63,9 +63,7 @@ struct bg_batch {
     bg::SphereDev* spheres = nullptr;
     bg::SphereAux* sphere_aux = nullptr;
     float* tri_orig = nullptr;
-    uint8_t* tri_flip = nullptr;
-    uint32_t* prim_mat = nullptr;
-    int32_t* prim_light = nullptr;
+    uint2* prim_info = nullptr;
     bg::MaterialDev* mats = nullptr;
     bg::LightDev* lights = nullptr;
     bg::SphereShadeDev* sph_shade = nullptr;
@@ -85,8 +83,8 @@ struct bg_batch {
 namespace {
 
 void free_batch(bg_batch* b) {
-    void* ptrs[] = {b->pairs, b->leaves, b->tri_verts, b->slot_prim, b->spheres, b->sphere_aux, b->tri_orig, b->tri_flip,
-                    b->prim_mat, b->prim_light, b->mats, b->lights, b->sph_shade, b->d_items, b->film,
+    void* ptrs[] = {b->pairs, b->leaves, b->tri_verts, b->slot_prim, b->spheres, b->sphere_aux, b->tri_orig,
+                    b->prim_info, b->mats, b->lights, b->sph_shade, b->d_items, b->film,
                     b->d_pass, b->d_queue, b->d_root};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -219,8 +217,8 @@ int bg_batch_create(bg_ctx* ctx, const bg_batch_item* items, uint32_t n, bg_batc
     const uint64_t np = bt->n_tris + bt->n_spheres;
     std::vector<bg::PairRecord> pairs(bt->n_pairs);
     std::vector<uint32_t> leaves(2 * bt->n_leaves);
-    std::vector<uint32_t> slots(bt->n_slots), pmat(np);
-    std::vector<int32_t> plight(np, -1);
+    std::vector<uint32_t> slots(bt->n_slots);
+    std::vector<uint2> pinfo(np);
     std::vector<bg::LightDev> lts(bt->n_lights);
     for (size_t u = 0; u < U; ++u) {
         const bg_scene* s = uniq[u];
@@ -245,8 +243,8 @@ int bg_batch_create(bg_ctx* ctx, const bg_batch_item* items, uint32_t n, bg_batc
         };
         for (size_t k = 0; k < s->bvh.slot_prim.size(); ++k) slots[slot_base[u] + k] = (uint32_t)global(s->bvh.slot_prim[k]);
         for (uint64_t p = 0; p < nt + s->n_spheres; ++p) {
-            pmat[global(p)] = s->prim_material[p] + (uint32_t)mat_base[u];
-            if (s->prim_light[p] >= 0) plight[global(p)] = s->prim_light[p] + (int32_t)light_base[u];
+            const int32_t li = s->prim_light[p] >= 0 ? s->prim_light[p] + (int32_t)light_base[u] : -1;
+            pinfo[global(p)] = bg::prim_info_of(li, s->prim_material[p] + (uint32_t)mat_base[u], p < nt && s->tri_flip[p] != 0);
         }
     }
     uint64_t bytes = 0;
@@ -261,12 +259,10 @@ int bg_batch_create(bg_ctx* ctx, const bg_batch_item* items, uint32_t n, bg_batc
     up((bg::PairRecord**)&bt->pairs, pairs.data(), pairs.size());
     up((uint32_t**)&bt->leaves, leaves.data(), leaves.size());
     up(&bt->slot_prim, slots.data(), slots.size());
-    up(&bt->prim_mat, pmat.data(), pmat.size());
-    up(&bt->prim_light, plight.data(), plight.size());
+    up(&bt->prim_info, pinfo.data(), pinfo.size());
     // device-to-device copies of the per-scene arrays that need no rebasing
     if (!rc) rc = dalloc(ctx, &bt->tri_verts, 9 * bt->n_slots);
     if (!rc) rc = dalloc(ctx, &bt->tri_orig, 9 * bt->n_tris);
-    if (!rc) rc = dalloc(ctx, &bt->tri_flip, bt->n_tris);
     if (!rc) rc = dalloc(ctx, &bt->spheres, bt->n_spheres);
     if (!rc) rc = dalloc(ctx, &bt->sphere_aux, bt->n_spheres);
     if (!rc) rc = dalloc(ctx, &bt->sph_shade, bt->n_spheres);
@@ -281,7 +277,6 @@ int bg_batch_create(bg_ctx* ctx, const bg_batch_item* items, uint32_t n, bg_batc
         };
         d2d(bt->tri_verts + 9 * slot_base[u], s->d_tri_verts, 9 * ns * sizeof(float));
         d2d(bt->tri_orig + 9 * tri_base[u], s->d_tri_orig, 9 * s->n_tris * sizeof(float));
-        d2d(bt->tri_flip + tri_base[u], s->d_tri_flip, s->n_tris);
         d2d(bt->spheres + sph_base[u], s->d_spheres, s->n_spheres * sizeof(bg::SphereDev));
         d2d(bt->sphere_aux + sph_base[u], s->d_sphere_aux, s->n_spheres * sizeof(bg::SphereAux));
         d2d(bt->sph_shade + sph_base[u], s->d_sph_shade, s->n_spheres * sizeof(bg::SphereShadeDev));
@@ -400,9 +395,7 @@ int bg_render_batch(bg_ctx* ctx, bg_batch* bt, const bg_batch_desc* d, bg_stats*
     std::memset(&p.cam, 0, sizeof(p.cam));
     bg::ShadeSceneDev sc;
     sc.tri_orig = bt->tri_orig;
-    sc.tri_flip = bt->tri_flip;
-    sc.prim_mat = bt->prim_mat;
-    sc.prim_light = bt->prim_light;
+    sc.prim_info = bt->prim_info;
     sc.mats = bt->mats;
     sc.lights = bt->lights;
     sc.spheres = bt->sph_shade;

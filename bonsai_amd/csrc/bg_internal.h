@@ -146,11 +146,20 @@ struct SphereShadeDev {
     uint32_t flip;     // the normal's flip: reverse_orientation xor the transform swaps handedness
     uint32_t reverse;  // reverse_orientation alone (Sphere::sample, sphere.rs:333-335)
 };
+// ShadeSceneDev::prim_info: the orientation flag in the material word, and the record of a primitive.
+constexpr uint32_t kPrimFlip = 0x80000000u;
+inline uint2 prim_info_of(int32_t light, uint32_t material, bool flip) {
+    uint2 r;
+    r.x = (uint32_t)light;
+    r.y = material | (flip ? kPrimFlip : 0u);
+    return r;
+}
 struct ShadeSceneDev {
     const float* tri_orig;      // 9 floats per triangle, scene order
-    const uint8_t* tri_flip;
-    const uint32_t* prim_mat;
-    const int32_t* prim_light;
+    // per primitive, scene order: x its light index (as int32, -1: none), y its material index, with
+    // kPrimFlip set for a triangle of reversed orientation; one 8-byte record, so shading a hit reads
+    // one scattered line for these where three arrays took three
+    const uint2* prim_info;
     const MaterialDev* mats;
     const LightDev* lights;
     const SphereShadeDev* spheres;
@@ -418,9 +427,7 @@ struct bg_scene {
     bg::SphereAux* d_sphere_aux = nullptr;
     // shading data
     float* d_tri_orig = nullptr;
-    uint8_t* d_tri_flip = nullptr;
-    uint32_t* d_prim_mat = nullptr;
-    int32_t* d_prim_light = nullptr;
+    uint2* d_prim_info = nullptr;  // ShadeSceneDev::prim_info
     bg::MaterialDev* d_mats = nullptr;
     bg::LightDev* d_lights = nullptr;
     bg::SphereShadeDev* d_sph_shade = nullptr;

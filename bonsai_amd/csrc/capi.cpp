@@ -347,6 +347,10 @@ int bg_scene_create(bg_ctx* ctx, const bg_scene_desc* d, bg_scene** out) {
     s->prim_material.assign(np, 0);
     if (d->tri_material) std::copy(d->tri_material, d->tri_material + nt, s->prim_material.begin());
     if (d->sphere_material) std::copy(d->sphere_material, d->sphere_material + ns, s->prim_material.begin() + nt);
+    if (d->n_materials >= bg::kPrimFlip) {
+        delete s;
+        return fail(ctx, BG_ERR_INVALID, "too many materials");
+    }
     if (d->n_materials) s->materials.assign(d->materials, d->materials + d->n_materials);
     for (uint32_t m : s->prim_material)
         if (!s->materials.empty() && m >= s->materials.size()) {
@@ -412,9 +416,12 @@ int bg_scene_create(bg_ctx* ctx, const bg_scene_desc* d, bg_scene** out) {
     if (!rc) rc = upload(ctx, &s->d_sphere_aux, aux.data(), aux.size(), &bytes);
     // shading records
     if (!rc) rc = upload(ctx, &s->d_tri_orig, d->tri_positions, 9 * nt, &bytes);
-    if (!rc) rc = upload(ctx, &s->d_tri_flip, s->tri_flip.data(), nt, &bytes);
-    if (!rc) rc = upload(ctx, &s->d_prim_mat, s->prim_material.data(), np, &bytes);
-    if (!rc) rc = upload(ctx, &s->d_prim_light, s->prim_light.data(), np, &bytes);
+    {
+        std::vector<uint2> info(np);
+        for (uint64_t p = 0; p < np; ++p)
+            info[p] = bg::prim_info_of(s->prim_light[p], s->prim_material[p], p < nt && s->tri_flip[p] != 0);
+        if (!rc) rc = upload(ctx, &s->d_prim_info, info.data(), np, &bytes);
+    }
     std::vector<bg::MaterialDev> mats(s->materials.size());
     for (size_t i = 0; i < mats.size(); ++i) {
         const bg_material& m = s->materials[i];
@@ -515,7 +522,7 @@ void bg_scene_destroy(bg_scene* s) {
     if (s->d_slot_prim) hipFree(s->d_slot_prim);
     if (s->d_spheres) hipFree(s->d_spheres);
     if (s->d_sphere_aux) hipFree(s->d_sphere_aux);
-    void* extra[] = {s->d_tri_orig, s->d_tri_flip, s->d_prim_mat, s->d_prim_light, s->d_mats, s->d_lights, s->d_sph_shade};
+    void* extra[] = {s->d_tri_orig, s->d_prim_info, s->d_mats, s->d_lights, s->d_sph_shade};
     for (void* p : extra)
         if (p) hipFree(p);
     delete s;
@@ -615,7 +622,7 @@ int bg_surface_interactions(bg_ctx* ctx, const bg_scene* s, const bg_ray* rays, 
     HIP_TRY(ctx, hipMemcpyAsync(base + rb, hits, hb, hipMemcpyHostToDevice, ctx->stream));
     bg::ShadeSceneDev sc;
     sc.tri_orig = s->d_tri_orig;
-    sc.tri_flip = s->d_tri_flip;
+    sc.prim_info = s->d_prim_info;
     sc.spheres = s->d_sph_shade;
     sc.n_tris = (uint32_t)s->n_tris;
     HIP_TRY(ctx, bg::launch_surface(sc, (const float4*)base, (const uint4*)(base + rb), n, (float4*)(base + rb + hb),

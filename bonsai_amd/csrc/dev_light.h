@@ -58,7 +58,7 @@ __device__ __forceinline__ bool hit_surface(const ShadeSceneDev& sc, float4 r0, 
         rs.ox = r0.x; rs.oy = r0.y; rs.oz = r0.z; rs.t_max = r0.w;
         rs.dx = r1.x; rs.dy = r1.y; rs.dz = r1.z;
         ray_setup(rs);
-        ok = tri_interaction(rs, sc.tri_orig + 9 * (uint64_t)prim, sc.tri_flip[prim] != 0, s, t);
+        ok = tri_interaction(rs, sc.tri_orig + 9 * (uint64_t)prim, (sc.prim_info[prim].y & kPrimFlip) != 0, s, t);
     } else {
         RayState rs;
         rs.ox = r0.x; rs.oy = r0.y; rs.oz = r0.z; rs.t_max = r0.w;
@@ -246,7 +246,7 @@ __device__ __forceinline__ LightPoint sample_light_point(const ShadeSceneDev& sc
         const float c0 = 1.0f - sq, c1 = ul1 * sq, c2 = 1.0f - c0 - c1;
         o.p = c0 * p0 + c1 * p1 + c2 * p2;
         V ln = unit(cross(p1 - p0, p2 - p0));
-        if (sc.tri_flip[light.prim]) ln = ln * -1.0f;
+        if (sc.prim_info[light.prim].y & kPrimFlip) ln = ln * -1.0f;
         o.n = ln;
         const V labs = vabs(c0 * p0) + vabs(c1 * p1) + vabs(c2 * p2);
         o.err = gamma_n(6.0f) * labs;
@@ -322,7 +322,7 @@ __device__ __forceinline__ float light_point_pdf(const ShadeSceneDev& sc, const 
         tri_verts(v, &p0, &p1, &p2);
         hp = b0 * p0 + b1 * p1 + b2 * p2;
         hn = unit(cross(p0 - p2, p1 - p2));
-        if (sc.tri_flip[l.prim]) hn = -hn;
+        if (sc.prim_info[l.prim].y & kPrimFlip) hn = -hn;
         area = l.area;
     }
     float pdf = len2(s.p - hp) / (__builtin_fabsf(dot(hn, -wi)) * area);
@@ -348,7 +348,7 @@ __device__ __forceinline__ C3 emission_at(const ShadeSceneDev& sc, int32_t li, u
         V p0, p1, p2;
         tri_verts(sc.tri_orig + 9 * (uint64_t)prim, &p0, &p1, &p2);
         n = unit(cross(p0 - p2, p1 - p2));
-        if (sc.tri_flip[prim]) n = -n;
+        if (sc.prim_info[prim].y & kPrimFlip) n = -n;
     }
     return emission(sc.lights[li], n, mkv(-r1.x, -r1.y, -r1.z));
 }

@@ -99,9 +99,10 @@ __global__ __launch_bounds__(128) void k_direct_shade(RenderParamsDev p, ShadeSc
     const uint64_t key = key_of(b.key[i]);
     uint32_t dim = b.state[i].x;
     C3 L = mkc(0.0f);
-    const int32_t hit_light = sc.prim_light[prim];
+    const uint2 info = sc.prim_info[prim];
+    const int32_t hit_light = (int32_t)info.x;
     if (hit_light >= 0) L = L + emission(sc.lights[hit_light], s.n, s.wo);  // it.emitted_radiance(&it.direction)
-    const BsdfN<MULTI ? 2 : 1> bs = make_bsdf<MULTI ? 2 : 1>(sc.mats[sc.prim_mat[prim]], s, ref);
+    const BsdfN<MULTI ? 2 : 1> bs = make_bsdf<MULTI ? 2 : 1>(sc.mats[info.y & ~kPrimFlip], s, ref);
     const uint32_t nl = sc.n_lights;
     if (KIND == 0) {  // integrator.rs:274-306, one sample per light (light.rs:52-54)
         for (uint32_t j = 0; j < nl; ++j) {
@@ -162,7 +163,7 @@ __global__ __launch_bounds__(256) void k_direct_resolve(RenderParamsDev p, Shade
             ++queries;
             const uint32_t hp = d.mhit[slot].x;
             if (hp != 0xFFFFFFFFu) {
-                const int32_t li = sc.prim_light[hp];
+                const int32_t li = (int32_t)sc.prim_info[hp].x;
                 const float2 c = d.c[slot];
                 // integrator.rs:412-419: the reference accepts any area light, pbrt only the sampled one
                 const bool same = p.mode == 0 ? (li >= 0) : (li == (int32_t)__float_as_uint(c.y));

@@ -172,14 +172,24 @@ __device__ __forceinline__ uint32_t shade_path(const RenderParamsDev& p, const S
     // rays this path has issued: low 16 bits all, high 16 bits any-hit (shadow) queries
     uint32_t nrays = __float_as_uint(b4.w);
     C3 beta = C3{b4.x, b4.y, b4.z};
-    const float4 l4 = b.L[i];
+    // radiance so far (.w: the continuation pdf of the previous vertex, guided only). The plain
+    // estimator changes it here only where the ray hit an emitter, so only there does it read and
+    // write it.
+    float4 l4 = GUIDED ? b.L[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     C3 L = C3{l4.x, l4.y, l4.z};
+    bool l_changed = GUIDED;
     const V rd = mkv(r1.x, r1.y, r1.z);
 
     // path.rs:78-86
-    const int32_t hit_light = sc.prim_light[prim];
+    const uint2 info = sc.prim_info[prim];
+    const int32_t hit_light = (int32_t)info.x;
     if (hit_light >= 0) {
         if (bounce == 0 || (flags & kSpecFlag)) {
+            if (!GUIDED) {
+                l4 = b.L[i];
+                L = C3{l4.x, l4.y, l4.z};
+                l_changed = true;
+            }
             L = L + beta * emission(sc.lights[hit_light], s.n, -rd);
         } else if (GUIDED) {
             // guided estimator: emission reached by the continuation, MIS-weighted against light
@@ -197,7 +207,7 @@ __device__ __forceinline__ uint32_t shade_path(const RenderParamsDev& p, const S
     float cont_pdf = 0.0f;
     if (alive) {
         // matte.rs:20-41 / conductor / plastic.rs:24-52; bsdf.rs:33-52
-        const MaterialDev& m = sc.mats[sc.prim_mat[prim]];
+        const MaterialDev& m = sc.mats[info.y & ~kPrimFlip];
         const BsdfN<MULTI ? 2 : 1> bs = make_bsdf<MULTI ? 2 : 1>(m, s, ref);
         const uint32_t nonspec = kAll & ~kSpecular;
         // network mixture of this vertex (guided estimator with guide_alpha > 0)
@@ -349,7 +359,7 @@ __device__ __forceinline__ uint32_t shade_path(const RenderParamsDev& p, const S
     if (!alive) kill_ray(b.ray, i);
     else nrays += 1u;
     b.beta[i] = make_float4(beta.r, beta.g, beta.b, __uint_as_float(nrays));
-    b.L[i] = make_float4(L.r, L.g, L.b, cont_pdf);  // .w: continuation pdf, for the next vertex's MIS
+    if (l_changed) b.L[i] = make_float4(L.r, L.g, L.b, cont_pdf);  // .w: continuation pdf, for the next vertex's MIS
     b.state[i] = make_uint2(dim, alive ? (flags | kAlive) : 0u);
     return (alive && ext_t > 0.0f ? 1u : 0u) | (shadow_t > 0.0f ? 2u : 0u) | (mis_t > 0.0f ? 4u : 0u);
 }
@@ -377,7 +387,7 @@ __device__ __forceinline__ void nee_resolve_path(const RenderParamsDev& p, const
     if (nee & kHasBsdf) {
         const uint32_t hp = b.mhit[i].x;
         if (hp != 0xFFFFFFFFu) {
-            const int32_t li = sc.prim_light[hp];
+            const int32_t li = (int32_t)sc.prim_info[hp].x;
             const float4 kb = b.nee_b[i];
             const bool same = p.mode == 0 ? (li >= 0) : (li == (int32_t)__float_as_uint(kb.w));
             if (same) {

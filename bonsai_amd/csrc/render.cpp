@@ -404,9 +404,7 @@ int check_render(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, const 
 bg::ShadeSceneDev shade_scene(const bg_scene* s) {
     bg::ShadeSceneDev sc;
     sc.tri_orig = s->d_tri_orig;
-    sc.tri_flip = s->d_tri_flip;
-    sc.prim_mat = s->d_prim_mat;
-    sc.prim_light = s->d_prim_light;
+    sc.prim_info = s->d_prim_info;
     sc.mats = s->d_mats;
     sc.lights = s->d_lights;
     sc.spheres = s->d_sph_shade;
