@@ -456,17 +456,26 @@ inline uint32_t has_multi_lobe(const std::vector<bg_material>& mats) {
 }
 }  // namespace bg
 
+// Scenes from this many primitives up (triangles and spheres) are rendered at the context's render
+// shape, smaller ones at its query shape (measured on both sides: 500k and 1.5M-2M primitives).
+constexpr uint64_t kRenderShapeMinPrims = 1ull << 20;
 struct bg_ctx {
     int device = 0;
     int num_cus = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
-    bg::LaunchShape shape;  // bg_intersect* queries, direct-lighting and Whitted renders
-    // path and guided renders: measured on the 1080p path-tracer scene (64 spp, 2.05M triangles + 256
+    bg::LaunchShape shape;  // bg_intersect* queries, direct-lighting and Whitted renders, small scenes
+    // path and guided renders of scenes of at least kRenderShapeMinPrims primitives: measured on the 1080p path-tracer scene (64 spp, 2.05M triangles + 256
     // spheres): 0.130 s against 0.138 s with the queries' shape (256, 8, 32, variant 4), the best
     // of 72 shapes; on the door-gap room (triangles only) 0.192 s against 0.209 s (plain, 32 spp)
     bg::LaunchShape render_shape{128, 8, 16, 5, true};
+    // the path and guided renders' shape for a scene: the atomic-queue refill pays on large trees
+    // and loses on small ones (the 64 batch scenes of 1k-500k triangles rendered one call each:
+    // 0.49 s against 0.38-0.39 s with the queries' shape; every film size lost)
+    const bg::LaunchShape& render_shape_for(uint64_t n_prims) const {
+        return n_prims >= kRenderShapeMinPrims ? render_shape : shape;
+    }
     // reusable device buffers
     uint32_t* spill = nullptr;
     unsigned long long* counters = nullptr;

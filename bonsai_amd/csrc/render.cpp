@@ -192,7 +192,8 @@ bg::TraverseArgs traverse_args(const bg_scene* s, bg_ctx* ctx, const float4* ray
 hipError_t trace_pass(const bg_scene* s, bg_ctx* ctx, const float4* rays, uint4* hits, uint64_t n, bool any, int grid,
                       int set, const uint64_t* mask, hipStream_t st) {
     bg::TraverseArgs a = traverse_args(s, ctx, rays, hits, n, set);
-    if (bg::traverse_compacts(a, ctx->render_shape, grid)) {
+    const bg::LaunchShape& shape = ctx->render_shape_for(s->n_tris + s->n_spheres);
+    if (bg::traverse_compacts(a, shape, grid)) {
         const bg::PathBuffers& b = ctx->paths;
         const hipError_t e =
             bg::launch_ray_compact(rays, n, b.rblk[set], b.rmask[set], b.ridx[set], b.rcount[set], hits, st, mask);
@@ -200,7 +201,7 @@ hipError_t trace_pass(const bg_scene* s, bg_ctx* ctx, const float4* rays, uint4*
         a.idx = b.ridx[set];
         a.n_idx = b.rcount[set];
     }
-    return bg::profiled_traverse(ctx, a, any, ctx->render_shape, grid, st);
+    return bg::profiled_traverse(ctx, a, any, shape, grid, st);
 }
 
 }  // namespace
@@ -436,8 +437,9 @@ int render_impl(bg_ctx* ctx, const bg_scene* s, const bg_render_desc* d, bg_film
     const uint64_t cap = npix * spp_pass;
     rc = bg::ensure_paths(ctx, cap);
     if (rc) return rc;
-    const int block = ctx->render_shape.block;
-    const uint64_t grid_cap = ctx->render_shape.blocks_per_cu > 0 ? (uint64_t)ctx->num_cus * ctx->render_shape.blocks_per_cu : (1u << 20);
+    const bg::LaunchShape& shape = ctx->render_shape_for(s->n_tris + s->n_spheres);
+    const int block = shape.block;
+    const uint64_t grid_cap = shape.blocks_per_cu > 0 ? (uint64_t)ctx->num_cus * shape.blocks_per_cu : (1u << 20);
     const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((cap + block - 1) / block, std::min<uint64_t>(grid_cap, 1u << 20)));
     // spill rows for each of the three ray sets, which trace at once
     const size_t per_set = (((size_t)grid * block * (s->bvh.max_depth + 1) * bg::kSpillWords * sizeof(uint32_t) + 255) / 256) * 256;

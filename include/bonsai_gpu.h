@@ -237,10 +237,11 @@ int bg_surface_interactions(bg_ctx* ctx, const bg_scene* scene, const bg_ray* ra
                             bg_surface* out);
 
 /* Traversal launch shape (chosen by the autotuner or the caller). A context
- * starts with two: bg_intersect* queries and the direct-lighting and Whitted
- * renders use 256 threads, an 8-entry stack, 32 blocks per CU and variant 4;
- * the path and guided renders use 128 threads, an 8-entry stack, 16 blocks
- * per CU and variant 5. bg_set_launch_config sets the one shape both then use;
+ * starts with two: bg_intersect* queries, the direct-lighting and Whitted
+ * renders and the renders of scenes under 2^20 primitives use 256 threads, an
+ * 8-entry stack, 32 blocks per CU and variant 4; the path and guided renders
+ * of larger scenes use 128 threads, an 8-entry stack, 16 blocks per CU and
+ * variant 5. bg_set_launch_config sets the one shape both then use;
  * bg_get_launch_config reads the first. Scene batches tune their own
  * (bg_batch_launch_config). */
 typedef struct bg_launch_config {
