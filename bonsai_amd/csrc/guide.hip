@@ -257,10 +257,10 @@ __global__ __launch_bounds__(kInferBlock) __attribute__((amdgpu_waves_per_eu(kIn
         f16v acc[2];
         // an offset the compiler cannot see through, so the weight reads stay in this loop
         // instead of being hoisted out of it into registers that would then spill
-        int wo = 0;
-        asm volatile("" : "+s"(wo));
+        int w_off = 0;
+        asm volatile("" : "+s"(w_off));
         for (int l = 0; l < kLayers; ++l) {
-            layer_mm<true>(wl + wo + l * kLayerLds, xb, lane, acc, bl + l * 64, l == kLayers - 1);
+            layer_mm<true>(wl + w_off + l * kLayerLds, xb, lane, acc, bl + l * 64, l == kLayers - 1);
             if (l < kLayers - 1) relu_pack(acc, xb);
         }
         // this lane holds outputs 20h .. 20h + 19 of sample l % 32 (lobes 4h .. 4h + 3): accumulators
