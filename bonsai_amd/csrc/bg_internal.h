@@ -312,7 +312,8 @@ hipError_t launch_batch_film(const RenderParamsDev& p, const PathBuffers& b, uin
 hipError_t launch_pass_advance(uint32_t* pass_first, uint32_t spp, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long* p, hipStream_t s);
 // A plain pass as one persistent launch (k_path_persistent): zeroes *queue, then `grid` blocks of
-// 256 threads run whole paths of 64-path groups taken from it. The three argument sets are the
+// 256 threads run whole paths, each lane taking a new path slot (from runs of 64 its wavefront
+// claims off the counter) when its path ends. The three argument sets are the
 // extension, shadow and MIS-branch queries (their spill rows: ext's, one row per thread).
 hipError_t launch_path_persistent(const RenderParamsDev& p, const ShadeSceneDev& sc, const PathBuffers& b,
                                   const TraverseArgs& ext, const TraverseArgs& sha, const TraverseArgs& mis,

@@ -59,7 +59,7 @@ def test_batch_equals_per_scene(ctx, items, mode, graph):
 @pytest.mark.parametrize("mode", [MODE_REF, MODE_CORRECTED])
 @pytest.mark.parametrize("graph", [True, False])
 def test_persistent_schedule_equals_per_scene(ctx, items, mode, graph):
-    # one persistent path kernel per pass (64-path groups from an atomic counter): the same films
+    # one persistent path kernel per pass (lanes take path slots from an atomic counter): the same films
     # and ray counts as the wavefront schedule and as each item rendered alone
     spp, depth = 4, 6
     ref = per_item(ctx, items, spp, depth, mode)
